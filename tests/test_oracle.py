@@ -1,0 +1,190 @@
+"""Pins of the FP64 oracle: golden vectors, closed forms, invariants (CPU).
+
+Every reference-held fixture for the path is checked here at the tolerance
+the reference's own test uses:
+
+* ``ref_TQU_signal.dat`` / ``ref_T1T2_signal.dat`` (rtol 1e-12, atol 1e-20;
+  ``cosmocore/tests/test_signal.py:30-212``),
+* the nside-8 Fortran suite (signal 1e-10, invcov 1e-12, Fisher diag 1e-6,
+  spectra 1e-10, noise bias 1e-6; ``qube/tests/test_fortran_validation.py``),
+* nside-4 T/QU/TQU/TEB Fisher (atol 1e-3, rtol 1e-5; ``qube/tests/test_fisher.py``),
+* v1.1.0 pixel-direct pins (rel 1e-10; ``cosmocore/tests/test_core_pipeline.py``),
+* PICSLike Fortran chi2 / lnL (1e-9; ``picslike/tests/test_fortran_validation.py``).
+
+Closed forms (independent of the recurrences): P_l from scipy, P^02 from
+``scipy.special.lpmv``, and F1 +- F2 = (2l+1)/(4 pi) d^l_{2,-+2} from Jacobi
+polynomials, up to l = 128.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy.special import eval_jacobi, eval_legendre, lpmv
+
+from oracle import healpix
+from oracle import kernels as K
+from oracle import linalg as LA
+from oracle import pipeline as P
+from tests import cases
+from tests import oracle_run as R
+
+XS = [-1.0, -0.999999, -0.93, -0.41, 0.0, 0.123, 0.5, 0.87, 0.999999, 1.0]
+
+
+@pytest.mark.parametrize("x", XS)
+def test_scalar_recurrences_python_equals_c(x):
+    assert np.array_equal(K.legendre_00_py(x, 128), K.legendre_c("00", x, 128))
+    for a, b in zip(K.legendre_22_py(x, 128), K.legendre_c("22", x, 128)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(K.legendre_02_py(x, 128), K.legendre_c("02", x, 128))
+
+
+@pytest.mark.parametrize("x", XS)
+def test_legendre_closed_forms(x):
+    L = 128
+    ell = np.arange(L + 1)
+    n = (2 * ell + 1) / (4 * math.pi)
+    p = K.legendre_00_py(x, L)
+    np.testing.assert_allclose(p, n * eval_legendre(ell, x), rtol=1e-11, atol=1e-13)
+    p02 = K.legendre_02_py(x, L)
+    e = ell[2:]
+    ref02 = n[2:] * lpmv(2, e, x) / np.sqrt((e + 2.0) * (e + 1) * e * (e - 1))
+    scale = np.max(np.abs(ref02)) + 1e-300
+    np.testing.assert_allclose(p02[2:] / scale, ref02 / scale, atol=1e-12)
+    _, f1, f2 = K.legendre_22_py(x, L)
+    d22 = ((1 + x) / 2) ** 2 * np.array([eval_jacobi(k - 2, 0, 4, x) for k in e])
+    d2m2 = ((1 - x) / 2) ** 2 * np.array([eval_jacobi(k - 2, 4, 0, x) for k in e])
+    s = np.max(np.abs(np.r_[n[2:] * d22, n[2:] * d2m2])) + 1e-300
+    np.testing.assert_allclose((f1[2:] + f2[2:]) / s, n[2:] * d2m2 / s, atol=1e-12)
+    np.testing.assert_allclose((f1[2:] - f2[2:]) / s, n[2:] * d22 / s, atol=1e-12)
+    assert np.all(f1[:2] == 0) and np.all(f2[:2] == 0)
+
+
+def test_rotation_angle_special_cases():
+    z = np.array([0.0, 0.0, 1.0])
+    assert K.rotation_angle(z, z) == (0.0, 0.0)
+    assert K.rotation_angle(z, -z) == (0.0, 0.0)
+    # Two equator points: r12 = +z, projected frames (0, 1) and (-1, 0); both
+    # dots vanish, so a12 = 2 acos(0) = pi and a21 = -pi (the flipped r12 has z < 0).
+    a12, a21 = K.rotation_angle(np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
+    assert a12 == pytest.approx(math.pi, abs=1e-15) and a21 == pytest.approx(-math.pi, abs=1e-15)
+    # Meridian pair in the x-z plane: r12 = +y, so dot1 = 1 (a12 = 0, sign from
+    # r12_z = 0 is negative) and dot2 = -1 (a21 = -2 pi).
+    r1 = np.array([math.sin(0.3), 0.0, math.cos(0.3)])
+    r2 = np.array([math.sin(0.9), 0.0, math.cos(0.9)])
+    a12, a21 = K.rotation_angle(r1, r2)
+    assert a12 == pytest.approx(0.0, abs=1e-7) and a21 == pytest.approx(-2 * math.pi, abs=1e-7)
+
+
+def test_ring_pointings_match_geometry_dat():
+    c = cases.case_fortran_b()
+    g = c.expect["geometry"]
+    v = healpix.pointings(8, g[:, 0].astype(int), nest=False)
+    np.testing.assert_array_equal(v, g[:, 1:4])
+
+
+@pytest.mark.parametrize("builder", [cases.case_signal_tqu, cases.case_signal_t1t2])
+def test_signal_golden(builder):
+    c = builder()
+    lay = R.layout(c)
+    cls_s, _ = R.smoothed(c, lay)
+    S = K.signal_matrix(lay, cls_s, c.lmax_signal)
+    np.testing.assert_allclose(S, c.expect["S"], rtol=1e-12, atol=1e-20)
+
+
+def test_fortran_suite():
+    c = cases.case_fortran_b()
+    lay = R.layout(c)
+    cls_s, beams = R.smoothed(c, lay)
+    prm = R.params(c, lay, beams)
+    F, Cinv, S = P.fisher(lay, cls_s, c.lmax_signal, c.noise, prm)
+    ref = c.expect["S"]
+    rel = np.where(np.abs(ref) > 1e-20, np.abs((S - ref) / ref), 0)
+    assert rel.max() < 1e-10
+    d = c.expect["invcov_diag"]
+    assert np.max(np.abs((np.diag(Cinv) - d) / d)) < 1e-12
+    bs = R.beam_smoothing(c, lay, beams)
+    e = np.arange(c.lmin, c.lmax + 1.0)
+    dd = e * (e + 1) / (2 * np.pi)
+    fr = F / np.outer(bs, bs) * np.outer(dd, dd)
+    fd = np.diag(c.expect["fisher"])
+    assert np.max(np.abs((np.diag(fr) - fd) / fd)) < 1e-6
+    q, nb, ch, nbh = P.qml(lay, Cinv, c.noise, c.maps, F, prm)
+    sp = c.expect["spectra"]
+    m = min(sp.size, ch.shape[1])
+    assert np.max(np.abs((ch[0, :m] - sp[:m]) / sp[:m])) < 1e-10
+    nr = c.expect["noise_bias"]
+    m = min(nr.size, nbh.size)
+    assert np.max(np.abs((nbh[:m] - nr[:m]) / nr[:m])) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["T", "QU", "TQU", "TEB"])
+def test_fisher_nside4(name):
+    c = cases.case_fisher_nside4(name)
+    lay = R.layout(c)
+    cls_s, beams = R.smoothed(c, lay)
+    prm = R.params(c, lay, beams)
+    F, _, _ = P.fisher(lay, cls_s, c.lmax_signal, c.noise, prm)
+    bs = R.beam_smoothing(c, lay, beams)
+    np.testing.assert_allclose(F / np.outer(bs, bs), c.expect["fisher"], atol=1e-3, rtol=1e-5)
+
+
+def test_v110_pins():
+    c = cases.case_v110()
+    lay = R.layout(c)
+    cls_s, _ = R.smoothed(c, lay)
+    S = K.signal_matrix(lay, cls_s, c.lmax_signal)
+    C = S + c.noise
+    x = c.maps[:, 0]
+    got = {"logdet": LA.matrix_slogdet_symm(C)[1],
+           "quadform": float(x @ LA.matrix_inverse_symm(C) @ x),
+           "cov_trace": float(np.trace(C)), "cov_fro": float(np.linalg.norm(C))}
+    for k, v in c.expect.items():
+        assert got[k] == pytest.approx(v, rel=1e-10), k
+
+
+def test_picslike_fortran():
+    c, points, res = cases.case_picslike_b()
+    lay = R.layout(c)
+    lnls = []
+    for k, cl in enumerate(points):
+        cls_s, _ = R.smoothed(c, lay, cl)
+        chi2, logdet, lnl = P.picslike_point(lay, cls_s, c.lmax_signal, c.noise, c.maps)
+        assert abs(chi2[0] / res[k, 3] - 1) < 1e-9
+        lnls.append(lnl[0])
+    scale = np.abs(res[:, 2]).mean()
+    assert np.max(np.abs(np.array(lnls) - res[:, 2])) / scale < 1e-9
+
+
+def test_signal_is_linear_in_cl_and_bins_sum_per_ell():
+    """S(C) = sum_l C_l dS/dC_l and a binned derivative = sum of per-l ones."""
+    c = cases.case_signal_tqu()
+    lay = R.layout(c)
+    cls_s, _ = R.smoothed(c, lay)
+    rng = np.random.default_rng(3)
+    cls_s = dict(cls_s)
+    cls_s["EB"] = 1e-6 * rng.random(c.lmax_signal + 1)
+    cls_s["TB"] = 1e-6 * rng.random(c.lmax_signal + 1)
+    S = K.signal_matrix(lay, cls_s, c.lmax_signal)
+    acc = np.zeros_like(S)
+    for fi, fj, mode, lab in lay.spectra_list():
+        for ell in range(c.lmax_signal + 1):
+            w = np.zeros(c.lmax_signal + 1)
+            w[ell] = cls_s[lab][ell]
+            if w[ell] != 0:
+                acc += K.binned_derivative(lay, fi, fj, mode, w)
+    np.testing.assert_allclose(acc, S, rtol=1e-11, atol=1e-11 * np.abs(S).max())
+
+
+def test_inverse_and_errors():
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((50, 50))
+    M = A @ A.T + 50 * np.eye(50)
+    np.testing.assert_allclose(LA.matrix_inverse_symm(M) @ M, np.eye(50), atol=1e-12)
+    assert LA.matrix_slogdet_symm(M)[1] == pytest.approx(np.linalg.slogdet(M)[1], rel=1e-12)
+    M[3, 3] = -1.0
+    with pytest.raises(ValueError, match="dpotrf failed with info="):
+        LA.matrix_inverse_symm(M)
+    with pytest.raises(np.linalg.LinAlgError):
+        LA.cholesky_factor(M)
