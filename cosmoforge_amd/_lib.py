@@ -1,0 +1,81 @@
+"""ctypes binding of libcfmi.so (the C ABI declared in include/cfmi.h).
+
+The library is loaded from this package directory.  If it is missing the
+import raises: there is no CPU fallback for the product path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfmi.so")
+
+CFMI_OK = 0
+CFMI_F64 = 0
+CFMI_F32 = 1
+CFMI_BF16 = 2
+
+_c = ctypes
+vp = _c.c_void_p
+i64 = _c.c_int64
+ci = _c.c_int
+cd = _c.c_double
+
+# name -> (restype, argtypes); kept in the order of include/cfmi.h.
+SIGNATURES = {
+    "cfmi_version": (ci, []),
+    "cfmi_create": (ci, [ci, vp, _c.POINTER(vp)]),
+    "cfmi_destroy": (ci, [vp]),
+    "cfmi_last_error": (_c.c_char_p, [vp]),
+    "cfmi_synchronize": (ci, [vp]),
+    "cfmi_pointings": (ci, [vp, i64, ci, vp, i64, vp]),
+    "cfmi_pointings_host": (ci, [i64, ci, vp, i64, vp]),
+    "cfmi_signal_00": (ci, [vp, vp, ci, vp, i64, vp, i64, ci, vp, i64, vp, i64]),
+    "cfmi_signal_22": (ci, [vp, vp, vp, vp, ci, vp, i64, vp, i64]),
+    "cfmi_signal_02": (ci, [vp, vp, vp, ci, vp, i64, vp, i64, vp, i64, vp, i64]),
+    "cfmi_deriv_stack_00": (ci, [vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, ci, vp, i64, i64, vp, i64, i64]),
+    "cfmi_deriv_stack_22": (ci, [vp, vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, vp, vp, i64, i64]),
+    "cfmi_deriv_stack_02": (ci, [vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, vp, vp, i64, i64, vp, vp,
+                                 i64, i64]),
+    "cfmi_gemm": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, ci]),
+    "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
+    "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
+    "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),
+    "cfmi_logdet_chol": (ci, [vp, i64, vp, i64, vp, _c.POINTER(cd)]),
+    "cfmi_scale": (ci, [vp, ci, i64, i64, vp, i64, vp]),
+    "cfmi_axpby": (ci, [vp, i64, i64, cd, vp, i64, cd, vp, i64]),
+    "cfmi_convert": (ci, [vp, ci, i64, i64, vp, i64, vp, i64]),
+    "cfmi_fill": (ci, [vp, ci, i64, i64, vp, i64, cd]),
+    "cfmi_gram": (ci, [vp, ci, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
+}
+
+_lib = None
+
+
+class CfmiError(RuntimeError):
+    """A libcfmi call returned an API / HIP error."""
+
+
+def load():
+    """Load libcfmi.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m cosmoforge_amd.build` "
+            "(the HIP path has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)

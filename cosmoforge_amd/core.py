@@ -1,0 +1,273 @@
+"""Shared input handling of Fisher / Spectra / PICSLike (host side).
+
+Mirrors the reference's ``Core`` seams for in-memory inputs
+(``src/cosmoforge.cosmocore/cosmocore/core.py:224-766``, ADR-0017):
+
+* ``params``: dict, YAML path or an object with attributes; defaults follow
+  ``settings.py:178-253``; the multipole chain
+  ``max(lmin_signal) <= lmin <= lmax <= lmax_signal`` clamps ``lmax``
+  (``core.py:378-401``);
+* ``mask`` (npix,) or (npix, ncomp), active = > 0.5, Q/U columns must agree
+  (``core.py:403-472``);
+* ``noise_cov1``: reduced (n, n) float64 in active-row order, used as-is;
+* ``cls_data``: {label: physical C_l}, l-indexed, every spectrum label
+  required (``spectra_io.py:257-326``);
+* ``beam``: (>= 3, >= lmax_signal + 1) T/E/B windows; wins over
+  ``smoothing_type`` (``beam.py:128-221``).
+"""
+
+from __future__ import annotations
+
+import math
+import os
+from types import SimpleNamespace
+
+import numpy as np
+
+from .layout import Layout, build_layout
+
+DEFAULTS = dict(
+    nside=16, spins=[0, 2], labels=["T", "E", "B"], physical_labels=None, feedback=1,
+    do_cross=False, smoothing_type="cosine_legacy", apply_pixwin=True, fwhmarcmin=440.0, lmax=64,
+    delta_ell=1, bin_lmins=None, bin_lmaxs=None, ordering="RING", nsims=None, remove_nb=True,
+    input_convention="Cl", output_convention="Cl", lmin_signal=2, lmax_signal=None, lmin=2,
+    parameters={}, fiducialfile=None,
+)
+
+
+def read_params(params) -> SimpleNamespace:
+    """Resolve ``params`` (dict / YAML path / object) over the defaults."""
+    values = dict(DEFAULTS)
+    if params is None:
+        pass
+    elif isinstance(params, dict):
+        values.update(params)
+    elif isinstance(params, (str, os.PathLike)):
+        import yaml
+
+        with open(params) as fh:
+            values.update(yaml.safe_load(fh) or {})
+    elif hasattr(params, "__dict__"):
+        values.update({k: v for k, v in vars(params).items() if not k.startswith("_")})
+    else:
+        raise TypeError("params must be an instance of InputParams, a string with the path to a "
+                        "parameter file, or a dictionary with parameters.")
+    p = SimpleNamespace(**values)
+    if isinstance(p.labels, str):
+        p.labels = list(p.labels)
+    p.nfields = sum(1 if s == 0 else 2 for s in p.spins)
+    ls = p.lmin_signal
+    ls = [ls] * len(p.spins) if isinstance(ls, int) else list(ls)
+    if len(ls) != len(p.spins):
+        raise ValueError(f"len(lmin_signal)={len(ls)} != len(spins)={len(p.spins)}")
+    for i, (lm, s) in enumerate(zip(ls, p.spins)):
+        if lm < abs(s):
+            raise ValueError(f"lmin_signal[{i}]={lm} invalid for spin-{s} component at index {i}; "
+                             f"minimum allowed is {abs(s)}")
+    p.lmin_signal = ls
+    lmax_signal = p.lmax_signal if p.lmax_signal is not None else 4 * p.nside
+    if p.lmin < max(ls):
+        raise ValueError(f"lmin={p.lmin} < max(lmin_signal)={max(ls)}")
+    if p.lmax is not None and p.lmax > lmax_signal:
+        p.lmax = lmax_signal
+    if p.lmax is not None and p.lmax < p.lmin:
+        raise ValueError(f"lmax={p.lmax} < lmin={p.lmin}")
+    if p.do_cross:
+        raise NotImplementedError("do_cross (two noise covariances) is not part of this build")
+    return p
+
+
+def resolve_basis(basis):
+    """``basis=`` semantics (ADR-0018).  Only the pixel-space estimator is
+    built here: ``None``/``"auto"`` resolve to it (the reference's auto picks
+    the harmonic basis, an out-of-scope algebra for the same estimator)."""
+    if basis is None or basis is False:
+        return "pixel"
+    if isinstance(basis, str):
+        b = basis.lower()
+        if b in ("auto", "pixel"):
+            return "pixel"
+        if b == "harmonic":
+            raise NotImplementedError("the harmonic basis is not part of this build; use basis=False")
+        raise ValueError(f"unknown basis {basis!r}")
+    if isinstance(basis, dict):
+        if basis.get("epsilon") or basis.get("mode_fraction") or basis.get("compression_target"):
+            raise NotImplementedError("compressed bases are not part of this build")
+        return resolve_basis(basis.get("method", "pixel"))
+    raise TypeError(f"basis must be None, False, str or dict; got {type(basis).__name__}")
+
+
+def resolve_mask(p, mask) -> np.ndarray:
+    if mask is None:
+        raise ValueError("mask= is required (file inputs are not part of this build)")
+    mask = np.asarray(mask, dtype=np.float64)
+    if mask.ndim == 1:
+        mask = mask[:, None]
+    npix = 12 * p.nside**2
+    if mask.shape[0] != npix:
+        raise ValueError(f"mask has npix={mask.shape[0]}, expected {npix} for nside={p.nside}")
+    if mask.shape[1] != p.nfields:
+        raise ValueError(f"mask has {mask.shape[1]} column(s), expected nfields={p.nfields}")
+    col = 0
+    for s in p.spins:
+        if s != 0:
+            if not np.array_equal(mask[:, col] > 0.5, mask[:, col + 1] > 0.5):
+                raise ValueError(f"mask columns {col} and {col + 1} differ, but they are the two "
+                                 "components of one spin-2 field and must share a single pixel set")
+            col += 2
+        else:
+            col += 1
+    return mask
+
+
+def coswinbeam(nside: int, ell1=None, ell2=None) -> np.ndarray:
+    """Raised-cosine window b(l) (``beam.py:27-74``)."""
+    ell1 = nside if ell1 is None else ell1
+    ell2 = 3 * nside if ell2 is None else ell2
+    if not (0 <= ell1 < ell2 <= 4 * nside):
+        raise ValueError("coswinbeam transition multipoles must satisfy 0 <= ell1 < ell2 <= 4*nside")
+    b = np.zeros(4 * nside + 1)
+    b[: ell1 + 1] = 1.0
+    ell = np.arange(ell1 + 1, ell2 + 1)
+    b[ell1 + 1: ell2 + 1] = 0.5 * (1.0 + np.cos((ell - ell1) * np.pi / (ell2 - ell1)))
+    return b
+
+
+def gaussian_beam(fwhm_rad: float, lmax: int) -> np.ndarray:
+    """(3, lmax+1) Gaussian T/E/B windows, E and B with exp(2 sigma^2)."""
+    sigma = fwhm_rad / math.sqrt(8.0 * math.log(2.0))
+    ell = np.arange(lmax + 1)
+    g = np.exp(-0.5 * ell * (ell + 1) * sigma**2)
+    pol = math.exp(2.0 * sigma**2)
+    return np.vstack([g, g * pol, g * pol])
+
+
+def resolve_beam(p, beam, lmax: int) -> np.ndarray:
+    if beam is not None:
+        b = np.asarray(beam, dtype=np.float64)
+        if b.shape[0] < 3:
+            raise ValueError(f"injected beam must have at least 3 rows (T, E, B), got {b.shape[0]}")
+        if b.shape[1] < lmax + 1:
+            raise ValueError(f"injected beam has {b.shape[1]} multipoles, need lmax_signal+1={lmax + 1}")
+        return b[:3, : lmax + 1].copy()
+    st = p.smoothing_type
+    if st == "none":
+        return np.ones((3, lmax + 1))
+    if st == "gaussian":
+        return gaussian_beam(math.radians(p.fwhmarcmin / 60.0), lmax)
+    if st in ("cosine_legacy", "cosine_npipe"):
+        b = coswinbeam(p.nside, ell1=p.nside if st == "cosine_legacy" else 1, ell2=3 * p.nside)
+        if b.size < lmax + 1:
+            raise ValueError("cosine beam shorter than lmax_signal + 1")
+        b = b[: lmax + 1]
+        return np.vstack([b, b, b])
+    if st == "file":
+        raise ValueError("smoothing_type='file' needs the beam= array (file inputs are not part of this build)")
+    raise ValueError(f"Unknown smoothtype='{st}'")
+
+
+def beam_by_label(layout: Layout, beam3: np.ndarray) -> dict:
+    """Field-map label -> window (``beam.py:261-286``)."""
+    named = {"T": 0, "E": 1, "B": 2}
+    out = {}
+    for f in layout.fields:
+        if f.spin == 0:
+            out[f.labels[0]] = beam3[named.get(f.labels[0], 0)]
+        else:
+            out[f.labels[0]] = beam3[1]
+            out[f.labels[1]] = beam3[2]
+    return out
+
+
+def smoothing_factors(layout: Layout, beams: dict, lmax: int) -> dict:
+    out = {}
+    for (i, j, _), lab in layout.spectra_map.items():
+        a, b = layout.split_label(lab, i, j)
+        if beams[a].size < lmax + 1 or beams[b].size < lmax + 1:
+            raise ValueError(f"Beam for spectrum '{lab}' is too short: expected at least {lmax + 1} multipoles")
+        out[lab] = beams[a][: lmax + 1] * beams[b][: lmax + 1]
+    return out
+
+
+def resolve_cls(layout: Layout, cls_data, lmax: int) -> dict:
+    if cls_data is None:
+        raise ValueError("cls_data= is required (file inputs are not part of this build)")
+    if isinstance(cls_data, np.ndarray):
+        if cls_data.ndim != 2 or cls_data.shape[1] != len(layout.labels):
+            raise ValueError(f"Expected {len(layout.labels)} spectra columns, got {cls_data.shape}")
+        cls_data = {lab: cls_data[:, k] for k, lab in enumerate(layout.labels)}
+    out = {}
+    for lab in layout.labels:
+        if lab not in cls_data:
+            raise ValueError(f"Missing power spectrum for {lab}")
+        arr = np.asarray(cls_data[lab], dtype=np.float64)[: lmax + 1]
+        if arr.size < lmax + 1:
+            raise ValueError(f"power spectrum {lab} has {arr.size} multipoles, need {lmax + 1}")
+        out[lab] = arr.copy()
+    return out
+
+
+def resolve_noise(n: int, noise_cov) -> np.ndarray:
+    if noise_cov is None:
+        raise ValueError("noise_cov1= is required (file inputs are not part of this build)")
+    N = np.asarray(noise_cov)
+    if N.shape != (n, n):
+        raise ValueError(f"noise_cov1 has shape {N.shape}, expected ({n}, {n}) for the active-pixel count")
+    return N
+
+
+def resolve_maps(n: int, nsims, maps) -> np.ndarray:
+    if maps is None:
+        raise ValueError("maps1= is required (file inputs are not part of this build)")
+    m = np.asarray(maps, dtype=np.float64)
+    if m.ndim == 1:
+        m = m[:, None]
+    if nsims is None:
+        nsims = m.shape[1]
+    if m.shape != (n, nsims):
+        raise ValueError(f"injected maps have shape {m.shape}, expected ({n}, {nsims}) for the "
+                         "active-pixel count and nsims")
+    return m
+
+
+def bins_from_params(p):
+    if p.bin_lmins is not None and p.bin_lmaxs is not None:
+        lo, hi = np.asarray(p.bin_lmins, int), np.asarray(p.bin_lmaxs, int)
+        keep = (hi >= 2) & (lo >= 2)
+        return list(zip(lo[keep].tolist(), hi[keep].tolist()))
+    nb = (p.lmax - p.lmin + 1) // p.delta_ell
+    lo = p.lmin + np.arange(nb) * p.delta_ell
+    return list(zip(lo.tolist(), (lo + p.delta_ell - 1).tolist()))
+
+
+def shape_weights(lmax: int, convention: str) -> np.ndarray:
+    key = str(convention).strip().lower()
+    if key not in ("cl", "dl"):
+        raise ValueError(f"Unknown shape convention '{convention}'. Must be 'Cl' or 'Dl'.")
+    if key == "cl":
+        return np.ones(lmax + 1)
+    ell = np.arange(lmax + 1, dtype=np.float64)
+    w = np.zeros(lmax + 1)
+    w[1:] = 2 * np.pi / (ell[1:] * (ell[1:] + 1))
+    return w
+
+
+class Inputs:
+    """Resolved in-memory inputs shared by the three estimators."""
+
+    def __init__(self, params, mask, noise_cov1, cls_data, beam):
+        self.params = read_params(params)
+        p = self.params
+        self.lmax_signal = p.lmax_signal if p.lmax_signal is not None else 4 * p.nside
+        self.mask = resolve_mask(p, mask)
+        self.layout = build_layout(self.mask, p.spins, p.labels)
+        self.n = self.layout.n
+        self.noise = resolve_noise(self.n, noise_cov1)
+        self.cls = resolve_cls(self.layout, cls_data, self.lmax_signal)
+        self.beam3 = resolve_beam(p, beam, self.lmax_signal)
+        self.beams = beam_by_label(self.layout, self.beam3)
+        self.smoothing = smoothing_factors(self.layout, self.beams, self.lmax_signal)
+
+    def smoothed_cls(self, cls=None) -> dict:
+        cls = self.cls if cls is None else cls
+        return {lab: cls[lab][: self.lmax_signal + 1] * self.smoothing[lab] for lab in self.layout.labels}

@@ -1,0 +1,597 @@
+// Dense linear algebra for gfx950: MFMA GEMM (FP64 / FP32), blocked
+// Cholesky, triangular inverse, Jacobi scaling, log-determinant, Gram.
+//
+// GEMM: 128 x 128 x 16 tiles, 256 threads (4 waves in a 2 x 2 grid, each
+// wave owns 64 x 64 = 4 x 4 MFMA 16x16x4 accumulators).  Global loads for
+// the next K-slab are issued into registers while the current slab is
+// consumed from LDS.  Split-K writes per-slab partials that a second
+// kernel reduces in a fixed order, so results do not depend on scheduling.
+#include "cfmi_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                  const T* __restrict__ A, int64_t lda,
+                                                  const T* __restrict__ B, int64_t ldb, T beta,
+                                                  T* __restrict__ C, int64_t ldc, int tri,
+                                                  int64_t kchunk, T* __restrict__ part)
+{
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
+    const int tid = threadIdx.x;
+    const int64_t m0 = (int64_t)blockIdx.x * BM;
+    const int64_t n0 = (int64_t)blockIdx.y * BN;
+    const int z = blockIdx.z;
+    const bool lower_only = (tri & 16) != 0;
+    if (lower_only && n0 >= m0 + BM) return;
+
+    int64_t kbeg = (int64_t)z * kchunk;
+    int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
+    if (tri & 1) kend = kend < m0 + BM ? kend : m0 + BM;
+    if (tri & 2) kbeg = kbeg > m0 ? kbeg : m0;
+    if (tri & 4) kend = kend < n0 + BN ? kend : n0 + BN;
+    if (tri & 8) kbeg = kbeg > n0 ? kbeg : n0;
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int lm = lane & 15, lk = lane >> 4;
+
+    v4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
+
+    T ra[8], rb[8];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int idx = tid + NT * r;
+            int m, k;
+            if (TA == 0) { m = idx & (BM - 1); k = idx >> 7; }
+            else { k = idx & (BK - 1); m = idx >> 4; }
+            int64_t gm = m0 + m, gk = k0 + k;
+            T v = 0;
+            if (gm < M && gk < kend)
+                v = TA == 0 ? A[gm + gk * lda] : A[gk + gm * lda];
+            ra[r] = v;
+            int n;
+            if (TB == 0) { k = idx & (BK - 1); n = idx >> 4; }
+            else { n = idx & (BN - 1); k = idx >> 7; }
+            int64_t gn = n0 + n;
+            gk = k0 + k;
+            T w = 0;
+            if (gn < N && gk < kend)
+                w = TB == 0 ? B[gk + gn * ldb] : B[gn + gk * ldb];
+            rb[r] = w;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int idx = tid + NT * r;
+            int m, k, n;
+            if (TA == 0) { m = idx & (BM - 1); k = idx >> 7; }
+            else { k = idx & (BK - 1); m = idx >> 4; }
+            As[k][m] = ra[r];
+            if (TB == 0) { k = idx & (BK - 1); n = idx >> 4; }
+            else { n = idx & (BN - 1); k = idx >> 7; }
+            Bs[k][n] = rb[r];
+        }
+    };
+
+    if (kbeg < kend) {
+        load(kbeg);
+        for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+            __syncthreads();
+            store();
+            __syncthreads();
+            if (k0 + BK < kend) load(k0 + BK);
+#pragma unroll
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = As[kk * 4 + lk][wm + i * 16 + lm];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = Bs[kk * 4 + lk][wn + j * 16 + lm];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(a[i], b[j], acc[i][j]);
+            }
+        }
+    }
+
+    // Epilogue: lane holds column lm of each 16 x 16 tile; its four values
+    // are rows lk + 4r (f64 16x16x4) or 4 lk + r (f32 16x16x4), as measured
+    // on gfx950 with an explicit layout probe.
+    constexpr bool kF64 = sizeof(T) == 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t gm = m0 + wm + i * 16 + (kF64 ? lk + 4 * r : lk * 4 + r);
+                int64_t gn = n0 + wn + j * 16 + lm;
+                if (gm >= M || gn >= N) continue;
+                if (lower_only && gn > gm) continue;
+                T v = alpha * acc[i][j][r];
+                if (part) {
+                    part[(int64_t)z * M * N + gm + gn * M] = v;
+                } else {
+                    T* p = C + gm + gn * ldc;
+                    *p = beta == T(0) ? v : v + beta * *p;
+                }
+            }
+}
+
+template <typename T>
+__global__ void splitk_reduce_kernel(int64_t M, int64_t N, int nz, const T* __restrict__ part,
+                                     T beta, T* __restrict__ C, int64_t ldc, int lower_only)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * N) return;
+    int64_t m = idx % M, n = idx / M;
+    if (lower_only && n > m) return;
+    T s = 0;
+    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * M * N + idx];
+    T* p = C + m + n * ldc;
+    *p = beta == T(0) ? s : s + beta * *p;
+}
+
+template <typename T>
+int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+           const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C,
+           int64_t ldc, int tri)
+{
+    if (M <= 0 || N <= 0) return CFMI_OK;
+    int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    // Split K when the output grid alone cannot fill the 256 CUs.
+    int64_t nz = 1;
+    int64_t tiles = tm * tn;
+    if (K > 4 * BK && tiles < 512) {
+        nz = std::min<int64_t>((1024 + tiles - 1) / tiles, (K + 255) / 256);
+        nz = std::max<int64_t>(nz, 1);
+        // Bound the partial buffer to 1 GiB.
+        while (nz > 1 && (size_t)nz * M * N * sizeof(T) > ((size_t)1 << 30)) nz = (nz + 1) / 2;
+    }
+    int64_t kchunk = BK;
+    if (K > 0) {
+        kchunk = (K + nz - 1) / nz;
+        kchunk = ((kchunk + BK - 1) / BK) * BK;
+        nz = (K + kchunk - 1) / kchunk;
+    }
+    if (nz < 1) nz = 1;
+    T* part = nullptr;
+    if (nz > 1) {
+        part = (T*)cfmi_workspace(c, (size_t)nz * M * N * sizeof(T));
+        if (!part) return cfmi_fail(c, CFMI_ENOMEM, "gemm: workspace allocation failed");
+    }
+    dim3 grid((unsigned)tm, (unsigned)tn, (unsigned)nz);
+    T a = (T)alpha, b = (T)beta;
+#define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                 \
+    hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(NT), 0, c->stream, M, N,            \
+                       (K > 0 ? K : 0), a, (const T*)A, lda, (const T*)B, ldb, b, (T*)C, ldc, tri, \
+                       kchunk, part)
+    if (!ta && !tb) CFMI_GEMM_LAUNCH(0, 0);
+    else if (!ta && tb) CFMI_GEMM_LAUNCH(0, 1);
+    else if (ta && !tb) CFMI_GEMM_LAUNCH(1, 0);
+    else CFMI_GEMM_LAUNCH(1, 1);
+#undef CFMI_GEMM_LAUNCH
+    CFMI_LAUNCH_CHECK(c);
+    if (nz > 1) {
+        int64_t tot = M * N;
+        hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                           0, c->stream, M, N, (int)nz, part, b, (T*)C, ldc, (tri & 16) ? 1 : 0);
+        CFMI_LAUNCH_CHECK(c);
+    }
+    return CFMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky: 64-column panels.  The diagonal block is factored in LDS by one
+// workgroup, which also forms its inverse; the panel below is multiplied by
+// that inverse (TRSM), and the trailing lower triangle is updated by GEMM.
+// ---------------------------------------------------------------------------
+constexpr int NB = 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info,
+                                                         int64_t koff, T* linv, T* U, int64_t ldu)
+{
+    __shared__ T s[NB][NB + 1];
+    __shared__ T si[NB][NB + 1];
+    __shared__ int bad;
+    if (*info != 0) return;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int r = idx & (NB - 1), q = idx >> 6;
+        s[r][q] = (r < kb && q < kb && r >= q) ? A[r + q * lda] : T(0);
+        si[r][q] = T(0);
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int q = 0; q < kb; ++q) {
+        if (tid == 0) {
+            T d = s[q][q];
+            if (!(d > T(0))) bad = q + 1;
+            else s[q][q] = sqrt(d);
+        }
+        __syncthreads();
+        if (bad) break;
+        T piv = s[q][q];
+        for (int r = q + 1 + tid; r < kb; r += 256) s[r][q] = s[r][q] / piv;
+        __syncthreads();
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            int r = idx & (NB - 1), p = idx >> 6;
+            if (p > q && r >= p && r < kb) s[r][p] -= s[r][q] * s[p][q];
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) atomicCAS(info, 0, (int)(koff + bad));
+        return;
+    }
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int r = idx & (NB - 1), q = idx >> 6;
+        if (r < kb && q < kb) A[r + q * lda] = r >= q ? s[r][q] : T(0);
+    }
+    // Inverse of the lower-triangular block, one column per thread.
+    if (tid < kb) {
+        int j = tid;
+        for (int i = j; i < kb; ++i) {
+            T sum = (i == j) ? T(1) : T(0);
+            for (int k = j; k < i; ++k) sum -= s[i][k] * si[k][j];
+            si[i][j] = sum / s[i][i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int r = idx & (NB - 1), q = idx >> 6;
+        T v = (r < kb && q < kb && r >= q) ? si[r][q] : T(0);
+        linv[r + q * NB] = v;
+        if (U && r < kb && q < kb) U[r + q * ldu] = v;
+    }
+}
+
+// P (m x kb) <- P * Linv^T, 64 rows per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int64_t m, int kb,
+                                                         const T* linv, const int* info)
+{
+    __shared__ T sp[NB][NB + 1];
+    __shared__ T sl[NB][NB + 1];
+    if (*info != 0) return;
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int r = idx & (NB - 1), q = idx >> 6;
+        sp[r][q] = (r0 + r < m && q < kb) ? P[(r0 + r) + q * ldp] : T(0);
+        sl[r][q] = linv[r + q * NB];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int r = idx & (NB - 1), q = idx >> 6;
+        if (r0 + r >= m || q >= kb) continue;
+        T sum = 0;
+        for (int j = 0; j <= q; ++j) sum += sp[r][j] * sl[q][j];
+        P[(r0 + r) + q * ldp] = sum;
+    }
+}
+
+template <typename T>
+__global__ void fill_kernel(int64_t m, int64_t n, T* A, int64_t lda, T v, int mode)
+{
+    // mode 0: whole block; mode 1: strict upper triangle only.
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    int64_t i = idx % m, j = idx / m;
+    if (mode == 1 && j <= i) return;
+    A[i + j * lda] = v;
+}
+
+template <typename T>
+int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, int mode)
+{
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    int64_t tot = m * n;
+    hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       m, n, (T*)A, lda, (T)v, mode);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+template <typename T>
+int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu, int* info)
+{
+    T* A = (T*)Av;
+    T* U = (T*)Uv;
+    T* linv = (T*)c->d_red;  // 64 x 64 staging inside the reduction buffer
+    CFMI_HIP(c, hipMemsetAsync(c->d_info, 0, sizeof(int), c->stream));
+    if (U) {
+        int rc = fill_t<T>(c, n, n, U, ldu, 0.0, 0);
+        if (rc) return rc;
+    }
+    for (int64_t k = 0; k < n; k += NB) {
+        int kb = (int)std::min<int64_t>(NB, n - k);
+        hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(256), 0, c->stream, A + k + k * lda,
+                           lda, kb, c->d_info, k, linv, U ? U + k + k * ldu : (T*)nullptr, ldu);
+        CFMI_LAUNCH_CHECK(c);
+        int64_t m = n - k - kb;
+        if (m > 0) {
+            hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + NB - 1) / NB)), dim3(256), 0,
+                               c->stream, A + (k + kb) + k * lda, lda, m, kb, (const T*)linv,
+                               (const int*)c->d_info);
+            CFMI_LAUNCH_CHECK(c);
+            T* P = A + (k + kb) + k * lda;
+            int rc = gemm_t<T>(c, 0, 1, m, m, kb, -1.0, P, lda, P, lda, 1.0, A + (k + kb) + (k + kb) * lda,
+                               lda, 16);
+            if (rc) return rc;
+        }
+    }
+    int rc = fill_t<T>(c, n, n, A, lda, 0.0, 1);
+    if (rc) return rc;
+    int h = 0;
+    CFMI_HIP(c, hipMemcpyAsync(&h, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CFMI_HIP(c, hipStreamSynchronize(c->stream));
+    if (info) *info = h;
+    return h > 0 ? h : CFMI_OK;
+}
+
+// U = L^-1 by recursive 2 x 2 blocking on 64-aligned splits; the diagonal
+// 64 x 64 blocks of U come from potrf.  U21 = -U22 (L21 U11).
+template <typename T>
+int trtri_rec(cfmi_ctx* c, int64_t lo, int64_t hi, const T* L, int64_t ldl, T* U, int64_t ldu, T* tmp)
+{
+    int64_t len = hi - lo;
+    if (len <= NB) return CFMI_OK;
+    int64_t half = ((len / 2 + NB - 1) / NB) * NB;
+    int64_t mid = lo + half;
+    int rc = trtri_rec<T>(c, lo, mid, L, ldl, U, ldu, tmp);
+    if (rc) return rc;
+    rc = trtri_rec<T>(c, mid, hi, L, ldl, U, ldu, tmp);
+    if (rc) return rc;
+    int64_t m2 = hi - mid, m1 = mid - lo;
+    // tmp (m2 x m1) = L21 * U11   (U11 lower: op(B)[k,n] = 0 for k < n)
+    rc = gemm_t<T>(c, 0, 0, m2, m1, m1, 1.0, L + mid + lo * ldl, ldl, U + lo + lo * ldu, ldu, 0.0, tmp,
+                   m2, 8);
+    if (rc) return rc;
+    // U21 = -U22 * tmp   (U22 lower: op(A)[m,k] = 0 for k > m)
+    return gemm_t<T>(c, 0, 0, m2, m1, m2, -1.0, U + mid + mid * ldu, ldu, tmp, m2, 0.0,
+                     U + mid + lo * ldu, ldu, 1);
+}
+
+__global__ void diag_scale_kernel(int64_t n, const double* A, int64_t lda, double* d)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = fabs(A[i + i * lda]);
+    if (a == 0.0) a = 1.0;
+    d[i] = 1.0 / sqrt(a);
+}
+
+__global__ void scale_kernel(int mode, int64_t m, int64_t n, double* A, int64_t lda, const double* d)
+{
+    // mode 0: columns, 1: rows, 2: both (D A D)
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    int64_t i = idx % m, j = idx / m;
+    double s = mode == 0 ? d[j] : mode == 1 ? d[i] : d[i] * d[j];
+    A[i + j * lda] *= s;
+}
+
+__global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* A, int64_t lda, double beta,
+                             double* B, int64_t ldb)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    int64_t i = idx % m, j = idx / m;
+    double* p = B + i + j * ldb;
+    *p = alpha * A[i + j * lda] + (beta == 0.0 ? 0.0 : beta * *p);
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(int64_t m, int64_t n, const S* src, int64_t lds, D* dst, int64_t ldd)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    int64_t i = idx % m, j = idx / m;
+    dst[i + j * ldd] = (D)src[i + j * lds];
+}
+
+// Deterministic single-workgroup reduction: 2 sum log L_ii - 2 sum log d_i.
+__global__ __launch_bounds__(256) void logdet_kernel(int64_t n, const double* L, int64_t ldl,
+                                                     const double* d, double* out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        s += 2.0 * log(L[i + i * ldl]);
+        if (d) s -= 2.0 * log(d[i]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+
+int cfmi_gemm_impl(cfmi_ctx* c, int dtype, int ta, int tb, int64_t M, int64_t N, int64_t K,
+                   double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                   double beta, void* C, int64_t ldc, int tri)
+{
+    if (dtype == CFMI_F64) return gemm_t<double>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    if (dtype == CFMI_F32) return gemm_t<float>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    return cfmi_fail(c, CFMI_EINVAL, "gemm: unsupported dtype %d", dtype);
+}
+
+extern "C" {
+
+int cfmi_gemm(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const void* A, int64_t lda, const void* B, int64_t ldb, double beta,
+              void* C, int64_t ldc, int flags)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m < 0 || n < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "gemm: negative size");
+    if (ldc < (m > 1 ? m : 1)) return cfmi_fail(c, CFMI_EINVAL, "gemm: ldc < m");
+    if (lda < 1 || ldb < 1) return cfmi_fail(c, CFMI_EINVAL, "gemm: bad leading dimension");
+    return cfmi_gemm_impl(c, dtype, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                          ldc, flags);
+}
+
+int cfmi_potrf(cfmi_ctx* c, int dtype, int64_t n, void* A, int64_t lda, void* U, int64_t ldu, int* info)
+{
+    if (!c) return CFMI_EINVAL;
+    if (n < 0 || lda < n) return cfmi_fail(c, CFMI_EINVAL, "potrf: bad size n=%lld lda=%lld",
+                                           (long long)n, (long long)lda);
+    if (U && ldu < n) return cfmi_fail(c, CFMI_EINVAL, "potrf: ldu < n");
+    if (info) *info = 0;
+    if (n == 0) return CFMI_OK;
+    if (dtype == CFMI_F64) return potrf_t<double>(c, n, A, lda, U, ldu, info);
+    if (dtype == CFMI_F32) return potrf_t<float>(c, n, A, lda, U, ldu, info);
+    return cfmi_fail(c, CFMI_EINVAL, "potrf: unsupported dtype %d", dtype);
+}
+
+int cfmi_trtri(cfmi_ctx* c, int dtype, int64_t n, const void* L, int64_t ldl, void* U, int64_t ldu)
+{
+    if (!c) return CFMI_EINVAL;
+    if (n <= NB) return CFMI_OK;
+    size_t esz = dtype == CFMI_F64 ? 8 : 4;
+    // The largest temporary is at the top split: (hi - mid) x (mid - lo).
+    int64_t half = ((n / 2 + NB - 1) / NB) * NB;
+    int64_t big = std::max<int64_t>(half, n - half);
+    // Kept apart from the shared workspace, which gemm's split-K may grow.
+    void* tmp2 = nullptr;
+    CFMI_HIP(c, hipMallocAsync(&tmp2, (size_t)big * big * esz, c->stream));
+    int rc;
+    if (dtype == CFMI_F64)
+        rc = trtri_rec<double>(c, 0, n, (const double*)L, ldl, (double*)U, ldu, (double*)tmp2);
+    else if (dtype == CFMI_F32)
+        rc = trtri_rec<float>(c, 0, n, (const float*)L, ldl, (float*)U, ldu, (float*)tmp2);
+    else
+        rc = cfmi_fail(c, CFMI_EINVAL, "trtri: unsupported dtype %d", dtype);
+    hipFreeAsync(tmp2, c->stream);
+    return rc;
+}
+
+int cfmi_jacobi_scale(cfmi_ctx* c, int64_t n, double* A, int64_t lda, double* d)
+{
+    if (!c) return CFMI_EINVAL;
+    if (n <= 0) return CFMI_OK;
+    hipLaunchKernelGGL(diag_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       (const double*)A, lda, d);
+    CFMI_LAUNCH_CHECK(c);
+    int64_t tot = n * n;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, 2, n, n,
+                       A, lda, (const double*)d);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_scale(cfmi_ctx* c, int side, int64_t m, int64_t n, double* A, int64_t lda, const double* d)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    int64_t tot = m * n;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       side ? 1 : 0, m, n, A, lda, d);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_axpby(cfmi_ctx* c, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, double beta,
+               double* B, int64_t ldb)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    int64_t tot = m * n;
+    hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, m, n,
+                       alpha, A, lda, beta, B, ldb);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_convert(cfmi_ctx* c, int dir, int64_t m, int64_t n, const void* src, int64_t lds, void* dst,
+                 int64_t ldd)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    int64_t tot = m * n;
+    dim3 g((unsigned)((tot + 255) / 256));
+    if (dir == 0)
+        hipLaunchKernelGGL((convert_kernel<double, float>), g, dim3(256), 0, c->stream, m, n,
+                           (const double*)src, lds, (float*)dst, ldd);
+    else
+        hipLaunchKernelGGL((convert_kernel<float, double>), g, dim3(256), 0, c->stream, m, n,
+                           (const float*)src, lds, (double*)dst, ldd);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_fill(cfmi_ctx* c, int dtype, int64_t m, int64_t n, void* A, int64_t lda, double v)
+{
+    if (!c) return CFMI_EINVAL;
+    if (dtype == CFMI_F64) return fill_t<double>(c, m, n, A, lda, v, 0);
+    if (dtype == CFMI_F32) return fill_t<float>(c, m, n, A, lda, v, 0);
+    return cfmi_fail(c, CFMI_EINVAL, "fill: unsupported dtype %d", dtype);
+}
+
+int cfmi_logdet_chol(cfmi_ctx* c, int64_t n, const double* L, int64_t ldl, const double* d, double* out)
+{
+    if (!c) return CFMI_EINVAL;
+    double* dev = c->d_red + 8192;
+    hipLaunchKernelGGL(logdet_kernel, dim3(1), dim3(256), 0, c->stream, n, L, ldl, d, dev);
+    CFMI_LAUNCH_CHECK(c);
+    CFMI_HIP(c, hipMemcpyAsync(out, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CFMI_HIP(c, hipStreamSynchronize(c->stream));
+    return CFMI_OK;
+}
+
+int cfmi_gram(cfmi_ctx* c, int dtype, int64_t m, int64_t n, int64_t K, const void* X, int64_t ldx,
+              const void* Y, int64_t ldy, double* G, int64_t ldg)
+{
+    if (!c) return CFMI_EINVAL;
+    if (dtype == CFMI_F64)
+        return gemm_t<double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, 0);
+    if (dtype == CFMI_F32) {
+        // FP32 inputs accumulate per split-K slab in FP32 and are reduced in
+        // FP64: run the slabs into an FP32 partial, then widen.
+        float* g32 = nullptr;
+        CFMI_HIP(c, hipMallocAsync((void**)&g32, (size_t)m * n * sizeof(float), c->stream));
+        int rc = gemm_t<float>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, g32, m, 0);
+        if (!rc) rc = cfmi_convert(c, 1, m, n, g32, m, G, ldg);
+        hipFreeAsync(g32, c->stream);
+        return rc;
+    }
+    return cfmi_fail(c, CFMI_EINVAL, "gram: unsupported dtype %d", dtype);
+}
+
+}  // extern "C"

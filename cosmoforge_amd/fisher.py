@@ -1,0 +1,144 @@
+"""``Fisher``: drop-in for ``qube.Fisher`` on the pixel-space path.
+
+Constructor kwargs and getters follow ``src/cosmoforge.qube/qube/fisher.py``
+(:141-266 constructor, :622-820 ``run``, :826-1232 getters).  The matrix is
+F_ij = 1/2 Tr[C^-1 dC_i C^-1 dC_j] over parameters ordered spectrum-major
+(``spectra_list`` order, then bins), beam-smoothed as the reference's.
+All device work runs in libcfmi on one GPU.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import engine, pipeline
+from .core import Inputs, bins_from_params, resolve_basis
+
+
+class Fisher:
+    def __init__(self, params=None, basis=None, cache_derivatives: bool = False, symmetry_mode=None,
+                 mask=None, noise_cov1=None, noise_cov2=None, cls_data=None, fiducial_cls=None, beam=None,
+                 *, params_file=None, device: int = 0, context: engine.Context | None = None, **kwargs):
+        if params is None and params_file is not None:
+            params = params_file
+        if noise_cov2 is not None:
+            raise NotImplementedError("noise_cov2 / do_cross is not part of this build")
+        if kwargs.get("pixel_filter") is not None:
+            raise NotImplementedError("pixel_filter is not part of this build")
+        if symmetry_mode is not None:
+            name = getattr(symmetry_mode, "name", symmetry_mode)
+            if not isinstance(name, str):
+                raise TypeError(f"symmetry_mode must be SymmetryMode | str | None; got "
+                                f"{type(symmetry_mode).__name__}")
+            if name.upper() not in ("SYMMETRIC", "DIRECTIONAL"):
+                raise ValueError(f"unknown symmetry_mode {symmetry_mode!r}; expected one of "
+                                 "['SYMMETRIC', 'DIRECTIONAL']")
+        self.basis = resolve_basis(basis)
+        self._cache_derivatives = cache_derivatives
+        self.inputs = Inputs(params, mask, noise_cov1, cls_data, beam)
+        self.params = self.inputs.params
+        self.fiducial_cls = fiducial_cls  # inert on the pixel-space path, as in the reference
+        self.ctx = context if context is not None else engine.Context(device)
+        self.cov: pipeline.Covariance | None = None
+        self.fisher = None
+        self.beam_smoothing = None
+        self.spectra_list = None
+        self.bins = None
+        self.V = None  # device stack of whitened derivatives (reused by Spectra)
+        self.timings = {}
+
+    # -- pipeline -----------------------------------------------------------
+    @property
+    def lmax_signal(self) -> int:
+        return self.inputs.lmax_signal
+
+    @property
+    def reduced_noise_cov1(self) -> np.ndarray:
+        return self.inputs.noise
+
+    def set_binning(self, bins):
+        """bins: list of (lmin_b, lmax_b) or an object with lmins/lmaxs."""
+        if hasattr(bins, "lmins"):
+            bins = list(zip(np.asarray(bins.lmins).tolist(), np.asarray(bins.lmaxs).tolist()))
+        bins = [(int(a), int(b)) for a, b in bins]
+        if bins[0][0] != self.params.lmin:
+            raise ValueError(f"bins.lmin={bins[0][0]} != params.lmin={self.params.lmin}")
+        if max(b for _, b in bins) > self.params.lmax:
+            raise ValueError("binning cannot extend above the inference ceiling.")
+        self.bins = bins
+
+    def prepare_covariance_matrices(self):
+        if self.cov is None:
+            self.cov = pipeline.Covariance(self.ctx, self.inputs)
+        self.cov.build()
+
+    def run(self) -> None:
+        import time
+
+        t0 = time.perf_counter()
+        self.prepare_covariance_matrices()
+        if self.bins is None:
+            self.bins = bins_from_params(self.params)
+        lay = self.inputs.layout
+        self.spectra_list = [s.label for s in lay.spectra_list()]
+        lmin, lmax = self.params.lmin, self.params.lmax
+        sm = self.inputs.smoothing
+        self.beam_smoothing = np.concatenate([sm[lab][lmin: lmax + 1] for lab in self.spectra_list])
+        self.n_params = len(self.spectra_list) * len(self.bins)
+        self.ctx.sync()
+        t1 = time.perf_counter()
+        self.fisher, self.V = pipeline.fisher(self.cov, self.bins, self.params.output_convention)
+        self.ctx.sync()
+        self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
+
+    compute = run
+
+    def release(self):
+        """Drop the device derivative stack (it holds n_params * n^2 doubles)."""
+        self.V = None
+
+    # -- getters ------------------------------------------------------------
+    def get_fisher_matrix(self) -> np.ndarray | None:
+        return self.fisher
+
+    @property
+    def signal_matrix(self) -> np.ndarray | None:
+        if self.cov is None or self.cov.S is None:
+            return None
+        return engine.to_numpy_cm(self.cov.S)
+
+    @property
+    def inverse_covariance(self) -> np.ndarray:
+        """C^-1 (the reference keeps it in ``noise_cov1`` after ``run``)."""
+        return engine.to_numpy_cm(self.cov.inverse())
+
+    def get_error_bars(self, *, as_dict: bool = False):
+        if self.fisher is None:
+            return None
+        cov = pipeline.inverse_symm(self.ctx, self.fisher)
+        err = np.sqrt(np.diag(cov))
+        if not as_dict:
+            return err
+        nb = len(self.bins)
+        return {lab: err[k * nb:(k + 1) * nb] for k, lab in enumerate(self.spectra_list)}
+
+    def get_bandpower_slices(self):
+        if self.spectra_list is None:
+            return None
+        nb = len(self.bins)
+        return {lab: slice(k * nb, (k + 1) * nb) for k, lab in enumerate(self.spectra_list)}
+
+    def get_fisher_block(self, label_i: str, label_j: str | None = None):
+        F, sl = self.get_fisher_matrix(), self.get_bandpower_slices()
+        if F is None or sl is None:
+            return None
+        label_j = label_i if label_j is None else label_j
+        for lab in (label_i, label_j):
+            if lab not in sl:
+                raise KeyError(f"unknown spectrum label {lab!r}; legal labels: {list(sl)}")
+        return F[sl[label_i], sl[label_j]]
+
+    def get_window_matrix(self):
+        """The beam-smoothed Fisher matrix, as the reference defines the
+        window of the convolved mode (``fisher.py:1000-1049``)."""
+        return self.get_fisher_matrix()

@@ -1,0 +1,142 @@
+"""``Spectra``: drop-in for ``qube.Spectra`` (QML power spectra) on the
+pixel-space path.
+
+Follows ``src/cosmoforge.qube/qube/spectra.py``: constructor :205-334
+(``mask``, ``noise_cov*``, ``cls_data``, ``fiducial_cls`` and ``beam`` are
+refused together with ``fisher=``), ``run`` :1220-1290, the traditional
+estimator :1124-1186 (q_b = 1/2 y^T dC_b y - nb_b, nb_b = 1/2 Tr[C^-1 N C^-1
+dC_b], y = C^-1 x), the F^-1 normalisation :607-661,1340-1356 and the
+getters :1358-1733.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import engine, pipeline
+from .core import resolve_maps
+from .fisher import Fisher
+
+
+class Spectra:
+    def __init__(self, params=None, fisher: Fisher | None = None, basis=None, mask=None, noise_cov1=None,
+                 noise_cov2=None, cls_data=None, fiducial_cls=None, beam=None, maps1=None, maps2=None, *,
+                 params_file=None, device: int = 0, **kwargs):
+        if params is None and params_file is not None:
+            params = params_file
+        if maps2 is not None:
+            raise NotImplementedError("maps2 / do_cross is not part of this build")
+        if fisher is not None:
+            clash = [k for k, v in dict(mask=mask, noise_cov1=noise_cov1, noise_cov2=noise_cov2,
+                                         cls_data=cls_data, fiducial_cls=fiducial_cls, beam=beam).items()
+                     if v is not None]
+            if clash:
+                raise ValueError(f"{', '.join(clash)} cannot be injected together with fisher=: the Fisher "
+                                 "instance already owns these inputs")
+            self.fisher_instance = fisher
+        else:
+            self.fisher_instance = Fisher(params, basis=basis, mask=mask, noise_cov1=noise_cov1,
+                                          cls_data=cls_data, fiducial_cls=fiducial_cls, beam=beam,
+                                          device=device, **kwargs)
+        self.params = self.fisher_instance.params
+        if params is not None and fisher is not None:
+            from .core import read_params
+
+            extra = read_params(params)
+            for key in ("nsims", "remove_nb"):
+                setattr(self.params, key, getattr(extra, key))
+        self._maps_in = maps1
+        self.maps1 = None
+        self.qml_results = None
+        self.qml_noise_bias = None
+        self.invfisher = None
+        self.fisher_normalized = None
+        self.inv_fisher_sqrt = None
+
+    @property
+    def ctx(self):
+        return self.fisher_instance.ctx
+
+    def setup_maps(self):
+        self.maps1 = resolve_maps(self.fisher_instance.inputs.n, self.params.nsims, self._maps_in)
+
+    def setup_fisher_inversion(self):
+        F = self.fisher_instance.get_fisher_matrix()
+        if F is None:
+            raise ValueError("Fisher matrix not available")
+        self.fisher_normalized = F.copy()
+        self.invfisher = pipeline.inverse_symm(self.ctx, F)
+
+    def run(self):
+        f = self.fisher_instance
+        if f.fisher is None:
+            f.run()
+        if f.V is None:
+            raise RuntimeError("the Fisher instance released its derivative stack; run it again")
+        self.setup_maps()
+        self.setup_fisher_inversion()
+        q, nb = pipeline.qml(f.cov, f.V, self.maps1, remove_nb=bool(self.params.remove_nb))
+        self.qml_results = q
+        self.qml_noise_bias = nb
+
+    compute = run
+
+    # -- getters ------------------------------------------------------------
+    def _normalize_spectra(self, spectra: np.ndarray) -> np.ndarray:
+        if self.invfisher is None:
+            raise ValueError("Fisher inversion must be set up first.")
+        return spectra @ self.invfisher
+
+    def _as_dict(self, arr):
+        f = self.fisher_instance
+        nb = len(f.bins)
+        return {lab: arr[..., k * nb:(k + 1) * nb] for k, lab in enumerate(f.spectra_list)}
+
+    def get_power_spectra(self, mode: str = "deconvolved", *, as_dict: bool = False):
+        if self.qml_results is None:
+            return None
+        if mode == "deconvolved":
+            out = self._normalize_spectra(self.qml_results)
+        elif mode == "decorrelated":
+            if self.inv_fisher_sqrt is None:
+                w, v = np.linalg.eigh(self.fisher_normalized)
+                self.inv_fisher_sqrt = (v / np.sqrt(w)) @ v.T
+            out = self.qml_results @ self.inv_fisher_sqrt
+        elif mode == "convolved":
+            W = self.fisher_instance.get_window_matrix()
+            y = self.qml_results
+
+            def convolve_theory(cl_theory):
+                if isinstance(cl_theory, dict):
+                    labs = self.fisher_instance.spectra_list
+                    missing = [lab for lab in labs if lab not in cl_theory]
+                    if missing:
+                        raise KeyError(f"convolve_theory dict missing entries for: {missing}. "
+                                       f"Expected keys: {labs}")
+                    cl_theory = np.concatenate([np.asarray(cl_theory[lab]) for lab in labs])
+                return W @ np.asarray(cl_theory)
+
+            return ((self._as_dict(y) if as_dict else y), W, convolve_theory)
+        else:
+            raise ValueError(f"unknown mode {mode!r}; expected deconvolved, decorrelated or convolved")
+        return self._as_dict(out) if as_dict else out
+
+    def get_noise_bias(self):
+        if self.qml_noise_bias is None:
+            return None
+        return self._normalize_spectra(self.qml_noise_bias)
+
+    def get_covariance(self):
+        return self.invfisher
+
+    def get_error_bars(self, *, as_dict: bool = False):
+        if self.invfisher is None:
+            return None
+        err = np.sqrt(np.diag(self.invfisher))
+        return self._as_dict(err) if as_dict else err
+
+    def get_effective_ells(self, use_midpoint: bool = False):
+        bins = self.fisher_instance.bins
+        if use_midpoint:
+            return np.array([0.5 * (a + b) for a, b in bins])
+        raise NotImplementedError("window-centroid effective ells are not part of this build")

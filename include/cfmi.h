@@ -1,0 +1,174 @@
+/*
+ * cfmi.h -- C ABI of libcfmi, the MI355X (gfx950) pixel-space covariance
+ * library behind cosmoforge_amd's Fisher / Spectra / PICSLike classes.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer owned by the caller (e.g. a
+ *     PyTorch-ROCm tensor's data_ptr()), unless its comment says "host".
+ *   - Matrices are column-major (Fortran order, as the reference allocates
+ *     them) with an explicit leading dimension.  A symmetric n x n matrix is
+ *     stored in full.
+ *   - Pointing vectors are (n, 3) row-major unit vectors (x, y, z).
+ *   - C_l / weight arrays are l-indexed, length lmax + 1.
+ *   - Return code: 0 = ok; < 0 = API / HIP error, message in
+ *     cfmi_last_error(); > 0 = LAPACK-style info (first non-positive pivot,
+ *     1-based) from a Cholesky factorisation.
+ *   - All work is enqueued on the context's stream; a call returns when the
+ *     work is enqueued, except calls that return a host scalar (they
+ *     synchronise the stream).  A context is not thread-safe; separate
+ *     contexts are.  No exception or abort crosses this boundary.
+ *
+ * Reference interfaces replaced (ggalloni/CosmoForge 1.3.0; paths relative
+ * to src/cosmoforge.cosmocore/cosmocore/ unless stated):
+ *   cfmi_pointings      compute_pointings / _unit_vectors   signal_kernels.py:113-161
+ *   cfmi_signal_00      compute_00_contribution             signal_kernels.py:164-242
+ *   cfmi_signal_22      compute_22_contribution             signal_kernels.py:245-333
+ *   cfmi_signal_02      compute_02_contribution             signal_kernels.py:336-387
+ *   cfmi_deriv_stack_*  get_binned_derivative_direct        basis/pixel.py:802-950
+ *                       (every bin of one spectrum in one launch)
+ *   cfmi_potrf          lapack.dpotrf in matrix_inverse_symm / cholesky_factor
+ *                                                           basics/linalg.py:197-214,362-406
+ *   cfmi_trtri          lapack.dpotri (via L^-1)            basics/linalg.py:212-215
+ *   cfmi_jacobi_scale   D M D preconditioning               basics/linalg.py:197-207
+ *   cfmi_logdet_chol    matrix_slogdet_symm                 basics/linalg.py:430-474
+ *   cfmi_gemm           matrix_mult (np.matmul)             basics/linalg.py:18-32
+ *   cfmi_gram           matrix_trace over all pairs         basics/linalg.py:66-98,
+ *                                                           qube/qube/fisher.py:557-603
+ *   cfmi_batched_*      PICSLike._compute_likelihood_point  picslike/picslike/picslike.py:568-664
+ *   cfmi_mlp_*          (no reference counterpart: learned likelihood surrogate)
+ */
+#ifndef CFMI_H
+#define CFMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFMI_OK 0
+#define CFMI_EINVAL (-1)
+#define CFMI_EHIP (-2)
+#define CFMI_ENOMEM (-3)
+
+#define CFMI_F64 0
+#define CFMI_F32 1
+#define CFMI_BF16 2
+
+/* Largest multipole the recurrence kernels accept. */
+#define CFMI_MAX_ELL 4096
+
+typedef struct cfmi_ctx cfmi_ctx;
+
+int cfmi_version(void);
+
+/* Create a context on `device`; `stream` is a hipStream_t (NULL: the
+ * context creates its own). */
+int cfmi_create(int device, void *stream, cfmi_ctx **out);
+int cfmi_destroy(cfmi_ctx *ctx);
+const char *cfmi_last_error(const cfmi_ctx *ctx);
+int cfmi_synchronize(cfmi_ctx *ctx);
+
+/* ---- A1: pixel centres -> unit vectors ------------------------------- */
+/* vec[i] = unit vector of HEALPix pixel pix[i] (RING if nest == 0), on the
+ * device.  Device sin/cos may differ from the platform libm by an ulp.     */
+int cfmi_pointings(cfmi_ctx *ctx, int64_t nside, int nest, const int64_t *pix, int64_t n,
+                   double *vec);
+/* The same on the host with the platform libm (bit-compatible with the
+ * reference's scalar loop); pix and vec are HOST arrays.  Spin-2 entries of
+ * pixel pairs on a common meridian depend on the last bit of these vectors,
+ * so the reference-compatible path uses this variant.                      */
+int cfmi_pointings_host(int64_t nside, int nest, const int64_t *pix, int64_t n, double *vec);
+
+/* ---- A2-A9: signal covariance blocks (FP64) ---------------------------- */
+/* Spin-0 x spin-0.  cl: beam-smoothed C_l (device, lmax+1).
+ * symmetric != 0: vec_r == vec_c (n_r == n_c), writes the full symmetric
+ *   block S[j,i] = sum_l cl_l (2l+1)/(4pi) P_l(v_j . v_i) into out.
+ * symmetric == 0: writes out[j + i*ld] for all j < n_r, i < n_c, and, if
+ *   mirror != NULL, mirror[i + j*ld_mirror] with the same value.            */
+int cfmi_signal_00(cfmi_ctx *ctx, const double *cl, int lmax, const double *vec_r, int64_t n_r,
+                   const double *vec_c, int64_t n_c, int symmetric, double *out, int64_t ld,
+                   double *mirror, int64_t ld_mirror);
+
+/* Spin-2 auto block: the full (2n x 2n) [Q;U] x [Q;U] block into out. */
+int cfmi_signal_22(cfmi_ctx *ctx, const double *cl_ee, const double *cl_bb, const double *cl_eb,
+                   int lmax, const double *vec, int64_t n, double *out, int64_t ld);
+
+/* Spin-0 x spin-2: the (2 n_p x n_t) [Q;U] x T block into out and, if
+ * mirror != NULL, its transpose (n_t x 2 n_p) into mirror.                 */
+int cfmi_signal_02(cfmi_ctx *ctx, const double *cl_te, const double *cl_tb, int lmax,
+                   const double *vec_t, int64_t n_t, const double *vec_p, int64_t n_p,
+                   double *out, int64_t ld, double *mirror, int64_t ld_mirror);
+
+/* ---- A11: binned derivative stacks (FP64) ------------------------------ */
+/* Bin b covers l in [bin_lo[b], bin_hi[b]] (device int32 arrays, bins
+ * ascending and disjoint); w is the l-indexed weight w_l * B1(l) B2(l).
+ * Output b of a stack starts at out + b * stride.  Geometry and mirror
+ * conventions are those of the matching cfmi_signal_* call.  In the 22 and
+ * 02 calls any output pointer may be NULL (not produced).                  */
+int cfmi_deriv_stack_00(cfmi_ctx *ctx, const double *w, int lmax, const int *bin_lo,
+                        const int *bin_hi, int nbins, const double *vec_r, int64_t n_r,
+                        const double *vec_c, int64_t n_c, int symmetric, double *out, int64_t ld,
+                        int64_t stride, double *mirror, int64_t ld_mirror, int64_t stride_mirror);
+int cfmi_deriv_stack_22(cfmi_ctx *ctx, const double *w_ee, const double *w_bb, const double *w_eb,
+                        int lmax, const int *bin_lo, const int *bin_hi, int nbins,
+                        const double *vec, int64_t n, double *out_ee, double *out_bb,
+                        double *out_eb, int64_t ld, int64_t stride);
+int cfmi_deriv_stack_02(cfmi_ctx *ctx, const double *w_te, const double *w_tb, int lmax,
+                        const int *bin_lo, const int *bin_hi, int nbins, const double *vec_t,
+                        int64_t n_t, const double *vec_p, int64_t n_p, double *out_te,
+                        double *out_tb, int64_t ld, int64_t stride, double *mirror_te,
+                        double *mirror_tb, int64_t ld_mirror, int64_t stride_mirror);
+
+/* ---- A12-A15: dense linear algebra ------------------------------------- */
+/* C = alpha op(A) op(B) + beta C; dtype CFMI_F64 or CFMI_F32; trans* != 0
+ * selects the transpose.  flags: bit 0 op(A) lower-triangular, bit 1 op(A)
+ * upper, bit 2 op(B) upper, bit 3 op(B) lower (zero parts are skipped, they
+ * must hold zeros), bit 4 write only the lower triangle of C.             */
+int cfmi_gemm(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const void *A, int64_t lda, const void *B, int64_t ldb, double beta,
+              void *C, int64_t ldc, int flags);
+
+/* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
+int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
+
+/* Lower Cholesky A = L L^T in place (upper triangle zeroed).  If U != NULL
+ * the diagonal 64 x 64 blocks of L^-1 are written into U (rest zeroed), to
+ * be completed by cfmi_trtri.  *info (host) receives the LAPACK info.      */
+int cfmi_potrf(cfmi_ctx *ctx, int dtype, int64_t n, void *A, int64_t lda, void *U, int64_t ldu,
+               int *info);
+
+/* Complete U = L^-1 from cfmi_potrf's L and diagonal blocks of U.        */
+int cfmi_trtri(cfmi_ctx *ctx, int dtype, int64_t n, const void *L, int64_t ldl, void *U,
+               int64_t ldu);
+
+/* log det of D^-1 L L^T D^-1 = 2 sum log L_ii - 2 sum log d_i (d may be
+ * NULL); the result is written to *out (host).                             */
+int cfmi_logdet_chol(cfmi_ctx *ctx, int64_t n, const double *L, int64_t ldl, const double *d,
+                     double *out);
+
+/* A[:, j] *= d[j] (side 0) or A[i, :] *= d[i] (side 1); FP64.           */
+int cfmi_scale(cfmi_ctx *ctx, int side, int64_t m, int64_t n, double *A, int64_t lda,
+               const double *d);
+
+/* B = alpha A + beta B elementwise on an m x n block; FP64.             */
+int cfmi_axpby(cfmi_ctx *ctx, int64_t m, int64_t n, double alpha, const double *A, int64_t lda,
+               double beta, double *B, int64_t ldb);
+
+/* Convert between FP64 and FP32 matrices (dir 0: f64 -> f32, 1: f32 -> f64). */
+int cfmi_convert(cfmi_ctx *ctx, int dir, int64_t m, int64_t n, const void *src, int64_t lds,
+                 void *dst, int64_t ldd);
+
+/* Fill an m x n block with value v (FP64 or FP32 per dtype). */
+int cfmi_fill(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, void *A, int64_t lda, double v);
+
+/* G[i, j] = sum_k X[k, i] Y[k, j] for i < m, j < n over k < K (FP64 or
+ * FP32 inputs, FP64 output), with fixed-order split-K partial sums.  X and
+ * Y are K x m and K x n column-major (ldx, ldy >= K).                     */
+int cfmi_gram(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t K, const void *X,
+              int64_t ldx, const void *Y, int64_t ldy, double *G, int64_t ldg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFMI_H */

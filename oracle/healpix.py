@@ -10,7 +10,7 @@ The reference calls ``hp.pix2ang`` at ``src/cosmoforge.cosmocore/cosmocore/
 signal_kernels.py:137`` and turns the angles into unit vectors with a scalar
 sin/cos loop at ``signal_kernels.py:144-161``; ``unit_vectors`` below keeps
 that scalar loop (Python's ``math`` is the platform libm, as Numba's scalar
-path is).
+path is; squares are products, as Numba emits for ``x**2``).
 
 Pinned by the reference's ``geometry.dat`` fixtures (RING, nside 8) and the
 NESTED golden signal matrix (nside 4).
@@ -128,7 +128,9 @@ def unit_vectors(theta, phi):
         x = st * math.cos(phi[i])
         y = st * math.sin(phi[i])
         z = math.cos(theta[i])
-        norm = math.sqrt(x**2 + y**2 + z**2)
+        # Numba lowers ``x**2`` with a literal integer exponent to x*x; CPython
+        # would call pow(x, 2.0), which can differ by an ulp.
+        norm = math.sqrt(x * x + y * y + z * z)
         out[i, 0] = x / norm
         out[i, 1] = y / norm
         out[i, 2] = z / norm

@@ -1,0 +1,195 @@
+"""Kernel-level parity of libcfmi against the oracle (GPU).
+
+Each test calls the library through the C ABI (``cosmoforge_amd.engine``)
+and compares with the FP64 oracle or NumPy on the same seeded inputs:
+GEMM and the MFMA operand layout (all transposes, ragged sizes, triangular
+flags), pointings (RING ``geometry.dat`` and NESTED), signal blocks against
+the reference's golden matrices, derivative stacks against the oracle's
+binned derivative, and Cholesky / inverse / log-determinant.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cosmoforge_amd import engine
+
+    return engine.Context(0)
+
+
+def _cm(ctx, a, dtype=None):
+    import torch
+
+    return ctx.upload_cm(a, dtype=dtype or torch.float64)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_layouts(ctx, dt, ta, tb):
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(11)
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    tol = 1e-12 if dt == "f64" else 2e-5
+    for (M, N, K) in [(37, 53, 29), (130, 129, 200), (257, 64, 1000)]:
+        A = rng.standard_normal((K, M) if ta else (M, K))
+        B = rng.standard_normal((N, K) if tb else (K, N))
+        C0 = rng.standard_normal((M, N))
+        dA, dB, dC = _cm(ctx, A, tdt), _cm(ctx, B, tdt), _cm(ctx, C0, tdt)
+        engine.gemm(ctx, bool(ta), bool(tb), M, N, K, 0.5, dA, 0, A.shape[0], dB, 0, B.shape[0], -2.0, dC, 0, M)
+        got = engine.to_numpy_cm(dC).astype(np.float64)
+        ref = 0.5 * (A.T if ta else A) @ (B.T if tb else B) - 2.0 * C0
+        assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < tol, (M, N, K)
+
+
+def test_gemm_triangular_flags(ctx):
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(5)
+    n = 300
+    Lw = np.tril(rng.standard_normal((n, n)))
+    B = rng.standard_normal((n, 77))
+    dL, dB = _cm(ctx, Lw), _cm(ctx, B)
+    dC = ctx.empty(n, 77)
+    engine.gemm(ctx, False, False, n, 77, n, 1.0, dL, 0, n, dB, 0, n, 0.0, dC, 0, n, 1)
+    assert np.allclose(engine.to_numpy_cm(dC), Lw @ B, rtol=0, atol=1e-11)
+    # lower-only output of L L^T
+    dS = ctx.zeros(n, n)
+    engine.gemm(ctx, False, True, n, n, n, 1.0, dL, 0, n, dL, 0, n, 0.0, dS, 0, n, 4 | 16)
+    got = engine.to_numpy_cm(dS)
+    ref = Lw @ Lw.T
+    assert np.allclose(np.tril(got), np.tril(ref), atol=1e-10)
+    assert np.all(np.triu(got, 1) == 0)
+
+
+def test_pointings(ctx):
+    from cosmoforge_amd import engine
+    from oracle import healpix
+    from tests import cases
+
+    g = cases.case_fortran_b().expect["geometry"]
+    v = engine.pointings(ctx, 8, g[:, 0].astype(np.int64), nest=False).cpu().numpy()
+    np.testing.assert_array_equal(v, g[:, 1:4])
+    for nside in (1, 4, 16):
+        pix = np.arange(12 * nside * nside)
+        for nest in (False, True):
+            r = healpix.pointings(nside, pix, nest)
+            v = engine.pointings(ctx, nside, pix, nest).cpu().numpy()
+            np.testing.assert_array_equal(v, r)
+            # device evaluation: same formulas, device libm (within 2 ulp)
+            v = engine.pointings(ctx, nside, pix, nest, libm_exact=False).cpu().numpy()
+            assert np.max(np.abs(v - r)) <= 4.5e-16, (nside, nest)
+
+
+def _device_signal(ctx, case):
+    import torch
+    from cosmoforge_amd import engine, layout as LY
+    from tests import oracle_run as R
+
+    lay = LY.build_layout(case.mask, case.spins, case.labels)
+    vecs = [engine.pointings(ctx, case.nside, f.pixels, case.nest) for f in lay.fields]
+    olay = R.layout(case)
+    cls_s, _ = R.smoothed(case, olay)
+    dcls = {k: ctx.upload(v) for k, v in cls_s.items()}
+    S = engine.signal_matrix(ctx, lay, vecs, dcls, case.lmax_signal)
+    torch.cuda.synchronize()
+    return engine.to_numpy_cm(S), lay, vecs, olay, cls_s
+
+
+@pytest.mark.parametrize("name", ["signal_tqu", "signal_t1t2"])
+def test_signal_golden(ctx, name):
+    from tests import cases
+
+    case = cases.case_signal_tqu() if name == "signal_tqu" else cases.case_signal_t1t2()
+    S, *_ = _device_signal(ctx, case)
+    np.testing.assert_allclose(S, case.expect["S"], rtol=1e-12, atol=1e-20)
+
+
+def test_signal_vs_oracle_with_eb_tb(ctx):
+    """Nonzero EB/TB (no reference fixture has them): GPU vs oracle."""
+    import torch
+    from cosmoforge_amd import engine, layout as LY
+    from oracle import kernels as K
+    from tests import cases
+    from tests import oracle_run as R
+
+    case = cases.case_fisher_nside4("TQU")
+    case.nside = 4
+    olay = R.layout(case)
+    cls_s, _ = R.smoothed(case, olay)
+    rng = np.random.default_rng(9)
+    cls_s["EB"] = 0.1 * cls_s["EE"] * rng.random(cls_s["EE"].size)
+    cls_s["TB"] = 0.1 * cls_s["TE"] * rng.random(cls_s["TE"].size)
+    So = K.signal_matrix(olay, cls_s, case.lmax_signal)
+    lay = LY.build_layout(case.mask, case.spins, case.labels)
+    vecs = [engine.pointings(ctx, 4, f.pixels, False) for f in lay.fields]
+    S = engine.signal_matrix(ctx, lay, vecs, {k: ctx.upload(v) for k, v in cls_s.items()}, case.lmax_signal)
+    torch.cuda.synchronize()
+    S = engine.to_numpy_cm(S)
+    assert np.max(np.abs(S - So)) <= 1e-12 * np.max(np.abs(So))
+
+
+def test_deriv_stacks_vs_oracle(ctx):
+    import torch
+    from cosmoforge_amd import engine, layout as LY
+    from oracle import kernels as K
+    from oracle import pipeline as OP
+    from tests import cases
+    from tests import oracle_run as R
+
+    case = cases.case_fisher_nside4("TQU")
+    olay = R.layout(case)
+    _, beams = R.smoothed(case, olay)
+    prm = R.params(case, olay, beams)
+    lay = LY.build_layout(case.mask, case.spins, case.labels)
+    vecs = [engine.pointings(ctx, 4, f.pixels, False) for f in lay.fields]
+    bins = OP.bins_fromdeltal(case.lmin, case.lmax)
+    nb = len(bins)
+    sm = OP.smoothing_factors(olay, beams, case.lmax_signal)
+    wts = {lab: ctx.upload(sm[lab]) for lab in lay.labels}
+    blocks = engine.deriv_stacks(ctx, lay, vecs, wts, case.lmax_signal, bins)
+    torch.cuda.synchronize()
+    n = lay.n
+    for s, blk in enumerate(blocks):
+        for b in range(nb):
+            fi, fj, mode, _, w = prm[s * nb + b]
+            ref = K.binned_derivative(olay, fi, fj, mode, w)
+            D = blk.stack[b].cpu().numpy().T
+            full = np.zeros((n, n))
+            full[blk.ra:blk.ra + blk.ma, blk.rb:blk.rb + blk.mb] = D
+            if blk.ra != blk.rb:
+                full[blk.rb:blk.rb + blk.mb, blk.ra:blk.ra + blk.ma] = D.T
+            scale = np.max(np.abs(ref)) + 1e-300
+            assert np.max(np.abs(full - ref)) / scale < 1e-12, (s, b)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 1000])
+def test_factor_and_inverse(ctx, n):
+    from cosmoforge_amd import engine
+    from oracle import linalg as LA
+
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    C = A @ A.T + n * np.diag(rng.uniform(0.5, 2.0, n)) * 10 ** rng.uniform(-3, 3, n)
+    C = 0.5 * (C + C.T)
+    dC = _cm(ctx, C)
+    f = engine.factor(ctx, dC, n)
+    Ci = engine.to_numpy_cm(engine.inverse_from_whitener(ctx, f.W, n))
+    ref = LA.matrix_inverse_symm(C)
+    assert np.max(np.abs(Ci - ref)) / np.max(np.abs(ref)) < 1e-12
+    assert f.logdet == pytest.approx(LA.matrix_slogdet_symm(C)[1], rel=1e-12, abs=1e-10)
+
+
+def test_factor_not_pd_raises(ctx):
+    from cosmoforge_amd import engine
+
+    n = 200
+    C = np.eye(n)
+    C[150, 150] = -1.0
+    with pytest.raises(ValueError, match="dpotrf failed with info=151"):
+        engine.factor(ctx, _cm(ctx, C), n)

@@ -1,0 +1,160 @@
+"""End-to-end parity of the drop-in classes (GPU) with the oracle and the
+reference's fixtures.
+
+Tolerances: F and the deconvolved C_l against the FP64 oracle at 1e-10
+relative (max-abs difference over max-abs value, and elementwise on the
+diagonal); the fixtures at the reference tests' own tolerances.
+"""
+
+import numpy as np
+import pytest
+
+from tests import cases
+from tests import oracle_run as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(case, **extra):
+    p = dict(nside=case.nside, spins=case.spins, labels=case.labels, lmin=case.lmin, lmax=case.lmax,
+             lmax_signal=case.lmax_signal, ordering="NESTED" if case.nest else "RING",
+             output_convention=case.convention, nsims=None if case.maps is None else case.maps.shape[1])
+    p.update(extra)
+    return p
+
+
+def _gpu_fisher(case):
+    from cosmoforge_amd import Fisher
+
+    f = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam)
+    f.run()
+    return f
+
+
+def _oracle_fisher(case):
+    from oracle import pipeline as OP
+
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    F, Cinv, S = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, prm)
+    return F, Cinv, S, lay, prm, beams
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("name", ["T", "QU", "TQU", "TEB"])
+def test_fisher_nside4(name):
+    case = cases.case_fisher_nside4(name)
+    f = _gpu_fisher(case)
+    F = f.get_fisher_matrix()
+    Fo, *_ = _oracle_fisher(case)
+    assert _rel(F, Fo) < 1e-10
+    d, do = np.diag(F), np.diag(Fo)
+    assert np.max(np.abs(d - do) / np.abs(do)) < 1e-10
+    bs = f.beam_smoothing
+    np.testing.assert_allclose(F / np.outer(bs, bs), case.expect["fisher"], atol=1e-3, rtol=1e-5)
+
+
+def test_fortran_suite_gpu():
+    from cosmoforge_amd import Spectra
+
+    case = cases.case_fortran_b()
+    f = _gpu_fisher(case)
+    S = f.signal_matrix
+    ref = case.expect["S"]
+    rel = np.where(np.abs(ref) > 1e-20, np.abs((S - ref) / ref), 0)
+    assert rel.max() < 1e-10
+    dinv = np.diag(f.inverse_covariance)
+    d = case.expect["invcov_diag"]
+    assert np.max(np.abs((dinv - d) / d)) < 1e-12
+    F = f.get_fisher_matrix()
+    Fo, Cinv, _, lay, prm, beams = _oracle_fisher(case)
+    assert _rel(F, Fo) < 1e-10
+    e = np.arange(case.lmin, case.lmax + 1.0)
+    dd = e * (e + 1) / (2 * np.pi)
+    fr = F / np.outer(f.beam_smoothing, f.beam_smoothing) * np.outer(dd, dd)
+    fd = np.diag(case.expect["fisher"])
+    assert np.max(np.abs((np.diag(fr) - fd) / fd)) < 1e-6
+    sp = Spectra(fisher=f, maps1=case.maps)
+    sp.run()
+    ch = sp.get_power_spectra()[0]
+    ref_sp = case.expect["spectra"]
+    m = min(ref_sp.size, ch.size)
+    assert np.max(np.abs((ch[:m] - ref_sp[:m]) / ref_sp[:m])) < 1e-10
+    nbh = sp.get_noise_bias()
+    nr = case.expect["noise_bias"]
+    m = min(nr.size, nbh.size)
+    assert np.max(np.abs((nbh[:m] - nr[:m]) / nr[:m])) < 1e-6
+    # against the oracle's QML on the same inputs
+    from oracle import pipeline as OP
+
+    q, nb, cho, nbho = OP.qml(lay, Cinv, case.noise, case.maps, Fo, prm)
+    assert _rel(sp.qml_results, q) < 1e-10
+    assert _rel(sp.qml_noise_bias, nb) < 1e-10
+    assert _rel(sp.get_power_spectra(), cho) < 1e-10
+
+
+def test_qml_tqu_with_eb_tb_vs_oracle():
+    """TQU with nonzero EB/TB spectra and a seeded map: every spectrum's
+    Fisher row and QML estimate against the oracle."""
+    from cosmoforge_amd import Fisher, Spectra
+    from oracle import pipeline as OP
+
+    case = cases.case_fisher_nside4("TQU")
+    rng = np.random.default_rng(20261203)
+    cls = dict(case.cls)
+    cls["EB"] = 0.2 * np.sqrt(cls["EE"] * cls["BB"])
+    cls["TB"] = 0.1 * np.sqrt(cls["TT"] * cls["BB"])
+    case.cls = cls
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    Fo, Cinv, S, = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, prm)
+    Lc = np.linalg.cholesky(S + case.noise)
+    maps = Lc @ rng.standard_normal((lay.n, 2))
+    q, nb, cho, nbho = OP.qml(lay, Cinv, case.noise, maps, Fo, prm)
+    f = Fisher(_params(case, nsims=2), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=cls,
+               beam=case.beam)
+    f.run()
+    assert _rel(f.get_fisher_matrix(), Fo) < 1e-10
+    sp = Spectra(fisher=f, maps1=maps)
+    sp.run()
+    assert _rel(sp.qml_results, q) < 1e-10
+    assert _rel(sp.qml_noise_bias, nb) < 1e-10
+    assert _rel(sp.get_power_spectra(), cho) < 1e-10
+    assert _rel(sp.get_noise_bias(), nbho) < 1e-10
+
+
+def test_v110_pins_gpu():
+    from cosmoforge_amd import PICSLike
+
+    case = cases.case_v110()
+    pl = PICSLike(_params(case, nsims=1), basis=False, mask=case.mask, noise_cov1=case.noise,
+                  cls_data=case.cls, beam=case.beam, maps1=case.maps)
+    chi2, lnl = pl._compute_likelihood_point(None)
+    logdet = -2.0 * lnl[0] - chi2[0]
+    assert logdet == pytest.approx(case.expect["logdet"], rel=1e-10)
+    assert chi2[0] == pytest.approx(case.expect["quadform"], rel=1e-10)
+    S = pl.cov.S.cpu().numpy().T
+    C = S + case.noise
+    assert np.trace(C) == pytest.approx(case.expect["cov_trace"], rel=1e-10)
+    assert np.linalg.norm(C) == pytest.approx(case.expect["cov_fro"], rel=1e-10)
+
+
+def test_picslike_fortran_gpu():
+    from cosmoforge_amd import PICSLike
+
+    case, points, res = cases.case_picslike_b()
+    theory = {(k,): cl for k, cl in enumerate(points)}
+    pl = PICSLike(_params(case, nsims=1), basis=False, mask=case.mask, noise_cov1=case.noise,
+                  beam=case.beam, maps1=case.maps, theoretical_spectra=theory)
+    pl.compute()
+    chi2 = pl.get_chi_squared()
+    lnl = pl.get_log_likelihood()
+    assert np.max(np.abs(chi2 / res[:, 3] - 1)) < 1e-9
+    assert np.max(np.abs(lnl - res[:, 2])) / np.abs(res[:, 2]).mean() < 1e-9
+    assert pl.get_best_fit() == {"point": int(np.argmax(lnl))}
