@@ -40,6 +40,8 @@ SIGNATURES = {
     "cfmi_deriv_stack_02": (ci, [vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, vp, vp, i64, i64, vp, vp,
                                  i64, i64]),
     "cfmi_gemm": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, ci]),
+    "cfmi_gemm_packed": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, i64, i64, cd,
+                              cd, ci]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),

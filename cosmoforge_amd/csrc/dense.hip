@@ -29,98 +29,230 @@ template <> struct Mfma<float> {
     }
 };
 
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+#ifndef CFMI_GEMM_BK
+#define CFMI_GEMM_BK 16
+#endif
+#ifndef CFMI_GEMM_WAVES
+#define CFMI_GEMM_WAVES 2
+#endif
+constexpr int BM = 128, BN = 128, BK = CFMI_GEMM_BK, NT = 256, PAD = 2;
+constexpr int BSMALL = 96;  // tile edge for outputs of at most ~300 x 300 (Gram matrices)
 
-template <typename T, int TA, int TB>
-__global__ __launch_bounds__(NT) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
-                                                  const T* __restrict__ A, int64_t lda,
-                                                  const T* __restrict__ B, int64_t ldb, T beta,
-                                                  T* __restrict__ C, int64_t ldc, int tri,
-                                                  int64_t kchunk, T* __restrict__ part)
+// Where the output block sits and how it is stored.
+struct OutMap {
+    int64_t row0 = 0, col0 = 0;  // absolute coordinates of C(0, 0)
+    int64_t pn = 0;              // > 0: packed lower triangle of a pn x pn matrix
+    double pdiag = 1.0;          // packed: scale of diagonal entries
+    double poff = 1.0;           // packed: scale of off-diagonal entries
+};
+
+template <typename T>
+struct GemmArgs {
+    int64_t M, N, K;
+    T alpha, beta;
+    const T* A;
+    int64_t lda;
+    const T* B;
+    int64_t ldb;
+    T* C;
+    int64_t ldc;
+    int tri;
+    int64_t kchunk;
+    T* part;
+    int64_t tm, tn;
+    OutMap o;
+};
+
+__device__ __forceinline__ int64_t packed_index(int64_t i, int64_t j, int64_t n)
+{
+    return j * (2 * n - j + 1) / 2 + (i - j);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_out(const GemmArgs<T>& g, int64_t gm, int64_t gn, T v)
+{
+    const int64_t i = g.o.row0 + gm, j = g.o.col0 + gn;
+    T* p;
+    if (g.o.pn > 0) {
+        v *= (T)(i == j ? g.o.pdiag : g.o.poff);
+        p = g.C + packed_index(i, j, g.o.pn);
+    } else {
+        p = g.C + gm + gn * g.ldc;
+    }
+    *p = g.beta == T(0) ? v : v + g.beta * *p;
+}
+
+// Tile order: with a triangular K-trim the heaviest tiles are issued first
+// (interleaved over the XCDs); otherwise each XCD gets a contiguous range of
+// tiles so that tiles sharing a B panel meet in the same L2.
+__device__ __forceinline__ void tile_of(int64_t b, int64_t tm, int64_t tn, int tri, int64_t& ti, int64_t& tj)
+{
+    const int64_t G = tm * tn;
+    int64_t t = b;
+    if ((tri & 31) == 0 && G >= 64) {
+        const int64_t q = G / 8, r = G % 8, x = b % 8, loc = b / 8;
+        t = x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+    }
+    ti = t % tm;
+    tj = t / tm;
+    if (tri & 1) ti = tm - 1 - ti;
+    if (tri & 4) tj = tn - 1 - tj;
+}
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+// Per thread BK/2 A and BK/2 B elements of a BM x BK / BK x BN slab, loaded as
+// pairs contiguous in memory: (m, m+1) for column-major A, (k, k+1) for a
+// transposed A; likewise for B.  Interior slabs with 2-element alignment use
+// unguarded vector loads; edge slabs use guarded scalar loads.
+template <typename T, int TA, int TB, int TBM, int TBN>
+struct Slab {
+    static constexpr int PA = TBM * BK / 2 / NT;  // element pairs per thread
+    static constexpr int PB = TBN * BK / 2 / NT;
+    static_assert(PA * 2 * NT == TBM * BK && PB * 2 * NT == TBN * BK, "tile does not split evenly");
+    T ra[2 * PA], rb[2 * PB];
+
+    __device__ __forceinline__ static void amap(int idx, int& m, int& k)
+    {
+        if (TA == 0) { m = (idx % (TBM / 2)) * 2; k = idx / (TBM / 2); }
+        else { k = (idx % (BK / 2)) * 2; m = idx / (BK / 2); }
+    }
+    __device__ __forceinline__ static void bmap(int idx, int& k, int& n)
+    {
+        if (TB == 0) { k = (idx % (BK / 2)) * 2; n = idx / (BK / 2); }
+        else { n = (idx % (TBN / 2)) * 2; k = idx / (TBN / 2); }
+    }
+
+    __device__ __forceinline__ void load_fast(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+                                              int64_t ldb, int64_t m0, int64_t n0, int64_t k0, int tid)
+    {
+        typedef typename Vec2<T>::type V2;
+#pragma unroll
+        for (int r = 0; r < PA; ++r) {
+            int idx = tid + NT * r, m, k, n;
+            amap(idx, m, k);
+            const T* pa = TA == 0 ? A + (m0 + m) + (k0 + k) * lda : A + (k0 + k) + (m0 + m) * lda;
+            V2 va = *reinterpret_cast<const V2*>(pa);
+            ra[2 * r] = va.x;
+            ra[2 * r + 1] = va.y;
+            bmap(idx, k, n);
+            const T* pb = TB == 0 ? B + (k0 + k) + (n0 + n) * ldb : B + (n0 + n) + (k0 + k) * ldb;
+            V2 vb = *reinterpret_cast<const V2*>(pb);
+            rb[2 * r] = vb.x;
+            rb[2 * r + 1] = vb.y;
+        }
+    }
+
+    __device__ __forceinline__ void load_guarded(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+                                                 int64_t ldb, int64_t m0, int64_t n0, int64_t k0, int64_t M,
+                                                 int64_t N, int64_t kend, int tid)
+    {
+#pragma unroll
+        for (int r = 0; r < PA; ++r) {
+            int idx = tid + NT * r, m, k, n;
+            amap(idx, m, k);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                int64_t gm = m0 + m + (TA == 0 ? e : 0), gk = k0 + k + (TA == 0 ? 0 : e);
+                T v = 0;
+                if (gm < M && gk < kend) v = TA == 0 ? A[gm + gk * lda] : A[gk + gm * lda];
+                ra[2 * r + e] = v;
+            }
+            bmap(idx, k, n);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                int64_t gk = k0 + k + (TB == 0 ? e : 0), gn = n0 + n + (TB == 0 ? 0 : e);
+                T v = 0;
+                if (gn < N && gk < kend) v = TB == 0 ? B[gk + gn * ldb] : B[gn + gk * ldb];
+                rb[2 * r + e] = v;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(T (*As)[TBM + PAD], T (*Bs)[TBN + PAD], int tid) const
+    {
+#pragma unroll
+        for (int r = 0; r < PA; ++r) {
+            int idx = tid + NT * r, m, k, n;
+            amap(idx, m, k);
+            if (TA == 0) { As[k][m] = ra[2 * r]; As[k][m + 1] = ra[2 * r + 1]; }
+            else { As[k][m] = ra[2 * r]; As[k + 1][m] = ra[2 * r + 1]; }
+            bmap(idx, k, n);
+            if (TB == 0) { Bs[k][n] = rb[2 * r]; Bs[k + 1][n] = rb[2 * r + 1]; }
+            else { Bs[k][n] = rb[2 * r]; Bs[k][n + 1] = rb[2 * r + 1]; }
+        }
+    }
+};
+
+template <typename T, int TA, int TB, int TBM, int TBN>
+__global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g)
 {
     typedef typename Mfma<T>::v4 v4;
-    __shared__ T As[BK][BM + 1];
-    __shared__ T Bs[BK][BN + 1];
+    constexpr int BM = TBM, BN = TBN;
+    constexpr int FM = BM / 32, FN = BN / 32;  // 16 x 16 MFMA tiles per wave (2 x 2 waves)
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
     const int tid = threadIdx.x;
-    const int64_t m0 = (int64_t)blockIdx.x * BM;
-    const int64_t n0 = (int64_t)blockIdx.y * BN;
-    const int z = blockIdx.z;
-    const bool lower_only = (tri & 16) != 0;
-    if (lower_only && n0 >= m0 + BM) return;
+    int64_t ti, tj;
+    tile_of(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
+    const int64_t m0 = ti * BM;
+    const int64_t n0 = tj * BN;
+    const int z = blockIdx.y;
+    const int64_t M = g.M, N = g.N, K = g.K;
+    const bool lower_only = (g.tri & 16) != 0;
+    if (lower_only && g.o.col0 + n0 > g.o.row0 + m0 + BM - 1) return;
+    if ((g.tri & 64) && tj > ti) return;  // symmetric result: lower tiles only
 
-    int64_t kbeg = (int64_t)z * kchunk;
-    int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
-    if (tri & 1) kend = kend < m0 + BM ? kend : m0 + BM;
-    if (tri & 2) kbeg = kbeg > m0 ? kbeg : m0;
-    if (tri & 4) kend = kend < n0 + BN ? kend : n0 + BN;
-    if (tri & 8) kbeg = kbeg > n0 ? kbeg : n0;
+    int64_t kbeg = (int64_t)z * g.kchunk;
+    int64_t kend = kbeg + g.kchunk < K ? kbeg + g.kchunk : K;
+    if (g.tri & 1) kend = kend < m0 + BM ? kend : m0 + BM;
+    if (g.tri & 2) kbeg = kbeg > m0 ? kbeg : m0;
+    if (g.tri & 4) kend = kend < n0 + BN ? kend : n0 + BN;
+    if (g.tri & 8) kbeg = kbeg > n0 ? kbeg : n0;
 
     const int wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
     const int lm = lane & 15, lk = lane >> 4;
+    const T* __restrict__ A = g.A;
+    const T* __restrict__ B = g.B;
+    const int64_t lda = g.lda, ldb = g.ldb;
+    // Vector loads need 2-element alignment of every column start.
+    const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % (2 * sizeof(T)) == 0)
+                         && (lda % 2 == 0) && (ldb % 2 == 0) && (kbeg % 2 == 0);
+    const bool interior_mn = (m0 + BM <= M) && (n0 + BN <= N);
 
-    v4 acc[4][4];
+    v4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
-    T ra[8], rb[8];
-    auto load = [&](int64_t k0) {
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            int idx = tid + NT * r;
-            int m, k;
-            if (TA == 0) { m = idx & (BM - 1); k = idx >> 7; }
-            else { k = idx & (BK - 1); m = idx >> 4; }
-            int64_t gm = m0 + m, gk = k0 + k;
-            T v = 0;
-            if (gm < M && gk < kend)
-                v = TA == 0 ? A[gm + gk * lda] : A[gk + gm * lda];
-            ra[r] = v;
-            int n;
-            if (TB == 0) { k = idx & (BK - 1); n = idx >> 4; }
-            else { n = idx & (BN - 1); k = idx >> 7; }
-            int64_t gn = n0 + n;
-            gk = k0 + k;
-            T w = 0;
-            if (gn < N && gk < kend)
-                w = TB == 0 ? B[gk + gn * ldb] : B[gn + gk * ldb];
-            rb[r] = w;
-        }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            int idx = tid + NT * r;
-            int m, k, n;
-            if (TA == 0) { m = idx & (BM - 1); k = idx >> 7; }
-            else { k = idx & (BK - 1); m = idx >> 4; }
-            As[k][m] = ra[r];
-            if (TB == 0) { k = idx & (BK - 1); n = idx >> 4; }
-            else { n = idx & (BN - 1); k = idx >> 7; }
-            Bs[k][n] = rb[r];
-        }
+    Slab<T, TA, TB, BM, BN> sl;
+    auto fetch = [&](int64_t k0) {
+        if (aligned && interior_mn && k0 + BK <= kend) sl.load_fast(A, lda, B, ldb, m0, n0, k0, tid);
+        else sl.load_guarded(A, lda, B, ldb, m0, n0, k0, M, N, kend, tid);
     };
 
     if (kbeg < kend) {
-        load(kbeg);
+        fetch(kbeg);
         for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
             __syncthreads();
-            store();
+            sl.store(As, Bs, tid);
             __syncthreads();
-            if (k0 + BK < kend) load(k0 + BK);
+            if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
             for (int kk = 0; kk < BK / 4; ++kk) {
-                T a[4], b[4];
+                T a[FM], b[FN];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = As[kk * 4 + lk][wm + i * 16 + lm];
+                for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + lk][wm + i * 16 + lm];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = Bs[kk * 4 + lk][wn + j * 16 + lm];
+                for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + lk][wn + j * 16 + lm];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(a[i], b[j], acc[i][j]);
+                    for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::run(a[i], b[j], acc[i][j]);
             }
         }
     }
@@ -130,46 +262,53 @@ __global__ __launch_bounds__(NT) void gemm_kernel(int64_t M, int64_t N, int64_t 
     // on gfx950 with an explicit layout probe.
     constexpr bool kF64 = sizeof(T) == 8;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int64_t gm = m0 + wm + i * 16 + (kF64 ? lk + 4 * r : lk * 4 + r);
                 int64_t gn = n0 + wn + j * 16 + lm;
                 if (gm >= M || gn >= N) continue;
-                if (lower_only && gn > gm) continue;
-                T v = alpha * acc[i][j][r];
-                if (part) {
-                    part[(int64_t)z * M * N + gm + gn * M] = v;
-                } else {
-                    T* p = C + gm + gn * ldc;
-                    *p = beta == T(0) ? v : v + beta * *p;
-                }
+                if (lower_only && g.o.col0 + gn > g.o.row0 + gm) continue;
+                T v = g.alpha * acc[i][j][r];
+                if (g.part) g.part[(int64_t)z * M * N + gm + gn * M] = v;
+                else store_out(g, gm, gn, v);
             }
 }
 
 template <typename T>
-__global__ void splitk_reduce_kernel(int64_t M, int64_t N, int nz, const T* __restrict__ part,
-                                     T beta, T* __restrict__ C, int64_t ldc, int lower_only)
+__global__ void splitk_reduce_kernel(GemmArgs<T> g, int nz)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t M = g.M, N = g.N;
     if (idx >= M * N) return;
     int64_t m = idx % M, n = idx / M;
-    if (lower_only && n > m) return;
+    if ((g.tri & 16) && g.o.col0 + n > g.o.row0 + m) return;
+    if ((g.tri & 64) && n > m) return;
     T s = 0;
-    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * M * N + idx];
-    T* p = C + m + n * ldc;
-    *p = beta == T(0) ? s : s + beta * *p;
+    for (int z = 0; z < nz; ++z) s += g.part[(int64_t)z * M * N + idx];
+    store_out(g, m, n, s);
 }
 
 template <typename T>
-int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
-           const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C,
-           int64_t ldc, int tri)
+__global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    int64_t i = idx % n, j = idx / n;
+    if (j > i) C[i + j * ldc] = C[j + i * ldc];
+}
+
+template <typename T>
+int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+           int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc, int tri,
+           const OutMap& om = OutMap())
 {
     if (M <= 0 || N <= 0) return CFMI_OK;
-    int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    const bool small = M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096;
+    const int tbm = small ? BSMALL : BM, tbn = small ? BSMALL : BN;
+    int64_t tm = (M + tbm - 1) / tbm, tn = (N + tbn - 1) / tbn;
     // Split K when the output grid alone cannot fill the 256 CUs.
     int64_t nz = 1;
     int64_t tiles = tm * tn;
@@ -191,12 +330,18 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
         part = (T*)cfmi_workspace(c, (size_t)nz * M * N * sizeof(T));
         if (!part) return cfmi_fail(c, CFMI_ENOMEM, "gemm: workspace allocation failed");
     }
-    dim3 grid((unsigned)tm, (unsigned)tn, (unsigned)nz);
-    T a = (T)alpha, b = (T)beta;
-#define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                 \
-    hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(NT), 0, c->stream, M, N,            \
-                       (K > 0 ? K : 0), a, (const T*)A, lda, (const T*)B, ldb, b, (T*)C, ldc, tri, \
-                       kchunk, part)
+    GemmArgs<T> g;
+    g.M = M; g.N = N; g.K = K > 0 ? K : 0;
+    g.alpha = (T)alpha; g.beta = (T)beta;
+    g.A = (const T*)A; g.lda = lda; g.B = (const T*)B; g.ldb = ldb; g.C = (T*)C; g.ldc = ldc;
+    g.tri = tri; g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = om;
+    dim3 grid((unsigned)(tm * tn), (unsigned)nz);
+#define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                        \
+    do {                                                                                                  \
+        if (small) hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BSMALL, BSMALL>), grid, dim3(NT), 0,      \
+                                      c->stream, g);                                                      \
+        else hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BM, BN>), grid, dim3(NT), 0, c->stream, g);     \
+    } while (0)
     if (!ta && !tb) CFMI_GEMM_LAUNCH(0, 0);
     else if (!ta && tb) CFMI_GEMM_LAUNCH(0, 1);
     else if (ta && !tb) CFMI_GEMM_LAUNCH(1, 0);
@@ -205,8 +350,13 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     CFMI_LAUNCH_CHECK(c);
     if (nz > 1) {
         int64_t tot = M * N;
-        hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                           0, c->stream, M, N, (int)nz, part, b, (T*)C, ldc, (tri & 16) ? 1 : 0);
+        hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                           c->stream, g, (int)nz);
+        CFMI_LAUNCH_CHECK(c);
+    }
+    if ((tri & 64) && M == N) {
+        hipLaunchKernelGGL((mirror_lower_kernel<T>), dim3((unsigned)((M * M + 255) / 256)), dim3(256), 0,
+                           c->stream, M, (T*)C, ldc);
         CFMI_LAUNCH_CHECK(c);
     }
     return CFMI_OK;
@@ -467,6 +617,24 @@ int cfmi_gemm(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, int64_t
                           ldc, flags);
 }
 
+int cfmi_gemm_packed(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+                     double alpha, const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* P,
+                     int64_t pn, int64_t row0, int64_t col0, double pdiag, double poff, int flags)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m < 0 || n < 0 || k < 0 || pn <= 0 || row0 < 0 || col0 < 0 || row0 + m > pn || col0 + n > pn)
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: block outside the %lld x %lld matrix", (long long)pn,
+                         (long long)pn);
+    OutMap om;
+    om.row0 = row0; om.col0 = col0; om.pn = pn; om.pdiag = pdiag; om.poff = poff;
+    int tri = (flags & 15) | 16;
+    if (dtype == CFMI_F64)
+        return gemm_t<double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om);
+    if (dtype == CFMI_F32)
+        return gemm_t<float>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om);
+    return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: unsupported dtype %d", dtype);
+}
+
 int cfmi_potrf(cfmi_ctx* c, int dtype, int64_t n, void* A, int64_t lda, void* U, int64_t ldu, int* info)
 {
     if (!c) return CFMI_EINVAL;
@@ -579,14 +747,15 @@ int cfmi_gram(cfmi_ctx* c, int dtype, int64_t m, int64_t n, int64_t K, const voi
               const void* Y, int64_t ldy, double* G, int64_t ldg)
 {
     if (!c) return CFMI_EINVAL;
+    const int sym = (X == Y && m == n && ldx == ldy) ? 64 : 0;
     if (dtype == CFMI_F64)
-        return gemm_t<double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, 0);
+        return gemm_t<double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, sym);
     if (dtype == CFMI_F32) {
         // FP32 inputs accumulate per split-K slab in FP32 and are reduced in
         // FP64: run the slabs into an FP32 partial, then widen.
         float* g32 = nullptr;
         CFMI_HIP(c, hipMallocAsync((void**)&g32, (size_t)m * n * sizeof(float), c->stream));
-        int rc = gemm_t<float>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, g32, m, 0);
+        int rc = gemm_t<float>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, g32, m, sym);
         if (!rc) rc = cfmi_convert(c, 1, m, n, g32, m, G, ldg);
         hipFreeAsync(g32, c->stream);
         return rc;

@@ -301,12 +301,29 @@ def inverse_from_whitener(ctx: Context, W: torch.Tensor, n: int) -> torch.Tensor
 # A14 - A17: Fisher and QML through the whitened derivatives
 # ---------------------------------------------------------------------------
 
+SQRT_HALF = 0.7071067811865476
+
+
+def packed_size(n: int) -> int:
+    return n * (n + 1) // 2
+
+
+def gemm_packed(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int,
+                B, offb: int, ldb: int, beta: float, Pk: torch.Tensor, pn: int, row0: int, col0: int,
+                pdiag: float = SQRT_HALF, poff: float = 1.0, flags: int = 0):
+    ctx.check(ctx.lib.cfmi_gemm_packed(ctx.h, dtype_code(Pk), 1 if ta else 0, 1 if tb else 0, m, n, k, alpha,
+                                       P(A, offa), lda, P(B, offb), ldb, beta, P(Pk), pn, row0, col0, pdiag,
+                                       poff, flags), "gemm_packed")
+
+
 def whitened_derivatives(ctx: Context, W: torch.Tensor, n: int, blocks: list, nbins: int,
                          out: torch.Tensor, zbuf: torch.Tensor | None = None):
-    """out[:, p] = vec(W dC_p W^T) for every parameter p = spectrum*nbins + bin.
+    """out[p] = packed lower triangle of W dC_p W^T, diagonal scaled by
+    1/sqrt(2), for every parameter p = spectrum * nbins + bin.
 
-    ``out`` is (n_params, n*n) (column-major n^2 x n_params), zero-filled."""
-    nn = n * n
+    With that scaling <out[i], out[j]> = 1/2 Tr[W dC_i W^T W dC_j W^T] =
+    1/2 Tr[C^-1 dC_i C^-1 dC_j] = F_ij.  ``out`` is (n_params, n(n+1)/2),
+    zero-filled."""
     for s, blk in enumerate(blocks):
         ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
         need = (n - ra) * mb
@@ -315,16 +332,15 @@ def whitened_derivatives(ctx: Context, W: torch.Tensor, n: int, blocks: list, nb
         for b in range(nbins):
             p = s * nbins + b
             D = blk.stack[b]
-            base = p * nn
             # Z = W[ra:, ra:ra+ma] D          ((n-ra) x mb; W block lower)
             gemm(ctx, False, False, n - ra, mb, ma, 1.0, W, ra + ra * n, n, D, 0, ma, 0.0, zbuf, 0, n - ra, 1)
-            # X = Z W[rb:, rb:rb+mb]^T  -> out[ra:, rb:]
-            gemm(ctx, False, True, n - ra, n - rb, mb, 1.0, zbuf, 0, n - ra, W, rb + rb * n, n, 0.0, out,
-                 base + ra + rb * n, n, 4)
+            # X = Z W[rb:, rb:rb+mb]^T at rows ra.., cols rb.. (lower part)
+            gemm_packed(ctx, False, True, n - ra, n - rb, mb, 1.0, zbuf, 0, n - ra, W, rb + rb * n, n, 0.0,
+                        out[p], n, ra, rb, flags=4)
             if ra != rb or ma != mb:
-                # + X^T = W[rb:, rb:rb+mb] Z^T -> out[rb:, ra:]
-                gemm(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, zbuf, 0, n - ra, 1.0, out,
-                     base + rb + ra * n, n, 1)
+                # + X^T = W[rb:, rb:rb+mb] Z^T at rows rb.., cols ra..
+                gemm_packed(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, zbuf, 0, n - ra, 1.0,
+                            out[p], n, rb, ra, flags=1)
     return zbuf
 
 

@@ -88,15 +88,22 @@ class StageTimer:
 
 
 def whitening_flops(n: int, blocks, nbins: int) -> float:
-    """Multiply-add flops of the whitened-derivative GEMMs, counting only the
-    nonzero (lower-triangular) parts of W that the kernels touch."""
+    """Multiply-add flops (x2) of the whitened-derivative GEMMs, counting the
+    products the kernels need: the nonzero (lower-triangular) part of W and,
+    for the output, its lower triangle only."""
     tot = 0.0
     for blk in blocks:
         ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
-        z = 2.0 * mb * (ma * (ma + 1) / 2 + (n - ra - ma) * ma)
-        x = 2.0 * (n - ra) * (mb * (mb + 1) / 2 + (n - rb - mb) * mb)
-        tot += (z + x * (2 if (ra != rb or ma != mb) else 1)) * nbins
-    return tot
+        i = np.arange(n - ra)
+        z = 2.0 * mb * np.minimum(i + 1, ma).sum()
+        j = np.arange(n - rb)
+        kj = np.minimum(j + 1, mb)
+        if ra == rb and ma == mb:
+            x = 2.0 * ((n - ra - j) * kj).sum()
+        else:
+            x = 2.0 * (n - ra) * kj.sum()
+        tot += (z + x) * nbins
+    return float(tot)
 
 
 def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None):
@@ -114,7 +121,8 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     blocks = engine.deriv_stacks(ctx, lay, cov.vecs, dev_w, inp.lmax_signal, bins)
     nb = len(bins)
     npar = len(blocks) * nb
-    V = torch.zeros((npar, n * n), dtype=torch.float64, device=ctx.device)
+    K = engine.packed_size(n)
+    V = torch.zeros((npar, K), dtype=torch.float64, device=ctx.device)
     if timer:
         timer.mark("white0")
         timer.flops = whitening_flops(n, blocks, nb)
@@ -122,10 +130,10 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     if timer:
         timer.mark("white1")
     del blocks
-    G = engine.gram(ctx, npar, npar, n * n, V, V)
+    G = engine.gram(ctx, npar, npar, K, V, V)
     if timer:
         timer.mark("gram1")
-    F = 0.25 * (G + G.T)
+    F = 0.5 * (G + G.T)
     return F, V
 
 
@@ -146,16 +154,18 @@ def qml(cov: Covariance, V: torch.Tensor, maps: np.ndarray, remove_nb: bool = Tr
     ns = maps.shape[1]
     npar = V.shape[0]
     Z = whitened_maps(cov, maps)
-    Q = torch.empty((ns + 1, n * n), dtype=torch.float64, device=ctx.device)
+    K = engine.packed_size(n)
+    # Packed like V (diagonal / sqrt 2), so <pack(A), V_b> = 1/2 <A, W dC_b W^T>.
+    Q = torch.empty((ns + 1, K), dtype=torch.float64, device=ctx.device)
     for s in range(ns):
-        engine.gemm(ctx, False, True, n, n, 1, 1.0, Z, s * n, n, Z, s * n, n, 0.0, Q[s], 0, n)
+        engine.gemm_packed(ctx, False, True, n, n, 1, 1.0, Z, s * n, n, Z, s * n, n, 0.0, Q[s], n, 0, 0)
     T = ctx.empty(n, n)
     engine.gemm(ctx, False, False, n, n, n, 1.0, W, 0, n, cov.N, 0, n, 0.0, T, 0, n, 1)
-    engine.gemm(ctx, False, True, n, n, n, 1.0, T, 0, n, W, 0, n, 0.0, Q[ns], 0, n, 4)
+    engine.gemm_packed(ctx, False, True, n, n, n, 1.0, T, 0, n, W, 0, n, 0.0, Q[ns], n, 0, 0, flags=4)
     del T
-    G = engine.gram(ctx, ns + 1, npar, n * n, Q, V)
-    nb = 0.5 * G[ns]
-    q = 0.5 * G[:ns]
+    G = engine.gram(ctx, ns + 1, npar, K, Q, V)
+    nb = G[ns]
+    q = G[:ns]
     if remove_nb:
         q = q - nb[None, :]
     return q, nb

@@ -129,6 +129,16 @@ int cfmi_gemm(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64
               double alpha, const void *A, int64_t lda, const void *B, int64_t ldb, double beta,
               void *C, int64_t ldc, int flags);
 
+/* Same product, written into the packed lower triangle of a pn x pn
+ * matrix (column-major packing: element (i, j), i >= j, at
+ * j (2 pn - j + 1) / 2 + i - j).  C's element (r, c) is (row0 + r,
+ * col0 + c); only entries with row0 + r >= col0 + c are produced, scaled by
+ * pdiag on the diagonal and poff elsewhere before beta-accumulation.      */
+int cfmi_gemm_packed(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64_t n,
+                     int64_t k, double alpha, const void *A, int64_t lda, const void *B,
+                     int64_t ldb, double beta, void *P, int64_t pn, int64_t row0, int64_t col0,
+                     double pdiag, double poff, int flags);
+
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
 
