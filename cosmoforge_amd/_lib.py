@@ -42,6 +42,10 @@ SIGNATURES = {
     "cfmi_gemm": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, ci]),
     "cfmi_gemm_packed": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, i64, i64, cd,
                               cd, ci]),
+    "cfmi_gemm_batched": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, i64, vp, i64, i64, cd, vp, i64, i64,
+                               i64, ci]),
+    "cfmi_gemm_packed_batched": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, i64, vp, i64, i64, cd, vp, i64,
+                                      i64, i64, i64, cd, cd, i64, ci]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),

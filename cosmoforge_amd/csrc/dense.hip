@@ -61,6 +61,7 @@ struct GemmArgs {
     T* part;
     int64_t tm, tn;
     OutMap o;
+    int64_t sA = 0, sB = 0, sC = 0;  // per-batch element strides (blockIdx.z)
 };
 
 __device__ __forceinline__ int64_t packed_index(int64_t i, int64_t j, int64_t n)
@@ -201,6 +202,11 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
     const int64_t n0 = tj * BN;
     const int z = blockIdx.y;
     const int64_t M = g.M, N = g.N, K = g.K;
+    if (blockIdx.z) {
+        g.A += blockIdx.z * g.sA;
+        g.B += blockIdx.z * g.sB;
+        g.C += blockIdx.z * g.sC;
+    }
     const bool lower_only = (g.tri & 16) != 0;
     if (lower_only && g.o.col0 + n0 > g.o.row0 + m0 + BM - 1) return;
     if ((g.tri & 64) && tj > ti) return;  // symmetric result: lower tiles only
@@ -303,16 +309,16 @@ __global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc)
 template <typename T>
 int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
            int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc, int tri,
-           const OutMap& om = OutMap())
+           const OutMap& om = OutMap(), int64_t batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0)
 {
-    if (M <= 0 || N <= 0) return CFMI_OK;
+    if (M <= 0 || N <= 0 || batch <= 0) return CFMI_OK;
     const bool small = M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096;
     const int tbm = small ? BSMALL : BM, tbn = small ? BSMALL : BN;
     int64_t tm = (M + tbm - 1) / tbm, tn = (N + tbn - 1) / tbn;
     // Split K when the output grid alone cannot fill the 256 CUs.
     int64_t nz = 1;
-    int64_t tiles = tm * tn;
-    if (K > 4 * BK && tiles < 512) {
+    int64_t tiles = tm * tn * batch;
+    if (K > 4 * BK && tiles < 512 && batch == 1) {
         nz = std::min<int64_t>((1024 + tiles - 1) / tiles, (K + 255) / 256);
         nz = std::max<int64_t>(nz, 1);
         // Bound the partial buffer to 1 GiB.
@@ -335,7 +341,8 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     g.alpha = (T)alpha; g.beta = (T)beta;
     g.A = (const T*)A; g.lda = lda; g.B = (const T*)B; g.ldb = ldb; g.C = (T*)C; g.ldc = ldc;
     g.tri = tri; g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = om;
-    dim3 grid((unsigned)(tm * tn), (unsigned)nz);
+    g.sA = sA; g.sB = sB; g.sC = sC;
+    dim3 grid((unsigned)(tm * tn), (unsigned)nz, (unsigned)batch);
 #define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                        \
     do {                                                                                                  \
         if (small) hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BSMALL, BSMALL>), grid, dim3(NT), 0,      \
@@ -621,18 +628,47 @@ int cfmi_gemm_packed(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, 
                      double alpha, const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* P,
                      int64_t pn, int64_t row0, int64_t col0, double pdiag, double poff, int flags)
 {
+    return cfmi_gemm_packed_batched(c, dtype, transa, transb, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, P, 0,
+                                    pn, row0, col0, pdiag, poff, 1, flags);
+}
+
+int cfmi_gemm_packed_batched(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb,
+                             int64_t sB, double beta, void* P, int64_t sP, int64_t pn, int64_t row0, int64_t col0,
+                             double pdiag, double poff, int64_t batch, int flags)
+{
     if (!c) return CFMI_EINVAL;
     if (m < 0 || n < 0 || k < 0 || pn <= 0 || row0 < 0 || col0 < 0 || row0 + m > pn || col0 + n > pn)
         return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: block outside the %lld x %lld matrix", (long long)pn,
                          (long long)pn);
+    if (batch < 1 || batch > 65535) return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: batch out of range");
     OutMap om;
     om.row0 = row0; om.col0 = col0; om.pn = pn; om.pdiag = pdiag; om.poff = poff;
     int tri = (flags & 15) | 16;
     if (dtype == CFMI_F64)
-        return gemm_t<double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om);
+        return gemm_t<double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om,
+                              batch, sA, sB, sP);
     if (dtype == CFMI_F32)
-        return gemm_t<float>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om);
+        return gemm_t<float>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om,
+                             batch, sA, sB, sP);
     return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: unsupported dtype %d", dtype);
+}
+
+int cfmi_gemm_batched(cfmi_ctx* c, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+                      double alpha, const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
+                      double beta, void* C, int64_t ldc, int64_t sC, int64_t batch, int flags)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m < 0 || n < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_batched: negative size");
+    if (batch < 1 || batch > 65535) return cfmi_fail(c, CFMI_EINVAL, "gemm_batched: batch out of range");
+    if (ldc < (m > 1 ? m : 1)) return cfmi_fail(c, CFMI_EINVAL, "gemm_batched: ldc < m");
+    if (dtype == CFMI_F64)
+        return gemm_t<double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, flags & 31,
+                              OutMap(), batch, sA, sB, sC);
+    if (dtype == CFMI_F32)
+        return gemm_t<float>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, flags & 31,
+                             OutMap(), batch, sA, sB, sC);
+    return cfmi_fail(c, CFMI_EINVAL, "gemm_batched: unsupported dtype %d", dtype);
 }
 
 int cfmi_potrf(cfmi_ctx* c, int dtype, int64_t n, void* A, int64_t lda, void* U, int64_t ldu, int* info)

@@ -316,6 +316,23 @@ def gemm_packed(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha:
                                        poff, flags), "gemm_packed")
 
 
+def gemm_batched(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int,
+                 sA: int, B, offb: int, ldb: int, sB: int, beta: float, C, offc: int, ldc: int, sC: int, batch: int,
+                 flags: int = 0):
+    ctx.check(ctx.lib.cfmi_gemm_batched(ctx.h, dtype_code(C), 1 if ta else 0, 1 if tb else 0, m, n, k, alpha,
+                                        P(A, offa), lda, sA, P(B, offb), ldb, sB, beta, P(C, offc), ldc, sC, batch,
+                                        flags), "gemm_batched")
+
+
+def gemm_packed_batched(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int,
+                        lda: int, sA: int, B, offb: int, ldb: int, sB: int, beta: float, Pk: torch.Tensor, sP: int,
+                        pn: int, row0: int, col0: int, batch: int, pdiag: float = SQRT_HALF, poff: float = 1.0,
+                        flags: int = 0):
+    ctx.check(ctx.lib.cfmi_gemm_packed_batched(ctx.h, dtype_code(Pk), 1 if ta else 0, 1 if tb else 0, m, n, k,
+                                               alpha, P(A, offa), lda, sA, P(B, offb), ldb, sB, beta, P(Pk), sP, pn,
+                                               row0, col0, pdiag, poff, batch, flags), "gemm_packed_batched")
+
+
 def whitened_derivatives(ctx: Context, W: torch.Tensor, n: int, blocks: list, nbins: int,
                          out: torch.Tensor, zbuf: torch.Tensor | None = None):
     """out[p] = packed lower triangle of W dC_p W^T, diagonal scaled by
@@ -323,24 +340,27 @@ def whitened_derivatives(ctx: Context, W: torch.Tensor, n: int, blocks: list, nb
 
     With that scaling <out[i], out[j]> = 1/2 Tr[W dC_i W^T W dC_j W^T] =
     1/2 Tr[C^-1 dC_i C^-1 dC_j] = F_ij.  ``out`` is (n_params, n(n+1)/2),
-    zero-filled."""
+    zero-filled.  All bins of a spectrum go through one batched launch per
+    product."""
+    K = packed_size(n)
     for s, blk in enumerate(blocks):
         ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
-        need = (n - ra) * mb
+        zs = (n - ra) * mb
+        need = zs * nbins
         if zbuf is None or zbuf.numel() < need:
+            zbuf = None
             zbuf = torch.empty(need, dtype=torch.float64, device=ctx.device)
-        for b in range(nbins):
-            p = s * nbins + b
-            D = blk.stack[b]
-            # Z = W[ra:, ra:ra+ma] D          ((n-ra) x mb; W block lower)
-            gemm(ctx, False, False, n - ra, mb, ma, 1.0, W, ra + ra * n, n, D, 0, ma, 0.0, zbuf, 0, n - ra, 1)
-            # X = Z W[rb:, rb:rb+mb]^T at rows ra.., cols rb.. (lower part)
-            gemm_packed(ctx, False, True, n - ra, n - rb, mb, 1.0, zbuf, 0, n - ra, W, rb + rb * n, n, 0.0,
-                        out[p], n, ra, rb, flags=4)
-            if ra != rb or ma != mb:
-                # + X^T = W[rb:, rb:rb+mb] Z^T at rows rb.., cols ra..
-                gemm_packed(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, zbuf, 0, n - ra, 1.0,
-                            out[p], n, rb, ra, flags=1)
+        Vs = out[s * nbins:(s + 1) * nbins]
+        # Z_b = W[ra:, ra:ra+ma] D_b      ((n-ra) x mb each; W block lower)
+        gemm_batched(ctx, False, False, n - ra, mb, ma, 1.0, W, ra + ra * n, n, 0, blk.stack, 0, ma, ma * mb, 0.0,
+                     zbuf, 0, n - ra, zs, nbins, 1)
+        # X_b = Z_b W[rb:, rb:rb+mb]^T at rows ra.., cols rb.. (lower part)
+        gemm_packed_batched(ctx, False, True, n - ra, n - rb, mb, 1.0, zbuf, 0, n - ra, zs, W, rb + rb * n, n, 0,
+                            0.0, Vs, K, n, ra, rb, nbins, flags=4)
+        if ra != rb or ma != mb:
+            # + X_b^T = W[rb:, rb:rb+mb] Z_b^T at rows rb.., cols ra..
+            gemm_packed_batched(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, 0, zbuf, 0, n - ra,
+                                zs, 1.0, Vs, K, n, rb, ra, nbins, flags=1)
     return zbuf
 
 

@@ -139,6 +139,18 @@ int cfmi_gemm_packed(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m
                      int64_t ldb, double beta, void *P, int64_t pn, int64_t row0, int64_t col0,
                      double pdiag, double poff, int flags);
 
+/* Strided batches of the two products above: batch b uses A + b sA,
+ * B + b sB and C + b sC (P + b sP); batch <= 65535.                       */
+int cfmi_gemm_batched(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64_t n,
+                      int64_t k, double alpha, const void *A, int64_t lda, int64_t sA, const void *B,
+                      int64_t ldb, int64_t sB, double beta, void *C, int64_t ldc, int64_t sC,
+                      int64_t batch, int flags);
+int cfmi_gemm_packed_batched(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64_t n,
+                             int64_t k, double alpha, const void *A, int64_t lda, int64_t sA,
+                             const void *B, int64_t ldb, int64_t sB, double beta, void *P,
+                             int64_t sP, int64_t pn, int64_t row0, int64_t col0, double pdiag,
+                             double poff, int64_t batch, int flags);
+
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
 

@@ -193,3 +193,34 @@ def test_factor_not_pd_raises(ctx):
     C[150, 150] = -1.0
     with pytest.raises(ValueError, match="dpotrf failed with info=151"):
         engine.factor(ctx, _cm(ctx, C), n)
+
+
+def test_gemm_batched_and_packed(ctx):
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(21)
+    M, N, K, nb = 150, 131, 77, 5
+    A = rng.standard_normal((nb, M, K))
+    B = rng.standard_normal((K, N))
+    dA = torch.as_tensor(np.ascontiguousarray(A.transpose(0, 2, 1))).cuda()  # col-major blocks
+    dB = ctx.upload_cm(B)
+    dC = torch.zeros((nb, N, M), dtype=torch.float64, device="cuda")
+    engine.gemm_batched(ctx, False, False, M, N, K, 1.0, dA, 0, M, M * K, dB, 0, K, 0, 0.0, dC, 0, M, M * N, nb)
+    got = dC.cpu().numpy().transpose(0, 2, 1)
+    assert np.max(np.abs(got - A @ B)) < 1e-11
+    # packed lower triangle of a symmetric product, offset inside a larger matrix
+    n, r0 = 200, 37
+    X = rng.standard_normal((n - r0, 40))
+    dX = ctx.upload_cm(X)
+    Kp = engine.packed_size(n)
+    Pk = torch.zeros((2, Kp), dtype=torch.float64, device="cuda")
+    engine.gemm_packed_batched(ctx, False, True, n - r0, n - r0, 40, 1.0, dX, 0, n - r0, 0, dX, 0, n - r0, 0, 0.0,
+                               Pk, Kp, n, r0, r0, 2)
+    full = np.zeros((n, n))
+    full[r0:, r0:] = X @ X.T
+    i, j = np.tril_indices(n)
+    ref = full[i, j] * np.where(i == j, engine.SQRT_HALF, 1.0)
+    order = np.lexsort((i, j))  # column-major packing
+    for b in range(2):
+        assert np.max(np.abs(Pk[b].cpu().numpy() - ref[order])) < 1e-11
