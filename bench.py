@@ -25,6 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+# HBM bytes per whitening launch on the nside-16 TQU workload, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each scaled by what the
+# counter reads for a 1 GiB copy: x2.0 and x1.0).  Not measured live.
+WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {1: 113.6e9}
+WHITENING_LAUNCHES = {0: 2, 1: 14}
 
 
 def make_map(ctx, cfg, seed):
@@ -147,6 +152,7 @@ def main():
     white_ms = timer.elapsed_ms("white0", "white1") / args.steps
     gram_ms = timer.elapsed_ms("white1", "gram1") / args.steps
     deriv_ms = timer.elapsed_ms("deriv0", "white0") / args.steps
+    launches = WHITENING_LAUNCHES.get(args.config)
     achieved = timer.flops / (white_ms * 1e-3) / 1e12
     F = results[0]
     out = {
@@ -167,9 +173,11 @@ def main():
         "stages_ms": {"deriv_stacks": round(deriv_ms, 2), "whitened_derivatives": round(white_ms, 2),
                       "fisher_gram": round(gram_ms, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "gemm_kernel<double> (whitened derivatives W dC_b W^T)",
-                     "flops_per_step": timer.flops},
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": WHITENING_TRAFFIC_BYTES_PER_LAUNCH.get(args.config),
+                     "kernel": "gemm_kernel<double> (whitened derivatives W dC_b W^T, batched over bins)",
+                     "launches_per_step": launches, "flops_per_launch": timer.flops / launches if launches else None,
+                     "ms_per_launch": round(white_ms / launches, 3) if launches else None},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, maps, args.cpu_params)
