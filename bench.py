@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
-"""Benchmark: Fisher + QML time-to-solution on one MI355X (FP64).
+"""Benchmark: Fisher + QML time-to-solution on one MI355X, mixed vs FP64.
 
 One step = one complete Fisher + QML solve of the nside-16 TQU, lmax-47
 configuration (n = 5952 rows, 276 parameters): pointings, S, C = S + N and
 its Cholesky, the 6 derivative stacks, the whitened derivatives, the Fisher
 Gram matrix, the QML estimates, noise bias and their F^-1 normalisation.
-Inputs (mask, spectra, beam, noise, one seeded map) are resident before the
-timed region.  Prints one JSON line (see the README of this repository).
+Each precision (FP64, mixed) is timed over K steps after W warm-ups; the
+line reports the mixed-precision time as ``value``, the FP64 time, the
+speedup and the mixed-vs-FP64 errors.  Inputs (mask, spectra, beam, noise,
+one seeded map) are resident before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--config 0|1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config 0|1|2] [--precisions fp64,mixed] [--no-cpu]
 """
 
 from __future__ import annotations
@@ -25,11 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
-# HBM bytes per whitening launch on the nside-16 TQU workload, measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (each scaled by what the
-# counter reads for a 1 GiB copy: x2.0 and x1.0).  Not measured live.
-WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {1: 113.6e9}
-WHITENING_LAUNCHES = {0: 2, 1: 14}
+FP32_MFMA_PEAK_TFLOPS = 157.3
+# HBM bytes per whitening launch, measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE on this workload (each scaled by what the counter reads for
+# a 1 GiB copy: x2.0 and x1.0); not measured live.  Keyed by (config, dtype).
+WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 113.6e9 / 1.0}
 
 
 def make_map(ctx, cfg, seed):
@@ -100,12 +102,17 @@ def cpu_baseline(cfg, maps, k_params: int):
     }
 
 
+def rel(a, b) -> float:
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--precisions", default="fp64,mixed", help="comma list of fp64 / mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-params", type=int, default=2)
     args = ap.parse_args()
@@ -123,63 +130,81 @@ def main():
     maps = make_map(ctx, cfg, workloads.SEED0 + args.config)
     print(f"[bench] config {args.config} ({cfg.name}): n={cfg.n}", file=sys.stderr, flush=True)
 
-    def step(timer=None):
+    def step(prec, timer=None, lt=None):
         f = Fisher(cfg.params, basis=False, mask=cfg.mask, noise_cov1=cfg.noise, cls_data=cfg.cls,
-                   beam=cfg.beam, context=ctx, timer=timer)
+                   beam=cfg.beam, context=ctx, timer=timer, launch_timer=lt, precision=prec)
         f.run()
         s = Spectra(fisher=f, maps1=maps)
         s.run()
-        return f, s
-
-    for w in range(args.warmup):
-        f, s = step()
-        del f, s
-        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    timer = pipeline.StageTimer()
-    t0 = time.perf_counter()
-    results = []
-    for k in range(args.steps):
-        f, s = step(timer)
-        results = (f.get_fisher_matrix(), s.get_power_spectra(), s.get_noise_bias())
+        out = (f.get_fisher_matrix(), s.get_power_spectra(), s.qml_noise_bias, s.qml_results,
+               f.get_error_bars())
         f.release()
         del f, s
-        print(f"[bench] step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    per_step = elapsed / args.steps
+        return out
 
-    white_ms = timer.elapsed_ms("white0", "white1") / args.steps
-    gram_ms = timer.elapsed_ms("white1", "gram1") / args.steps
-    deriv_ms = timer.elapsed_ms("deriv0", "white0") / args.steps
-    launches = WHITENING_LAUNCHES.get(args.config)
+    runs = {}
+    for prec in args.precisions.split(","):
+        for w in range(args.warmup):
+            step(prec)
+            print(f"[bench] {prec} warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        timer = pipeline.StageTimer()
+        lt = pipeline.LaunchTimer()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            res = step(prec, timer, lt)
+            print(f"[bench] {prec} step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        per_step = (time.perf_counter() - t0) / args.steps
+        runs[prec] = dict(s=per_step, res=res, timer=timer, lt=lt)
+
+    main_prec = "mixed" if "mixed" in runs else "fp64"
+    r = runs[main_prec]
+    timer, lt = r["timer"], r["lt"]
+    white_ms = lt.total_ms() / args.steps
+    launches = len(lt.launches) / args.steps
     achieved = timer.flops / (white_ms * 1e-3) / 1e12
-    F = results[0]
+    peak = FP32_MFMA_PEAK_TFLOPS if main_prec == "mixed" else FP64_MFMA_PEAK_TFLOPS
+    dt = "f32" if main_prec == "mixed" else "f64"
+    F = r["res"][0]
     out = {
-        "metric": "Fisher+QML time-to-solution (FP64)",
-        "value": round(per_step, 4),
-        "unit": "s",
+        "metric": "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error",
+        "value": round(r["s"], 4),
+        "unit": "s per Fisher+QML solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(per_step * 1e3, 2),
+        "ms_per_step": round(r["s"] * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 MFMA (FP64 recurrences, factorisation, Gram accumulation, QML)" if main_prec == "mixed"
+                 else "f64",
         "data": "synthetic (seeded map of S+N; reference-recipe spectra, mask, beam, noise)",
         "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(F.shape[0]),
-                   "lmax_signal": cfg.params["lmax_signal"], "nsims": 1},
-        "stages_ms": {"deriv_stacks": round(deriv_ms, 2), "whitened_derivatives": round(white_ms, 2),
-                      "fisher_gram": round(gram_ms, 2)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4),
-                     "traffic": WHITENING_TRAFFIC_BYTES_PER_LAUNCH.get(args.config),
-                     "kernel": "gemm_kernel<double> (whitened derivatives W dC_b W^T, batched over bins)",
-                     "launches_per_step": launches, "flops_per_launch": timer.flops / launches if launches else None,
-                     "ms_per_launch": round(white_ms / launches, 3) if launches else None},
+                   "lmax_signal": cfg.params["lmax_signal"], "nsims": 1, "precision": main_prec,
+                   "row_bands": len(timer.bands)},
+        "stages_ms": {"deriv_stacks": round(timer.elapsed_ms("deriv0", "white0") / args.steps, 2),
+                      "whitened_derivatives": round(timer.elapsed_ms("white0", "white1") / args.steps, 2),
+                      "fisher_gram": round(timer.elapsed_ms("white1", "gram1") / args.steps, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4),
+                     "traffic": WHITENING_TRAFFIC_BYTES_PER_LAUNCH.get((args.config, dt)),
+                     "kernel": f"gemm_kernel<{'float' if dt == 'f32' else 'double'}> (whitened derivatives "
+                               "W dC_b W^T, batched over bins)",
+                     "launches_per_step": launches, "flops_per_launch": timer.flops / max(launches, 1),
+                     "ms_per_launch": round(white_ms / max(launches, 1), 3)},
     }
-    if not args.no_cpu:
+    for prec, rr in runs.items():
+        out[f"{prec}_s"] = round(rr["s"], 4)
+    if "fp64" in runs and "mixed" in runs:
+        (F, C, nb, q, sig), (Fm, Cm, nbm, qm, _) = runs["fp64"]["res"], runs["mixed"]["res"]
+        out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["s"] / runs["mixed"]["s"], 3)
+        out["mixed_error_vs_fp64"] = {
+            "F_maxrel": rel(Fm, F), "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.abs(np.diag(F)))),
+            "Cl_maxrel": rel(Cm, C), "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
+            "noise_bias_maxrel": rel(nbm, nb), "qml_raw_maxrel": rel(qm, q), "cond_F": float(np.linalg.cond(F))}
+    if not args.no_cpu and args.config < 2:
         out["cpu_baseline"] = cpu_baseline(cfg, maps, args.cpu_params)
     print(json.dumps(out))
 

@@ -16,6 +16,7 @@ CFMI_OK = 0
 CFMI_F64 = 0
 CFMI_F32 = 1
 CFMI_BF16 = 2
+CFMI_GEMM_ACC64 = 128
 
 _c = ctypes
 vp = _c.c_void_p
@@ -35,10 +36,12 @@ SIGNATURES = {
     "cfmi_signal_00": (ci, [vp, vp, ci, vp, i64, vp, i64, ci, vp, i64, vp, i64]),
     "cfmi_signal_22": (ci, [vp, vp, vp, vp, ci, vp, i64, vp, i64]),
     "cfmi_signal_02": (ci, [vp, vp, vp, ci, vp, i64, vp, i64, vp, i64, vp, i64]),
-    "cfmi_deriv_stack_00": (ci, [vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, ci, vp, i64, i64, vp, i64, i64]),
-    "cfmi_deriv_stack_22": (ci, [vp, vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, vp, vp, i64, i64]),
-    "cfmi_deriv_stack_02": (ci, [vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, vp, vp, i64, i64, vp, vp,
-                                 i64, i64]),
+    "cfmi_deriv_state_doubles": (i64, [i64, i64, ci]),
+    "cfmi_deriv_stack_00": (ci, [vp, ci, vp, ci, vp, vp, ci, vp, i64, vp, i64, ci, vp, i64, i64, vp, i64, i64, vp,
+                                 ci]),
+    "cfmi_deriv_stack_22": (ci, [vp, ci, vp, vp, vp, ci, vp, vp, ci, vp, i64, vp, vp, vp, i64, i64, vp, ci]),
+    "cfmi_deriv_stack_02": (ci, [vp, ci, vp, vp, ci, vp, vp, ci, vp, i64, vp, i64, vp, vp, i64, i64, vp, vp,
+                                 i64, i64, vp, ci]),
     "cfmi_gemm": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, ci]),
     "cfmi_gemm_packed": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, vp, i64, cd, vp, i64, i64, i64, cd,
                               cd, ci]),
@@ -46,6 +49,8 @@ SIGNATURES = {
                                i64, ci]),
     "cfmi_gemm_packed_batched": (ci, [vp, ci, ci, ci, i64, i64, i64, cd, vp, i64, i64, vp, i64, i64, cd, vp, i64,
                                       i64, i64, i64, cd, cd, i64, ci]),
+    "cfmi_gemm_ex": (ci, [vp, ci, ci, ci, ci, i64, i64, i64, cd, vp, i64, i64, i64, vp, i64, i64, i64, cd, vp,
+                          i64, i64, i64, ci, ci, i64, i64, i64, cd, cd]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),
@@ -53,6 +58,7 @@ SIGNATURES = {
     "cfmi_scale": (ci, [vp, ci, i64, i64, vp, i64, vp]),
     "cfmi_axpby": (ci, [vp, i64, i64, cd, vp, i64, cd, vp, i64]),
     "cfmi_convert": (ci, [vp, ci, i64, i64, vp, i64, vp, i64]),
+    "cfmi_copy": (ci, [vp, ci, i64, i64, vp, i64, vp, i64]),
     "cfmi_fill": (ci, [vp, ci, i64, i64, vp, i64, cd]),
     "cfmi_gram": (ci, [vp, ci, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
 }

@@ -39,26 +39,35 @@ constexpr int BM = 128, BN = 128, BK = CFMI_GEMM_BK, NT = 256, PAD = 2;
 constexpr int BSMALL = 96;  // tile edge for outputs of at most ~300 x 300 (Gram matrices)
 
 // Where the output block sits and how it is stored.
+//   mode 0: dense column-major (ldc)
+//   mode 1: packed lower triangle, column-major, of a pn x pn matrix
+//   mode 2: packed lower triangle, row-major: element (i, j <= i) at
+//           i (i + 1) / 2 + j - pbase (a contiguous band of rows)
 struct OutMap {
+    int mode = 0;
     int64_t row0 = 0, col0 = 0;  // absolute coordinates of C(0, 0)
-    int64_t pn = 0;              // > 0: packed lower triangle of a pn x pn matrix
+    int64_t pn = 0;
+    int64_t pbase = 0;
     double pdiag = 1.0;          // packed: scale of diagonal entries
     double poff = 1.0;           // packed: scale of off-diagonal entries
 };
 
-template <typename T>
+// T: input type; TO: output type; TE: epilogue / partial-sum type (double
+// when FP32 products are accumulated in FP64 across K-chunks).
+template <typename T, typename TO, typename TE>
 struct GemmArgs {
     int64_t M, N, K;
-    T alpha, beta;
+    double alpha, beta;
     const T* A;
     int64_t lda;
     const T* B;
     int64_t ldb;
-    T* C;
+    TO* C;
     int64_t ldc;
     int tri;
+    int64_t offA = 0, offB = 0;  // diagonal offsets of the triangular K-trims
     int64_t kchunk;
-    T* part;
+    TE* part;
     int64_t tm, tn;
     OutMap o;
     int64_t sA = 0, sB = 0, sC = 0;  // per-batch element strides (blockIdx.z)
@@ -69,18 +78,21 @@ __device__ __forceinline__ int64_t packed_index(int64_t i, int64_t j, int64_t n)
     return j * (2 * n - j + 1) / 2 + (i - j);
 }
 
-template <typename T>
-__device__ __forceinline__ void store_out(const GemmArgs<T>& g, int64_t gm, int64_t gn, T v)
+template <typename T, typename TO, typename TE>
+__device__ __forceinline__ void store_out(const GemmArgs<T, TO, TE>& g, int64_t gm, int64_t gn, TE v)
 {
     const int64_t i = g.o.row0 + gm, j = g.o.col0 + gn;
-    T* p;
-    if (g.o.pn > 0) {
-        v *= (T)(i == j ? g.o.pdiag : g.o.poff);
+    TO* p;
+    if (g.o.mode == 1) {
+        v *= (TE)(i == j ? g.o.pdiag : g.o.poff);
         p = g.C + packed_index(i, j, g.o.pn);
+    } else if (g.o.mode == 2) {
+        v *= (TE)(i == j ? g.o.pdiag : g.o.poff);
+        p = g.C + (i * (i + 1) / 2 + j - g.o.pbase);
     } else {
         p = g.C + gm + gn * g.ldc;
     }
-    *p = g.beta == T(0) ? v : v + g.beta * *p;
+    *p = g.beta == 0.0 ? (TO)v : (TO)(v + (TE)g.beta * (TE)*p);
 }
 
 // Tile order: with a triangular K-trim the heaviest tiles are issued first
@@ -187,10 +199,15 @@ struct Slab {
     }
 };
 
-template <typename T, int TA, int TB, int TBM, int TBN>
-__global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g)
+// FP32 products accumulated in FP64: the MFMA accumulators are flushed into
+// FP64 registers every FLUSH_SLABS K-slabs (FLUSH_SLABS * BK terms).
+constexpr int FLUSH_SLABS = 16;
+
+template <typename T, typename TO, typename TE, int TA, int TB, int TBM, int TBN>
+__global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, TO, TE> g)
 {
     typedef typename Mfma<T>::v4 v4;
+    constexpr bool ACC64 = sizeof(TE) > sizeof(T);
     constexpr int BM = TBM, BN = TBN;
     constexpr int FM = BM / 32, FN = BN / 32;  // 16 x 16 MFMA tiles per wave (2 x 2 waves)
     __shared__ T As[BK][BM + PAD];
@@ -213,10 +230,10 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
 
     int64_t kbeg = (int64_t)z * g.kchunk;
     int64_t kend = kbeg + g.kchunk < K ? kbeg + g.kchunk : K;
-    if (g.tri & 1) kend = kend < m0 + BM ? kend : m0 + BM;
-    if (g.tri & 2) kbeg = kbeg > m0 ? kbeg : m0;
-    if (g.tri & 4) kend = kend < n0 + BN ? kend : n0 + BN;
-    if (g.tri & 8) kbeg = kbeg > n0 ? kbeg : n0;
+    if (g.tri & 1) kend = kend < m0 + BM + g.offA ? kend : m0 + BM + g.offA;
+    if (g.tri & 2) kbeg = kbeg > m0 + g.offA ? kbeg : m0 + g.offA;
+    if (g.tri & 4) kend = kend < n0 + BN + g.offB ? kend : n0 + BN + g.offB;
+    if (g.tri & 8) kbeg = kbeg > n0 + g.offB ? kbeg : n0 + g.offB;
 
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
@@ -234,6 +251,27 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
+    TE acc64[ACC64 ? FM : 1][ACC64 ? FN : 1][4];
+    if (ACC64) {
+#pragma unroll
+        for (int i = 0; i < (ACC64 ? FM : 1); ++i)
+#pragma unroll
+            for (int j = 0; j < (ACC64 ? FN : 1); ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc64[i][j][r] = 0;
+    }
+    auto flush = [&]() {
+        if (ACC64) {
+#pragma unroll
+            for (int i = 0; i < (ACC64 ? FM : 1); ++i)
+#pragma unroll
+                for (int j = 0; j < (ACC64 ? FN : 1); ++j) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[i][j][r] += (TE)acc[i][j][r];
+                    acc[i][j] = v4{0, 0, 0, 0};
+                }
+        }
+    };
 
     Slab<T, TA, TB, BM, BN> sl;
     auto fetch = [&](int64_t k0) {
@@ -243,6 +281,7 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
 
     if (kbeg < kend) {
         fetch(kbeg);
+        int slab = 0;
         for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
             __syncthreads();
             sl.store(As, Bs, tid);
@@ -260,8 +299,13 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
 #pragma unroll
                     for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::run(a[i], b[j], acc[i][j]);
             }
+            if (ACC64 && ++slab == FLUSH_SLABS) {
+                flush();
+                slab = 0;
+            }
         }
     }
+    flush();
 
     // Epilogue: lane holds column lm of each 16 x 16 tile; its four values
     // are rows lk + 4r (f64 16x16x4) or 4 lk + r (f32 16x16x4), as measured
@@ -277,14 +321,17 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T> g
                 int64_t gn = n0 + wn + j * 16 + lm;
                 if (gm >= M || gn >= N) continue;
                 if (lower_only && g.o.col0 + gn > g.o.row0 + gm) continue;
-                T v = g.alpha * acc[i][j][r];
+                TE v;
+                if (ACC64) v = acc64[ACC64 ? i : 0][ACC64 ? j : 0][r];
+                else v = (TE)acc[i][j][r];
+                v *= (TE)g.alpha;
                 if (g.part) g.part[(int64_t)z * M * N + gm + gn * M] = v;
                 else store_out(g, gm, gn, v);
             }
 }
 
-template <typename T>
-__global__ void splitk_reduce_kernel(GemmArgs<T> g, int nz)
+template <typename T, typename TO, typename TE>
+__global__ void splitk_reduce_kernel(GemmArgs<T, TO, TE> g, int nz)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t M = g.M, N = g.N;
@@ -292,7 +339,7 @@ __global__ void splitk_reduce_kernel(GemmArgs<T> g, int nz)
     int64_t m = idx % M, n = idx / M;
     if ((g.tri & 16) && g.o.col0 + n > g.o.row0 + m) return;
     if ((g.tri & 64) && n > m) return;
-    T s = 0;
+    TE s = 0;
     for (int z = 0; z < nz; ++z) s += g.part[(int64_t)z * M * N + idx];
     store_out(g, m, n, s);
 }
@@ -306,13 +353,22 @@ __global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc)
     if (j > i) C[i + j * ldc] = C[j + i * ldc];
 }
 
-template <typename T>
-int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+struct GemmShape {
+    int64_t offA = 0, offB = 0;
+    OutMap om;
+    int64_t batch = 1, sA = 0, sB = 0, sC = 0;
+};
+
+template <typename T, typename TO, typename TE>
+int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
            int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc, int tri,
-           const OutMap& om = OutMap(), int64_t batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0)
+           const GemmShape& sh)
 {
+    const int64_t batch = sh.batch;
     if (M <= 0 || N <= 0 || batch <= 0) return CFMI_OK;
-    const bool small = M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096;
+    // FP64 accumulators of FP32 products fit in registers only with 96 x 96 tiles.
+    constexpr bool kAcc64 = sizeof(TE) > sizeof(T);
+    const bool small = kAcc64 || (M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096);
     const int tbm = small ? BSMALL : BM, tbn = small ? BSMALL : BN;
     int64_t tm = (M + tbm - 1) / tbm, tn = (N + tbn - 1) / tbn;
     // Split K when the output grid alone cannot fill the 256 CUs.
@@ -322,7 +378,7 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
         nz = std::min<int64_t>((1024 + tiles - 1) / tiles, (K + 255) / 256);
         nz = std::max<int64_t>(nz, 1);
         // Bound the partial buffer to 1 GiB.
-        while (nz > 1 && (size_t)nz * M * N * sizeof(T) > ((size_t)1 << 30)) nz = (nz + 1) / 2;
+        while (nz > 1 && (size_t)nz * M * N * sizeof(TE) > ((size_t)1 << 30)) nz = (nz + 1) / 2;
     }
     int64_t kchunk = BK;
     if (K > 0) {
@@ -331,23 +387,30 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
         nz = (K + kchunk - 1) / kchunk;
     }
     if (nz < 1) nz = 1;
-    T* part = nullptr;
+    TE* part = nullptr;
     if (nz > 1) {
-        part = (T*)cfmi_workspace(c, (size_t)nz * M * N * sizeof(T));
+        part = (TE*)cfmi_workspace(c, (size_t)nz * M * N * sizeof(TE));
         if (!part) return cfmi_fail(c, CFMI_ENOMEM, "gemm: workspace allocation failed");
     }
-    GemmArgs<T> g;
+    GemmArgs<T, TO, TE> g;
     g.M = M; g.N = N; g.K = K > 0 ? K : 0;
-    g.alpha = (T)alpha; g.beta = (T)beta;
-    g.A = (const T*)A; g.lda = lda; g.B = (const T*)B; g.ldb = ldb; g.C = (T*)C; g.ldc = ldc;
-    g.tri = tri; g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = om;
-    g.sA = sA; g.sB = sB; g.sC = sC;
+    g.alpha = alpha; g.beta = beta;
+    g.A = (const T*)A; g.lda = lda; g.B = (const T*)B; g.ldb = ldb; g.C = (TO*)C; g.ldc = ldc;
+    g.tri = tri; g.offA = sh.offA; g.offB = sh.offB;
+    g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = sh.om;
+    g.sA = sh.sA; g.sB = sh.sB; g.sC = sh.sC;
     dim3 grid((unsigned)(tm * tn), (unsigned)nz, (unsigned)batch);
 #define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                        \
     do {                                                                                                  \
-        if (small) hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BSMALL, BSMALL>), grid, dim3(NT), 0,      \
-                                      c->stream, g);                                                      \
-        else hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BM, BN>), grid, dim3(NT), 0, c->stream, g);     \
+        if constexpr (kAcc64) {                                                                           \
+            hipLaunchKernelGGL((gemm_kernel<T, TO, TE, TA_, TB_, BSMALL, BSMALL>), grid, dim3(NT), 0,     \
+                               c->stream, g);                                                             \
+        } else {                                                                                          \
+            if (small) hipLaunchKernelGGL((gemm_kernel<T, TO, TE, TA_, TB_, BSMALL, BSMALL>), grid,       \
+                                          dim3(NT), 0, c->stream, g);                                     \
+            else hipLaunchKernelGGL((gemm_kernel<T, TO, TE, TA_, TB_, BM, BN>), grid, dim3(NT), 0,        \
+                                    c->stream, g);                                                        \
+        }                                                                                                 \
     } while (0)
     if (!ta && !tb) CFMI_GEMM_LAUNCH(0, 0);
     else if (!ta && tb) CFMI_GEMM_LAUNCH(0, 1);
@@ -357,16 +420,27 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     CFMI_LAUNCH_CHECK(c);
     if (nz > 1) {
         int64_t tot = M * N;
-        hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL((splitk_reduce_kernel<T, TO, TE>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                            c->stream, g, (int)nz);
         CFMI_LAUNCH_CHECK(c);
     }
-    if ((tri & 64) && M == N) {
-        hipLaunchKernelGGL((mirror_lower_kernel<T>), dim3((unsigned)((M * M + 255) / 256)), dim3(256), 0,
-                           c->stream, M, (T*)C, ldc);
+    if ((tri & 64) && M == N && sh.om.mode == 0) {
+        hipLaunchKernelGGL((mirror_lower_kernel<TO>), dim3((unsigned)((M * M + 255) / 256)), dim3(256), 0,
+                           c->stream, M, (TO*)C, ldc);
         CFMI_LAUNCH_CHECK(c);
     }
     return CFMI_OK;
+}
+
+// Same-type GEMM (FP64 or FP32 accumulation), the form used inside the library.
+template <typename T>
+int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+           int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc, int tri,
+           const OutMap& om = OutMap(), int64_t batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0)
+{
+    GemmShape sh;
+    sh.om = om; sh.batch = batch; sh.sA = sA; sh.sB = sB; sh.sC = sC;
+    return gemm_x<T, T, T>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -643,7 +717,7 @@ int cfmi_gemm_packed_batched(cfmi_ctx* c, int dtype, int transa, int transb, int
                          (long long)pn);
     if (batch < 1 || batch > 65535) return cfmi_fail(c, CFMI_EINVAL, "gemm_packed: batch out of range");
     OutMap om;
-    om.row0 = row0; om.col0 = col0; om.pn = pn; om.pdiag = pdiag; om.poff = poff;
+    om.mode = 1; om.row0 = row0; om.col0 = col0; om.pn = pn; om.pdiag = pdiag; om.poff = poff;
     int tri = (flags & 15) | 16;
     if (dtype == CFMI_F64)
         return gemm_t<double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, P, 1, tri, om,
@@ -760,6 +834,24 @@ int cfmi_convert(cfmi_ctx* c, int dir, int64_t m, int64_t n, const void* src, in
     return CFMI_OK;
 }
 
+int cfmi_copy(cfmi_ctx* c, int dtype, int64_t m, int64_t n, const void* src, int64_t lds, void* dst, int64_t ldd)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    int64_t tot = m * n;
+    dim3 g((unsigned)((tot + 255) / 256));
+    if (dtype == CFMI_F64)
+        hipLaunchKernelGGL((convert_kernel<double, double>), g, dim3(256), 0, c->stream, m, n, (const double*)src,
+                           lds, (double*)dst, ldd);
+    else if (dtype == CFMI_F32)
+        hipLaunchKernelGGL((convert_kernel<float, float>), g, dim3(256), 0, c->stream, m, n, (const float*)src,
+                           lds, (float*)dst, ldd);
+    else
+        return cfmi_fail(c, CFMI_EINVAL, "copy: unsupported dtype %d", dtype);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
 int cfmi_fill(cfmi_ctx* c, int dtype, int64_t m, int64_t n, void* A, int64_t lda, double v)
 {
     if (!c) return CFMI_EINVAL;
@@ -784,19 +876,53 @@ int cfmi_gram(cfmi_ctx* c, int dtype, int64_t m, int64_t n, int64_t K, const voi
 {
     if (!c) return CFMI_EINVAL;
     const int sym = (X == Y && m == n && ldx == ldy) ? 64 : 0;
+    GemmShape sh;
     if (dtype == CFMI_F64)
-        return gemm_t<double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, sym);
-    if (dtype == CFMI_F32) {
-        // FP32 inputs accumulate per split-K slab in FP32 and are reduced in
-        // FP64: run the slabs into an FP32 partial, then widen.
-        float* g32 = nullptr;
-        CFMI_HIP(c, hipMallocAsync((void**)&g32, (size_t)m * n * sizeof(float), c->stream));
-        int rc = gemm_t<float>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, g32, m, sym);
-        if (!rc) rc = cfmi_convert(c, 1, m, n, g32, m, G, ldg);
-        hipFreeAsync(g32, c->stream);
-        return rc;
-    }
+        return gemm_x<double, double, double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, sym, sh);
+    // FP32 products, FP32 MFMA accumulation within K-chunks, FP64 across them.
+    if (dtype == CFMI_F32)
+        return gemm_x<float, double, double>(c, 1, 0, m, n, K, 1.0, X, ldx, Y, ldy, 0.0, G, ldg, sym, sh);
     return cfmi_fail(c, CFMI_EINVAL, "gram: unsupported dtype %d", dtype);
+}
+
+int cfmi_gemm_ex(cfmi_ctx* c, int dtype, int out_dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const void* A, int64_t lda, int64_t sA, int64_t off_a, const void* B, int64_t ldb,
+                 int64_t sB, int64_t off_b, double beta, void* C, int64_t ldc, int64_t sC, int64_t batch, int flags,
+                 int out_mode, int64_t row0, int64_t col0, int64_t pbase, double pdiag, double poff)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m < 0 || n < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: negative size");
+    if (batch < 1 || batch > 65535) return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: batch out of range");
+    if (lda < 1 || ldb < 1) return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: bad leading dimension");
+    if (out_mode < 0 || out_mode > 2) return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: bad out_mode %d", out_mode);
+    if (out_mode == 0 && ldc < (m > 1 ? m : 1)) return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: ldc < m");
+    if (out_mode == 1 && (row0 < 0 || col0 < 0 || row0 + m > ldc || col0 + n > ldc))
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: block outside the packed %lld x %lld matrix", (long long)ldc,
+                         (long long)ldc);
+    if (out_mode == 2 && (row0 < 0 || col0 < 0 || pbase < 0 || pbase > row0 * (row0 + 1) / 2))
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: packed row band starts after row %lld", (long long)row0);
+    GemmShape sh;
+    sh.offA = off_a; sh.offB = off_b;
+    sh.om.mode = out_mode; sh.om.row0 = row0; sh.om.col0 = col0; sh.om.pn = out_mode == 1 ? ldc : 0;
+    sh.om.pbase = pbase; sh.om.pdiag = pdiag; sh.om.poff = poff;
+    sh.batch = batch; sh.sA = sA; sh.sB = sB; sh.sC = sC;
+    int tri = flags & 31;
+    if (out_mode != 0) tri |= 16;
+    if (out_mode != 0) ldc = 1;
+    const bool acc64 = (flags & CFMI_GEMM_ACC64) != 0;
+    if (dtype == CFMI_F64 && out_dtype == CFMI_F64)
+        return gemm_x<double, double, double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                              ldc, tri, sh);
+    if (dtype == CFMI_F32 && out_dtype == CFMI_F32 && !acc64)
+        return gemm_x<float, float, float>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                           ldc, tri, sh);
+    if (dtype == CFMI_F32 && out_dtype == CFMI_F32)
+        return gemm_x<float, float, double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                            ldc, tri, sh);
+    if (dtype == CFMI_F32 && out_dtype == CFMI_F64)
+        return gemm_x<float, double, double>(c, transa != 0, transb != 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                                             ldc, tri, sh);
+    return cfmi_fail(c, CFMI_EINVAL, "gemm_ex: unsupported dtype pair %d -> %d", dtype, out_dtype);
 }
 
 }  // extern "C"

@@ -72,11 +72,13 @@ struct PairArgs {
     const int* bhi;
     int nbins;
     int sum_mode;      // 1: single summed output (signal), 0: per-spectrum stacks
-    double* out[3];
-    double* mir[3];
+    void* out[3];      // double*, or float* for FP32 derivative stacks
+    void* mir[3];
     int64_t ld, stride, ldm, stride_m;
     int64_t np;        // plane offset: n (22) or n_p (02)
     int64_t ntile;     // tiles per side (symmetric kinds)
+    double* state;     // per-pair recurrence state (P_l, P_{l-1}) after the last bin, or NULL
+    int lresume;       // > 0: start at l = lresume from the saved state
 };
 
 // cos/sin of the rotation angles of geometry.py:49-110, trig-free.
@@ -130,8 +132,9 @@ __device__ __forceinline__ double dot3_exact(const double* a, const double* b)
 // Write a tile plane: value v[q] belongs to pair (j0 + tj, i0 + ti0 + 8q).
 // direct: dst[j + i*ld]; transposed (if dstT): dstT[i + j*ldT].
 // diag_tile: only pairs with j >= i are valid (others skipped).
-__device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double* v, double* dst, int64_t ld,
-                                            double* dstT, int64_t ldT, int64_t j0, int64_t i0, int64_t nr,
+template <typename TO>
+__device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double* v, TO* dst, int64_t ld,
+                                            TO* dstT, int64_t ldT, int64_t j0, int64_t i0, int64_t nr,
                                             int64_t nc, bool diag_tile)
 {
     const int t = threadIdx.x;
@@ -141,7 +144,7 @@ __device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double
         for (int q = 0; q < PPT; ++q) {
             int ti = ti0 + 8 * q;
             int64_t j = j0 + tj, i = i0 + ti;
-            if (j < nr && i < nc && (!diag_tile || tj >= ti)) dst[j + i * ld] = v[q];
+            if (j < nr && i < nc && (!diag_tile || tj >= ti)) dst[j + i * ld] = (TO)v[q];
         }
     }
     if (dstT) {
@@ -154,12 +157,12 @@ __device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double
         for (int q = 0; q < PPT; ++q) {
             int b = b0 + 8 * q;  // local row index j
             int64_t i = i0 + a, j = j0 + b;
-            if (j < nr && i < nc && (!diag_tile || b >= a)) dstT[i + j * ldT] = sT[a][b];
+            if (j < nr && i < nc && (!diag_tile || b >= a)) dstT[i + j * ldT] = (TO)sT[a][b];
         }
     }
 }
 
-template <int KIND>
+template <int KIND, typename TO>
 __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
 {
     __shared__ double sT[TILE][TILE + 1];
@@ -230,10 +233,23 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
         pm2[q] = 0.0;
     }
 
+    // Recurrence state slot of this thread's pairs (tile-major).
+    double* st = p.state ? p.state + ((int64_t)(blockIdx.x + (int64_t)blockIdx.y * gridDim.x) * 256 + t) * (2 * PPT)
+                         : nullptr;
+    int lstart = (KIND == K00S || KIND == K00G) ? 0 : 2;
+    if (st && p.lresume > lstart) {
+        lstart = p.lresume;
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            pm1[q] = st[2 * q];
+            pm2[q] = st[2 * q + 1];
+        }
+    }
+
     // A null bin table means one bin covering l = 0..L (the signal matrix).
     int b = 0;
     int lo = p.blo ? p.blo[0] : 0, hi = p.bhi ? p.bhi[0] : p.L;
-    for (int l = (KIND == K00S || KIND == K00G) ? 0 : 2; l <= p.lrun; ++l) {
+    for (int l = lstart; l <= p.lrun; ++l) {
         double P[PPT], F1[PPT], F2[PPT];
         if (KIND == K00S || KIND == K00G) {
 #pragma unroll
@@ -290,14 +306,14 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
         if (l == hi) {
             // ---- emit bin b ----
             if (KIND == K00S || KIND == K00G) {
-                double* o = p.out[0] ? p.out[0] + b * p.stride : nullptr;
-                double* m = KIND == K00S ? (o) : (p.mir[0] ? p.mir[0] + b * p.stride_m : nullptr);
+                TO* o = p.out[0] ? (TO*)p.out[0] + b * p.stride : nullptr;
+                TO* m = KIND == K00S ? (o) : (p.mir[0] ? (TO*)p.mir[0] + b * p.stride_m : nullptr);
                 int64_t ldm = KIND == K00S ? p.ld : p.ldm;
                 write_plane(sT, acc[0], o, p.ld, m, ldm, j0, i0, p.nr, p.nc, diag_tile);
             } else if (KIND == K22) {
                 const int nspec = p.sum_mode ? 1 : 3;
                 for (int s = 0; s < nspec; ++s) {
-                    double* o = p.out[s];
+                    TO* o = (TO*)p.out[s];
                     if (!o) continue;
                     o += b * p.stride;
                     double va[PPT], vb[PPT], vc[PPT], vd[PPT];
@@ -323,10 +339,10 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
                         vd[q] = -qq * c1s2 + uu * s1c2 + qu * (c1c2 - s1s2);
                     }
                     const int64_t n = p.np, ld = p.ld;
-                    double* QQ = o;
-                    double* UU = o + n + n * ld;
-                    double* UQ = o + n;
-                    double* QU = o + n * ld;
+                    TO* QQ = o;
+                    TO* UU = o + n + n * ld;
+                    TO* UQ = o + n;
+                    TO* QU = o + n * ld;
                     write_plane(sT, va, QQ, ld, QQ, ld, j0, i0, p.nr, p.nc, diag_tile);
                     write_plane(sT, vb, UU, ld, UU, ld, j0, i0, p.nr, p.nc, diag_tile);
                     write_plane(sT, vc, UQ, ld, QU, ld, j0, i0, p.nr, p.nc, diag_tile);
@@ -335,8 +351,8 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
             } else {  // K02: rows j are spin-2 pixels, cols i spin-0
                 const int nspec = p.sum_mode ? 1 : 2;
                 for (int s = 0; s < nspec; ++s) {
-                    double* o = p.out[s] ? p.out[s] + b * p.stride : nullptr;
-                    double* m = p.mir[s] ? p.mir[s] + b * p.stride_m : nullptr;
+                    TO* o = p.out[s] ? (TO*)p.out[s] + b * p.stride : nullptr;
+                    TO* m = p.mir[s] ? (TO*)p.mir[s] + b * p.stride_m : nullptr;
                     if (!o && !m) continue;
                     double vq[PPT], vu[PPT];
 #pragma unroll
@@ -364,10 +380,20 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
             hi = p.bhi[b];
         }
     }
+    if (st) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            st[2 * q] = pm1[q];
+            st[2 * q + 1] = pm2[q];
+        }
+    }
 }
 
-int launch_pairs(cfmi_ctx* c, int kind, PairArgs& a, const double* w0, const double* w1, const double* w2)
+
+int launch_pairs(cfmi_ctx* c, int kind, PairArgs& a, const double* w0, const double* w1, const double* w2,
+                 int dtype = CFMI_F64)
 {
+    if (dtype != CFMI_F64 && dtype != CFMI_F32) return cfmi_fail(c, CFMI_EINVAL, "unsupported output dtype %d", dtype);
     if (a.L < 0 || a.L > CFMI_MAX_ELL) return cfmi_fail(c, CFMI_EINVAL, "lmax %d out of range", a.L);
     if (a.nr <= 0 || a.nc <= 0 || a.nbins <= 0) return CFMI_OK;
     if (!c->d_coef) CFMI_HIP(c, hipMalloc((void**)&c->d_coef, 5 * (CFMI_MAX_ELL + 1) * sizeof(double)));
@@ -376,15 +402,22 @@ int launch_pairs(cfmi_ctx* c, int kind, PairArgs& a, const double* w0, const dou
     CFMI_LAUNCH_CHECK(c);
     a.tab = c->d_coef;
     int64_t tr = (a.nr + TILE - 1) / TILE, tc = (a.nc + TILE - 1) / TILE;
+    const bool f32 = dtype == CFMI_F32;
+#define CFMI_PAIRS(K_, grid_)                                                                           \
+    do {                                                                                               \
+        if (f32) hipLaunchKernelGGL((pair_kernel<K_, float>), grid_, dim3(256), 0, c->stream, a);       \
+        else hipLaunchKernelGGL((pair_kernel<K_, double>), grid_, dim3(256), 0, c->stream, a);          \
+    } while (0)
     if (kind == K00S || kind == K22) {
-        int64_t nt = tr * (tr + 1) / 2;
-        if (kind == K00S) hipLaunchKernelGGL((pair_kernel<K00S>), dim3((unsigned)nt), dim3(256), 0, c->stream, a);
-        else hipLaunchKernelGGL((pair_kernel<K22>), dim3((unsigned)nt), dim3(256), 0, c->stream, a);
+        dim3 g((unsigned)(tr * (tr + 1) / 2));
+        if (kind == K00S) CFMI_PAIRS(K00S, g);
+        else CFMI_PAIRS(K22, g);
     } else {
         dim3 g((unsigned)tr, (unsigned)tc);
-        if (kind == K00G) hipLaunchKernelGGL((pair_kernel<K00G>), g, dim3(256), 0, c->stream, a);
-        else hipLaunchKernelGGL((pair_kernel<K02>), g, dim3(256), 0, c->stream, a);
+        if (kind == K00G) CFMI_PAIRS(K00G, g);
+        else CFMI_PAIRS(K02, g);
     }
+#undef CFMI_PAIRS
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
@@ -640,10 +673,18 @@ int cfmi_signal_02(cfmi_ctx* c, const double* cl_te, const double* cl_tb, int lm
     return launch_pairs(c, K02, a, cl_te, cl_tb, nullptr);
 }
 
-int cfmi_deriv_stack_00(cfmi_ctx* c, const double* w, int lmax, const int* bin_lo, const int* bin_hi, int nbins,
-                        const double* vec_r, int64_t n_r, const double* vec_c, int64_t n_c, int symmetric,
-                        double* out, int64_t ld, int64_t stride, double* mirror, int64_t ld_mirror,
-                        int64_t stride_mirror)
+int64_t cfmi_deriv_state_doubles(int64_t n_r, int64_t n_c, int symmetric)
+{
+    if (n_r <= 0 || n_c <= 0) return 0;
+    int64_t tr = (n_r + TILE - 1) / TILE, tc = (n_c + TILE - 1) / TILE;
+    int64_t tiles = symmetric ? tr * (tr + 1) / 2 : tr * tc;
+    return tiles * 256 * 2 * PPT;
+}
+
+int cfmi_deriv_stack_00(cfmi_ctx* c, int dtype, const double* w, int lmax, const int* bin_lo, const int* bin_hi,
+                        int nbins, const double* vec_r, int64_t n_r, const double* vec_c, int64_t n_c, int symmetric,
+                        void* out, int64_t ld, int64_t stride, void* mirror, int64_t ld_mirror,
+                        int64_t stride_mirror, double* state, int lresume)
 {
     if (!c) return CFMI_EINVAL;
     if (nbins <= 0) return CFMI_OK;
@@ -653,12 +694,14 @@ int cfmi_deriv_stack_00(cfmi_ctx* c, const double* w, int lmax, const int* bin_l
     a.vr = vec_r; a.vc = vec_c; a.nr = n_r; a.nc = n_c; a.L = lmax; a.lrun = lmax;
     a.blo = bin_lo; a.bhi = bin_hi; a.nbins = nbins; a.sum_mode = 0;
     a.out[0] = out; a.mir[0] = mirror; a.ld = ld; a.stride = stride; a.ldm = ld_mirror; a.stride_m = stride_mirror;
-    return launch_pairs(c, symmetric ? K00S : K00G, a, w, nullptr, nullptr);
+    a.state = state; a.lresume = lresume;
+    return launch_pairs(c, symmetric ? K00S : K00G, a, w, nullptr, nullptr, dtype);
 }
 
-int cfmi_deriv_stack_22(cfmi_ctx* c, const double* w_ee, const double* w_bb, const double* w_eb, int lmax,
-                        const int* bin_lo, const int* bin_hi, int nbins, const double* vec, int64_t n,
-                        double* out_ee, double* out_bb, double* out_eb, int64_t ld, int64_t stride)
+int cfmi_deriv_stack_22(cfmi_ctx* c, int dtype, const double* w_ee, const double* w_bb, const double* w_eb,
+                        int lmax, const int* bin_lo, const int* bin_hi, int nbins, const double* vec, int64_t n,
+                        void* out_ee, void* out_bb, void* out_eb, int64_t ld, int64_t stride, double* state,
+                        int lresume)
 {
     if (!c) return CFMI_EINVAL;
     if (nbins <= 0) return CFMI_OK;
@@ -666,13 +709,15 @@ int cfmi_deriv_stack_22(cfmi_ctx* c, const double* w_ee, const double* w_bb, con
     a.vr = vec; a.vc = vec; a.nr = n; a.nc = n; a.L = lmax; a.lrun = lmax;
     a.blo = bin_lo; a.bhi = bin_hi; a.nbins = nbins; a.sum_mode = 0;
     a.out[0] = out_ee; a.out[1] = out_bb; a.out[2] = out_eb; a.ld = ld; a.stride = stride; a.np = n;
-    return launch_pairs(c, K22, a, w_ee, w_bb, w_eb);
+    a.state = state; a.lresume = lresume;
+    return launch_pairs(c, K22, a, w_ee, w_bb, w_eb, dtype);
 }
 
-int cfmi_deriv_stack_02(cfmi_ctx* c, const double* w_te, const double* w_tb, int lmax, const int* bin_lo,
-                        const int* bin_hi, int nbins, const double* vec_t, int64_t n_t, const double* vec_p,
-                        int64_t n_p, double* out_te, double* out_tb, int64_t ld, int64_t stride, double* mirror_te,
-                        double* mirror_tb, int64_t ld_mirror, int64_t stride_mirror)
+int cfmi_deriv_stack_02(cfmi_ctx* c, int dtype, const double* w_te, const double* w_tb, int lmax,
+                        const int* bin_lo, const int* bin_hi, int nbins, const double* vec_t, int64_t n_t,
+                        const double* vec_p, int64_t n_p, void* out_te, void* out_tb, int64_t ld, int64_t stride,
+                        void* mirror_te, void* mirror_tb, int64_t ld_mirror, int64_t stride_mirror, double* state,
+                        int lresume)
 {
     if (!c) return CFMI_EINVAL;
     if (nbins <= 0) return CFMI_OK;
@@ -681,7 +726,8 @@ int cfmi_deriv_stack_02(cfmi_ctx* c, const double* w_te, const double* w_tb, int
     a.blo = bin_lo; a.bhi = bin_hi; a.nbins = nbins; a.sum_mode = 0;
     a.out[0] = out_te; a.out[1] = out_tb; a.mir[0] = mirror_te; a.mir[1] = mirror_tb;
     a.ld = ld; a.stride = stride; a.ldm = ld_mirror; a.stride_m = stride_mirror; a.np = n_p;
-    return launch_pairs(c, K02, a, w_te, w_tb, nullptr);
+    a.state = state; a.lresume = lresume;
+    return launch_pairs(c, K02, a, w_te, w_tb, nullptr, dtype);
 }
 
 }  // extern "C"

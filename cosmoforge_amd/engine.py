@@ -191,65 +191,117 @@ class Block:
     ma: int
     rb: int
     mb: int
-    stack: torch.Tensor  # (nbins, mb, ma) col-major blocks
+    stack: torch.Tensor | None = None  # (nbins, mb, ma) col-major blocks
+
+    @property
+    def sym(self) -> bool:
+        return self.ra == self.rb and self.ma == self.mb
 
 
-def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: int,
-                 bins: list) -> list:
-    """For every spectrum of ``layout.spectra_list()``, every bin's D block.
-
-    ``weights``: label -> device l-indexed w_l B1 B2 (lmax + 1).
-    Returns a list of Block, one per spectrum, in spectra-list order."""
-    lo = ctx.upload(np.array([b[0] for b in bins], dtype=np.int32), dtype=torch.int32)
-    hi = ctx.upload(np.array([b[1] for b in bins], dtype=np.int32), dtype=torch.int32)
-    nb = len(bins)
-    lib, h = ctx.lib, ctx.h
+def block_geometry(layout: Layout) -> list:
+    """Block (without stack) of every spectrum of ``layout.spectra_list()``."""
     out = []
-    cache22 = {}
-    cache02 = {}
     for sp in layout.spectra_list():
         fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
-        if fi.spin == 0 and fj.spin == 0:
-            if sp.fi == sp.fj:
-                st = torch.empty((nb, fi.rows, fi.rows), dtype=torch.float64, device=ctx.device)
-                ctx.check(lib.cfmi_deriv_stack_00(h, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
-                                                  P(vecs[sp.fi]), fi.npix, P(vecs[sp.fi]), fi.npix, 1, P(st),
-                                                  fi.rows, fi.rows * fi.rows, None, 0, 0), "deriv_stack_00")
-                out.append(Block(fi.offset, fi.rows, fi.offset, fi.rows, st))
-            else:
-                st = torch.empty((nb, fi.rows, fj.rows), dtype=torch.float64, device=ctx.device)
-                ctx.check(lib.cfmi_deriv_stack_00(h, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
-                                                  P(vecs[sp.fj]), fj.npix, P(vecs[sp.fi]), fi.npix, 0, P(st),
-                                                  fj.rows, fi.rows * fj.rows, None, 0, 0), "deriv_stack_00")
-                out.append(Block(fj.offset, fj.rows, fi.offset, fi.rows, st))
-        elif fi.spin == 2 and fj.spin == 2:
-            if sp.fi != sp.fj:
-                raise NotImplementedError("spin-2 x spin-2 cross-field derivative")
-            key = sp.fi
-            if key not in cache22:
-                labs = [layout.spectra_map[(sp.fi, sp.fi, m)] for m in range(3)]
-                sts = [torch.empty((nb, fi.rows, fi.rows), dtype=torch.float64, device=ctx.device)
-                       for _ in range(3)]
-                ctx.check(lib.cfmi_deriv_stack_22(h, P(weights[labs[0]]), P(weights[labs[1]]),
-                                                  P(weights[labs[2]]), lmax, P(lo), P(hi), nb, P(vecs[key]),
-                                                  fi.npix, P(sts[0]), P(sts[1]), P(sts[2]), fi.rows,
-                                                  fi.rows * fi.rows), "deriv_stack_22")
-                cache22[key] = sts
-            out.append(Block(fi.offset, fi.rows, fi.offset, fi.rows, cache22[key][sp.mode]))
+        if fi.spin == 2 and fj.spin == 2 and sp.fi != sp.fj:
+            raise NotImplementedError("spin-2 x spin-2 cross-field derivative")
+        if fi.spin == 2 and fj.spin == 0:
+            raise NotImplementedError("spin-2 field before spin-0 field")
+        if sp.fi == sp.fj:
+            out.append(Block(fi.offset, fi.rows, fi.offset, fi.rows))
         else:
-            if fi.spin == 2:
-                raise NotImplementedError("spin-2 field before spin-0 field")
-            key = (sp.fi, sp.fj)
-            if key not in cache02:
-                labs = [layout.spectra_map[(sp.fi, sp.fj, m)] for m in range(2)]
-                sts = [torch.empty((nb, fi.rows, fj.rows), dtype=torch.float64, device=ctx.device)
-                       for _ in range(2)]
-                ctx.check(lib.cfmi_deriv_stack_02(h, P(weights[labs[0]]), P(weights[labs[1]]), lmax, P(lo),
-                                                  P(hi), nb, P(vecs[sp.fi]), fi.npix, P(vecs[sp.fj]), fj.npix,
-                                                  P(sts[0]), P(sts[1]), fj.rows, fi.rows * fj.rows, None, None,
-                                                  0, 0), "deriv_stack_02")
-                cache02[key] = sts
-            out.append(Block(fj.offset, fj.rows, fi.offset, fi.rows, cache02[key][sp.mode]))
+            out.append(Block(fj.offset, fj.rows, fi.offset, fi.rows))
+    return out
+
+
+def deriv_state(ctx: Context, layout: Layout, group_spectrum: int) -> torch.Tensor:
+    """Recurrence-state buffer for the recurrence pass of one spectrum group."""
+    sp = layout.spectra_list()[group_spectrum]
+    fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
+    if sp.fi == sp.fj:
+        k = ctx.lib.cfmi_deriv_state_doubles(fi.npix, fi.npix, 1)
+    elif fi.spin == 0 and fj.spin == 0:
+        k = ctx.lib.cfmi_deriv_state_doubles(fj.npix, fi.npix, 0)
+    else:
+        k = ctx.lib.cfmi_deriv_state_doubles(fj.npix, fi.npix, 0)
+    return torch.empty(int(k), dtype=torch.float64, device=ctx.device)
+
+
+def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: int, bins: list,
+                 dtype=torch.float64, only=None, bin_range=None, state: torch.Tensor | None = None,
+                 lresume: int = 0) -> list:
+    """For every spectrum of ``layout.spectra_list()``, every bin's D block.
+
+    ``weights``: label -> device l-indexed w_l B1 B2 (lmax + 1).  ``only``:
+    indices of the spectra to produce (the others get ``stack=None``);
+    ``bin_range``: (b0, b1), produce bins b0 .. b1-1 only.  Stacks are FP64
+    sums of the FP64 recurrences stored as ``dtype``.  The three spin-2
+    spectra of a field share one recurrence pass, and TE/TB another.
+    ``state`` (``deriv_state``, one recurrence group per call): the pass
+    saves its final recurrence state there and, with ``lresume`` > 0,
+    resumes at l = lresume from it."""
+    b0, b1 = bin_range if bin_range is not None else (0, len(bins))
+    sel = bins[b0:b1]
+    lo = ctx.upload(np.array([b[0] for b in sel], dtype=np.int32), dtype=torch.int32)
+    hi = ctx.upload(np.array([b[1] for b in sel], dtype=np.int32), dtype=torch.int32)
+    nb = len(sel)
+    dcode = F64 if dtype == torch.float64 else F32
+    lib, h = ctx.lib, ctx.h
+    specs = layout.spectra_list()
+    geo = block_geometry(layout)
+    want = set(range(len(specs)) if only is None else only)
+    out = [Block(g.ra, g.ma, g.rb, g.mb) for g in geo]
+    done = set()
+    for k, sp in enumerate(specs):
+        if k not in want or k in done:
+            continue
+        fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
+        blk = out[k]
+        if fi.spin == 0 and fj.spin == 0:
+            st = torch.empty((nb, blk.mb, blk.ma), dtype=dtype, device=ctx.device)
+            if sp.fi == sp.fj:
+                ctx.check(lib.cfmi_deriv_stack_00(h, dcode, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
+                                                  P(vecs[sp.fi]), fi.npix, P(vecs[sp.fi]), fi.npix, 1, P(st),
+                                                  fi.rows, fi.rows * fi.rows, None, 0, 0, P(state), lresume),
+                          "deriv_stack_00")
+            else:
+                ctx.check(lib.cfmi_deriv_stack_00(h, dcode, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
+                                                  P(vecs[sp.fj]), fj.npix, P(vecs[sp.fi]), fi.npix, 0, P(st),
+                                                  fj.rows, fi.rows * fj.rows, None, 0, 0, P(state), lresume),
+                          "deriv_stack_00")
+            blk.stack = st
+            done.add(k)
+        elif fi.spin == 2:
+            # EE, BB, EB of this field: one pass, only the requested planes.
+            members = [m for m, s2 in enumerate(specs) if s2.fi == sp.fi and s2.fj == sp.fj]
+            sts = [None, None, None]
+            for m in members:
+                if m in want:
+                    sts[specs[m].mode] = torch.empty((nb, fi.rows, fi.rows), dtype=dtype, device=ctx.device)
+            labs = [layout.spectra_map[(sp.fi, sp.fi, md)] for md in range(3)]
+            ctx.check(lib.cfmi_deriv_stack_22(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]),
+                                              P(weights[labs[2]]), lmax, P(lo), P(hi), nb, P(vecs[sp.fi]),
+                                              fi.npix, P(sts[0]), P(sts[1]), P(sts[2]), fi.rows,
+                                              fi.rows * fi.rows, P(state), lresume), "deriv_stack_22")
+            for m in members:
+                if m in want:
+                    out[m].stack = sts[specs[m].mode]
+                    done.add(m)
+        else:
+            members = [m for m, s2 in enumerate(specs) if s2.fi == sp.fi and s2.fj == sp.fj]
+            sts = [None, None]
+            for m in members:
+                if m in want:
+                    sts[specs[m].mode] = torch.empty((nb, fi.rows, fj.rows), dtype=dtype, device=ctx.device)
+            labs = [layout.spectra_map[(sp.fi, sp.fj, md)] for md in range(2)]
+            ctx.check(lib.cfmi_deriv_stack_02(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]), lmax, P(lo),
+                                              P(hi), nb, P(vecs[sp.fi]), fi.npix, P(vecs[sp.fj]), fj.npix,
+                                              P(sts[0]), P(sts[1]), fj.rows, fi.rows * fj.rows, None, None,
+                                              0, 0, P(state), lresume), "deriv_stack_02")
+            for m in members:
+                if m in want:
+                    out[m].stack = sts[specs[m].mode]
+                    done.add(m)
     return out
 
 
@@ -333,38 +385,104 @@ def gemm_packed_batched(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int
                                                row0, col0, pdiag, poff, batch, flags), "gemm_packed_batched")
 
 
-def whitened_derivatives(ctx: Context, W: torch.Tensor, n: int, blocks: list, nbins: int,
-                         out: torch.Tensor, zbuf: torch.Tensor | None = None):
-    """out[p] = packed lower triangle of W dC_p W^T, diagonal scaled by
-    1/sqrt(2), for every parameter p = spectrum * nbins + bin.
+def gemm_ex(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int,
+            B, offb: int, ldb: int, beta: float, C, offc: int = 0, ldc: int = 1, *, sA: int = 0, sB: int = 0,
+            sC: int = 0, batch: int = 1, flags: int = 0, tri_a: int = 0, tri_b: int = 0, out_mode: int = 0,
+            row0: int = 0, col0: int = 0, pbase: int = 0, pdiag: float = 1.0, poff: float = 1.0):
+    """``cfmi_gemm_ex``: input dtype from A, output dtype from C."""
+    if m <= 0 or n <= 0:
+        return
+    ctx.check(ctx.lib.cfmi_gemm_ex(ctx.h, dtype_code(A), dtype_code(C), 1 if ta else 0, 1 if tb else 0, m, n, k,
+                                   alpha, P(A, offa), lda, sA, tri_a, P(B, offb), ldb, sB, tri_b, beta,
+                                   P(C, offc), ldc, sC, batch, flags, out_mode, row0, col0, pbase, pdiag, poff),
+              "gemm_ex")
 
-    With that scaling <out[i], out[j]> = 1/2 Tr[W dC_i W^T W dC_j W^T] =
-    1/2 Tr[C^-1 dC_i C^-1 dC_j] = F_ij.  ``out`` is (n_params, n(n+1)/2),
-    zero-filled.  All bins of a spectrum go through one batched launch per
-    product."""
-    K = packed_size(n)
-    for s, blk in enumerate(blocks):
-        ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
-        zs = (n - ra) * mb
-        need = zs * nbins
-        if zbuf is None or zbuf.numel() < need:
+
+def _gemm_ex(*a, **k):
+    return gemm_ex(*a, **k)
+
+
+def band_size(r0: int, r1: int) -> int:
+    """Entries of lower-triangle rows r0 .. r1-1."""
+    return (r1 * (r1 + 1) - r0 * (r0 + 1)) // 2
+
+
+def whiten_band(ctx: Context, W: torch.Tensor, n: int, blk: Block, X: torch.Tensor, r0: int, r1: int,
+                zbuf: torch.Tensor | None = None, flags: int = 0, zcap: int | None = None,
+                counter: list | None = None) -> torch.Tensor | None:
+    """X[b] += rows r0..r1-1 of the lower triangle of W dC_b W^T, row-major
+    packed (element (i, c <= i) at i(i+1)/2 + c - r0(r0+1)/2), diagonal
+    scaled by 1/sqrt(2), for every bin b of ``blk.stack``.
+
+    With that scaling the sum over bands of <X_i, X_j> is
+    1/2 Tr[W dC_i W^T W dC_j W^T] = 1/2 Tr[C^-1 dC_i C^-1 dC_j].
+    W is lower triangular (its zero parts are skipped); ``X`` is
+    (nbins, band_size(r0, r1)) and must hold zeros (or earlier terms).
+    ``zcap``: at most this many elements of intermediate products at once
+    (bins are then taken in sub-batches)."""
+    D = blk.stack
+    if zcap is not None:
+        rows = r1 - min(max(r0, blk.ra), max(r0, blk.rb))
+        per = max(rows, 1) * max(blk.ma, blk.mb)
+        step = max(1, zcap // per)
+        if step < D.shape[0]:
+            for b0 in range(0, D.shape[0], step):
+                sub = Block(blk.ra, blk.ma, blk.rb, blk.mb, D[b0:b0 + step])
+                zbuf = whiten_band(ctx, W, n, sub, X[b0:b0 + step], r0, r1, zbuf, flags, counter=counter)
+            return zbuf
+    nbb = D.shape[0]
+    ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
+    sX = X.shape[1]
+    pbase = r0 * (r0 + 1) // 2
+    full = r0 == 0 and r1 == n
+    fl = flags
+
+    def gemm_ex(*a, **k):  # noqa: F811  (counts the launches of this call)
+        if counter is not None:
+            counter.append(1)
+        _gemm_ex(*a, **k)
+
+    def need(k):
+        nonlocal zbuf
+        if zbuf is None or zbuf.numel() < k or zbuf.dtype != W.dtype:
             zbuf = None
-            zbuf = torch.empty(need, dtype=torch.float64, device=ctx.device)
-        Vs = out[s * nbins:(s + 1) * nbins]
-        # Z_b = W[ra:, ra:ra+ma] D_b      ((n-ra) x mb each; W block lower)
-        gemm_batched(ctx, False, False, n - ra, mb, ma, 1.0, W, ra + ra * n, n, 0, blk.stack, 0, ma, ma * mb, 0.0,
-                     zbuf, 0, n - ra, zs, nbins, 1)
-        # X_b = Z_b W[rb:, rb:rb+mb]^T at rows ra.., cols rb.. (lower part)
-        gemm_packed_batched(ctx, False, True, n - ra, n - rb, mb, 1.0, zbuf, 0, n - ra, zs, W, rb + rb * n, n, 0,
-                            0.0, Vs, K, n, ra, rb, nbins, flags=4)
-        if ra != rb or ma != mb:
-            # + X_b^T = W[rb:, rb:rb+mb] Z_b^T at rows rb.., cols ra..
-            gemm_packed_batched(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, 0, zbuf, 0, n - ra,
-                                zs, 1.0, Vs, K, n, rb, ra, nbins, flags=1)
+            zbuf = torch.empty(k, dtype=W.dtype, device=ctx.device)
+        return zbuf
+
+    # term 1: X[i, c] += sum_k (W[i, a] D)[k] W[c, rb + k],   i >= ra, c >= rb
+    i0 = max(r0, ra)
+    rr = r1 - i0
+    if rr > 0:
+        Z = need(rr * mb * nbb)
+        gemm_ex(ctx, False, False, rr, mb, ma, 1.0, W, i0 + ra * n, n, D, 0, ma, 0.0, Z, 0, rr, sB=ma * mb,
+                sC=rr * mb, batch=nbb, flags=1 | fl, tri_a=i0 - ra)
+        gemm_ex(ctx, False, True, rr, r1 - rb, mb, 1.0, Z, 0, rr, W, rb + rb * n, n, 1.0, X, 0, sA=rr * mb,
+                sC=sX, batch=nbb, flags=4 | fl, out_mode=2, row0=i0, col0=rb, pbase=pbase, pdiag=SQRT_HALF)
+    if blk.sym:
+        return zbuf
+    # term 2: the transpose, X[i, c] += (W dC^T W^T)[i, c] = sum_k' (W[i, b] D^T)[k'] W[c, ra + k']
+    if full and rr > 0:
+        # Z above holds all rows of W[:, a] D: reuse it as op(B) = Z^T.
+        gemm_ex(ctx, False, True, n - rb, n - ra, mb, 1.0, W, rb + rb * n, n, Z, 0, rr, 1.0, X, 0, sB=rr * mb,
+                sC=sX, batch=nbb, flags=1 | fl, out_mode=2, row0=rb, col0=ra, pbase=pbase, pdiag=SQRT_HALF)
+        return zbuf
+    j0 = max(r0, rb)
+    rj = r1 - j0
+    if rj <= 0 or r1 <= ra:
+        return zbuf
+    Z2 = need(rj * ma * nbb)
+    gemm_ex(ctx, False, True, rj, ma, mb, 1.0, W, j0 + rb * n, n, D, 0, ma, 0.0, Z2, 0, rj, sB=ma * mb,
+            sC=rj * ma, batch=nbb, flags=1 | fl, tri_a=j0 - rb)
+    gemm_ex(ctx, False, True, rj, r1 - ra, ma, 1.0, Z2, 0, rj, W, ra + ra * n, n, 1.0, X, 0, sA=rj * ma, sC=sX,
+            batch=nbb, flags=4 | fl, out_mode=2, row0=j0, col0=ra, pbase=pbase, pdiag=SQRT_HALF)
     return zbuf
 
 
-def gram(ctx: Context, m: int, nn: int, K: int, X: torch.Tensor, Y: torch.Tensor) -> np.ndarray:
-    G = torch.empty((nn, m), dtype=torch.float64, device=ctx.device)
-    ctx.check(ctx.lib.cfmi_gram(ctx.h, dtype_code(X), m, nn, K, P(X), K, P(Y), K, P(G), m), "gram")
-    return to_numpy_cm(G)
+def gram(ctx: Context, m: int, nn: int, K: int, X: torch.Tensor, Y: torch.Tensor,
+         out: torch.Tensor | None = None, beta: float = 0.0):
+    """G = X^T Y (m x nn, FP64) over K-long columns; with ``out`` the device
+    G is accumulated (beta) and returned, otherwise a host copy."""
+    G = out if out is not None else torch.empty((nn, m), dtype=torch.float64, device=ctx.device)
+    sym = 64 if (X.data_ptr() == Y.data_ptr() and m == nn) else 0
+    gemm_ex(ctx, True, False, m, nn, K, 1.0, X, 0, K, Y, 0, K, beta, G, 0, m, flags=sym | _lib.CFMI_GEMM_ACC64)
+    return G if out is not None else to_numpy_cm(G)

@@ -4,7 +4,10 @@ Constructor kwargs and getters follow ``src/cosmoforge.qube/qube/fisher.py``
 (:141-266 constructor, :622-820 ``run``, :826-1232 getters).  The matrix is
 F_ij = 1/2 Tr[C^-1 dC_i C^-1 dC_j] over parameters ordered spectrum-major
 (``spectra_list`` order, then bins), beam-smoothed as the reference's.
-All device work runs in libcfmi on one GPU.
+All device work runs in libcfmi on one GPU.  ``precision="mixed"`` (no
+reference counterpart) runs the derivative stacks, the whitener and the
+whitening products in FP32 on MFMA, with FP64 recurrences, factorisation,
+FP64-accumulated Gram matrix and FP64 F^-1.
 """
 
 from __future__ import annotations
@@ -18,7 +21,8 @@ from .core import Inputs, bins_from_params, resolve_basis
 class Fisher:
     def __init__(self, params=None, basis=None, cache_derivatives: bool = False, symmetry_mode=None,
                  mask=None, noise_cov1=None, noise_cov2=None, cls_data=None, fiducial_cls=None, beam=None,
-                 *, params_file=None, device: int = 0, context: engine.Context | None = None, **kwargs):
+                 *, params_file=None, device: int = 0, context: engine.Context | None = None,
+                 precision: str = "fp64", **kwargs):
         if params is None and params_file is not None:
             params = params_file
         if noise_cov2 is not None:
@@ -44,9 +48,13 @@ class Fisher:
         self.beam_smoothing = None
         self.spectra_list = None
         self.bins = None
-        self.V = None  # device stack of whitened derivatives (reused by Spectra)
+        if precision not in pipeline.PRECISIONS:
+            raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
+        self.precision = precision
+        self.derivs = None  # device derivative stacks (reused by Spectra when resident)
         self.timings = {}
         self.timer = kwargs.get("timer")
+        self.launch_timer = kwargs.get("launch_timer")
 
     # -- pipeline -----------------------------------------------------------
     @property
@@ -88,16 +96,17 @@ class Fisher:
         self.n_params = len(self.spectra_list) * len(self.bins)
         self.ctx.sync()
         t1 = time.perf_counter()
-        self.fisher, self.V = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
-                                              timer=self.timer)
+        self.fisher, self.derivs = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
+                                                   timer=self.timer, precision=self.precision,
+                                                   launch_timer=self.launch_timer)
         self.ctx.sync()
         self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
 
     compute = run
 
     def release(self):
-        """Drop the device derivative stack (it holds n_params * n^2 doubles)."""
-        self.V = None
+        """Drop the device derivative stacks (Spectra regenerates them)."""
+        self.derivs = None
 
     # -- getters ------------------------------------------------------------
     def get_fisher_matrix(self) -> np.ndarray | None:

@@ -20,6 +20,7 @@ import torch
 
 from . import engine
 from .core import Inputs, shape_weights
+from . import _lib
 from .engine import P
 
 
@@ -36,6 +37,7 @@ class Covariance:
         self.N = ctx.upload_cm(inputs.noise)
         self.S = None
         self.factor = None
+        self._w32 = None
 
     def signal(self, cls=None) -> torch.Tensor:
         ctx = self.ctx
@@ -52,6 +54,7 @@ class Covariance:
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(S), n, 0.0, P(C), n), "axpby")
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
         self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener)
+        self._w32 = None
         return self.factor
 
     def inverse(self) -> torch.Tensor:
@@ -87,54 +90,275 @@ class StageTimer:
         return sum(a.elapsed_time(b) for a, b in zip(self.events[start], self.events[end]))
 
 
-def whitening_flops(n: int, blocks, nbins: int) -> float:
+def whitening_flops(n: int, blocks, nbins: int, bands=None) -> float:
     """Multiply-add flops (x2) of the whitened-derivative GEMMs, counting the
     products the kernels need: the nonzero (lower-triangular) part of W and,
-    for the output, its lower triangle only."""
+    for the output, its lower triangle only.  ``bands``: the row bands the
+    work is split into (one band: the whole matrix)."""
+    bands = bands or [(0, n)]
     tot = 0.0
     for blk in blocks:
         ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
-        i = np.arange(n - ra)
-        z = 2.0 * mb * np.minimum(i + 1, ma).sum()
-        j = np.arange(n - rb)
-        kj = np.minimum(j + 1, mb)
-        if ra == rb and ma == mb:
-            x = 2.0 * ((n - ra - j) * kj).sum()
-        else:
-            x = 2.0 * (n - ra) * kj.sum()
-        tot += (z + x) * nbins
+        for r0, r1 in bands:
+            i = np.arange(max(r0, ra), r1)
+            tot += 2.0 * mb * np.minimum(i - ra + 1, ma).sum() * nbins      # Z = W[i, a] D
+            tot += 2.0 * _tri_sum(i, rb, mb) * nbins                        # Z W[c, b]^T
+            if blk.sym:
+                continue
+            if r0 == 0 and r1 == n:                                         # W[i, b] Z^T
+                j = np.arange(rb, n)
+                tot += 2.0 * (np.minimum(j - rb + 1, mb) * np.maximum(j - ra + 1, 0)).sum() * nbins
+            else:                                                           # Z2 = W[j, b] D^T, Z2 W[c, a]^T
+                j = np.arange(max(r0, rb), r1)
+                tot += 2.0 * ma * np.minimum(j - rb + 1, mb).sum() * nbins
+                tot += 2.0 * _tri_sum(j, ra, ma) * nbins
     return float(tot)
 
 
-def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None):
-    """(F, V): F is (n_p, n_p) on the host, V the device (n_p, n^2) stack of
-    whitened derivatives W dC_p W^T (kept for QML)."""
-    ctx, inp, n = cov.ctx, cov.inp, cov.n
-    lay = inp.layout
-    for lo, hi in bins:
-        if hi > inp.lmax_signal:
-            raise ValueError(f"bin [{lo}, {hi}] extends beyond lmax_signal={inp.lmax_signal}")
-    w = parameter_weights(inp, bins, convention)
-    dev_w = {k: ctx.upload(v) for k, v in w.items()}
+def _tri_sum(rows: np.ndarray, c0: int, kmax: int) -> float:
+    """sum over rows i and columns c in [c0, i] of min(c - c0 + 1, kmax)."""
+    m = np.maximum(rows - c0 + 1, 0)            # columns per row
+    # sum_{t=1..m} min(t, kmax)
+    full = np.minimum(m, kmax)
+    return float((full * (full + 1) / 2 + np.maximum(m - kmax, 0) * kmax).sum())
+
+
+def plan_bands(n: int, npar: int, esize: int, budget: int, align: int = 128) -> list:
+    """Row bands [r0, r1) of the lower triangle such that one band's packed
+    whitened derivatives for all npar parameters fit in ``budget`` bytes."""
+    if npar * esize * band_size(0, n) <= budget:
+        return [(0, n)]
+    bands = []
+    r0 = 0
+    cap = budget // (npar * esize)
+    while r0 < n:
+        lo, hi = 1, n - r0
+        while lo < hi:                                   # largest r with band_size(r0, r0 + r) <= cap
+            mid = (lo + hi + 1) // 2
+            if band_size(r0, r0 + mid) <= cap:
+                lo = mid
+            else:
+                hi = mid - 1
+        r = lo
+        if r0 + r < n and r > align:
+            r = (r // align) * align
+        if band_size(r0, r0 + r) > cap:
+            raise MemoryError(f"one row of {n} packed whitened derivatives for {npar} parameters exceeds "
+                              f"the {budget / 2**30:.1f} GiB budget")
+        bands.append((r0, r0 + r))
+        r0 += r
+    return bands
+
+
+def band_size(r0: int, r1: int) -> int:
+    return (r1 * (r1 + 1) - r0 * (r0 + 1)) // 2
+
+
+PRECISIONS = ("fp64", "mixed")
+
+
+def _dtype(precision: str):
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}; got {precision!r}")
+    return torch.float64 if precision == "fp64" else torch.float32
+
+
+def whitener(cov: Covariance, precision: str) -> torch.Tensor:
+    """W in the arithmetic type of ``precision`` (FP32: W rounded once)."""
+    W = cov.factor.W
+    if precision == "fp64":
+        return W
+    if getattr(cov, "_w32", None) is None:
+        n = cov.n
+        w32 = torch.empty((n, n), dtype=torch.float32, device=cov.ctx.device)
+        cov.ctx.check(cov.ctx.lib.cfmi_convert(cov.ctx.h, 0, n, n, P(W), n, P(w32), n), "convert")
+        cov._w32 = w32
+    return cov._w32
+
+
+class DerivativeSource:
+    """The binned derivative stacks of one (Covariance, bins) pair: kept on
+    the device when they fit, regenerated spectrum by spectrum (and in bin
+    chunks) otherwise."""
+
+    def __init__(self, cov: Covariance, bins, convention: str, dtype, budget: int | None = None):
+        self.cov, self.bins, self.dtype, self.convention = cov, bins, dtype, convention
+        inp = cov.inp
+        for lo, hi in bins:
+            if hi > inp.lmax_signal:
+                raise ValueError(f"bin [{lo}, {hi}] extends beyond lmax_signal={inp.lmax_signal}")
+        w = parameter_weights(inp, bins, convention)
+        self.weights = {k: cov.ctx.upload(v) for k, v in w.items()}
+        self.geometry = engine.block_geometry(inp.layout)
+        es = 4 if dtype == torch.float32 else 8
+        self.bytes_per_bin = [g.ma * g.mb * es for g in self.geometry]
+        self.total_bytes = sum(self.bytes_per_bin) * len(bins)
+        self.resident = None
+        self.budget = budget
+
+    def fits(self, budget: int) -> bool:
+        return self.total_bytes <= budget
+
+    def make_resident(self):
+        if self.resident is None:
+            self.resident = self._make(None, None)
+        return self.resident
+
+    def _make(self, only, bin_range, state=None, lresume=0):
+        inp = self.cov.inp
+        return engine.deriv_stacks(self.cov.ctx, inp.layout, self.cov.vecs, self.weights, inp.lmax_signal,
+                                   self.bins, dtype=self.dtype, only=only, bin_range=bin_range, state=state,
+                                   lresume=lresume)
+
+    def chunks(self, chunk_budget: int):
+        """Yield (spectrum index, first bin, Block with stack) over all
+        spectra and bins; sibling spectra sharing a recurrence pass are
+        produced together."""
+        if self.resident is not None:
+            for s, blk in enumerate(self.resident):
+                yield s, 0, blk
+            return
+        specs = self.cov.inp.layout.spectra_list()
+        nb = len(self.bins)
+        groups = []
+        seen = set()
+        for s, sp in enumerate(specs):
+            if s in seen:
+                continue
+            g = [m for m, s2 in enumerate(specs) if (s2.fi, s2.fj) == (sp.fi, sp.fj)]
+            seen.update(g)
+            groups.append(g)
+        for g in groups:
+            per_bin = sum(self.bytes_per_bin[m] for m in g)
+            cb = max(1, min(nb, chunk_budget // max(per_bin, 1)))
+            # Consecutive bin chunks continue one recurrence pass.
+            state = engine.deriv_state(self.cov.ctx, self.cov.inp.layout, g[0]) if cb < nb else None
+            for b0 in range(0, nb, cb):
+                lres = self.bins[b0 - 1][1] + 1 if b0 > 0 else 0
+                blocks = self._make(g, (b0, min(nb, b0 + cb)), state, lres)
+                for m in g:
+                    yield m, b0, blocks[m]
+                del blocks
+            del state
+
+
+def _free_bytes(ctx) -> int:
+    """Device memory available to new buffers: free memory plus what torch's
+    caching allocator holds but does not use."""
+    free, _ = torch.cuda.mem_get_info(ctx.device)
+    cached = torch.cuda.memory_reserved(ctx.device) - torch.cuda.memory_allocated(ctx.device)
+    return int(free + max(cached, 0))
+
+
+class LaunchTimer:
+    """HIP events around individual kernel launches on the context's stream
+    (the stream the library enqueues on)."""
+
+    def __init__(self):
+        self.pairs = []
+        self._open = None
+        self.launches = []
+
+    def start(self):
+        self._open = torch.cuda.Event(enable_timing=True)
+        self._open.record()
+
+    def stop(self):
+        b = torch.cuda.Event(enable_timing=True)
+        b.record()
+        self.pairs.append((self._open, b))
+        self._open = None
+
+    def total_ms(self) -> float:
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in self.pairs)
+
+
+def plan_memory(n: int, npar: int, es: int, geometry, nbins: int, stack_bytes: int, free: int):
+    """(bands, stacks_resident, chunk_budget, z_budget) for the banded Fisher.
+
+    One band holds npar packed row bands of es-byte entries; the derivative
+    stacks stay resident when they fit beside it, otherwise they are
+    regenerated per band in chunks of at most ``chunk_budget`` bytes."""
+    maxw = max(max(g.ma, g.mb) for g in geometry)
+    zfull = max((n - min(g.ra, g.rb)) * maxw for g in geometry) * nbins * es
+    if stack_bytes + npar * es * band_size(0, n) + zfull <= free:
+        return [(0, n)], True, stack_bytes, zfull
+    resident = stack_bytes <= free // 4
+    chunk = stack_bytes if resident else free // 4
+    per_bin = max(g.ma * g.mb for g in geometry) * es * 3
+    cbins = nbins if resident else max(1, min(nbins, chunk // per_bin))
+    zres = free // 16
+    bands = plan_bands(n, npar, es, free - chunk - zres)
+    h = max(r1 - r0 for r0, r1 in bands)
+    if h * maxw * es > zres:          # one bin of the tallest band must fit
+        zres = h * maxw * es
+        bands = plan_bands(n, npar, es, free - chunk - zres)
+    return bands, resident, chunk, zres
+
+
+def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None,
+           precision: str = "fp64", keep_stacks: bool = True, mem_fraction: float = 0.8,
+           launch_timer: LaunchTimer | None = None, mem_budget: int | None = None, progress=None):
+    """(F, derivative source): F is (n_p, n_p) on the host.
+
+    F_ij = 1/2 <W dC_i W^T, W dC_j W^T>, accumulated over row bands of the
+    packed lower triangles (one band when everything fits on the device).
+    ``precision="mixed"``: derivative stacks, W and the whitened derivatives
+    in FP32, FP32 MFMA products accumulated in FP64 across 256-term chunks,
+    the Gram matrix in FP64; recurrences, factorisation and F^-1 stay FP64."""
+    ctx, n = cov.ctx, cov.n
+    T = _dtype(precision)
+    es = 4 if T == torch.float32 else 8
+    W = whitener(cov, precision)
+    src = DerivativeSource(cov, bins, convention, T)
+    nb = len(bins)
+    npar = len(src.geometry) * nb
+    # Whitening products (K <= a few 10^4, terms of both signs) accumulate in
+    # FP32 inside the MFMA (FP64 chunk accumulation there was measured to
+    # change nothing: the FP32 error of F comes from rounding W and D); the
+    # Gram matrix (K ~ n^2 / 2, positive diagonal terms) accumulates in FP64
+    # across 256-term chunks.
+    flags = 0
+    free = int(mem_budget) if mem_budget is not None else int(_free_bytes(ctx) * mem_fraction)
+    bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes, free)
     if timer:
         timer.mark("deriv0")
-    blocks = engine.deriv_stacks(ctx, lay, cov.vecs, dev_w, inp.lmax_signal, bins)
-    nb = len(bins)
-    npar = len(blocks) * nb
-    K = engine.packed_size(n)
-    V = torch.zeros((npar, K), dtype=torch.float64, device=ctx.device)
+    if resident:
+        src.make_resident()
     if timer:
-        timer.mark("white0")
-        timer.flops = whitening_flops(n, blocks, nb)
-    engine.whitened_derivatives(ctx, cov.factor.W, n, blocks, nb, V)
-    if timer:
-        timer.mark("white1")
-    del blocks
-    G = engine.gram(ctx, npar, npar, K, V, V)
-    if timer:
-        timer.mark("gram1")
-    F = 0.5 * (G + G.T)
-    return F, V
+        timer.flops = whitening_flops(n, src.geometry, nb, bands)
+        timer.bands = bands
+    G = torch.zeros((npar, npar), dtype=torch.float64, device=ctx.device)
+    zbuf = None
+    for bi, (r0, r1) in enumerate(bands):
+        X = torch.zeros((npar, band_size(r0, r1)), dtype=T, device=ctx.device)
+        if timer:
+            timer.mark("white0")
+        for s, b0, blk in src.chunks(chunk_budget):
+            k = blk.stack.shape[0]
+            Xs = X[s * nb + b0: s * nb + b0 + k]
+            if launch_timer is not None:
+                launch_timer.start()
+            zbuf = engine.whiten_band(ctx, W, n, blk, Xs, r0, r1, zbuf, flags, zbytes // es,
+                                      counter=launch_timer.launches if launch_timer is not None else None)
+            if launch_timer is not None:
+                launch_timer.stop()
+            del blk, Xs                  # the next chunk is allocated before the loop variable is rebound
+        if timer:
+            timer.mark("white1")
+        engine.gram(ctx, npar, npar, X.shape[1], X, X, out=G, beta=0.0 if bi == 0 else 1.0)
+        if timer:
+            timer.mark("gram1")
+        del X
+        if progress is not None:
+            progress(bi + 1, len(bands))
+    del zbuf
+    if not keep_stacks:
+        src.resident = None
+    Gh = engine.to_numpy_cm(G)
+    F = 0.5 * (Gh + Gh.T)
+    return F, src
 
 
 def whitened_maps(cov: Covariance, maps: np.ndarray) -> torch.Tensor:
@@ -147,25 +371,116 @@ def whitened_maps(cov: Covariance, maps: np.ndarray) -> torch.Tensor:
     return Z
 
 
-def qml(cov: Covariance, V: torch.Tensor, maps: np.ndarray, remove_nb: bool = True):
-    """(q, nb): raw QML estimates (nsims, n_p) and noise bias (n_p,)."""
+def solve(cov: Covariance, maps: np.ndarray, precision: str = "fp64", iters: int = 3) -> torch.Tensor:
+    """y = C^-1 x (n x nsims, FP64, column-major).
+
+    fp64: y = W^T (W x).  mixed: FP32 solves through W (rounded to FP32)
+    refined with FP64 residuals r = x - (S + N) y."""
     ctx, n = cov.ctx, cov.n
-    W = cov.factor.W
     ns = maps.shape[1]
-    npar = V.shape[0]
-    Z = whitened_maps(cov, maps)
-    K = engine.packed_size(n)
-    # Packed like V (diagonal / sqrt 2), so <pack(A), V_b> = 1/2 <A, W dC_b W^T>.
-    Q = torch.empty((ns + 1, K), dtype=torch.float64, device=ctx.device)
-    for s in range(ns):
-        engine.gemm_packed(ctx, False, True, n, n, 1, 1.0, Z, s * n, n, Z, s * n, n, 0.0, Q[s], n, 0, 0)
-    T = ctx.empty(n, n)
-    engine.gemm(ctx, False, False, n, n, n, 1.0, W, 0, n, cov.N, 0, n, 0.0, T, 0, n, 1)
-    engine.gemm_packed(ctx, False, True, n, n, n, 1.0, T, 0, n, W, 0, n, 0.0, Q[ns], n, 0, 0, flags=4)
-    del T
-    G = engine.gram(ctx, ns + 1, npar, K, Q, V)
-    nb = G[ns]
-    q = G[:ns]
+    X = ctx.upload_cm(maps)
+    if precision == "fp64":
+        W = cov.factor.W
+        Z = ctx.empty(n, ns)
+        engine.gemm(ctx, False, False, n, ns, n, 1.0, W, 0, n, X, 0, n, 0.0, Z, 0, n, 1)
+        Y = ctx.empty(n, ns)
+        engine.gemm(ctx, True, False, n, ns, n, 1.0, W, 0, n, Z, 0, n, 0.0, Y, 0, n, 2)
+        return Y
+    W32 = whitener(cov, precision)
+
+    def apply_inv32(R64):
+        r32 = torch.empty((ns, n), dtype=torch.float32, device=ctx.device)
+        ctx.check(ctx.lib.cfmi_convert(ctx.h, 0, n, ns, P(R64), n, P(r32), n), "convert")
+        z32 = torch.empty_like(r32)
+        engine.gemm(ctx, False, False, n, ns, n, 1.0, W32, 0, n, r32, 0, n, 0.0, z32, 0, n, 1)
+        y32 = torch.empty_like(r32)
+        engine.gemm(ctx, True, False, n, ns, n, 1.0, W32, 0, n, z32, 0, n, 0.0, y32, 0, n, 2)
+        y64 = ctx.empty(n, ns)
+        ctx.check(ctx.lib.cfmi_convert(ctx.h, 1, n, ns, P(y32), n, P(y64), n), "convert")
+        return y64
+
+    Y = apply_inv32(X)
+    R = ctx.empty(n, ns)
+    for _ in range(iters):
+        # R = X - S Y - N Y   (FP64)
+        ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, ns, 1.0, P(X), n, 0.0, P(R), n), "axpby")
+        engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.S, 0, n, Y, 0, n, 1.0, R, 0, n)
+        engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.N, 0, n, Y, 0, n, 1.0, R, 0, n)
+        dY = apply_inv32(R)
+        ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, ns, 1.0, P(dY), n, 1.0, P(Y), n), "axpby")
+    return Y
+
+
+def noise_weighted_inverse(cov: Covariance, precision: str = "fp64") -> torch.Tensor:
+    """G = C^-1 N C^-1 (n x n) in the arithmetic type of ``precision``."""
+    ctx, n = cov.ctx, cov.n
+    T = _dtype(precision)
+    W = whitener(cov, precision)
+    Ci = torch.empty((n, n), dtype=T, device=ctx.device)
+    engine.gemm(ctx, True, False, n, n, n, 1.0, W, 0, n, W, 0, n, 0.0, Ci, 0, n, 2)
+    if T == torch.float64:
+        Nt = cov.N
+    else:
+        Nt = torch.empty((n, n), dtype=T, device=ctx.device)
+        ctx.check(ctx.lib.cfmi_convert(ctx.h, 0, n, n, P(cov.N), n, P(Nt), n), "convert")
+    T1 = torch.empty_like(Ci)
+    engine.gemm(ctx, False, False, n, n, n, 1.0, Ci, 0, n, Nt, 0, n, 0.0, T1, 0, n)
+    del Nt
+    G = torch.empty_like(Ci)
+    engine.gemm(ctx, False, False, n, n, n, 1.0, T1, 0, n, Ci, 0, n, 0.0, G, 0, n)
+    return G
+
+
+def qml(cov: Covariance, src: DerivativeSource, maps: np.ndarray, remove_nb: bool = True,
+        precision: str = "fp64", chunk_budget: int | None = None):
+    """(q, nb): raw QML estimates (nsims, n_p) and noise bias (n_p,).
+
+    q_b + nb_b = 1/2 y^T dC_b y with y = C^-1 x, and nb_b = 1/2 Tr[G dC_b]
+    with G = C^-1 N C^-1 (``spectra.py:1124-1186``).  With dC_b = e_a D_b e_b^T
+    (+ transpose) both are <M_ab, D_b> (x2 when a != b) for M = y y^T and G:
+    one FP64-accumulated GEMM per spectrum over the flattened D_b.
+
+    The contractions sum n^2 terms into results about n times smaller than
+    the terms, so FP32 rounding of G or D would leave ~n * 6e-8 relative
+    error (3e-5 measured at n = 768): they run in FP64 in both precisions.
+    ``precision`` selects how y is solved (``solve``)."""
+    ctx, n = cov.ctx, cov.n
+    ns = maps.shape[1]
+    Y = solve(cov, maps, precision)
+    if src.dtype != torch.float64:
+        src = DerivativeSource(cov, src.bins, src.convention, torch.float64)
+    G = noise_weighted_inverse(cov, "fp64")
+    nb_bins = len(src.bins)
+    npar = len(src.geometry) * nb_bins
+    out = np.zeros((ns + 1, npar))
+    if chunk_budget is None:
+        chunk_budget = int(_free_bytes(ctx) * 0.25)
+    cache = {}
+    for s, b0, blk in src.chunks(chunk_budget):
+        D = blk.stack
+        k = D.shape[0]
+        key = (blk.ra, blk.ma, blk.rb, blk.mb)
+        if key not in cache:
+            cache.clear()
+            ma, mb = blk.ma, blk.mb
+            M = torch.empty((ns + 1, mb, ma), dtype=torch.float64, device=ctx.device)
+            # G[a, b] block, column-major ma x mb
+            ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, ma, mb, P(G, blk.ra + blk.rb * n), n, P(M), ma), "copy")
+            for q in range(ns):
+                # y_a y_b^T (K = 1 product)
+                engine.gemm_ex(ctx, False, True, ma, mb, 1, 1.0, Y, blk.ra + q * n, ma, Y, blk.rb + q * n, mb,
+                               0.0, M, (q + 1) * ma * mb, ma)
+            cache[key] = M
+        M = cache[key]
+        R = torch.empty((ns + 1, k), dtype=torch.float64, device=ctx.device)
+        K = blk.ma * blk.mb
+        engine.gemm_ex(ctx, True, False, k, ns + 1, K, 1.0, D, 0, K, M, 0, K, 0.0, R, 0, k)
+        mult = 0.5 * (1.0 if blk.sym else 2.0)
+        out[:, s * nb_bins + b0: s * nb_bins + b0 + k] = mult * R.cpu().numpy()
+        del blk, D
+    del cache, G
+    nb = out[0]
+    q = out[1:]
     if remove_nb:
         q = q - nb[None, :]
     return q, nb

@@ -6,7 +6,9 @@ Follows ``src/cosmoforge.qube/qube/spectra.py``: constructor :205-334
 refused together with ``fisher=``), ``run`` :1220-1290, the traditional
 estimator :1124-1186 (q_b = 1/2 y^T dC_b y - nb_b, nb_b = 1/2 Tr[C^-1 N C^-1
 dC_b], y = C^-1 x), the F^-1 normalisation :607-661,1340-1356 and the
-getters :1358-1733.
+getters :1358-1733.  The Fisher instance's ``precision`` applies: in
+"mixed", y comes from FP32 solves refined against the FP64 C, and
+C^-1 N C^-1 and the contractions run on FP32 inputs with FP64 accumulation.
 """
 
 from __future__ import annotations
@@ -71,11 +73,14 @@ class Spectra:
         f = self.fisher_instance
         if f.fisher is None:
             f.run()
-        if f.V is None:
-            raise RuntimeError("the Fisher instance released its derivative stack; run it again")
+        src = f.derivs
+        if src is None:
+            src = pipeline.DerivativeSource(f.cov, f.bins, self.params.output_convention,
+                                            pipeline._dtype(f.precision))
         self.setup_maps()
         self.setup_fisher_inversion()
-        q, nb = pipeline.qml(f.cov, f.V, self.maps1, remove_nb=bool(self.params.remove_nb))
+        q, nb = pipeline.qml(f.cov, src, self.maps1, remove_nb=bool(self.params.remove_nb),
+                             precision=f.precision)
         self.qml_results = q
         self.qml_noise_bias = nb
 

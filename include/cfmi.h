@@ -100,25 +100,35 @@ int cfmi_signal_02(cfmi_ctx *ctx, const double *cl_te, const double *cl_tb, int 
                    const double *vec_t, int64_t n_t, const double *vec_p, int64_t n_p,
                    double *out, int64_t ld, double *mirror, int64_t ld_mirror);
 
-/* ---- A11: binned derivative stacks (FP64) ------------------------------ */
+/* ---- A11: binned derivative stacks (FP64 recurrences) ----------------- */
 /* Bin b covers l in [bin_lo[b], bin_hi[b]] (device int32 arrays, bins
  * ascending and disjoint); w is the l-indexed weight w_l * B1(l) B2(l).
- * Output b of a stack starts at out + b * stride.  Geometry and mirror
+ * Output b of a stack starts at out + b * stride, stored as dtype
+ * (CFMI_F64, or CFMI_F32: the FP64 sums rounded once).  Geometry and mirror
  * conventions are those of the matching cfmi_signal_* call.  In the 22 and
- * 02 calls any output pointer may be NULL (not produced).                  */
-int cfmi_deriv_stack_00(cfmi_ctx *ctx, const double *w, int lmax, const int *bin_lo,
+ * 02 calls any output pointer may be NULL (not produced).
+ * state (device, may be NULL) holds cfmi_deriv_state_doubles(n_r, n_c,
+ * symmetric) doubles (22: n_r = n_c = n, symmetric; 02: n_r = n_p,
+ * n_c = n_t, not symmetric): the per-pair recurrence state after the last
+ * bin is saved there, and with lresume > 0 the recurrence starts at
+ * l = lresume from the saved state (of a call that ended at lresume - 1
+ * with the same geometry), so consecutive bin ranges cost one pass.       */
+int64_t cfmi_deriv_state_doubles(int64_t n_r, int64_t n_c, int symmetric);
+int cfmi_deriv_stack_00(cfmi_ctx *ctx, int dtype, const double *w, int lmax, const int *bin_lo,
                         const int *bin_hi, int nbins, const double *vec_r, int64_t n_r,
-                        const double *vec_c, int64_t n_c, int symmetric, double *out, int64_t ld,
-                        int64_t stride, double *mirror, int64_t ld_mirror, int64_t stride_mirror);
-int cfmi_deriv_stack_22(cfmi_ctx *ctx, const double *w_ee, const double *w_bb, const double *w_eb,
-                        int lmax, const int *bin_lo, const int *bin_hi, int nbins,
-                        const double *vec, int64_t n, double *out_ee, double *out_bb,
-                        double *out_eb, int64_t ld, int64_t stride);
-int cfmi_deriv_stack_02(cfmi_ctx *ctx, const double *w_te, const double *w_tb, int lmax,
+                        const double *vec_c, int64_t n_c, int symmetric, void *out, int64_t ld,
+                        int64_t stride, void *mirror, int64_t ld_mirror, int64_t stride_mirror,
+                        double *state, int lresume);
+int cfmi_deriv_stack_22(cfmi_ctx *ctx, int dtype, const double *w_ee, const double *w_bb,
+                        const double *w_eb, int lmax, const int *bin_lo, const int *bin_hi,
+                        int nbins, const double *vec, int64_t n, void *out_ee, void *out_bb,
+                        void *out_eb, int64_t ld, int64_t stride, double *state, int lresume);
+int cfmi_deriv_stack_02(cfmi_ctx *ctx, int dtype, const double *w_te, const double *w_tb, int lmax,
                         const int *bin_lo, const int *bin_hi, int nbins, const double *vec_t,
-                        int64_t n_t, const double *vec_p, int64_t n_p, double *out_te,
-                        double *out_tb, int64_t ld, int64_t stride, double *mirror_te,
-                        double *mirror_tb, int64_t ld_mirror, int64_t stride_mirror);
+                        int64_t n_t, const double *vec_p, int64_t n_p, void *out_te,
+                        void *out_tb, int64_t ld, int64_t stride, void *mirror_te,
+                        void *mirror_tb, int64_t ld_mirror, int64_t stride_mirror, double *state,
+                        int lresume);
 
 /* ---- A12-A15: dense linear algebra ------------------------------------- */
 /* C = alpha op(A) op(B) + beta C; dtype CFMI_F64 or CFMI_F32; trans* != 0
@@ -151,6 +161,28 @@ int cfmi_gemm_packed_batched(cfmi_ctx *ctx, int dtype, int transa, int transb, i
                              int64_t sP, int64_t pn, int64_t row0, int64_t col0, double pdiag,
                              double poff, int64_t batch, int flags);
 
+/* General form of the products above.
+ *   dtype / out_dtype: (F64, F64), (F32, F32) or (F32, F64).
+ *   flags: bits 0-3 as cfmi_gemm, with the triangular trims shifted by a
+ *     diagonal offset: bit 0 skips k > m + off_a, bit 1 k < m + off_a,
+ *     bit 2 k > n + off_b, bit 3 k < n + off_b;
+ *     CFMI_GEMM_ACC64: FP32 products accumulated by the MFMA in FP32 over
+ *     chunks of 256 terms and in FP64 across chunks (always on for
+ *     out_dtype F64 with FP32 inputs).
+ *   out_mode 0: dense C (ldc); 1: packed lower triangle of an ldc x ldc
+ *     matrix as cfmi_gemm_packed; 2: packed lower rows, row-major: element
+ *     (i, j <= i) at C + i (i + 1) / 2 + j - pbase.  In modes 1 and 2,
+ *     (r, c) of the product is element (row0 + r, col0 + c), only its
+ *     lower-triangle entries are produced, scaled by pdiag / poff.
+ *   Batches as cfmi_gemm_batched (sC strides the output).                 */
+#define CFMI_GEMM_ACC64 128
+int cfmi_gemm_ex(cfmi_ctx *ctx, int dtype, int out_dtype, int transa, int transb, int64_t m,
+                 int64_t n, int64_t k, double alpha, const void *A, int64_t lda, int64_t sA,
+                 int64_t off_a, const void *B, int64_t ldb, int64_t sB, int64_t off_b,
+                 double beta, void *C, int64_t ldc, int64_t sC, int64_t batch, int flags,
+                 int out_mode, int64_t row0, int64_t col0, int64_t pbase, double pdiag,
+                 double poff);
+
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
 
@@ -181,12 +213,17 @@ int cfmi_axpby(cfmi_ctx *ctx, int64_t m, int64_t n, double alpha, const double *
 int cfmi_convert(cfmi_ctx *ctx, int dir, int64_t m, int64_t n, const void *src, int64_t lds,
                  void *dst, int64_t ldd);
 
+/* Copy an m x n block (FP64 or FP32 per dtype) between leading dimensions. */
+int cfmi_copy(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, const void *src, int64_t lds, void *dst,
+              int64_t ldd);
+
 /* Fill an m x n block with value v (FP64 or FP32 per dtype). */
 int cfmi_fill(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, void *A, int64_t lda, double v);
 
 /* G[i, j] = sum_k X[k, i] Y[k, j] for i < m, j < n over k < K (FP64 or
- * FP32 inputs, FP64 output), with fixed-order split-K partial sums.  X and
- * Y are K x m and K x n column-major (ldx, ldy >= K).                     */
+ * FP32 inputs, FP64 output), with fixed-order split-K partial sums; FP32
+ * inputs accumulate as in CFMI_GEMM_ACC64.  X and Y are K x m and K x n
+ * column-major (ldx, ldy >= K).                                           */
 int cfmi_gram(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t K, const void *X,
               int64_t ldx, const void *Y, int64_t ldy, double *G, int64_t ldg);
 

@@ -224,3 +224,87 @@ def test_gemm_batched_and_packed(ctx):
     order = np.lexsort((i, j))  # column-major packing
     for b in range(2):
         assert np.max(np.abs(Pk[b].cpu().numpy() - ref[order])) < 1e-11
+
+
+def test_gemm_ex_band_packing_and_offsets(ctx):
+    """cfmi_gemm_ex: row-major packed lower band output and shifted
+    triangular trims, FP64 and FP32, batched."""
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(21)
+    n = 300
+    W = np.tril(rng.standard_normal((n, n)))
+    r0, r1, ra = 96, 233, 40
+    m, K, N = r1 - r0, 150, 90
+    for dt, tol in [(torch.float64, 1e-12), (torch.float32, 3e-5)]:
+        Wd = ctx.upload_cm(W, dtype=dt)
+        Bm = rng.standard_normal((2, K, N))
+        Bd = torch.tensor(np.ascontiguousarray(np.transpose(Bm, (0, 2, 1))), dtype=dt, device=ctx.device)
+        # Z_b = W[r0:r1, ra:ra+K] B_b with A-lower trim offset r0 - ra
+        Z = torch.zeros((2, N, m), dtype=dt, device=ctx.device)
+        engine.gemm_ex(ctx, False, False, m, N, K, 1.0, Wd, r0 + ra * n, n, Bd, 0, K, 0.0, Z, 0, m, sB=K * N,
+                       sC=m * N, batch=2, flags=1, tri_a=r0 - ra)
+        for b in range(2):
+            want = W[r0:r1, ra:ra + K] @ Bm[b]
+            got = Z[b].cpu().numpy().T
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
+        # band output of Z_b W[0:r1, 0:N]^T (B-upper trim) into rows r0..r1
+        pb = r0 * (r0 + 1) // 2
+        X = torch.zeros((2, engine.band_size(r0, r1)), dtype=dt, device=ctx.device)
+        engine.gemm_ex(ctx, False, True, m, r1, N, 1.0, Z, 0, m, Wd, 0, n, 0.0, X, 0, sA=m * N,
+                       sC=X.shape[1], batch=2, flags=4, out_mode=2, row0=r0, col0=0, pbase=pb, pdiag=0.5)
+        for b in range(2):
+            P = Z[b].cpu().double().numpy().T @ W[:r1, :N].T
+            want = np.concatenate([P[i - r0, :i + 1] * np.r_[np.ones(i), 0.5] for i in range(r0, r1)])
+            got = X[b].cpu().double().numpy()
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
+
+
+def test_gram_fp32_accumulates_in_fp64(ctx):
+    """FP32 inputs over a long K: FP32 products, FP64 accumulation across
+    chunks, so the error stays at FP32 rounding of the inputs, not of a
+    10^7-term FP32 sum."""
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(3)
+    K, m = 4_000_000, 5
+    X = rng.random((m, K)).astype(np.float32)          # positive terms: worst case for a plain FP32 sum
+    Xd = torch.tensor(X, device=ctx.device)
+    G = engine.gram(ctx, m, m, K, Xd, Xd)
+    want = X.astype(np.float64) @ X.astype(np.float64).T
+    assert np.max(np.abs(G - want) / want) < 1e-6
+    Xd64 = Xd.double()
+    G64 = engine.gram(ctx, m, m, K, Xd64, Xd64)
+    assert np.max(np.abs(G64 - want) / want) < 1e-12
+
+
+def test_deriv_stacks_chunked_resume_and_fp32(ctx):
+    """Bin chunks that resume the recurrence from its saved state give the
+    same stacks as one pass (bit for bit); FP32 stacks are the FP64 sums
+    rounded once."""
+    import torch
+    from cosmoforge_amd import engine
+    from tests import cases
+
+    case = cases.case_fisher_nside4("TEB")
+    from cosmoforge_amd.core import Inputs, bins_from_params
+    from cosmoforge_amd import pipeline
+
+    p = dict(nside=case.nside, spins=case.spins, labels=case.labels, lmin=case.lmin, lmax=case.lmax,
+             lmax_signal=case.lmax_signal, ordering="NESTED" if case.nest else "RING")
+    inp = Inputs(p, case.mask, case.noise, case.cls, case.beam)
+    cov = pipeline.Covariance(ctx, inp)
+    bins = bins_from_params(inp.params)
+    full = pipeline.DerivativeSource(cov, bins, "Cl", torch.float64)
+    ref = full.make_resident()
+    got = {}
+    for s, b0, blk in full.__class__(cov, bins, "Cl", torch.float64).chunks(1):   # one bin per chunk
+        got[(s, b0)] = blk.stack.clone()
+    for s, blk in enumerate(ref):
+        for b in range(len(bins)):
+            assert torch.equal(got[(s, b)][0], blk.stack[b]), (s, b)
+    f32 = pipeline.DerivativeSource(cov, bins, "Cl", torch.float32).make_resident()
+    for a, b in zip(ref, f32):
+        assert torch.equal(a.stack.float(), b.stack)

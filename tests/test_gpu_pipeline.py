@@ -160,3 +160,53 @@ def test_picslike_fortran_gpu():
     assert np.max(np.abs(chi2 / res[:, 3] - 1)) < 1e-9
     assert np.max(np.abs(lnl - res[:, 2])) / np.abs(res[:, 2]).mean() < 1e-9
     assert pl.get_best_fit() == {"point": int(np.argmax(lnl))}
+
+
+def test_fisher_bands_equal_single_band():
+    """The banded Fisher (several row bands, stacks regenerated per band)
+    equals the single-band Fisher at FP64 rounding level."""
+    from cosmoforge_amd import Fisher, pipeline
+
+    case = cases.case_fisher_nside4("TQU")
+    f1 = _gpu_fisher(case)
+    F1 = f1.get_fisher_matrix()
+    f2 = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                beam=case.beam)
+    f2.prepare_covariance_matrices()
+    from cosmoforge_amd.core import bins_from_params
+
+    bins = bins_from_params(f2.params)
+    n, npar = f2.cov.n, 6 * len(bins)
+    budget = 3 * npar * 8 * (n * (n + 1) // 2) // 10   # forces bands; stacks no longer resident
+    timer = pipeline.StageTimer()
+    F2, src = pipeline.fisher(f2.cov, bins, f2.params.output_convention, timer=timer, mem_budget=budget)
+    assert len(timer.bands) > 1 and src.resident is None
+    assert _rel(F2, F1) < 1e-12
+
+
+def test_mixed_precision_fisher_qml_close_to_fp64():
+    """precision="mixed" against the FP64 path on the nside-8 Fortran case
+    and on nside-4 TQU with EB/TB.
+
+    F within 1e-6 (max-norm relative); the QML estimates and noise bias
+    within FP64 rounding (refined solves, FP64 contractions); so the error
+    of the deconvolved C_l is F's error propagated through F^-1 (printed:
+    it scales with cond(F))."""
+    from cosmoforge_amd import Fisher, Spectra
+
+    for case in [cases.case_fortran_b(), cases.case_fisher_nside4("TEB")]:
+        out = {}
+        for prec in ("fp64", "mixed"):
+            f = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                       beam=case.beam, precision=prec)
+            f.run()
+            x = case.maps if case.maps is not None else np.random.default_rng(1).standard_normal((f.cov.n, 1))
+            s = Spectra(fisher=f, maps1=x)
+            s.run()
+            out[prec] = (f.get_fisher_matrix(), s.get_power_spectra(), s.qml_noise_bias, s.qml_results)
+        (F, C, nb, q), (Fm, Cm, nbm, qm) = out["fp64"], out["mixed"]
+        errs = dict(F=_rel(Fm, F), nb=_rel(nbm, nb), q=_rel(qm, q), C=_rel(Cm, C), condF=np.linalg.cond(F))
+        print("mixed vs fp64:", errs)
+        assert errs["F"] < 1e-6 and errs["nb"] < 1e-12 and errs["q"] < 1e-9
+        if errs["condF"] < 1e8:   # the nside-4 TEB F is numerically singular (cond ~ 1e16)
+            assert errs["C"] < errs["condF"] * errs["F"]

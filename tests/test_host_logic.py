@@ -1,0 +1,170 @@
+"""Host-side logic of the banded whitening (CPU only).
+
+``engine.whiten_band`` is run with a NumPy emulator in place of
+``cfmi_gemm_ex``; the emulator asserts that every triangular K-trim the
+band code requests skips only zeros, and applies the packed row-band output
+mapping.  The result must equal the packed lower triangle of W dC W^T
+computed directly, for one band and for several."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from cosmoforge_amd import engine, pipeline
+
+
+def _view(flat, off, ld, rows, cols):
+    """Column-major rows x cols matrix at flat[off:] with leading dim ld."""
+    idx = off + np.arange(rows)[:, None] + ld * np.arange(cols)[None, :]
+    return flat[idx]
+
+
+def fake_gemm_ex(ctx, ta, tb, m, n, k, alpha, A, offa, lda, B, offb, ldb, beta, C, offc=0, ldc=1, *, sA=0, sB=0,
+                 sC=0, batch=1, flags=0, tri_a=0, tri_b=0, out_mode=0, row0=0, col0=0, pbase=0, pdiag=1.0,
+                 poff=1.0):
+    if m <= 0 or n <= 0:
+        return
+    a_flat = A.detach().reshape(-1).numpy()
+    b_flat = B.detach().reshape(-1).numpy()
+    c_flat = C.detach().reshape(-1).numpy()  # shares memory with C
+    for z in range(batch):
+        opA = _view(a_flat, offa + z * sA, lda, k, m).T if ta else _view(a_flat, offa + z * sA, lda, m, k)
+        opB = _view(b_flat, offb + z * sB, ldb, n, k).T if tb else _view(b_flat, offb + z * sB, ldb, k, n)
+        mm = np.arange(m)[:, None]
+        kk = np.arange(k)[None, :]
+        if flags & 1:
+            assert np.all(opA[kk > mm + tri_a] == 0), "A-lower trim skips nonzeros"
+        if flags & 2:
+            assert np.all(opA[kk < mm + tri_a] == 0), "A-upper trim skips nonzeros"
+        kk2 = np.arange(k)[:, None]
+        nn = np.arange(n)[None, :]
+        if flags & 4:
+            assert np.all(opB[kk2 > nn + tri_b] == 0), "B-upper trim skips nonzeros"
+        if flags & 8:
+            assert np.all(opB[kk2 < nn + tri_b] == 0), "B-lower trim skips nonzeros"
+        prod = alpha * (opA @ opB)
+        if out_mode == 0:
+            for j in range(n):
+                base = offc + z * sC + j * ldc
+                c_flat[base:base + m] = prod[:, j] + (beta * c_flat[base:base + m] if beta else 0.0)
+        else:
+            assert out_mode == 2
+            for r in range(m):
+                i = row0 + r
+                for cidx in range(n):
+                    c = col0 + cidx
+                    if c > i:
+                        continue
+                    v = prod[r, cidx] * (pdiag if i == c else poff)
+                    p = offc + z * sC + i * (i + 1) // 2 + c - pbase
+                    c_flat[p] = v + (beta * c_flat[p] if beta else 0.0)
+
+
+class FakeCtx:
+    device = torch.device("cpu")
+
+
+def _problem(rng, nf, nb):
+    n = 3 * nf
+    L = np.tril(rng.standard_normal((n, n))) + 4 * np.eye(n)
+    W = np.linalg.inv(L)                                  # lower triangular
+    W[np.triu_indices(n, 1)] = 0.0
+    blocks = []
+    for ra, ma, rb, mb in [(0, nf, 0, nf), (nf, 2 * nf, nf, 2 * nf), (nf, 2 * nf, 0, nf)]:
+        D = rng.standard_normal((nb, mb, ma))             # torch (nb, mb, ma) = col-major ma x mb
+        if ra == rb:
+            D = D + np.transpose(D, (0, 2, 1))
+        blocks.append((ra, ma, rb, mb, D))
+    return n, W, blocks
+
+
+def _direct(n, W, ra, ma, rb, mb, Db):
+    dC = np.zeros((n, n))
+    Dm = Db.T                                              # ma x mb
+    dC[ra:ra + ma, rb:rb + mb] += Dm
+    if not (ra == rb and ma == mb):
+        dC[rb:rb + mb, ra:ra + ma] += Dm.T
+    V = W @ dC @ W.T
+    out = []
+    for i in range(n):
+        for c in range(i + 1):
+            out.append(V[i, c] * (engine.SQRT_HALF if i == c else 1.0))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("bands,zcap", [([(0, 12)], None), ([(0, 5), (5, 12)], None),
+                                        ([(0, 3), (3, 4), (4, 9), (9, 12)], None), ([(0, 5), (5, 12)], 40)])
+def test_whiten_band_matches_direct(monkeypatch, bands, zcap):
+    rng = np.random.default_rng(5)
+    n, W, blocks = _problem(rng, 4, 3)
+    monkeypatch.setattr(engine, "gemm_ex", fake_gemm_ex)
+    Wt = torch.tensor(np.ascontiguousarray(W.T))          # col-major n x n
+    for ra, ma, rb, mb, D in blocks:
+        blk = engine.Block(ra, ma, rb, mb, torch.tensor(D))
+        got = []
+        for r0, r1 in bands:
+            X = torch.zeros((D.shape[0], engine.band_size(r0, r1)), dtype=torch.float64)
+            engine.whiten_band(FakeCtx(), Wt, n, blk, X, r0, r1, zcap=zcap)
+            got.append(X.numpy())
+        got = np.concatenate(got, axis=1)
+        for b in range(D.shape[0]):
+            want = _direct(n, W, ra, ma, rb, mb, D[b])
+            np.testing.assert_allclose(got[b], want, rtol=1e-12, atol=1e-12)
+
+
+def test_band_gram_is_half_trace():
+    """sum over bands of <X_i, X_j> = 1/2 Tr[V_i V_j] for symmetric V."""
+    rng = np.random.default_rng(9)
+    n = 7
+    Vs = [rng.standard_normal((n, n)) for _ in range(2)]
+    Vs = [v + v.T for v in Vs]
+    packs = [np.array([v[i, c] * (engine.SQRT_HALF if i == c else 1.0) for i in range(n) for c in range(i + 1)])
+             for v in Vs]
+    assert math.isclose(packs[0] @ packs[1], 0.5 * np.trace(Vs[0] @ Vs[1]), rel_tol=1e-13)
+
+
+def test_band_size_and_plan():
+    n = 1000
+    assert sum(engine.band_size(a, b) for a, b in [(0, 10), (10, 500), (500, 1000)]) == n * (n + 1) // 2
+    for budget in [10**6, 4 * 10**6, 10**9]:
+        bands = pipeline.plan_bands(n, 3, 8, budget, align=128)
+        assert bands[0][0] == 0 and bands[-1][1] == n
+        assert all(b0[1] == b1[0] for b0, b1 in zip(bands, bands[1:]))
+        assert all(3 * 8 * engine.band_size(a, b) <= budget for a, b in bands)
+        assert all((b - a) % 128 == 0 for a, b in bands[:-1] if b - a > 128)
+    assert pipeline.plan_bands(n, 3, 8, 10**9) == [(0, n)]
+    with pytest.raises(MemoryError):
+        pipeline.plan_bands(n, 3, 8, 3 * 8 * 500)
+
+
+def _brute_flops(n, blk, nb, bands):
+    ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
+    tot = 0
+    for r0, r1 in bands:
+        full = r0 == 0 and r1 == n
+        for i in range(max(r0, ra), r1):
+            tot += 2 * mb * min(i - ra + 1, ma)
+            for c in range(rb, i + 1):
+                tot += 2 * min(c - rb + 1, mb)
+        if blk.sym:
+            continue
+        if full:
+            for i in range(rb, n):
+                for c in range(ra, i + 1):
+                    tot += 2 * min(i - rb + 1, mb)
+        else:
+            for j in range(max(r0, rb), r1):
+                tot += 2 * ma * min(j - rb + 1, mb)
+                for c in range(ra, j + 1):
+                    tot += 2 * min(c - ra + 1, ma)
+    return tot * nb
+
+
+@pytest.mark.parametrize("bands", [None, [(0, 7), (7, 20), (20, 30)]])
+def test_whitening_flops_brute_force(bands):
+    n, nf = 30, 10
+    blocks = [engine.Block(0, nf, 0, nf), engine.Block(nf, 2 * nf, nf, 2 * nf), engine.Block(nf, 2 * nf, 0, nf)]
+    want = sum(_brute_flops(n, b, 3, bands or [(0, n)]) for b in blocks)
+    assert pipeline.whitening_flops(n, blocks, 3, bands) == pytest.approx(want, rel=1e-12)
