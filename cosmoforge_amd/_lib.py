@@ -61,6 +61,11 @@ SIGNATURES = {
     "cfmi_copy": (ci, [vp, ci, i64, i64, vp, i64, vp, i64]),
     "cfmi_fill": (ci, [vp, ci, i64, i64, vp, i64, cd]),
     "cfmi_gram": (ci, [vp, ci, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
+    "cfmi_batch_create": (ci, [vp, ci, ci, vp, ci, vp, vp, vp, vp, ci, vp, ci, ci, ci, _c.POINTER(vp)]),
+    "cfmi_batch_run": (ci, [vp, vp, vp, vp, vp]),
+    "cfmi_batch_capture": (ci, [vp, vp, vp, vp, vp]),
+    "cfmi_batch_replay": (ci, [vp]),
+    "cfmi_batch_destroy": (ci, [vp]),
 }
 
 _lib = None

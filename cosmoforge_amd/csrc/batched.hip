@@ -1,0 +1,728 @@
+// Batched exact likelihood: many covariance matrices of uneven size, each a
+// linear combination C_j = sum_k c_jk M_pk of per-patch FP64 component
+// matrices, factorised and solved in one launch per stage.
+//
+// Reference: PICSLike._compute_likelihood_point (picslike/picslike.py:568-664):
+// chi2 = d^T C^-1 d and ln|C| per parameter point, lnL = -(chi2 + ln|C|)/2,
+// with C^-1 from the Jacobi-scaled dpotrf/dpotri (basics/linalg.py:152-219)
+// and ln|C| from a second dpotrf (basics/linalg.py:430-474).
+//
+// Design (gfx950):
+//   * one persistent kernel: a grid of (CUs x k) workgroups pulls jobs from
+//     an atomic queue, largest matrices first; every workgroup leaves when
+//     the queue is exhausted;
+//   * one workgroup per matrix, right-looking blocked Cholesky in the job's
+//     workspace (FP32 in mixed mode, FP64 otherwise): the NB x NB diagonal
+//     block is factored in LDS in FP64 (its log-diagonal accumulates ln|C|
+//     in FP64) and inverted there, the panel is multiplied by that inverse
+//     on FP64 MFMA, and the trailing
+//     lower triangle is updated by 32 x 32 tiles of 16x16x4 MFMA per wave;
+//   * C is Jacobi-scaled (D C D, unit diagonal) as the reference scales it;
+//   * solves are blocked: a wave solves each diagonal block with lane
+//     shuffles while the workgroup applies the off-diagonal blocks;
+//   * refinement in the same workgroup: r = D x - (D C D) y from the FP64
+//     components, one more pair of solves, chi2 = (D x)^T y in FP64.
+#include "cfmi_internal.h"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+constexpr int BT = 256;     // threads per workgroup
+constexpr int TT = 32;      // trailing-update tile edge (2 x 2 MFMA tiles per wave)
+constexpr int VMAX = 2048;  // largest matrix (rows) a job may have (LDS vectors)
+constexpr int MAXC = 4;     // largest number of component matrices
+constexpr int RPT = VMAX / BT;
+
+struct BatchArgs {
+    int njobs;
+    int ncomp;
+    int iters;                 // refinement iterations (kernel 2)
+    const int* order;          // processing order of the jobs
+    const int* job_patch;
+    const int64_t* patch_n;
+    const int64_t* patch_comp;  // element offset of the patch's first component
+    const int64_t* patch_x;     // element offset of the patch's data vector
+    const double* comps;
+    const double* xs;
+    const double* coef;         // [njobs][ncomp]
+    void* work;                 // factors
+    const int64_t* job_work;    // element offset of the job's n x n factor
+    const int64_t* job_vec;     // element offset of the job's n-vectors
+    double* dsc;                // Jacobi scaling per job
+    double* logdet;
+    double* chi2;
+    int* info;
+    unsigned* queue;            // [0]: job counter
+};
+
+template <typename T> struct Mf;
+template <> struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ int next_job(unsigned* q, int* s_job)
+{
+    if (threadIdx.x == 0) *s_job = (int)atomicAdd(q, 1u);
+    __syncthreads();
+    int v = *s_job;
+    __syncthreads();
+    return v;
+}
+
+// Lower tiles of a t x t tile grid, enumerated row by row.
+__device__ __forceinline__ void tri_tile(int k, int& ti, int& tj)
+{
+    int r = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= k) ++r;
+    while (r * (r + 1) / 2 > k) --r;
+    ti = r;
+    tj = k - r * (r + 1) / 2;
+}
+
+// A[r0:, r0:] -= P P^T on the lower triangle, P = A[r0:, kc:kc+kb].
+// TS: storage type, TC: MFMA type.  Each wave owns 32 x 32 output tiles; all
+// NB/4 K-steps of its operands are loaded before the MFMA chain so that the
+// loads of a tile are in flight together.  The product accumulates from zero
+// and is added to the stored tile once.
+template <typename TS, typename TC, int NB>
+__device__ void trailing_update(int n, TS* A, int r0, int kc, int kb)
+{
+    typedef typename Mf<TC>::v4 v4;
+    constexpr bool kF64 = sizeof(TC) == 8;
+    constexpr int KS = 8;                       // K-steps (of 4) loaded together
+    const int m = n - r0;
+    if (m <= 0) return;
+    const int t = (m + TT - 1) / TT;
+    const int ntiles = t * (t + 1) / 2;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    for (int tile = wave; tile < ntiles; tile += BT / 64) {
+        int ti, tj;
+        tri_tile(tile, ti, tj);
+        const int i0 = r0 + ti * TT, j0 = r0 + tj * TT;
+        v4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
+        for (int k4 = 0; k4 < NB; k4 += 4 * KS) {
+            TC av[KS][2], bv[KS][2];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int kk = k4 + 4 * s + lk;
+                const bool kin = kk < kb;
+                const TS* col = A + (kc + (kin ? kk : 0)) * n;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int ri = i0 + 16 * h + lm, rj = j0 + 16 * h + lm;
+                    av[s][h] = (kin && ri < n) ? -(TC)col[ri] : TC(0);
+                    bv[s][h] = (kin && rj < n) ? (TC)col[rj] : TC(0);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) acc[a][b] = Mf<TC>::run(av[s][a], bv[s][b], acc[a][b]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int row = i0 + 16 * a + (kF64 ? lk + 4 * r : 4 * lk + r);
+                    int col = j0 + 16 * b + lm;
+                    if (row < n && col < n && row >= col) {
+                        TS* p = A + row + col * n;
+                        *p = (TS)((TS)*p + (TS)acc[a][b][r]);
+                    }
+                }
+    }
+}
+
+// Stage the kb x kb diagonal block at k0 (lower part) into LDS as FP64.
+template <typename T, int NB>
+__device__ __forceinline__ void load_diag(int n, const T* A, int k0, int kb, double (*sd)[NB + 1])
+{
+#pragma unroll
+    for (int s = 0; s < (NB * NB + BT - 1) / BT; ++s) {
+        const int idx = threadIdx.x + s * BT;
+        if (idx < NB * NB) {
+            int r = idx % NB, q = idx / NB;
+            sd[r][q] = (r < kb && q < kb && r >= q) ? (double)A[(k0 + r) + (k0 + q) * n] : 0.0;
+        }
+    }
+}
+
+// In-place Cholesky of the scaled matrix.  Returns 0 or the 1-based pivot
+// of the first non-positive diagonal; *logdet_s gets 2 sum log L_ii (FP64).
+// P <- P L_kk^-T for the panel rows below the diagonal block, FP64 on MFMA:
+// each wave owns 32-row tiles; a tile's rows are read completely before the
+// tile is overwritten.
+template <typename TS, int NB>
+__device__ void panel_solve(int n, TS* A, int k0, int kb, const double (*si)[NB + 1])
+{
+    typedef Mf<double>::v4 v4;
+    constexpr int KS = NB / 4, NT = NB / 16;
+    const int r0 = k0 + kb;
+    const int m = n - r0;
+    if (m <= 0) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int ntile = (m + TT - 1) / TT;
+    for (int tile = wave; tile < ntile; tile += BT / 64) {
+        const int i0 = r0 + tile * TT;
+        double av[KS][2];
+        double outv[NT][2][4];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int kk = 4 * s + lk;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ri = i0 + 16 * h + lm;
+                av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (k0 + kk) * n] : 0.0;
+            }
+        }
+        for (int b = 0; b < NT; ++b) {      // 16 output columns at a time
+            v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+            const int cc = 16 * b + lm;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int kk = 4 * s + lk;
+                // op(B)[k][c] = L^-T[k][c] = L^-1[c][k]
+                const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) acc[a] = Mf<double>::run(av[s][a], bv, acc[a]);
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = i0 + 16 * a + lk + 4 * r;
+                    if (row < n && cc < kb) outv[b][a][r] = acc[a][r];
+                }
+        }
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = i0 + 16 * a + lk + 4 * r;
+                    const int col = 16 * b + lm;
+                    if (row < n && col < kb) A[row + (k0 + col) * n] = (TS)outv[b][a][r];
+                }
+    }
+}
+
+template <typename TS, typename TC, int NB>
+__device__ int factor(int n, TS* A, double (*sd)[NB + 1], double (*si)[NB + 1], int* s_bad,
+                      double* logdet_s)
+{
+    const int tid = threadIdx.x;
+    double ld = 0.0;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = (int)(n - k0 < NB ? n - k0 : NB);
+        load_diag<TS, NB>(n, A, k0, kb, sd);
+        if (tid == 0) *s_bad = 0;
+        __syncthreads();
+        for (int q = 0; q < kb; ++q) {
+            if (tid == 0) {
+                double v = sd[q][q];
+                if (!(v > 0.0)) *s_bad = (int)(k0 + q + 1);
+                else sd[q][q] = sqrt(v);
+            }
+            __syncthreads();
+            if (*s_bad) break;
+            const double piv = sd[q][q];
+            for (int r = q + 1 + tid; r < kb; r += BT) sd[r][q] /= piv;
+            __syncthreads();
+            // rank-1 update of the remaining lower part (columns q+1..kb-1)
+            const int w = kb - q - 1;
+            for (int idx = tid; idx < w * w; idx += BT) {
+                int r = q + 1 + idx % w, pc = q + 1 + idx / w;
+                if (r >= pc) sd[r][pc] -= sd[r][q] * sd[pc][q];
+            }
+            __syncthreads();
+        }
+        const int bad = *s_bad;
+        if (bad) return bad;
+        if (tid == 0)
+            for (int q = 0; q < kb; ++q) ld += 2.0 * log(sd[q][q]);
+        for (int idx = tid; idx < NB * NB; idx += BT) {
+            int r = idx % NB, q = idx / NB;
+            if (r < kb && q < kb && r >= q) A[(k0 + r) + (k0 + q) * n] = (TS)sd[r][q];
+        }
+        // L_kk^-1 (FP64, lower) by columns, one thread per column.
+        if (tid < kb) {
+            const int j = tid;
+            for (int i = 0; i < NB; ++i) si[i][j] = 0.0;
+            for (int i = j; i < kb; ++i) {
+                double acc = i == j ? 1.0 : 0.0;
+                for (int k = j; k < i; ++k) acc -= sd[i][k] * si[k][j];
+                si[i][j] = acc / sd[i][i];
+            }
+        }
+        __syncthreads();
+        panel_solve<TS, NB>(n, A, k0, kb, si);
+        __syncthreads();
+        trailing_update<TS, TC, NB>(n, A, k0 + kb, k0, kb);
+        __syncthreads();
+    }
+    if (tid == 0) *logdet_s = ld;
+    __syncthreads();
+    return 0;
+}
+
+// v <- L^-1 v (forward) then L^-T v (backward); v in LDS (FP64, length n).
+// red needs (BT / 64) * NB doubles.
+template <typename T, int NB>
+__device__ void solve_inplace(int n, const T* A, double* v, double (*sd)[NB + 1], double* red)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (n + NB - 1) / NB;
+    for (int bk = 0; bk < nblk; ++bk) {
+        const int k0 = bk * NB;
+        const int kb = (int)(n - k0 < NB ? n - k0 : NB);
+        load_diag<T, NB>(n, A, k0, kb, sd);
+        __syncthreads();
+        if (wave == 0) {
+            double x = lane < kb ? v[k0 + lane] : 0.0;
+            for (int q = 0; q < kb; ++q) {
+                double xq = __shfl(x, q) / sd[q][q];
+                if (lane == q) x = xq;
+                else if (lane > q && lane < kb) x -= sd[lane][q] * xq;
+            }
+            if (lane < kb) v[k0 + lane] = x;
+        }
+        __syncthreads();
+        for (int i = k0 + kb + tid; i < n; i += BT) {
+            double s = 0.0;
+            for (int q0 = 0; q0 < NB; q0 += 16) {
+#pragma unroll
+                for (int q = q0; q < q0 + 16; ++q)
+                    if (q < kb) s += (double)A[i + (k0 + q) * n] * v[k0 + q];
+            }
+            v[i] -= s;
+        }
+        __syncthreads();
+    }
+    for (int bk = nblk - 1; bk >= 0; --bk) {
+        const int k0 = bk * NB;
+        const int kb = (int)(n - k0 < NB ? n - k0 : NB);
+        // t_c = sum_{i >= k0 + kb} L[i, k0 + c] v_i: per-thread partials over
+        // its rows (loads of a row in flight together), then a wave reduction.
+        for (int c0 = 0; c0 < NB; c0 += 16) {
+            double part[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) part[c] = 0.0;
+            for (int i = k0 + kb + tid; i < n; i += BT) {
+                const double vi = v[i];
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c0 + c < kb) part[c] += (double)A[i + (k0 + c0 + c) * n] * vi;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                double s = part[c];
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+                if (lane == 0) red[wave * NB + c0 + c] = s;
+            }
+        }
+        load_diag<T, NB>(n, A, k0, kb, sd);
+        __syncthreads();
+        if (wave == 0) {
+            double x = 0.0;
+            if (lane < kb) {
+                double t = 0.0;
+                for (int w = 0; w < BT / 64; ++w) t += red[w * NB + lane];
+                x = v[k0 + lane] - t;
+            }
+            for (int q = kb - 1; q >= 0; --q) {
+                double yq = __shfl(x, q) / sd[q][q];
+                if (lane == q) x = yq;
+                else if (lane < q) x -= sd[q][lane] * yq;
+            }
+            if (lane < kb) v[k0 + lane] = x;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ double block_sum(double s, double* red)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < BT / 64; ++w) t += red[w];
+    __syncthreads();
+    return t;  // valid on thread 0
+}
+
+// Per job: assemble D C D, factor, ln|C|, y = (D C D)^-1 D x refined with
+// FP64 residuals from the components, chi2 = (D x)^T y.
+template <typename TS, typename TC, int NB>
+__global__ __launch_bounds__(BT, 2) void batch_factor_kernel(BatchArgs a)
+{
+    __shared__ double sd[NB][NB + 1];
+    __shared__ double si[NB][NB + 1];
+    __shared__ double sv[VMAX];
+    __shared__ double sy[VMAX];
+    __shared__ double red[(BT / 64) * NB];
+    __shared__ int s_job, s_bad;
+    __shared__ double s_ld;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int q = next_job(&a.queue[0], &s_job);
+        if (q >= a.njobs) break;
+        const int job = a.order[q];
+        const int p = a.job_patch[job];
+        const int n = (int)a.patch_n[p];
+        const double* M = a.comps + a.patch_comp[p];
+        const double* x = a.xs + a.patch_x[p];
+        TS* A = (TS*)a.work + a.job_work[job];
+        double* d = a.dsc + a.job_vec[job];
+        double c[MAXC];
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) c[k] = k < a.ncomp ? a.coef[(int64_t)job * a.ncomp + k] : 0.0;
+        const int nn = n * n;
+        // Jacobi scaling from the FP64 diagonal
+        double ldd = 0.0;
+        for (int i = tid; i < n; i += BT) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < MAXC; ++k)
+                if (k < a.ncomp) v += c[k] * M[k * nn + i + i * n];
+            double di = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+            d[i] = di;
+            sv[i] = di;
+            ldd += log(di);
+        }
+        const double sum_logd = block_sum(ldd, red);
+        // lower triangle of D C D, column-major, flat over all n^2 entries
+        constexpr int U = 4;
+        for (int base = tid; base < nn; base += BT * U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + u * BT;
+                if (idx < nn) {
+                    const int j = idx / n, i = idx - j * n;
+                    if (i >= j) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < MAXC; ++k)
+                            if (k < a.ncomp) v += c[k] * M[k * nn + idx];
+                        A[idx] = (TS)(v * sv[i] * sv[j]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int bad = factor<TS, TC, NB>(n, A, sd, si, &s_bad, &s_ld);
+        if (bad) {
+            if (tid == 0) {
+                a.info[job] = bad;
+                a.logdet[job] = __builtin_nan("");
+                a.chi2[job] = __builtin_nan("");
+            }
+            __syncthreads();
+            continue;
+        }
+        // b = D x, then y = (DCD)^-1 b
+        for (int i = tid; i < n; i += BT) sv[i] = d[i] * x[i];
+        __syncthreads();
+        solve_inplace<TS, NB>(n, A, sv, sd, red);
+        // refinement: y += (DCD)^-1 (b - DCD y), the residual from the FP64 components
+        for (int it = 0; it < a.iters; ++it) {
+            for (int i = tid; i < n; i += BT) sy[i] = sv[i] * d[i];  // unscaled y
+            __syncthreads();
+            double acc[RPT];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) acc[r] = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double yj = sy[j];
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    const int i = tid + r * BT;
+                    if (i < n) {
+                        double mm = 0.0;
+#pragma unroll
+                        for (int k = 0; k < MAXC; ++k)
+                            if (k < a.ncomp) mm += c[k] * M[k * nn + i + j * n];
+                        acc[r] += mm * yj;
+                    }
+                }
+            }
+            __syncthreads();
+            // sy <- residual (scaled), sv keeps y
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = tid + r * BT;
+                if (i < n) sy[i] = d[i] * (x[i] - acc[r]);
+            }
+            __syncthreads();
+            solve_inplace<TS, NB>(n, A, sy, sd, red);
+            for (int i = tid; i < n; i += BT) sv[i] += sy[i];
+            __syncthreads();
+        }
+        double part = 0.0;
+        for (int i = tid; i < n; i += BT) part += d[i] * x[i] * sv[i];
+        const double chi = block_sum(part, red);
+        if (tid == 0) {
+            a.info[job] = 0;
+            a.logdet[job] = s_ld - 2.0 * sum_logd;
+            a.chi2[job] = chi;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void reset_queue_kernel(unsigned* q)
+{
+    if (threadIdx.x < 2) q[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+struct cfmi_batch {
+    cfmi_ctx* ctx = nullptr;
+    int mode = CFMI_BATCH_MIXED;
+    int nb = 32;
+    int wg_per_cu = 2;
+    int iters = 1;
+    int ncomp = 0;
+    int njobs = 0;
+    int ncu = 256;
+    BatchArgs args = {};
+    void* d_work = nullptr;
+    double* d_dsc = nullptr;
+    int* d_order = nullptr;
+    int* d_job_patch = nullptr;
+    int64_t* d_patch = nullptr;  // patch_n | patch_comp | patch_x
+    int64_t* d_job = nullptr;    // job_work | job_vec
+    unsigned* d_queue = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename TS, typename TC, int NB>
+void launch_factor(cfmi_batch* b, hipStream_t s)
+{
+    dim3 grid((unsigned)(b->ncu * b->wg_per_cu));
+    hipLaunchKernelGGL((batch_factor_kernel<TS, TC, NB>), grid, dim3(BT), 0, s, b->args);
+}
+
+template <typename TS, typename TC>
+void launch_nb(cfmi_batch* b, hipStream_t s)
+{
+    if (b->nb == 16) launch_factor<TS, TC, 16>(b, s);
+    else launch_factor<TS, TC, 32>(b, s);
+}
+
+int enqueue(cfmi_batch* b, hipStream_t s)
+{
+    cfmi_ctx* c = b->ctx;
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
+    CFMI_LAUNCH_CHECK(c);
+    if (b->mode == CFMI_BATCH_MIXED) launch_nb<float, float>(b, s);
+    else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
+    else launch_nb<double, double>(b, s);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+void release(cfmi_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    (void)hipFree(b->d_work);
+    (void)hipFree(b->d_dsc);
+    (void)hipFree(b->d_order);
+    (void)hipFree(b->d_job_patch);
+    (void)hipFree(b->d_patch);
+    (void)hipFree(b->d_job);
+    (void)hipFree(b->d_queue);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int ncomp, const double* comps,
+                      const int64_t* comp_off, const double* xs, const int64_t* x_off, int njobs,
+                      const int* job_patch, int nb, int wg_per_cu, int refine_iters, cfmi_batch** out)
+{
+    if (!c || !out) return CFMI_EINVAL;
+    *out = nullptr;
+    if (mode != CFMI_BATCH_MIXED && mode != CFMI_BATCH_MIXED_F64_STORAGE && mode != CFMI_BATCH_FP64)
+        return cfmi_fail(c, CFMI_EINVAL, "batch: unknown mode %d", mode);
+    if (npatch <= 0 || njobs <= 0) return cfmi_fail(c, CFMI_EINVAL, "batch: empty batch");
+    if (ncomp < 1 || ncomp > MAXC) return cfmi_fail(c, CFMI_EINVAL, "batch: 1..%d component matrices", MAXC);
+    if (nb != 16 && nb != 32) return cfmi_fail(c, CFMI_EINVAL, "batch: panel width must be 16 or 32");
+    if (wg_per_cu < 1 || wg_per_cu > 8) return cfmi_fail(c, CFMI_EINVAL, "batch: 1..8 workgroups per CU");
+    if (refine_iters < 0 || refine_iters > 8) return cfmi_fail(c, CFMI_EINVAL, "batch: 0..8 refinement iterations");
+    for (int p = 0; p < npatch; ++p)
+        if (n[p] < 1 || n[p] > VMAX)
+            return cfmi_fail(c, CFMI_EINVAL, "batch: patch %d has %lld rows (1..%d supported)", p, (long long)n[p],
+                             VMAX);
+    for (int j = 0; j < njobs; ++j)
+        if (job_patch[j] < 0 || job_patch[j] >= npatch)
+            return cfmi_fail(c, CFMI_EINVAL, "batch: job %d names patch %d of %d", j, job_patch[j], npatch);
+    cfmi_batch* b = new cfmi_batch();
+    b->ctx = c;
+    b->mode = mode;
+    b->nb = nb;
+    b->wg_per_cu = wg_per_cu;
+    b->iters = refine_iters;
+    b->ncomp = ncomp;
+    b->njobs = njobs;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+        b->ncu = ncu;
+    const size_t esz = mode == CFMI_BATCH_MIXED ? 4 : 8;
+    std::vector<int64_t> jw(njobs), jv(njobs);
+    int64_t w = 0, v = 0;
+    for (int j = 0; j < njobs; ++j) {
+        int64_t m = n[job_patch[j]];
+        jw[j] = w;
+        jv[j] = v;
+        w += m * m;
+        v += m;
+    }
+    std::vector<int> order(njobs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return n[job_patch[x]] > n[job_patch[y]]; });
+    std::vector<int64_t> pt(3 * (size_t)npatch);
+    for (int p = 0; p < npatch; ++p) {
+        pt[p] = n[p];
+        pt[npatch + p] = comp_off[p];
+        pt[2 * npatch + p] = x_off[p];
+    }
+    std::vector<int64_t> jb(2 * (size_t)njobs);
+    for (int j = 0; j < njobs; ++j) {
+        jb[j] = jw[j];
+        jb[njobs + j] = jv[j];
+    }
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    chk(hipMalloc(&b->d_work, (size_t)w * esz));
+    chk(hipMalloc((void**)&b->d_dsc, (size_t)v * sizeof(double)));
+    chk(hipMalloc((void**)&b->d_order, (size_t)njobs * sizeof(int)));
+    chk(hipMalloc((void**)&b->d_job_patch, (size_t)njobs * sizeof(int)));
+    chk(hipMalloc((void**)&b->d_patch, pt.size() * sizeof(int64_t)));
+    chk(hipMalloc((void**)&b->d_job, jb.size() * sizeof(int64_t)));
+    chk(hipMalloc((void**)&b->d_queue, 2 * sizeof(unsigned)));
+    if (e == hipSuccess) {
+        chk(hipMemcpy(b->d_order, order.data(), (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
+        chk(hipMemcpy(b->d_job_patch, job_patch, (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
+        chk(hipMemcpy(b->d_patch, pt.data(), pt.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        chk(hipMemcpy(b->d_job, jb.data(), jb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) {
+        release(b);
+        delete b;
+        return cfmi_fail(c, CFMI_ENOMEM, "batch: device allocation failed (%s)", hipGetErrorString(e));
+    }
+    BatchArgs& a = b->args;
+    a.njobs = njobs;
+    a.ncomp = ncomp;
+    a.iters = refine_iters;
+    a.order = b->d_order;
+    a.job_patch = b->d_job_patch;
+    a.patch_n = b->d_patch;
+    a.patch_comp = b->d_patch + npatch;
+    a.patch_x = b->d_patch + 2 * npatch;
+    a.comps = comps;
+    a.xs = xs;
+    a.work = b->d_work;
+    a.job_work = b->d_job;
+    a.job_vec = b->d_job + njobs;
+    a.dsc = b->d_dsc;
+    a.queue = b->d_queue;
+    *out = b;
+    return CFMI_OK;
+}
+
+int cfmi_batch_run(cfmi_batch* b, const double* coef, double* chi2, double* logdet, int* info)
+{
+    if (!b) return CFMI_EINVAL;
+    b->args.coef = coef;
+    b->args.chi2 = chi2;
+    b->args.logdet = logdet;
+    b->args.info = info;
+    return enqueue(b, b->ctx->stream);
+}
+
+int cfmi_batch_capture(cfmi_batch* b, const double* coef, double* chi2, double* logdet, int* info)
+{
+    if (!b) return CFMI_EINVAL;
+    cfmi_ctx* c = b->ctx;
+    if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    if (!b->cap_stream) CFMI_HIP(c, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    b->args.coef = coef;
+    b->args.chi2 = chi2;
+    b->args.logdet = logdet;
+    b->args.info = info;
+    CFMI_HIP(c, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return cfmi_fail(c, CFMI_EHIP, "batch: graph capture failed (%s)", hipGetErrorString(e));
+    b->graph = g;
+    CFMI_HIP(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return CFMI_OK;
+}
+
+int cfmi_batch_replay(cfmi_batch* b)
+{
+    if (!b) return CFMI_EINVAL;
+    if (!b->exec) return cfmi_fail(b->ctx, CFMI_EINVAL, "batch: no captured graph (call cfmi_batch_capture)");
+    CFMI_HIP(b->ctx, hipGraphLaunch(b->exec, b->ctx->stream));
+    return CFMI_OK;
+}
+
+int cfmi_batch_destroy(cfmi_batch* b)
+{
+    if (!b) return CFMI_OK;
+    release(b);
+    delete b;
+    return CFMI_OK;
+}
+
+}  // extern "C"

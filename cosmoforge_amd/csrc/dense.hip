@@ -38,6 +38,13 @@ template <> struct Mfma<float> {
 constexpr int BM = 128, BN = 128, BK = CFMI_GEMM_BK, NT = 256, PAD = 2;
 constexpr int BSMALL = 96;  // tile edge for outputs of at most ~300 x 300 (Gram matrices)
 
+// Grid of an elementwise grid-stride kernel over tot elements (256 threads).
+inline dim3 elem_grid(int64_t tot)
+{
+    int64_t b = (tot + 255) / 256;
+    return dim3((unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536));
+}
+
 // Where the output block sits and how it is stored.
 //   mode 0: dense column-major (ldc)
 //   mode 1: packed lower triangle, column-major, of a pn x pn matrix
@@ -333,24 +340,26 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
 template <typename T, typename TO, typename TE>
 __global__ void splitk_reduce_kernel(GemmArgs<T, TO, TE> g, int nz)
 {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t M = g.M, N = g.N;
-    if (idx >= M * N) return;
-    int64_t m = idx % M, n = idx / M;
-    if ((g.tri & 16) && g.o.col0 + n > g.o.row0 + m) return;
-    if ((g.tri & 64) && n > m) return;
-    TE s = 0;
-    for (int z = 0; z < nz; ++z) s += g.part[(int64_t)z * M * N + idx];
-    store_out(g, m, n, s);
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < M * N;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t m = idx % M, n = idx / M;
+        if ((g.tri & 16) && g.o.col0 + n > g.o.row0 + m) continue;
+        if ((g.tri & 64) && n > m) continue;
+        TE s = 0;
+        for (int z = 0; z < nz; ++z) s += g.part[(int64_t)z * M * N + idx];
+        store_out(g, m, n, s);
+    }
 }
 
 template <typename T>
 __global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc)
 {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * n) return;
-    int64_t i = idx % n, j = idx / n;
-    if (j > i) C[i + j * ldc] = C[j + i * ldc];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx % n, j = idx / n;
+        if (j > i) C[i + j * ldc] = C[j + i * ldc];
+    }
 }
 
 struct GemmShape {
@@ -420,12 +429,12 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     CFMI_LAUNCH_CHECK(c);
     if (nz > 1) {
         int64_t tot = M * N;
-        hipLaunchKernelGGL((splitk_reduce_kernel<T, TO, TE>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL((splitk_reduce_kernel<T, TO, TE>), elem_grid(tot), dim3(256), 0,
                            c->stream, g, (int)nz);
         CFMI_LAUNCH_CHECK(c);
     }
     if ((tri & 64) && M == N && sh.om.mode == 0) {
-        hipLaunchKernelGGL((mirror_lower_kernel<TO>), dim3((unsigned)((M * M + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL((mirror_lower_kernel<TO>), elem_grid(M * M), dim3(256), 0,
                            c->stream, M, (TO*)C, ldc);
         CFMI_LAUNCH_CHECK(c);
     }
@@ -538,11 +547,12 @@ template <typename T>
 __global__ void fill_kernel(int64_t m, int64_t n, T* A, int64_t lda, T v, int mode)
 {
     // mode 0: whole block; mode 1: strict upper triangle only.
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * n) return;
-    int64_t i = idx % m, j = idx / m;
-    if (mode == 1 && j <= i) return;
-    A[i + j * lda] = v;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx % m, j = idx / m;
+        if (mode == 1 && j <= i) continue;
+        A[i + j * lda] = v;
+    }
 }
 
 template <typename T>
@@ -550,7 +560,7 @@ int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, in
 {
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
-    hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((fill_kernel<T>), elem_grid(tot), dim3(256), 0, c->stream,
                        m, n, (T*)A, lda, (T)v, mode);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
@@ -618,40 +628,43 @@ int trtri_rec(cfmi_ctx* c, int64_t lo, int64_t hi, const T* L, int64_t ldl, T* U
 
 __global__ void diag_scale_kernel(int64_t n, const double* A, int64_t lda, double* d)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double a = fabs(A[i + i * lda]);
-    if (a == 0.0) a = 1.0;
-    d[i] = 1.0 / sqrt(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a = fabs(A[i + i * lda]);
+        if (a == 0.0) a = 1.0;
+        d[i] = 1.0 / sqrt(a);
+    }
 }
 
 __global__ void scale_kernel(int mode, int64_t m, int64_t n, double* A, int64_t lda, const double* d)
 {
     // mode 0: columns, 1: rows, 2: both (D A D)
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * n) return;
-    int64_t i = idx % m, j = idx / m;
-    double s = mode == 0 ? d[j] : mode == 1 ? d[i] : d[i] * d[j];
-    A[i + j * lda] *= s;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx % m, j = idx / m;
+        double s = mode == 0 ? d[j] : mode == 1 ? d[i] : d[i] * d[j];
+        A[i + j * lda] *= s;
+    }
 }
 
 __global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* A, int64_t lda, double beta,
                              double* B, int64_t ldb)
 {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * n) return;
-    int64_t i = idx % m, j = idx / m;
-    double* p = B + i + j * ldb;
-    *p = alpha * A[i + j * lda] + (beta == 0.0 ? 0.0 : beta * *p);
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx % m, j = idx / m;
+        double* p = B + i + j * ldb;
+        *p = alpha * A[i + j * lda] + (beta == 0.0 ? 0.0 : beta * *p);
+    }
 }
 
 template <typename S, typename D>
 __global__ void convert_kernel(int64_t m, int64_t n, const S* src, int64_t lds, D* dst, int64_t ldd)
 {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * n) return;
-    int64_t i = idx % m, j = idx / m;
-    dst[i + j * ldd] = (D)src[i + j * lds];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = idx % m, j = idx / m;
+        dst[i + j * ldd] = (D)src[i + j * lds];
+    }
 }
 
 // Deterministic single-workgroup reduction: 2 sum log L_ii - 2 sum log d_i.
@@ -784,11 +797,11 @@ int cfmi_jacobi_scale(cfmi_ctx* c, int64_t n, double* A, int64_t lda, double* d)
 {
     if (!c) return CFMI_EINVAL;
     if (n <= 0) return CFMI_OK;
-    hipLaunchKernelGGL(diag_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+    hipLaunchKernelGGL(diag_scale_kernel, elem_grid(n), dim3(256), 0, c->stream, n,
                        (const double*)A, lda, d);
     CFMI_LAUNCH_CHECK(c);
     int64_t tot = n * n;
-    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, 2, n, n,
+    hipLaunchKernelGGL(scale_kernel, elem_grid(tot), dim3(256), 0, c->stream, 2, n, n,
                        A, lda, (const double*)d);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
@@ -799,7 +812,7 @@ int cfmi_scale(cfmi_ctx* c, int side, int64_t m, int64_t n, double* A, int64_t l
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
-    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(scale_kernel, elem_grid(tot), dim3(256), 0, c->stream,
                        side ? 1 : 0, m, n, A, lda, d);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
@@ -811,7 +824,7 @@ int cfmi_axpby(cfmi_ctx* c, int64_t m, int64_t n, double alpha, const double* A,
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
-    hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, m, n,
+    hipLaunchKernelGGL(axpby_kernel, elem_grid(tot), dim3(256), 0, c->stream, m, n,
                        alpha, A, lda, beta, B, ldb);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
@@ -823,7 +836,7 @@ int cfmi_convert(cfmi_ctx* c, int dir, int64_t m, int64_t n, const void* src, in
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
-    dim3 g((unsigned)((tot + 255) / 256));
+    dim3 g = elem_grid(tot);
     if (dir == 0)
         hipLaunchKernelGGL((convert_kernel<double, float>), g, dim3(256), 0, c->stream, m, n,
                            (const double*)src, lds, (float*)dst, ldd);
@@ -839,7 +852,7 @@ int cfmi_copy(cfmi_ctx* c, int dtype, int64_t m, int64_t n, const void* src, int
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
-    dim3 g((unsigned)((tot + 255) / 256));
+    dim3 g = elem_grid(tot);
     if (dtype == CFMI_F64)
         hipLaunchKernelGGL((convert_kernel<double, double>), g, dim3(256), 0, c->stream, m, n, (const double*)src,
                            lds, (double*)dst, ldd);

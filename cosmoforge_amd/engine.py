@@ -229,7 +229,7 @@ def deriv_state(ctx: Context, layout: Layout, group_spectrum: int) -> torch.Tens
 
 def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: int, bins: list,
                  dtype=torch.float64, only=None, bin_range=None, state: torch.Tensor | None = None,
-                 lresume: int = 0) -> list:
+                 lresume: int = 0, pool: torch.Tensor | None = None) -> list:
     """For every spectrum of ``layout.spectra_list()``, every bin's D block.
 
     ``weights``: label -> device l-indexed w_l B1 B2 (lmax + 1).  ``only``:
@@ -239,7 +239,8 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
     spectra of a field share one recurrence pass, and TE/TB another.
     ``state`` (``deriv_state``, one recurrence group per call): the pass
     saves its final recurrence state there and, with ``lresume`` > 0,
-    resumes at l = lresume from it."""
+    resumes at l = lresume from it.  ``pool``: a flat ``dtype`` buffer the
+    stacks are carved from (instead of fresh allocations)."""
     b0, b1 = bin_range if bin_range is not None else (0, len(bins))
     sel = bins[b0:b1]
     lo = ctx.upload(np.array([b[0] for b in sel], dtype=np.int32), dtype=torch.int32)
@@ -252,13 +253,24 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
     want = set(range(len(specs)) if only is None else only)
     out = [Block(g.ra, g.ma, g.rb, g.mb) for g in geo]
     done = set()
+    used = [0]
+
+    def alloc(shape):
+        if pool is None:
+            return torch.empty(shape, dtype=dtype, device=ctx.device)
+        k = int(np.prod(shape))
+        if used[0] + k > pool.numel():
+            raise MemoryError("derivative stack chunk exceeds its pool")
+        t = pool[used[0]:used[0] + k].view(shape)
+        used[0] += k
+        return t
     for k, sp in enumerate(specs):
         if k not in want or k in done:
             continue
         fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
         blk = out[k]
         if fi.spin == 0 and fj.spin == 0:
-            st = torch.empty((nb, blk.mb, blk.ma), dtype=dtype, device=ctx.device)
+            st = alloc((nb, blk.mb, blk.ma))
             if sp.fi == sp.fj:
                 ctx.check(lib.cfmi_deriv_stack_00(h, dcode, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
                                                   P(vecs[sp.fi]), fi.npix, P(vecs[sp.fi]), fi.npix, 1, P(st),
@@ -277,7 +289,7 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
             sts = [None, None, None]
             for m in members:
                 if m in want:
-                    sts[specs[m].mode] = torch.empty((nb, fi.rows, fi.rows), dtype=dtype, device=ctx.device)
+                    sts[specs[m].mode] = alloc((nb, fi.rows, fi.rows))
             labs = [layout.spectra_map[(sp.fi, sp.fi, md)] for md in range(3)]
             ctx.check(lib.cfmi_deriv_stack_22(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]),
                                               P(weights[labs[2]]), lmax, P(lo), P(hi), nb, P(vecs[sp.fi]),
@@ -292,7 +304,7 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
             sts = [None, None]
             for m in members:
                 if m in want:
-                    sts[specs[m].mode] = torch.empty((nb, fi.rows, fj.rows), dtype=dtype, device=ctx.device)
+                    sts[specs[m].mode] = alloc((nb, fi.rows, fj.rows))
             labs = [layout.spectra_map[(sp.fi, sp.fj, md)] for md in range(2)]
             ctx.check(lib.cfmi_deriv_stack_02(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]), lmax, P(lo),
                                               P(hi), nb, P(vecs[sp.fi]), fi.npix, P(vecs[sp.fj]), fj.npix,

@@ -204,11 +204,11 @@ class DerivativeSource:
             self.resident = self._make(None, None)
         return self.resident
 
-    def _make(self, only, bin_range, state=None, lresume=0):
+    def _make(self, only, bin_range, state=None, lresume=0, pool=None):
         inp = self.cov.inp
         return engine.deriv_stacks(self.cov.ctx, inp.layout, self.cov.vecs, self.weights, inp.lmax_signal,
                                    self.bins, dtype=self.dtype, only=only, bin_range=bin_range, state=state,
-                                   lresume=lresume)
+                                   lresume=lresume, pool=pool)
 
     def chunks(self, chunk_budget: int):
         """Yield (spectrum index, first bin, Block with stack) over all
@@ -228,18 +228,24 @@ class DerivativeSource:
             g = [m for m, s2 in enumerate(specs) if (s2.fi, s2.fj) == (sp.fi, sp.fj)]
             seen.update(g)
             groups.append(g)
+        es = 4 if self.dtype == torch.float32 else 8
+        plan = []
         for g in groups:
             per_bin = sum(self.bytes_per_bin[m] for m in g)
-            cb = max(1, min(nb, chunk_budget // max(per_bin, 1)))
+            plan.append((g, max(1, min(nb, chunk_budget // max(per_bin, 1))), per_bin))
+        # One pool for every chunk (no per-chunk allocations of tens of GB).
+        pool = torch.empty(max(cb * pb for _, cb, pb in plan) // es, dtype=self.dtype, device=self.cov.ctx.device)
+        for g, cb, _ in plan:
             # Consecutive bin chunks continue one recurrence pass.
             state = engine.deriv_state(self.cov.ctx, self.cov.inp.layout, g[0]) if cb < nb else None
             for b0 in range(0, nb, cb):
                 lres = self.bins[b0 - 1][1] + 1 if b0 > 0 else 0
-                blocks = self._make(g, (b0, min(nb, b0 + cb)), state, lres)
+                blocks = self._make(g, (b0, min(nb, b0 + cb)), state, lres, pool)
                 for m in g:
                     yield m, b0, blocks[m]
                 del blocks
             del state
+        del pool
 
 
 def _free_bytes(ctx) -> int:
@@ -331,8 +337,10 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
         timer.bands = bands
     G = torch.zeros((npar, npar), dtype=torch.float64, device=ctx.device)
     zbuf = None
+    xbuf = torch.empty(npar * max(band_size(r0, r1) for r0, r1 in bands), dtype=T, device=ctx.device)
     for bi, (r0, r1) in enumerate(bands):
-        X = torch.zeros((npar, band_size(r0, r1)), dtype=T, device=ctx.device)
+        X = xbuf[:npar * band_size(r0, r1)].view(npar, band_size(r0, r1))
+        ctx.check(ctx.lib.cfmi_fill(ctx.h, engine.dtype_code(X), X.numel(), 1, P(X), X.numel(), 0.0), "fill")
         if timer:
             timer.mark("white0")
         for s, b0, blk in src.chunks(chunk_budget):
@@ -353,7 +361,7 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
         del X
         if progress is not None:
             progress(bi + 1, len(bands))
-    del zbuf
+    del zbuf, xbuf
     if not keep_stacks:
         src.resident = None
     Gh = engine.to_numpy_cm(G)

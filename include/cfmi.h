@@ -227,6 +227,37 @@ int cfmi_fill(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, void *A, int64_t l
 int cfmi_gram(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t K, const void *X,
               int64_t ldx, const void *Y, int64_t ldy, double *G, int64_t ldg);
 
+/* ---- A18 / A21: batched exact likelihood -------------------------------- */
+/* A batch of njobs covariance matrices C_j = sum_k coef[j][k] M_{p(j),k},
+ * p(j) = job_patch[j] (host array), from per-patch FP64 component matrices
+ * (device; patch p's k-th n_p x n_p column-major matrix at
+ * comps + comp_off[p] + k n_p^2, comp_off host) and data vectors (device,
+ * patch p's at xs + x_off[p], x_off host); n_p <= 2048, ncomp <= 4.
+ * Per job: chi2 = x^T C^-1 x and ln|C| of the Jacobi-scaled Cholesky.
+ *   mode CFMI_BATCH_MIXED: factor stored in FP32, trailing updates in FP32
+ *     on MFMA; diagonal blocks, panels, ln|C| and refinement in FP64;
+ *   mode CFMI_BATCH_MIXED_F64_STORAGE: factor stored in FP64, trailing
+ *     products in FP32 on MFMA added to it in FP64;
+ *   mode CFMI_BATCH_FP64: everything FP64.
+ *   nb: panel width 16, 32 or 64; wg_per_cu: persistent workgroups per CU;
+ *   refine_iters: refinement steps y += C^-1 (x - C y) (0..8).
+ * info[j] > 0 (chi2, logdet NaN) marks a non-positive-definite C_j.
+ * cfmi_batch_run enqueues on the context's stream; cfmi_batch_capture
+ * records the same work as a graph with fixed argument pointers, which
+ * cfmi_batch_replay launches (refill coef in place between replays).     */
+#define CFMI_BATCH_MIXED 1
+#define CFMI_BATCH_MIXED_F64_STORAGE 2
+#define CFMI_BATCH_FP64 3
+typedef struct cfmi_batch cfmi_batch;
+int cfmi_batch_create(cfmi_ctx *ctx, int mode, int npatch, const int64_t *n, int ncomp,
+                      const double *comps, const int64_t *comp_off, const double *xs,
+                      const int64_t *x_off, int njobs, const int *job_patch, int nb, int wg_per_cu,
+                      int refine_iters, cfmi_batch **out);
+int cfmi_batch_run(cfmi_batch *b, const double *coef, double *chi2, double *logdet, int *info);
+int cfmi_batch_capture(cfmi_batch *b, const double *coef, double *chi2, double *logdet, int *info);
+int cfmi_batch_replay(cfmi_batch *b);
+int cfmi_batch_destroy(cfmi_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
