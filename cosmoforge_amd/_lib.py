@@ -66,6 +66,14 @@ SIGNATURES = {
     "cfmi_batch_capture": (ci, [vp, vp, vp, vp, vp]),
     "cfmi_batch_replay": (ci, [vp]),
     "cfmi_batch_destroy": (ci, [vp]),
+    "cfmi_mlp_param_count": (i64, []),
+    "cfmi_mlp_workspace_bytes": (i64, [ci, ci]),
+    "cfmi_mlp_forward": (ci, [vp, ci, ci, vp, vp, vp, vp, vp, vp]),
+    "cfmi_mlp_loss": (ci, [vp, ci, ci, vp, vp, vp, vp, vp]),
+    "cfmi_mlp_backward": (ci, [vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci]),
+    "cfmi_mlp_adam": (ci, [vp, i64, vp, vp, vp, vp, cd, cd, cd, cd, ci, vp]),
+    "cfmi_mlp_to_bf16": (ci, [vp, i64, vp, vp]),
+    "cfmi_mlp_ensemble_stats": (ci, [vp, ci, ci, vp, vp, vp]),
 }
 
 _lib = None

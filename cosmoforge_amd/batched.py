@@ -268,3 +268,56 @@ def disc_coefficients(tau, r, logamp=None) -> np.ndarray:
 
 
 TRUTH = (0.06, 0.05, 0.0)
+
+
+class JointLikelihood:
+    """-2 ln L(theta) summed over independent patches, theta = (tau, r, A)
+    rows; evaluated with ``BatchedLikelihood`` in chunks of at most
+    ``max_jobs`` (patch, point) jobs, one plan per chunk size."""
+
+    def __init__(self, ctx: engine.Context, patches: list, precision: str = "mixed", nb: int = 32,
+                 wg_per_cu: int = 4, max_jobs: int = 16384):
+        self.ctx, self.patches, self.precision = ctx, patches, precision
+        self.nb, self.wg_per_cu = nb, wg_per_cu
+        self.per_call = max(1, max_jobs // len(patches))
+        self._plans = {}
+        self.evaluations = 0
+
+    def _plan(self, npts: int) -> BatchedLikelihood:
+        if npts not in self._plans:
+            if len(self._plans) > 2:
+                for b in self._plans.values():
+                    b.close()
+                self._plans = {}
+            jp = np.tile(np.arange(len(self.patches)), npts)
+            self._plans[npts] = BatchedLikelihood(self.ctx, self.patches, jp, self.precision, nb=self.nb,
+                                                  wg_per_cu=self.wg_per_cu)
+        return self._plans[npts]
+
+    def __call__(self, theta: np.ndarray, graph: bool = False) -> np.ndarray:
+        theta = np.atleast_2d(np.asarray(theta, dtype=np.float64))
+        out = np.empty(len(theta))
+        npatch = len(self.patches)
+        for s in range(0, len(theta), self.per_call):
+            th = theta[s:s + self.per_call]
+            coef = disc_coefficients(np.repeat(th[:, 0], npatch), np.repeat(th[:, 1], npatch),
+                                     np.repeat(th[:, 2], npatch))
+            v = self._plan(len(th)).minus2lnl(coef, graph=graph)
+            out[s:s + len(th)] = v.reshape(len(th), npatch).sum(axis=1)
+        self.evaluations += len(theta)
+        return out
+
+    def close(self):
+        for b in self._plans.values():
+            b.close()
+        self._plans = {}
+
+
+BOUNDS = np.array([[0.03, 0.09], [0.0, 0.1], [-0.1, 0.1]])
+
+
+def draw_points(n: int, seed: int, bounds=None) -> np.ndarray:
+    """Uniform (tau, r, A) inside ``bounds`` (default BOUNDS)."""
+    b = BOUNDS if bounds is None else np.asarray(bounds, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    return rng.uniform(b[:, 0], b[:, 1], (n, 3))

@@ -258,6 +258,33 @@ int cfmi_batch_capture(cfmi_batch *b, const double *coef, double *chi2, double *
 int cfmi_batch_replay(cfmi_batch *b);
 int cfmi_batch_destroy(cfmi_batch *b);
 
+/* ---- A21: ensemble MLP surrogate of -2 ln L ---------------------------- */
+/* E networks 3 -> 256 -> 256 -> 256 -> 1 (SiLU).  Parameters per member
+ * (cfmi_mlp_param_count() values, layout W1[256][3], b1, W2[256][256], b2,
+ * W3[256][256], b3, W4[1][256], b4): FP32 master w32 [E][P] and its BF16
+ * copy wb16 [E][P] (uint16 bit patterns), the copy used by every product;
+ * the 256 x 256 layers run on BF16 MFMA with FP32 accumulation.
+ * X: inputs [N][3] FP32; idx: [E][B] rows of X per member (NULL: row b);
+ * y: [E][B]; ws: cfmi_mlp_workspace_bytes(E, B) bytes of activations kept
+ * by forward for backward.  Backward writes dL/dparams into grad [E][P]
+ * from dy = dL/dy [E][B]; weight gradients reduce over batch chunks of
+ * `chunk` rows (multiple of 32) through part [E][ceil(B/chunk)][256][256].
+ * cfmi_mlp_loss: dy = 2 (y - T[idx]) / B and loss[e] = mean squared error. */
+int64_t cfmi_mlp_param_count(void);
+int64_t cfmi_mlp_workspace_bytes(int E, int B);
+int cfmi_mlp_forward(cfmi_ctx *ctx, int E, int B, const float *X, const int *idx, const float *w32,
+                     const void *wb16, float *y, void *ws);
+int cfmi_mlp_loss(cfmi_ctx *ctx, int E, int B, const float *y, const float *T, const int *idx, float *dy,
+                  float *loss);
+int cfmi_mlp_backward(cfmi_ctx *ctx, int E, int B, const float *X, const int *idx, const void *wb16,
+                      const float *dy, void *ws, float *grad, float *part, int chunk);
+/* Adam (bias-corrected, step >= 1) on total = E * P values; refreshes wb16. */
+int cfmi_mlp_adam(cfmi_ctx *ctx, int64_t total, float *w, const float *g, float *m, float *v, double lr,
+                  double beta1, double beta2, double eps, int step, void *wb16);
+int cfmi_mlp_to_bf16(cfmi_ctx *ctx, int64_t total, const float *w, void *wb16);
+/* mean and population standard deviation of y [E][B] over the members. */
+int cfmi_mlp_ensemble_stats(cfmi_ctx *ctx, int E, int B, const float *y, float *mean, float *sd);
+
 #ifdef __cplusplus
 }
 #endif
