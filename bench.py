@@ -102,6 +102,120 @@ def cpu_baseline(cfg, maps, k_params: int):
     }
 
 
+def picslike_leg(ctx, log):
+    """Config-3 batched exact likelihood: evals/s of the persistent batched
+    kernel (mixed and FP64, eager and graph replay) against the plain
+    per-point GPU path (the drop-in PICSLike point by point, FP64)."""
+    import torch
+
+    from cosmoforge_amd import batched, pipeline, workloads
+    from cosmoforge_amd.core import Inputs
+
+    cfg = workloads.config(3)
+    L = cfg.params["lmax_signal"]
+    truth = batched.disc_coefficients([batched.TRUTH[0]], [batched.TRUTH[1]], [batched.TRUTH[2]])[0]
+    patches = batched.qu_patches(ctx, 16, cfg.extra["masks"], batched.disc_basis(L), 1.0, L, truth,
+                                 seed=workloads.SEED0 + 3)
+    ns = np.array([p.n for p in patches])
+    tau, r = cfg.extra["tau"], cfg.extra["r"]
+    job_patch = np.repeat(np.arange(len(patches)), tau.shape[1])
+    coef = batched.disc_coefficients(tau.ravel(), r.ravel())
+    tun = batched.autotune(ctx, patches, job_patch, coef, "mixed", use_cache=False)
+    out = {"workload": cfg.name, "jobs": int(len(job_patch)), "n_min": int(ns.min()), "n_max": int(ns.max()),
+           "n_mean": round(float(ns.mean()), 1), "autotune": {"nb": tun["nb"], "wg_per_cu": tun["wg_per_cu"]}}
+    vals = {}
+    for prec in ("mixed", "fp64"):
+        b = batched.BatchedLikelihood(ctx, patches, job_patch, prec, nb=tun["nb"], wg_per_cu=tun["wg_per_cu"])
+        vals[prec] = b.minus2lnl(coef)
+        for graph in (False, True):
+            b.run(coef, graph=graph)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                b.run(coef, graph=graph)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            out[f"{prec}{'_graph' if graph else ''}_evals_per_s"] = round(b.njobs / dt, 1)
+        b.close()
+        log(f"[bench] picslike batched {prec} done")
+    sel = np.linspace(0, len(job_patch) - 1, 24).astype(int)
+    plain = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in sel:
+        p = job_patch[j]
+        mask = np.stack([cfg.extra["masks"][p]] * 2, axis=1)
+        inp = Inputs(dict(cfg.params), mask, np.eye(patches[p].n), workloads.qu_cls(L, tau.ravel()[j], r.ravel()[j]),
+                     cfg.beam)
+        cov = pipeline.Covariance(ctx, inp)
+        chi2, logdet = pipeline.likelihood(cov, patches[p].x.cpu().numpy()[:, None], None)
+        plain.append(chi2[0] + logdet)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / len(sel)
+    out["plain_per_point_evals_per_s"] = round(1.0 / dt, 1)
+    out["speedup_batched_mixed_vs_plain"] = round(out["mixed_graph_evals_per_s"] / out["plain_per_point_evals_per_s"], 1)
+    out["mixed_vs_fp64_max_abs_m2lnL"] = float(np.max(np.abs(vals["mixed"] - vals["fp64"])))
+    out["plain_vs_fp64_batched_max_abs_m2lnL"] = float(np.max(np.abs(np.array(plain) - vals["fp64"][sel])))
+    return out, (patches, tun)
+
+
+def surrogate_leg(ctx, patches, tun, log, n_train=20000, n_cand=65536):
+    """Config 4: 4-member BF16 MLP ensemble trained on exact joint -2 ln L of
+    the config-3 patches; 65,536 candidates screened (spread > 0.5 or within
+    25 of the best prediction -> exact)."""
+    import torch
+
+    from cosmoforge_amd import batched
+    from cosmoforge_amd.surrogate import EnsembleMLP, screen
+
+    like = batched.JointLikelihood(ctx, patches, "mixed", nb=tun["nb"], wg_per_cu=tun["wg_per_cu"])
+    train = batched.draw_points(n_train, seed=20261207)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    target = like(train, graph=True)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    log(f"[bench] surrogate training set: {n_train} exact joint evaluations in {t_data:.1f} s")
+    model = EnsembleMLP(ctx, members=4, seed=7, bounds=batched.BOUNDS)
+    t0 = time.perf_counter()
+    model.fit(train, target, epochs=120, batch=512, lr=3e-3, lr_final=5e-5)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"[bench] surrogate trained in {t_train:.1f} s")
+    cand = batched.draw_points(n_cand, seed=20261208)
+    model.predict(cand[:1024])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mu, sg = model.predict(cand)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t0
+    like.evaluations = 0
+    t0 = time.perf_counter()
+    rep = screen(model, cand, lambda th: like(th, graph=True), spread_threshold=0.5, keep_delta=25.0)
+    torch.cuda.synchronize()
+    t_screen = time.perf_counter() - t0
+    # the surrogate's error, judged by the exact likelihood on a sample of the rejected candidates
+    rng = np.random.default_rng(3)
+    rej = rep.rejected
+    chk = rng.choice(rej, size=min(2048, rej.size), replace=False) if rej.size else rej
+    exact_chk = like(cand[chk], graph=True) if chk.size else np.zeros(0)
+    ex = rep.exact
+    like.close()
+    jobs_rate = len(patches) * len(ex) / max(t_screen - t_inf, 1e-9)
+    return {"train_points": n_train, "train_data_s": round(t_data, 2), "train_s": round(t_train, 2),
+            "candidates": n_cand, "inference_s": round(t_inf, 4), "screen_s": round(t_screen, 2),
+            "screened_points_per_s": round(n_cand / t_screen, 1),
+            "exhaustive_batched_points_per_s": round(n_train / t_data, 1),
+            "exact_fraction": round(len(ex) / n_cand, 4), "fallback_by_spread": int(len(rep.fallback)),
+            "rejected": int(rej.size), "patch_jobs_per_s_in_screen": round(jobs_rate, 1),
+            "surrogate_error_on_rejected_sample": {
+                "n": int(chk.size),
+                "max_abs": float(np.max(np.abs(mu[chk] - exact_chk))) if chk.size else None,
+                "rms": float(np.sqrt(np.mean((mu[chk] - exact_chk) ** 2))) if chk.size else None,
+                "min_exact_minus_best_kept": float(np.min(exact_chk) - np.min(rep.value[ex])) if chk.size else None}}
+
+
 def rel(a, b) -> float:
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
@@ -111,7 +225,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4],
+                    help="0-2: Fisher+QML workloads; 3: batched likelihood; 4: surrogate-screened likelihood")
+    ap.add_argument("--no-picslike", action="store_true", help="skip the batched-likelihood leg")
     ap.add_argument("--precisions", default="fp64,mixed", help="comma list of fp64 / mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-params", type=int, default=2)
@@ -125,8 +241,33 @@ def main():
 
     from cosmoforge_amd import Fisher, Spectra, engine, pipeline, workloads
 
-    cfg = workloads.config(args.config)
     ctx = engine.Context(0)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if args.config in (3, 4):
+        pl, (patches, tun) = picslike_leg(ctx, log)
+        out = {"metric": "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error",
+               "n_gpus": 1, "steps": 3, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "data": "synthetic (seeded discs, maps and parameter draws)"}
+        if args.config == 3:
+            out.update({"value": pl["mixed_graph_evals_per_s"], "unit": "likelihood evaluations/s",
+                        "ms_per_step": round(1e3 * pl["jobs"] / pl["mixed_graph_evals_per_s"], 2),
+                        "dtype": "f32 MFMA (FP64 diagonal blocks, panels, ln|C|, refinement)",
+                        "config": {"workload": pl["workload"], "jobs": pl["jobs"]}, "picslike": pl})
+        else:
+            sl = surrogate_leg(ctx, patches, tun, log)
+            out.update({"value": sl["screened_points_per_s"], "unit": "parameter points/s (joint over 64 patches)",
+                        "ms_per_step": round(1e3 * sl["screen_s"], 2), "dtype": "bf16 MFMA surrogate + f32 MFMA exact",
+                        "config": {"workload": "nside16_QU_64discs_surrogate", "candidates": sl["candidates"]},
+                        "surrogate": sl, "picslike": pl,
+                        "speedup_screened_vs_plain_per_point": round(
+                            sl["screened_points_per_s"] / (pl["plain_per_point_evals_per_s"] / 64.0), 1)})
+        print(json.dumps(out))
+        return
+
+    cfg = workloads.config(args.config)
     maps = make_map(ctx, cfg, workloads.SEED0 + args.config)
     print(f"[bench] config {args.config} ({cfg.name}): n={cfg.n}", file=sys.stderr, flush=True)
 
@@ -204,6 +345,8 @@ def main():
             "F_maxrel": rel(Fm, F), "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.abs(np.diag(F)))),
             "Cl_maxrel": rel(Cm, C), "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
             "noise_bias_maxrel": rel(nbm, nb), "qml_raw_maxrel": rel(qm, q), "cond_F": float(np.linalg.cond(F))}
+    if not args.no_picslike:
+        out["picslike"] = picslike_leg(ctx, log)[0]
     if not args.no_cpu and args.config < 2:
         out["cpu_baseline"] = cpu_baseline(cfg, maps, args.cpu_params)
     print(json.dumps(out))
