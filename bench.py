@@ -31,7 +31,7 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # HBM bytes per whitening launch, measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE on this workload (each scaled by what the counter reads for
 # a 1 GiB copy: x2.0 and x1.0); not measured live.  Keyed by (config, dtype).
-WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 113.6e9 / 1.0}
+WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 113.6e9, (1, "f32"): 57.6e9}
 
 
 def make_map(ctx, cfg, seed):

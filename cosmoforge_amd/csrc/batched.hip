@@ -133,12 +133,15 @@ __device__ void trailing_update(int n, TS* A, int r0, int kc, int kb)
                     bv[s][h] = (kin && rj < n) ? (TC)col[rj] : TC(0);
                 }
             }
+        // D = P_j P_i^T (the transposed tile): a lane's outputs are rows
+        // i0 + 16 a + lm (contiguous in the column-major store) of columns
+        // j0 + 16 b + {4 lk + r | lk + 4 r}.
 #pragma unroll
             for (int s = 0; s < KS; ++s)
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) acc[a][b] = Mf<TC>::run(av[s][a], bv[s][b], acc[a][b]);
+                    for (int b = 0; b < 2; ++b) acc[a][b] = Mf<TC>::run(bv[s][b], av[s][a], acc[a][b]);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -146,8 +149,8 @@ __device__ void trailing_update(int n, TS* A, int r0, int kc, int kb)
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    int row = i0 + 16 * a + (kF64 ? lk + 4 * r : 4 * lk + r);
-                    int col = j0 + 16 * b + lm;
+                    int row = i0 + 16 * a + lm;
+                    int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
                     if (row < n && col < n && row >= col) {
                         TS* p = A + row + col * n;
                         *p = (TS)((TS)*p + (TS)acc[a][b][r]);
