@@ -238,35 +238,41 @@ __global__ void loss_kernel(int B, const float* y, const float* T, const int* id
     if (threadIdx.x == 0) loss[e] = red[0] / (float)B;
 }
 
-// delta3 = dy W4 SiLU'(z3) (BF16); gW4[j] = sum_b dy_b h3[b][j]; gb4 = sum_b dy_b.
+// delta3 = dy W4 SiLU'(z3) (BF16) for batch rows of chunk blockIdx.x; the
+// chunk's partial sums of gW4[j] = sum_b dy_b h3[b][j] and gb4 = sum_b dy_b
+// go to part[e][chunk][0..256] (reduced in a fixed order afterwards).
+constexpr int RCH = 64;  // batch rows per reduction chunk
+
 __global__ void l4_bwd_kernel(int B, const float* dy, const bf16* h3, const float* z3, int64_t sAct, const bf16* wb,
-                              bf16* d3, float* grad)
+                              bf16* d3, float* part, int nchunk)
 {
-    const int e = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;  // column
-    if (j >= H) return;
+    const int e = blockIdx.y, c = blockIdx.x;
+    const int j = threadIdx.x;  // column (256 threads)
     const float w4 = bf2f(wb[e * NPAR + OFF_W4 + j]);
     float gw = 0.0f, gb = 0.0f;
-    for (int b = 0; b < B; ++b) {
+    const int b1 = min(B, (c + 1) * RCH);
+    for (int b = c * RCH; b < b1; ++b) {
         const float g = dy[(int64_t)e * B + b];
         const int64_t o = e * sAct + (int64_t)b * H + j;
         gw += g * bf2f(h3[o]);
         gb += g;
         d3[o] = f2bf(g * w4 * dsilu(z3[o]));
     }
-    grad[e * NPAR + OFF_W4 + j] = gw;
-    if (j == 0) grad[e * NPAR + OFF_B4] = gb;
+    float* pp = part + ((int64_t)e * nchunk + c) * (H + 1);
+    pp[j] = gw;
+    if (j == 0) pp[H] = gb;
 }
 
-// db[j] = sum_b delta[b][j]; for the first layer also gW1[j][k] = sum_b delta[b][j] x[b][k].
-__global__ void colsum_kernel(int B, const bf16* d, int64_t sAct, float* grad, int64_t off_b, const float* X,
-                              const int* idx, int64_t off_w1)
+// Chunk partials of db[j] = sum_b delta[b][j] and, for the first layer,
+// gW1[j][k] = sum_b delta[b][j] x[b][k]: part[e][chunk][j * 4 + {0: db, 1..3: gW1}].
+__global__ void colsum_kernel(int B, const bf16* d, int64_t sAct, const float* X, const int* idx, float* part,
+                              int nchunk)
 {
-    const int e = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= H) return;
+    const int e = blockIdx.y, c = blockIdx.x;
+    const int j = threadIdx.x;
     float sb = 0.0f, sw[NIN] = {0.0f, 0.0f, 0.0f};
-    for (int b = 0; b < B; ++b) {
+    const int b1 = min(B, (c + 1) * RCH);
+    for (int b = c * RCH; b < b1; ++b) {
         const float v = bf2f(d[e * sAct + (int64_t)b * H + j]);
         sb += v;
         if (X) {
@@ -275,10 +281,39 @@ __global__ void colsum_kernel(int B, const bf16* d, int64_t sAct, float* grad, i
             for (int k = 0; k < NIN; ++k) sw[k] += v * X[(int64_t)row * NIN + k];
         }
     }
-    grad[e * NPAR + off_b + j] = sb;
-    if (X)
+    float* pp = part + ((int64_t)e * nchunk + c) * (4 * H);
+    pp[4 * j] = sb;
 #pragma unroll
-        for (int k = 0; k < NIN; ++k) grad[e * NPAR + off_w1 + j * NIN + k] = sw[k];
+    for (int k = 0; k < NIN; ++k) pp[4 * j + 1 + k] = sw[k];
+}
+
+// Fixed-order sums of the chunk partials into the gradient.
+__global__ void l4_reduce_kernel(int nchunk, const float* part, float* grad)
+{
+    const int e = blockIdx.y, j = threadIdx.x;
+    float gw = 0.0f, gb = 0.0f;
+    for (int c = 0; c < nchunk; ++c) {
+        const float* pp = part + ((int64_t)e * nchunk + c) * (H + 1);
+        gw += pp[j];
+        if (j == 0) gb += pp[H];
+    }
+    grad[e * NPAR + OFF_W4 + j] = gw;
+    if (j == 0) grad[e * NPAR + OFF_B4] = gb;
+}
+
+__global__ void colsum_reduce_kernel(int nchunk, const float* part, float* grad, int64_t off_b, int with_w1)
+{
+    const int e = blockIdx.y, j = threadIdx.x;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < nchunk; ++c) {
+        const float* pp = part + ((int64_t)e * nchunk + c) * (4 * H) + 4 * j;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s[k] += pp[k];
+    }
+    grad[e * NPAR + off_b + j] = s[0];
+    if (with_w1)
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) grad[e * NPAR + OFF_W1 + j * NIN + k] = s[1 + k];
 }
 
 // grad[e][off + i] = sum over splits of part[e][s][i] (fixed order).
@@ -425,9 +460,15 @@ int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx,
     const bf16* wb = (const bf16*)wb16;
     Acts a = acts_of(ws, E, B);
     const int nsplit = (B + chunk - 1) / chunk;
+    // Reduction partials reuse `part` (it holds at least E * nsplit * 256 * 256 floats).
+    const int nchunk = (B + RCH - 1) / RCH;
+    if ((int64_t)nchunk * 4 * H > (int64_t)nsplit * H * H)
+        return cfmi_fail(c, CFMI_EINVAL, "mlp: gradient chunk too large for the batch reductions");
     // output layer
-    hipLaunchKernelGGL(l4_bwd_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, B, dy, a.h[2], a.z[2], a.s, wb,
-                       a.d[2], grad);
+    hipLaunchKernelGGL(l4_bwd_kernel, dim3((unsigned)nchunk, (unsigned)E), dim3(256), 0, c->stream, B, dy, a.h[2],
+                       a.z[2], a.s, wb, a.d[2], part, nchunk);
+    CFMI_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(l4_reduce_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, nchunk, part, grad);
     CFMI_LAUNCH_CHECK(c);
     for (int l = 2; l >= 1; --l) {
         const int64_t offw = l == 2 ? OFF_W3 : OFF_W2, offb = l == 2 ? OFF_B3 : OFF_B2;
@@ -442,8 +483,11 @@ int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx,
         hipLaunchKernelGGL(split_sum_kernel, dim3(gcap((int64_t)H * H).x, (unsigned)E), dim3(256), 0, c->stream,
                            nsplit, (int64_t)H * H, part, (int64_t)nsplit * H * H, grad, offw);
         CFMI_LAUNCH_CHECK(c);
-        hipLaunchKernelGGL(colsum_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, B, a.d[l], a.s, grad, offb,
-                           (const float*)nullptr, (const int*)nullptr, (int64_t)0);
+        hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)nchunk, (unsigned)E), dim3(256), 0, c->stream, B, a.d[l],
+                           a.s, (const float*)nullptr, (const int*)nullptr, part, nchunk);
+        CFMI_LAUNCH_CHECK(c);
+        hipLaunchKernelGGL(colsum_reduce_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, nchunk, part, grad, offb,
+                           0);
         CFMI_LAUNCH_CHECK(c);
         // delta_{l-1} = (delta_l W_l) SiLU'(z_{l-1})
         MG d = {};
@@ -456,8 +500,11 @@ int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx,
         if (rc) return rc;
     }
     // first layer: gb1 and gW1 = delta_1^T x
-    hipLaunchKernelGGL(colsum_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, B, a.d[0], a.s, grad, OFF_B1, X,
-                       idx, OFF_W1);
+    hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)nchunk, (unsigned)E), dim3(256), 0, c->stream, B, a.d[0], a.s, X,
+                       idx, part, nchunk);
+    CFMI_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(colsum_reduce_kernel, dim3(1, (unsigned)E), dim3(256), 0, c->stream, nchunk, part, grad, OFF_B1,
+                       1);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
