@@ -16,6 +16,7 @@ CFMI_OK = 0
 CFMI_F64 = 0
 CFMI_F32 = 1
 CFMI_BF16 = 2
+CFMI_BF16X3 = 3
 CFMI_GEMM_ACC64 = 128
 
 _c = ctypes
@@ -51,6 +52,10 @@ SIGNATURES = {
                                       i64, i64, i64, cd, cd, i64, ci]),
     "cfmi_gemm_ex": (ci, [vp, ci, ci, ci, ci, i64, i64, i64, cd, vp, i64, i64, i64, vp, i64, i64, i64, cd, vp,
                           i64, i64, i64, ci, ci, i64, i64, i64, cd, cd]),
+    "cfmi_gemm_b3": (ci, [vp, ci, i64, i64, i64, cd, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, cd, vp, i64,
+                          i64, i64, i64, ci, i64, i64, i64, cd]),
+    "cfmi_split3": (ci, [vp, ci, i64, i64, vp, i64, vp, i64, i64]),
+    "cfmi_join3": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),

@@ -17,6 +17,7 @@
 #include "cfmi_internal.h"
 
 #include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -79,7 +80,44 @@ struct PairArgs {
     int64_t ntile;     // tiles per side (symmetric kinds)
     double* state;     // per-pair recurrence state (P_l, P_{l-1}) after the last bin, or NULL
     int lresume;       // > 0: start at l = lresume from the saved state
+    int64_t ps, ps_m;  // three-piece BF16 output: plane strides of out / mir
 };
+
+// Element type of a three-piece BF16 output (x = x0 + x1 + x2, planes ps apart).
+struct Bf3 {
+    unsigned short v;
+};
+
+__device__ __forceinline__ unsigned short f2bf16(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ double bf16f(unsigned short v)
+{
+    unsigned u = (unsigned)v << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+template <typename TO>
+__device__ __forceinline__ void put(TO* p, int64_t i, double v, int64_t) { p[i] = (TO)v; }
+template <>
+__device__ __forceinline__ void put<Bf3>(Bf3* p, int64_t i, double v, int64_t ps)
+{
+    unsigned short* q = reinterpret_cast<unsigned short*>(p) + i;
+    const unsigned short h = f2bf16((float)v);
+    double r = v - bf16f(h);
+    const unsigned short m = f2bf16((float)r);
+    r -= bf16f(m);
+    q[0] = h;
+    q[ps] = m;
+    q[2 * ps] = f2bf16((float)r);
+}
 
 // cos/sin of the rotation angles of geometry.py:49-110, trig-free.
 // For pairs on a common meridian d is +-1 up to rounding and sin(2 acos d)
@@ -135,7 +173,7 @@ __device__ __forceinline__ double dot3_exact(const double* a, const double* b)
 template <typename TO>
 __device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double* v, TO* dst, int64_t ld,
                                             TO* dstT, int64_t ldT, int64_t j0, int64_t i0, int64_t nr,
-                                            int64_t nc, bool diag_tile)
+                                            int64_t nc, bool diag_tile, int64_t ps = 0, int64_t psT = 0)
 {
     const int t = threadIdx.x;
     const int tj = t & (TILE - 1), ti0 = t >> 5;
@@ -144,7 +182,7 @@ __device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double
         for (int q = 0; q < PPT; ++q) {
             int ti = ti0 + 8 * q;
             int64_t j = j0 + tj, i = i0 + ti;
-            if (j < nr && i < nc && (!diag_tile || tj >= ti)) dst[j + i * ld] = (TO)v[q];
+            if (j < nr && i < nc && (!diag_tile || tj >= ti)) put(dst, j + i * ld, v[q], ps);
         }
     }
     if (dstT) {
@@ -157,7 +195,7 @@ __device__ __forceinline__ void write_plane(double (*sT)[TILE + 1], const double
         for (int q = 0; q < PPT; ++q) {
             int b = b0 + 8 * q;  // local row index j
             int64_t i = i0 + a, j = j0 + b;
-            if (j < nr && i < nc && (!diag_tile || b >= a)) dstT[i + j * ldT] = (TO)sT[a][b];
+            if (j < nr && i < nc && (!diag_tile || b >= a)) put(dstT, i + j * ldT, sT[a][b], psT);
         }
     }
 }
@@ -309,7 +347,8 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
                 TO* o = p.out[0] ? (TO*)p.out[0] + b * p.stride : nullptr;
                 TO* m = KIND == K00S ? (o) : (p.mir[0] ? (TO*)p.mir[0] + b * p.stride_m : nullptr);
                 int64_t ldm = KIND == K00S ? p.ld : p.ldm;
-                write_plane(sT, acc[0], o, p.ld, m, ldm, j0, i0, p.nr, p.nc, diag_tile);
+                write_plane(sT, acc[0], o, p.ld, m, ldm, j0, i0, p.nr, p.nc, diag_tile, p.ps,
+                            KIND == K00S ? p.ps : p.ps_m);
             } else if (KIND == K22) {
                 const int nspec = p.sum_mode ? 1 : 3;
                 for (int s = 0; s < nspec; ++s) {
@@ -343,10 +382,10 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
                     TO* UU = o + n + n * ld;
                     TO* UQ = o + n;
                     TO* QU = o + n * ld;
-                    write_plane(sT, va, QQ, ld, QQ, ld, j0, i0, p.nr, p.nc, diag_tile);
-                    write_plane(sT, vb, UU, ld, UU, ld, j0, i0, p.nr, p.nc, diag_tile);
-                    write_plane(sT, vc, UQ, ld, QU, ld, j0, i0, p.nr, p.nc, diag_tile);
-                    write_plane(sT, vd, QU, ld, UQ, ld, j0, i0, p.nr, p.nc, diag_tile);
+                    write_plane(sT, va, QQ, ld, QQ, ld, j0, i0, p.nr, p.nc, diag_tile, p.ps, p.ps);
+                    write_plane(sT, vb, UU, ld, UU, ld, j0, i0, p.nr, p.nc, diag_tile, p.ps, p.ps);
+                    write_plane(sT, vc, UQ, ld, QU, ld, j0, i0, p.nr, p.nc, diag_tile, p.ps, p.ps);
+                    write_plane(sT, vd, QU, ld, UQ, ld, j0, i0, p.nr, p.nc, diag_tile, p.ps, p.ps);
                 }
             } else {  // K02: rows j are spin-2 pixels, cols i spin-0
                 const int nspec = p.sum_mode ? 1 : 2;
@@ -365,9 +404,9 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
                         vu[q] = tq * s1[q] + tu * c1[q];
                     }
                     const int64_t n = p.np;
-                    write_plane(sT, vq, o, p.ld, m, p.ldm, j0, i0, p.nr, p.nc, false);
+                    write_plane(sT, vq, o, p.ld, m, p.ldm, j0, i0, p.nr, p.nc, false, p.ps, p.ps_m);
                     write_plane(sT, vu, o ? o + n : nullptr, p.ld, m ? m + n * p.ldm : nullptr, p.ldm, j0, i0,
-                                p.nr, p.nc, false);
+                                p.nr, p.nc, false, p.ps, p.ps_m);
                 }
             }
 #pragma unroll
@@ -393,7 +432,12 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
 int launch_pairs(cfmi_ctx* c, int kind, PairArgs& a, const double* w0, const double* w1, const double* w2,
                  int dtype = CFMI_F64)
 {
-    if (dtype != CFMI_F64 && dtype != CFMI_F32) return cfmi_fail(c, CFMI_EINVAL, "unsupported output dtype %d", dtype);
+    if (dtype != CFMI_F64 && dtype != CFMI_F32 && dtype != CFMI_BF16X3)
+        return cfmi_fail(c, CFMI_EINVAL, "unsupported output dtype %d", dtype);
+    if (dtype == CFMI_BF16X3) {
+        a.ps = a.stride * a.nbins;
+        a.ps_m = a.stride_m * a.nbins;
+    }
     if (a.L < 0 || a.L > CFMI_MAX_ELL) return cfmi_fail(c, CFMI_EINVAL, "lmax %d out of range", a.L);
     if (a.nr <= 0 || a.nc <= 0 || a.nbins <= 0) return CFMI_OK;
     if (!c->d_coef) CFMI_HIP(c, hipMalloc((void**)&c->d_coef, 5 * (CFMI_MAX_ELL + 1) * sizeof(double)));
@@ -402,10 +446,11 @@ int launch_pairs(cfmi_ctx* c, int kind, PairArgs& a, const double* w0, const dou
     CFMI_LAUNCH_CHECK(c);
     a.tab = c->d_coef;
     int64_t tr = (a.nr + TILE - 1) / TILE, tc = (a.nc + TILE - 1) / TILE;
-    const bool f32 = dtype == CFMI_F32;
 #define CFMI_PAIRS(K_, grid_)                                                                           \
     do {                                                                                               \
-        if (f32) hipLaunchKernelGGL((pair_kernel<K_, float>), grid_, dim3(256), 0, c->stream, a);       \
+        if (dtype == CFMI_F32) hipLaunchKernelGGL((pair_kernel<K_, float>), grid_, dim3(256), 0, c->stream, a); \
+        else if (dtype == CFMI_BF16X3)                                                                 \
+            hipLaunchKernelGGL((pair_kernel<K_, Bf3>), grid_, dim3(256), 0, c->stream, a);             \
         else hipLaunchKernelGGL((pair_kernel<K_, double>), grid_, dim3(256), 0, c->stream, a);          \
     } while (0)
     if (kind == K00S || kind == K22) {

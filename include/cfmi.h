@@ -54,6 +54,8 @@ extern "C" {
 #define CFMI_F64 0
 #define CFMI_F32 1
 #define CFMI_BF16 2
+/* Three BF16 planes per value, x = x0 + x1 + x2 (24 significant bits). */
+#define CFMI_BF16X3 3
 
 /* Largest multipole the recurrence kernels accept. */
 #define CFMI_MAX_ELL 4096
@@ -104,7 +106,9 @@ int cfmi_signal_02(cfmi_ctx *ctx, const double *cl_te, const double *cl_tb, int 
 /* Bin b covers l in [bin_lo[b], bin_hi[b]] (device int32 arrays, bins
  * ascending and disjoint); w is the l-indexed weight w_l * B1(l) B2(l).
  * Output b of a stack starts at out + b * stride, stored as dtype
- * (CFMI_F64, or CFMI_F32: the FP64 sums rounded once).  Geometry and mirror
+ * (CFMI_F64; CFMI_F32: the FP64 sums rounded once; CFMI_BF16X3: split
+ * into three BF16 planes, plane p of the stack at out + p * nbins * stride,
+ * likewise for the mirror).  Geometry and mirror
  * conventions are those of the matching cfmi_signal_* call.  In the 22 and
  * 02 calls any output pointer may be NULL (not produced).
  * state (device, may be NULL) holds cfmi_deriv_state_doubles(n_r, n_c,
@@ -182,6 +186,25 @@ int cfmi_gemm_ex(cfmi_ctx *ctx, int dtype, int out_dtype, int transa, int transb
                  double beta, void *C, int64_t ldc, int64_t sC, int64_t batch, int flags,
                  int out_mode, int64_t row0, int64_t col0, int64_t pbase, double pdiag,
                  double poff);
+
+/* FP32-accurate product on BF16 MFMA (CFMI_BF16X3 operands): C = alpha op
+ * with op(A)[m][k] = A[m lda + k] and op(B)[k][n] = B[n ldb + k] (planes pA,
+ * pB elements apart; batches sA, sB, sC apart), six piece products per
+ * term accumulated in FP32.  flags bit 0: A[m][k] = 0 for k > m + off_a;
+ * bit 2: B[n][k] = 0 for k > n + off_b.  out_mode 0: FP32 C[m + n ldc]
+ * (+ beta C); 1: three BF16 planes C[m ldc + n] (planes pC apart); 2: FP32
+ * packed lower row band as cfmi_gemm_ex mode 2 (+ beta C).               */
+int cfmi_gemm_b3(cfmi_ctx *ctx, int out_mode, int64_t m, int64_t n, int64_t k, double alpha,
+                 const void *A, int64_t lda, int64_t sA, int64_t pA, int64_t off_a, const void *B,
+                 int64_t ldb, int64_t sB, int64_t pB, int64_t off_b, double beta, void *C, int64_t ldc,
+                 int64_t sC, int64_t pC, int64_t batch, int flags, int64_t row0, int64_t col0,
+                 int64_t pbase, double pdiag);
+/* Column-major FP64/FP32 (rows x cols) -> three BF16 planes, row-major
+ * (element (i, j) at dst + i ldd + j), planes ps apart; and back (sum of the
+ * pieces, count values, FP64).                                            */
+int cfmi_split3(cfmi_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void *src, int64_t lds,
+                void *dst, int64_t ldd, int64_t ps);
+int cfmi_join3(cfmi_ctx *ctx, int64_t count, const void *src, int64_t ps, double *dst);
 
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
