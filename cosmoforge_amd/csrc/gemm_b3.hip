@@ -75,11 +75,15 @@ struct B3Args {
     int vecA, vecB;           // 16-byte vector loads allowed
 };
 
+// Tile order: column tiles of B in turn, the row tiles of A in each; with a
+// triangular K-trim the heaviest tiles (longest K) are issued first.
+// (Grouping tiles into patches per XCD was measured no faster: L2 reuse of
+// the operand tiles is not what bounds this kernel.)
 __device__ __forceinline__ void tile_order(int64_t b, int64_t tm, int64_t tn, int tri, int64_t& ti, int64_t& tj)
 {
     ti = b % tm;
     tj = b / tm;
-    if (tri & 1) ti = tm - 1 - ti;   // heaviest rows (longest K) first
+    if (tri & 1) ti = tm - 1 - ti;
     if (tri & 4) tj = tn - 1 - tj;
 }
 
@@ -111,6 +115,18 @@ struct Stage {
 #define CFMI_B3_LF(u, vv) vv = *reinterpret_cast<const u4*>(base + (u / 2) * ps + (u % 2) * 64 * ld);
         CFMI_B3_EACH(CFMI_B3_LF)
 #undef CFMI_B3_LF
+    }
+    // Whole vectors, rows beyond ``rows`` read as zeros (the slab is inside K).
+    __device__ __forceinline__ void load_rows(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
+                                              int64_t rows, int64_t k0, int tid)
+    {
+        const int64_t gr = r0 + (tid >> 2);
+        const bf16* base = P + gr * ld + k0 + (tid & 3) * 8;
+        const u4 z = {0u, 0u, 0u, 0u};
+#define CFMI_B3_LR(u, vv) \
+    vv = (gr + (u % 2) * 64 < rows) ? *reinterpret_cast<const u4*>(base + (u / 2) * ps + (u % 2) * 64 * ld) : z;
+        CFMI_B3_EACH(CFMI_B3_LR)
+#undef CFMI_B3_LR
     }
     __device__ __forceinline__ void load_guarded(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
                                                  int64_t rows, int64_t k0, int64_t kend, int tid)
@@ -155,13 +171,20 @@ __device__ __forceinline__ void slab_mfma(const bf16 (*As)[TM][LROW], const bf16
     }
 }
 
-template <bool FAST>
+// Staging variants: FAST (whole tile in range), ROWS (whole vectors, rows
+// predicated), ELEM (element-wise: partial last slab or unaligned operands).
+enum { FAST = 0, ROWS = 1, ELEM = 2 };
+
+template <int LD>
 __device__ __forceinline__ void fetch(const B3Args& g, const bf16* A, const bf16* B, int64_t m0, int64_t n0,
                                       int64_t k0, int64_t kend, Stage& sa, Stage& sb, int tid)
 {
-    if constexpr (FAST) {
+    if constexpr (LD == FAST) {
         sa.load_fast(A, g.lda, g.pA, m0, k0, tid);
         sb.load_fast(B, g.ldb, g.pB, n0, k0, tid);
+    } else if constexpr (LD == ROWS) {
+        sa.load_rows(A, g.lda, g.pA, m0, g.M, k0, tid);
+        sb.load_rows(B, g.ldb, g.pB, n0, g.N, k0, tid);
     } else {
         sa.load_guarded(A, g.lda, g.pA, m0, g.M, k0, kend, tid);
         sb.load_guarded(B, g.ldb, g.pB, n0, g.N, k0, kend, tid);
@@ -169,22 +192,21 @@ __device__ __forceinline__ void fetch(const B3Args& g, const bf16* A, const bf16
 }
 
 // K range [kbeg, kend) in 32-deep slabs: the next slab's global loads are in
-// flight (registers) while the current one is multiplied from LDS.  FAST:
-// whole 16-byte vectors, rows and K all in range (no per-element guards).
-template <bool FAST>
+// flight (registers) while the current one is multiplied from LDS.
+template <int LD>
 __device__ __forceinline__ void k_loop(const B3Args& g, const bf16* A, const bf16* B, int64_t m0, int64_t n0,
                                        int64_t kbeg, int64_t kend, bf16 (*As)[TM][LROW], bf16 (*Bs)[TN][LROW],
                                        f4 (&acc)[4][4], int tid, int wm, int wn, int lm, int lk)
 {
     Stage sa, sb;
-    fetch<FAST>(g, A, B, m0, n0, kbeg, kend, sa, sb, tid);
+    fetch<LD>(g, A, B, m0, n0, kbeg, kend, sa, sb, tid);
     int64_t k0 = kbeg;
     for (; k0 + TK < kend; k0 += TK) {
         __syncthreads();
         sa.store(As, tid);
         sb.store(Bs, tid);
         __syncthreads();
-        fetch<FAST>(g, A, B, m0, n0, k0 + TK, kend, sa, sb, tid);
+        fetch<LD>(g, A, B, m0, n0, k0 + TK, kend, sa, sb, tid);
         slab_mfma(As, Bs, acc, wm, wn, lm, lk);
     }
     __syncthreads();
@@ -200,12 +222,13 @@ __global__ __launch_bounds__(NTH, 2) void gemm_b3_kernel(B3Args g)
     __shared__ __attribute__((aligned(16))) bf16 As[3][TM][LROW];
     __shared__ __attribute__((aligned(16))) bf16 Bs[3][TN][LROW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t z = blockIdx.z;
     int64_t ti, tj;
     tile_order(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
     const int64_t m0 = ti * TM, n0 = tj * TN;
     if (MODE == 2 && g.col0 + n0 > g.row0 + m0 + TM - 1) return;  // tile above the diagonal
-    const bf16* A = g.A + blockIdx.z * g.sA;
-    const bf16* B = g.B + blockIdx.z * g.sB;
+    const bf16* A = g.A + z * g.sA;
+    const bf16* B = g.B + z * g.sB;
     int64_t kend = g.K;
     if (g.tri & 1) kend = kend < m0 + TM + g.offA ? kend : m0 + TM + g.offA;
     if (g.tri & 4) kend = kend < n0 + TN + g.offB ? kend : n0 + TN + g.offB;
@@ -218,15 +241,18 @@ __global__ __launch_bounds__(NTH, 2) void gemm_b3_kernel(B3Args g)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-    const bool fast = g.vecA && g.vecB && m0 + TM <= g.M && n0 + TN <= g.N;
-    const int64_t kf = fast ? kend / TK * TK : 0;
-    if (kf > 0) k_loop<true>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
-    if (kf < kend) k_loop<false>(g, A, B, m0, n0, kf, kend, As, Bs, acc, tid, wm, wn, lm, lk);
+    const bool vec = g.vecA && g.vecB;
+    const int64_t kf = vec ? kend / TK * TK : 0;
+    if (kf > 0) {
+        if (m0 + TM <= g.M && n0 + TN <= g.N) k_loop<FAST>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
+        else k_loop<ROWS>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
+    }
+    if (kf < kend) k_loop<ELEM>(g, A, B, m0, n0, kf, kend, As, Bs, acc, tid, wm, wn, lm, lk);
 
     // Output lane layout (measured on gfx950): row 4 (lane / 16) + r, column lane % 16.
     float* Cf = (float*)g.C;
     bf16* Cb = (bf16*)g.C;
-    const int64_t zoff = blockIdx.z * g.sC;
+    const int64_t zoff = z * g.sC;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -352,6 +378,7 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.tm = (m + TM - 1) / TM; g.tn = (n + TN - 1) / TN;
     g.vecA = aligned16(A, lda, pA) && (sA % 8 == 0);
     g.vecB = aligned16(B, ldb, pB) && (sB % 8 == 0);
+    if (g.tm * g.tn > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
     dim3 grid((unsigned)(g.tm * g.tn), 1, (unsigned)batch);
     if (out_mode == 0) hipLaunchKernelGGL((gemm_b3_kernel<0>), grid, dim3(NTH), 0, c->stream, g);
     else if (out_mode == 1) hipLaunchKernelGGL((gemm_b3_kernel<1>), grid, dim3(NTH), 0, c->stream, g);

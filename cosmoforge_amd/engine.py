@@ -33,6 +33,7 @@ from .layout import Layout
 
 F64 = _lib.CFMI_F64
 F32 = _lib.CFMI_F32
+BF16X3 = "bf16x3"   # stack dtype marker: three BF16 planes x = x0 + x1 + x2 (int16 storage)
 
 
 class LinAlgError(ValueError):
@@ -191,11 +192,16 @@ class Block:
     ma: int
     rb: int
     mb: int
-    stack: torch.Tensor | None = None  # (nbins, mb, ma) col-major blocks
+    stack: torch.Tensor | None = None   # (nbins, mb, ma) col-major blocks; BF16X3: (3, nbins, mb, ma)
+    mirror: torch.Tensor | None = None  # BF16X3 only: (3, nbins, ma, mb), the transposed blocks
 
     @property
     def sym(self) -> bool:
         return self.ra == self.rb and self.ma == self.mb
+
+    @property
+    def nbins(self) -> int:
+        return self.stack.shape[1] if self.stack.dim() == 4 else self.stack.shape[0]
 
 
 def block_geometry(layout: Layout) -> list:
@@ -229,7 +235,7 @@ def deriv_state(ctx: Context, layout: Layout, group_spectrum: int) -> torch.Tens
 
 def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: int, bins: list,
                  dtype=torch.float64, only=None, bin_range=None, state: torch.Tensor | None = None,
-                 lresume: int = 0, pool: torch.Tensor | None = None) -> list:
+                 lresume: int = 0, pool: torch.Tensor | None = None, mirrors: bool = False) -> list:
     """For every spectrum of ``layout.spectra_list()``, every bin's D block.
 
     ``weights``: label -> device l-indexed w_l B1 B2 (lmax + 1).  ``only``:
@@ -240,13 +246,22 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
     ``state`` (``deriv_state``, one recurrence group per call): the pass
     saves its final recurrence state there and, with ``lresume`` > 0,
     resumes at l = lresume from it.  ``pool``: a flat ``dtype`` buffer the
-    stacks are carved from (instead of fresh allocations)."""
+    stacks are carved from (instead of fresh allocations).
+
+    ``dtype=BF16X3``: every block as three BF16 planes (``split_planes``),
+    shape (3, nbins, mb, ma) on an int16 pool; with ``mirrors`` the blocks
+    of a != b spectra also get their transposes (``Block.mirror``), which
+    the banded three-piece whitening multiplies by."""
     b0, b1 = bin_range if bin_range is not None else (0, len(bins))
     sel = bins[b0:b1]
     lo = ctx.upload(np.array([b[0] for b in sel], dtype=np.int32), dtype=torch.int32)
     hi = ctx.upload(np.array([b[1] for b in sel], dtype=np.int32), dtype=torch.int32)
     nb = len(sel)
-    dcode = F64 if dtype == torch.float64 else F32
+    b3 = isinstance(dtype, str)
+    if b3 and dtype != BF16X3:
+        raise ValueError(f"unknown stack dtype {dtype!r}")
+    dcode = _lib.CFMI_BF16X3 if b3 else (F64 if dtype == torch.float64 else F32)
+    tdt = torch.int16 if b3 else dtype
     lib, h = ctx.lib, ctx.h
     specs = layout.spectra_list()
     geo = block_geometry(layout)
@@ -256,8 +271,10 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
     used = [0]
 
     def alloc(shape):
+        if b3:
+            shape = (3,) + tuple(shape)
         if pool is None:
-            return torch.empty(shape, dtype=dtype, device=ctx.device)
+            return torch.empty(shape, dtype=tdt, device=ctx.device)
         k = int(np.prod(shape))
         if used[0] + k > pool.numel():
             raise MemoryError("derivative stack chunk exceeds its pool")
@@ -277,10 +294,13 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
                                                   fi.rows, fi.rows * fi.rows, None, 0, 0, P(state), lresume),
                           "deriv_stack_00")
             else:
+                mr = alloc((nb, blk.ma, blk.mb)) if (b3 and mirrors) else None
                 ctx.check(lib.cfmi_deriv_stack_00(h, dcode, P(weights[sp.label]), lmax, P(lo), P(hi), nb,
                                                   P(vecs[sp.fj]), fj.npix, P(vecs[sp.fi]), fi.npix, 0, P(st),
-                                                  fj.rows, fi.rows * fj.rows, None, 0, 0, P(state), lresume),
+                                                  fj.rows, fi.rows * fj.rows, P(mr), blk.mb if mr is not None else 0,
+                                                  blk.ma * blk.mb if mr is not None else 0, P(state), lresume),
                           "deriv_stack_00")
+                blk.mirror = mr
             blk.stack = st
             done.add(k)
         elif fi.spin == 2:
@@ -302,17 +322,24 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
         else:
             members = [m for m, s2 in enumerate(specs) if s2.fi == sp.fi and s2.fj == sp.fj]
             sts = [None, None]
+            mrs = [None, None]
             for m in members:
                 if m in want:
                     sts[specs[m].mode] = alloc((nb, fi.rows, fj.rows))
+                    if b3 and mirrors:
+                        mrs[specs[m].mode] = alloc((nb, fj.rows, fi.rows))
             labs = [layout.spectra_map[(sp.fi, sp.fj, md)] for md in range(2)]
+            has_m = mrs[0] is not None or mrs[1] is not None
             ctx.check(lib.cfmi_deriv_stack_02(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]), lmax, P(lo),
                                               P(hi), nb, P(vecs[sp.fi]), fi.npix, P(vecs[sp.fj]), fj.npix,
-                                              P(sts[0]), P(sts[1]), fj.rows, fi.rows * fj.rows, None, None,
-                                              0, 0, P(state), lresume), "deriv_stack_02")
+                                              P(sts[0]), P(sts[1]), fj.rows, fi.rows * fj.rows, P(mrs[0]),
+                                              P(mrs[1]), fi.rows if has_m else 0,
+                                              fi.rows * fj.rows if has_m else 0, P(state), lresume),
+                      "deriv_stack_02")
             for m in members:
                 if m in want:
                     out[m].stack = sts[specs[m].mode]
+                    out[m].mirror = mrs[specs[m].mode]
                     done.add(m)
     return out
 
@@ -487,6 +514,123 @@ def whiten_band(ctx: Context, W: torch.Tensor, n: int, blk: Block, X: torch.Tens
             sC=rj * ma, batch=nbb, flags=1 | fl, tri_a=j0 - rb)
     gemm_ex(ctx, False, True, rj, r1 - ra, ma, 1.0, Z2, 0, rj, W, ra + ra * n, n, 1.0, X, 0, sA=rj * ma, sC=sX,
             batch=nbb, flags=4 | fl, out_mode=2, row0=j0, col0=ra, pbase=pbase, pdiag=SQRT_HALF)
+    return zbuf
+
+
+# ---------------------------------------------------------------------------
+# Three-piece BF16 operands: FP32-accurate products on the BF16 MFMA
+# ---------------------------------------------------------------------------
+
+def planes_ld(cols: int) -> int:
+    """Row pitch of three-piece planes (a multiple of 8 elements, so rows
+    start on 16-byte boundaries)."""
+    return (cols + 7) // 8 * 8
+
+
+def split_planes(ctx: Context, A: torch.Tensor, rows: int, cols: int, lda: int) -> torch.Tensor:
+    """Column-major FP64/FP32 (rows x cols, ld lda) -> (3, rows, planes_ld(cols))
+    int16 tensor of BF16 pieces, row-major: [p, i, j] is piece p of A[i, j]."""
+    ld = planes_ld(cols)
+    out = torch.empty((3, rows, ld), dtype=torch.int16, device=ctx.device)
+    ctx.check(ctx.lib.cfmi_split3(ctx.h, dtype_code(A), rows, cols, P(A), lda, P(out), ld, rows * ld), "split3")
+    return out
+
+
+def gemm_b3(ctx: Context, mode: int, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int, sA: int,
+            pA: int, B, offb: int, ldb: int, sB: int, pB: int, beta: float, C, offc: int = 0, ldc: int = 0,
+            sC: int = 0, pC: int = 0, *, batch: int = 1, flags: int = 0, tri_a: int = 0, tri_b: int = 0,
+            row0: int = 0, col0: int = 0, pbase: int = 0, pdiag: float = 1.0):
+    """``cfmi_gemm_b3``: C[m, n] = alpha sum_k A[m][k] B[n][k] (+ beta C) on
+    k-contiguous three-piece operands (offsets and strides in elements,
+    planes pA / pB apart).  mode 0: FP32 column-major C; 1: three-piece
+    row-major C (planes pC apart); 2: FP32 packed lower row band."""
+    if m <= 0 or n <= 0:
+        return
+    ctx.check(ctx.lib.cfmi_gemm_b3(ctx.h, mode, m, n, k, alpha, P(A, offa), lda, sA, pA, tri_a, P(B, offb), ldb,
+                                   sB, pB, tri_b, beta, P(C, offc), ldc, sC, pC, batch, flags, row0, col0, pbase,
+                                   pdiag), "gemm_b3")
+
+
+def _gemm_b3(*a, **k):
+    return gemm_b3(*a, **k)
+
+
+def _plane_buffer(ctx: Context, k: int) -> torch.Tensor:
+    """Flat storage for k three-piece plane entries (int16 BF16 bits)."""
+    return torch.empty(k, dtype=torch.int16, device=ctx.device)
+
+
+def whiten_band_b3(ctx: Context, Wp: torch.Tensor, n: int, blk: Block, X: torch.Tensor, r0: int, r1: int,
+                   zbuf: torch.Tensor | None = None, zcap: int | None = None,
+                   counter: list | None = None) -> torch.Tensor | None:
+    """``whiten_band`` on three-piece BF16 operands (FP32 accuracy at the
+    BF16 MFMA rate): ``Wp`` = ``split_planes`` of W (3, n, ldw);
+    ``blk.stack`` (3, nbins, mb, ma) planes of the column-major D blocks,
+    which are k-contiguous for W D as they stand; ``blk.mirror`` holds
+    D^T for a != b when the band is not all of [0, n) (W D^T has D^T as
+    its k-contiguous operand).  The products W D (W D^T) are stored as
+    three-piece row-major Z, the operand of the band product, and ``X``
+    (FP32) receives the packed band as in ``whiten_band``."""
+    D = blk.stack
+    nbb = D.shape[1]
+    ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
+    if zcap is not None:
+        rows = r1 - min(max(r0, ra), max(r0, rb))
+        step = max(1, zcap // (max(rows, 1) * max(ma, mb)))
+        if step < nbb:
+            for b0 in range(0, nbb, step):
+                Mi = blk.mirror[:, b0:b0 + step] if blk.mirror is not None else None
+                sub = Block(ra, ma, rb, mb, D[:, b0:b0 + step], Mi)
+                zbuf = whiten_band_b3(ctx, Wp, n, sub, X[b0:b0 + step], r0, r1, zbuf, counter=counter)
+            return zbuf
+    ldw, pW = Wp.shape[2], Wp.stride(0)
+    sX = X.stride(0)
+    pbase = r0 * (r0 + 1) // 2
+    full = r0 == 0 and r1 == n
+
+    def g(*a, **k):
+        if counter is not None:
+            counter.append(1)
+        _gemm_b3(ctx, *a, **k)
+
+    def need(k):
+        nonlocal zbuf
+        if zbuf is None or zbuf.numel() < 3 * k:
+            zbuf = None
+            zbuf = _plane_buffer(ctx, 3 * k)
+        return zbuf
+
+    # term 1: Z = W[i0:r1, ra:ra+ma] D (rr x mb), X[i, c] += sum_k Z[i, k] W[c, rb + k]
+    i0 = max(r0, ra)
+    rr = r1 - i0
+    if rr > 0:
+        Z = need(rr * mb * nbb)
+        pZ = rr * mb * nbb
+        g(1, rr, mb, ma, 1.0, Wp, i0 * ldw + ra, ldw, 0, pW, D, 0, ma, D.stride(1), D.stride(0), 0.0, Z, 0, mb,
+          rr * mb, pZ, batch=nbb, flags=1, tri_a=i0 - ra)
+        g(2, rr, r1 - rb, mb, 1.0, Z, 0, mb, rr * mb, pZ, Wp, rb * ldw + rb, ldw, 0, pW, 1.0, X, 0, 0, sX, 0,
+          batch=nbb, flags=4, row0=i0, col0=rb, pbase=pbase, pdiag=SQRT_HALF)
+    if blk.sym:
+        return zbuf
+    # term 2: X[i, c] += sum_k' (W[i, rb:] D^T)[k'] W[c, ra + k']
+    if full and rr > 0:
+        # Z holds all rows of W[:, a] D: X[i, c] += sum_k W[i, rb + k] Z[c - ra, k]
+        g(2, n - rb, n - ra, mb, 1.0, Wp, rb * ldw + rb, ldw, 0, pW, Z, 0, mb, rr * mb, pZ, 1.0, X, 0, 0, sX, 0,
+          batch=nbb, flags=1, row0=rb, col0=ra, pbase=pbase, pdiag=SQRT_HALF)
+        return zbuf
+    j0 = max(r0, rb)
+    rj = r1 - j0
+    if rj <= 0 or r1 <= ra:
+        return zbuf
+    Dt = blk.mirror
+    if Dt is None:
+        raise ValueError("whiten_band_b3: a != b block over a partial band needs the transposed stack (mirrors)")
+    Z2 = need(rj * ma * nbb)
+    pZ2 = rj * ma * nbb
+    g(1, rj, ma, mb, 1.0, Wp, j0 * ldw + rb, ldw, 0, pW, Dt, 0, mb, Dt.stride(1), Dt.stride(0), 0.0, Z2, 0, ma,
+      rj * ma, pZ2, batch=nbb, flags=1, tri_a=j0 - rb)
+    g(2, rj, r1 - ra, ma, 1.0, Z2, 0, ma, rj * ma, pZ2, Wp, ra * ldw + ra, ldw, 0, pW, 1.0, X, 0, 0, sX, 0,
+      batch=nbb, flags=4, row0=j0, col0=ra, pbase=pbase, pdiag=SQRT_HALF)
     return zbuf
 
 

@@ -5,8 +5,9 @@ Constructor kwargs and getters follow ``src/cosmoforge.qube/qube/fisher.py``
 F_ij = 1/2 Tr[C^-1 dC_i C^-1 dC_j] over parameters ordered spectrum-major
 (``spectra_list`` order, then bins), beam-smoothed as the reference's.
 All device work runs in libcfmi on one GPU.  ``precision="mixed"`` (no
-reference counterpart) runs the derivative stacks, the whitener and the
-whitening products in FP32 on MFMA, with FP64 recurrences, factorisation,
+reference counterpart) runs the whitening products at FP32 accuracy on the
+matrix cores (``mixed_gemm="bf16x3"``: three-piece BF16 operands, the
+default; ``"fp32"``: FP32 MFMA), with FP64 recurrences, factorisation,
 FP64-accumulated Gram matrix and FP64 F^-1.
 """
 
@@ -55,6 +56,9 @@ class Fisher:
         self.timings = {}
         self.timer = kwargs.get("timer")
         self.launch_timer = kwargs.get("launch_timer")
+        self.mixed_gemm = kwargs.get("mixed_gemm", "bf16x3")
+        if self.mixed_gemm not in pipeline.MIXED_GEMMS:
+            raise ValueError(f"mixed_gemm must be one of {pipeline.MIXED_GEMMS}; got {self.mixed_gemm!r}")
 
     # -- pipeline -----------------------------------------------------------
     @property
@@ -98,7 +102,7 @@ class Fisher:
         t1 = time.perf_counter()
         self.fisher, self.derivs = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
                                                    timer=self.timer, precision=self.precision,
-                                                   launch_timer=self.launch_timer)
+                                                   launch_timer=self.launch_timer, mixed_gemm=self.mixed_gemm)
         self.ctx.sync()
         self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
 

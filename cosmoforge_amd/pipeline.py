@@ -38,6 +38,7 @@ class Covariance:
         self.S = None
         self.factor = None
         self._w32 = None
+        self._wp = None
 
     def signal(self, cls=None) -> torch.Tensor:
         ctx = self.ctx
@@ -55,6 +56,7 @@ class Covariance:
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
         self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener)
         self._w32 = None
+        self._wp = None
         return self.factor
 
     def inverse(self) -> torch.Tensor:
@@ -176,12 +178,30 @@ def whitener(cov: Covariance, precision: str) -> torch.Tensor:
     return cov._w32
 
 
+def whitener_planes(cov: Covariance) -> torch.Tensor:
+    """W as three-piece BF16 row-major planes (3, n, ldw), split once from
+    FP64 (each entry to 2^-26 relative)."""
+    if getattr(cov, "_wp", None) is None:
+        cov._wp = engine.split_planes(cov.ctx, cov.factor.W, cov.n, cov.n, cov.n)
+    return cov._wp
+
+
+MIXED_GEMMS = ("bf16x3", "fp32")
+
+
+def _stack_bytes(dtype) -> int:
+    if dtype == engine.BF16X3:
+        return 6
+    return 4 if dtype == torch.float32 else 8
+
+
 class DerivativeSource:
     """The binned derivative stacks of one (Covariance, bins) pair: kept on
     the device when they fit, regenerated spectrum by spectrum (and in bin
     chunks) otherwise."""
 
-    def __init__(self, cov: Covariance, bins, convention: str, dtype, budget: int | None = None):
+    def __init__(self, cov: Covariance, bins, convention: str, dtype, budget: int | None = None,
+                 mirrors: bool = False):
         self.cov, self.bins, self.dtype, self.convention = cov, bins, dtype, convention
         inp = cov.inp
         for lo, hi in bins:
@@ -190,11 +210,19 @@ class DerivativeSource:
         w = parameter_weights(inp, bins, convention)
         self.weights = {k: cov.ctx.upload(v) for k, v in w.items()}
         self.geometry = engine.block_geometry(inp.layout)
-        es = 4 if dtype == torch.float32 else 8
-        self.bytes_per_bin = [g.ma * g.mb * es for g in self.geometry]
-        self.total_bytes = sum(self.bytes_per_bin) * len(bins)
         self.resident = None
         self.budget = budget
+        self.set_mirrors(mirrors)
+
+    def set_mirrors(self, on: bool):
+        """Three-piece stacks: also produce D^T of the a != b blocks (the
+        banded whitening's operand); counted in the byte totals."""
+        self.mirrors = bool(on) and self.dtype == engine.BF16X3
+        self.resident = None
+        es = _stack_bytes(self.dtype)
+        self.bytes_per_bin = [g.ma * g.mb * es * (2 if (self.mirrors and not g.sym) else 1)
+                              for g in self.geometry]
+        self.total_bytes = sum(self.bytes_per_bin) * len(self.bins)
 
     def fits(self, budget: int) -> bool:
         return self.total_bytes <= budget
@@ -208,7 +236,7 @@ class DerivativeSource:
         inp = self.cov.inp
         return engine.deriv_stacks(self.cov.ctx, inp.layout, self.cov.vecs, self.weights, inp.lmax_signal,
                                    self.bins, dtype=self.dtype, only=only, bin_range=bin_range, state=state,
-                                   lresume=lresume, pool=pool)
+                                   lresume=lresume, pool=pool, mirrors=self.mirrors)
 
     def chunks(self, chunk_budget: int):
         """Yield (spectrum index, first bin, Block with stack) over all
@@ -228,13 +256,14 @@ class DerivativeSource:
             g = [m for m, s2 in enumerate(specs) if (s2.fi, s2.fj) == (sp.fi, sp.fj)]
             seen.update(g)
             groups.append(g)
-        es = 4 if self.dtype == torch.float32 else 8
         plan = []
         for g in groups:
             per_bin = sum(self.bytes_per_bin[m] for m in g)
             plan.append((g, max(1, min(nb, chunk_budget // max(per_bin, 1))), per_bin))
         # One pool for every chunk (no per-chunk allocations of tens of GB).
-        pool = torch.empty(max(cb * pb for _, cb, pb in plan) // es, dtype=self.dtype, device=self.cov.ctx.device)
+        pdt = torch.int16 if self.dtype == engine.BF16X3 else self.dtype
+        pes = torch.empty((), dtype=pdt).element_size()
+        pool = torch.empty(max(cb * pb for _, cb, pb in plan) // pes, dtype=pdt, device=self.cov.ctx.device)
         for g, cb, _ in plan:
             # Consecutive bin chunks continue one recurrence pass.
             state = engine.deriv_state(self.cov.ctx, self.cov.inp.layout, g[0]) if cb < nb else None
@@ -280,44 +309,55 @@ class LaunchTimer:
         return sum(a.elapsed_time(b) for a, b in self.pairs)
 
 
-def plan_memory(n: int, npar: int, es: int, geometry, nbins: int, stack_bytes: int, free: int):
+def plan_memory(n: int, npar: int, es: int, geometry, nbins: int, stack_bytes: int, free: int,
+                es_z: int | None = None):
     """(bands, stacks_resident, chunk_budget, z_budget) for the banded Fisher.
 
     One band holds npar packed row bands of es-byte entries; the derivative
     stacks stay resident when they fit beside it, otherwise they are
-    regenerated per band in chunks of at most ``chunk_budget`` bytes."""
+    regenerated per band in chunks of at most ``chunk_budget`` bytes.  The
+    intermediate products W D take es_z bytes per entry (default es)."""
+    es_z = es if es_z is None else es_z
     maxw = max(max(g.ma, g.mb) for g in geometry)
-    zfull = max((n - min(g.ra, g.rb)) * maxw for g in geometry) * nbins * es
+    zfull = max((n - min(g.ra, g.rb)) * maxw for g in geometry) * nbins * es_z
     if stack_bytes + npar * es * band_size(0, n) + zfull <= free:
         return [(0, n)], True, stack_bytes, zfull
     resident = stack_bytes <= free // 4
     chunk = stack_bytes if resident else free // 4
-    per_bin = max(g.ma * g.mb for g in geometry) * es * 3
-    cbins = nbins if resident else max(1, min(nbins, chunk // per_bin))
     zres = free // 16
     bands = plan_bands(n, npar, es, free - chunk - zres)
     h = max(r1 - r0 for r0, r1 in bands)
-    if h * maxw * es > zres:          # one bin of the tallest band must fit
-        zres = h * maxw * es
+    if h * maxw * es_z > zres:        # one bin of the tallest band must fit
+        zres = h * maxw * es_z
         bands = plan_bands(n, npar, es, free - chunk - zres)
     return bands, resident, chunk, zres
 
 
 def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None,
            precision: str = "fp64", keep_stacks: bool = True, mem_fraction: float = 0.8,
-           launch_timer: LaunchTimer | None = None, mem_budget: int | None = None, progress=None):
+           launch_timer: LaunchTimer | None = None, mem_budget: int | None = None, progress=None,
+           mixed_gemm: str = "bf16x3"):
     """(F, derivative source): F is (n_p, n_p) on the host.
 
     F_ij = 1/2 <W dC_i W^T, W dC_j W^T>, accumulated over row bands of the
     packed lower triangles (one band when everything fits on the device).
-    ``precision="mixed"``: derivative stacks, W and the whitened derivatives
-    in FP32, FP32 MFMA products accumulated in FP64 across 256-term chunks,
-    the Gram matrix in FP64; recurrences, factorisation and F^-1 stay FP64."""
+    ``precision="mixed"``: the whitening products at FP32 accuracy, the
+    whitened derivatives stored in FP32, their Gram matrix from FP32
+    products accumulated in FP64 across 256-term chunks; recurrences,
+    factorisation and F^-1 stay FP64.  ``mixed_gemm`` picks the whitening
+    arithmetic: "bf16x3" (default: W and D as three BF16 pieces, six BF16
+    MFMA products per tile, ``cfmi_gemm_b3``) or "fp32" (W and D rounded to
+    FP32, FP32 MFMA)."""
     ctx, n = cov.ctx, cov.n
     T = _dtype(precision)
+    if mixed_gemm not in MIXED_GEMMS:
+        raise ValueError(f"mixed_gemm must be one of {MIXED_GEMMS}; got {mixed_gemm!r}")
+    b3 = precision == "mixed" and mixed_gemm == "bf16x3"
     es = 4 if T == torch.float32 else 8
-    W = whitener(cov, precision)
-    src = DerivativeSource(cov, bins, convention, T)
+    W = whitener_planes(cov) if b3 else whitener(cov, precision)
+    sdt = engine.BF16X3 if b3 else T
+    es_z = _stack_bytes(sdt)
+    src = DerivativeSource(cov, bins, convention, sdt)
     nb = len(bins)
     npar = len(src.geometry) * nb
     # Whitening products (K <= a few 10^4, terms of both signs) accumulate in
@@ -327,7 +367,12 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     # across 256-term chunks.
     flags = 0
     free = int(mem_budget) if mem_budget is not None else int(_free_bytes(ctx) * mem_fraction)
-    bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes, free)
+    bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes, free,
+                                                        es_z)
+    if b3 and len(bands) > 1 and not all(g.sym for g in src.geometry):
+        src.set_mirrors(True)             # partial bands multiply the a != b blocks by W D^T
+        bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes,
+                                                            free, es_z)
     if timer:
         timer.mark("deriv0")
     if resident:
@@ -344,12 +389,15 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
         if timer:
             timer.mark("white0")
         for s, b0, blk in src.chunks(chunk_budget):
-            k = blk.stack.shape[0]
+            k = blk.nbins
             Xs = X[s * nb + b0: s * nb + b0 + k]
             if launch_timer is not None:
                 launch_timer.start()
-            zbuf = engine.whiten_band(ctx, W, n, blk, Xs, r0, r1, zbuf, flags, zbytes // es,
-                                      counter=launch_timer.launches if launch_timer is not None else None)
+            cnt = launch_timer.launches if launch_timer is not None else None
+            if b3:
+                zbuf = engine.whiten_band_b3(ctx, W, n, blk, Xs, r0, r1, zbuf, zbytes // es_z, counter=cnt)
+            else:
+                zbuf = engine.whiten_band(ctx, W, n, blk, Xs, r0, r1, zbuf, flags, zbytes // es_z, counter=cnt)
             if launch_timer is not None:
                 launch_timer.stop()
             del blk, Xs                  # the next chunk is allocated before the loop variable is rebound

@@ -308,3 +308,49 @@ def test_deriv_stacks_chunked_resume_and_fp32(ctx):
     f32 = pipeline.DerivativeSource(cov, bins, "Cl", torch.float32).make_resident()
     for a, b in zip(ref, f32):
         assert torch.equal(a.stack.float(), b.stack)
+
+
+def test_deriv_stacks_bf16x3_and_mirrors(ctx):
+    """Three-piece BF16 stacks: the joined pieces equal the FP64 stacks to
+    2^-25 relative (three BF16 pieces carry 24 bits; above an FP64-noise
+    floor), and the transposed
+    stacks of the a != b blocks (spin-0 cross fields in TEB, TE/TB in TQU)
+    hold the same pieces transposed."""
+    import torch
+    from cosmoforge_amd import engine, pipeline
+    from cosmoforge_amd.core import Inputs, bins_from_params
+    from tests import cases
+
+    for name in ("TEB", "TQU"):
+        case = cases.case_fisher_nside4(name)
+        p = dict(nside=case.nside, spins=case.spins, labels=case.labels, lmin=case.lmin, lmax=case.lmax,
+                 lmax_signal=case.lmax_signal, ordering="NESTED" if case.nest else "RING")
+        inp = Inputs(p, case.mask, case.noise, case.cls, case.beam)
+        cov = pipeline.Covariance(ctx, inp)
+        bins = bins_from_params(inp.params)
+        ref = pipeline.DerivativeSource(cov, bins, "Cl", torch.float64).make_resident()
+        b3 = pipeline.DerivativeSource(cov, bins, "Cl", engine.BF16X3, mirrors=True).make_resident()
+        nmirror = 0
+        for a, b in zip(ref, b3):
+            nb, mb, ma = a.stack.shape
+            assert b.stack.shape == (3, nb, mb, ma)
+            cnt = nb * mb * ma
+            j = torch.empty(cnt, dtype=torch.float64, device=ctx.device)
+            ctx.check(ctx.lib.cfmi_join3(ctx.h, cnt, engine.P(b.stack), b.stack.stride(0), engine.P(j)), "join3")
+            x = a.stack.reshape(-1)
+            # The FP64 and three-piece kernels are separate instantiations: FMA
+            # contraction may differ at FP64 rounding (entries that cancel in the
+            # spin-2 rotation), hence the 2^-40 max|x| floor.
+            err = (j - x).abs()
+            bound = 2.0 ** -25 * x.abs() + 2.0 ** -40 * x.abs().max()
+            bad = err > bound
+            if bad.any():
+                print(name, "worst:", [(float(x[i]), float(j[i])) for i in torch.nonzero(bad)[:5, 0]])
+            assert not bad.any(), name
+            if a.sym:
+                assert b.mirror is None
+            else:
+                nmirror += 1
+                assert b.mirror.shape == (3, nb, ma, mb)
+                assert torch.equal(b.mirror, b.stack.transpose(2, 3)), name
+        assert nmirror == sum(1 for g in engine.block_geometry(inp.layout) if not g.sym) > 0

@@ -210,3 +210,34 @@ def test_mixed_precision_fisher_qml_close_to_fp64():
         assert errs["F"] < 1e-6 and errs["nb"] < 1e-12 and errs["q"] < 1e-9
         if errs["condF"] < 1e8:   # the nside-4 TEB F is numerically singular (cond ~ 1e16)
             assert errs["C"] < errs["condF"] * errs["F"]
+
+
+def test_mixed_whitening_engines_and_bands():
+    """Mixed Fisher on nside-4 TQU (TE/TB blocks) with the three-piece BF16
+    whitening in one band and in several (transposed stacks, regenerated per
+    band), and with the FP32 MFMA whitening: each within 1e-6 of FP64, the
+    banded three-piece result equal to the single-band one to FP32 rounding
+    of the summation order."""
+    from cosmoforge_amd import Fisher, pipeline
+    from cosmoforge_amd.core import bins_from_params
+
+    case = cases.case_fisher_nside4("TQU")
+    f = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam)
+    f.prepare_covariance_matrices()
+    bins = bins_from_params(f.params)
+    conv = f.params.output_convention
+    n, npar = f.cov.n, 6 * len(bins)
+    F64, _ = pipeline.fisher(f.cov, bins, conv)
+    Fb, _ = pipeline.fisher(f.cov, bins, conv, precision="mixed")
+    Ff, _ = pipeline.fisher(f.cov, bins, conv, precision="mixed", mixed_gemm="fp32")
+    timer = pipeline.StageTimer()
+    budget = 3 * npar * 4 * (n * (n + 1) // 2) // 10
+    Fbb, src = pipeline.fisher(f.cov, bins, conv, precision="mixed", timer=timer, mem_budget=budget)
+    assert len(timer.bands) > 1 and src.mirrors
+    errs = dict(b3=_rel(Fb, F64), fp32=_rel(Ff, F64), b3_bands=_rel(Fbb, F64), bands_vs_one=_rel(Fbb, Fb))
+    print("mixed whitening vs fp64:", errs)
+    assert errs["b3"] < 1e-6 and errs["fp32"] < 1e-6 and errs["b3_bands"] < 1e-6
+    assert errs["bands_vs_one"] < 1e-6
+    with pytest.raises(ValueError, match="mixed_gemm"):
+        pipeline.fisher(f.cov, bins, conv, precision="mixed", mixed_gemm="fp16")

@@ -168,3 +168,108 @@ def test_whitening_flops_brute_force(bands):
     blocks = [engine.Block(0, nf, 0, nf), engine.Block(nf, 2 * nf, nf, 2 * nf), engine.Block(nf, 2 * nf, 0, nf)]
     want = sum(_brute_flops(n, b, 3, bands or [(0, n)]) for b in blocks)
     assert pipeline.whitening_flops(n, blocks, 3, bands) == pytest.approx(want, rel=1e-12)
+
+
+# --- three-piece BF16 whitening (engine.whiten_band_b3) ----------------------
+
+FRAC = (0.5, 0.3, 0.2)   # the emulated "pieces" of a value: they sum to it
+
+
+def _rows(flat, off, ld, ps, rows, cols):
+    """k-contiguous rows x cols operand: sum over the three planes of
+    flat[off + p ps + r ld + c]."""
+    idx = off + np.arange(rows)[:, None] * ld + np.arange(cols)[None, :]
+    return flat[idx] + flat[idx + ps] + flat[idx + 2 * ps]
+
+
+def _flat(t):
+    """The storage from t's first element on, as the device pointer P(t) sees it
+    (strided views included)."""
+    t = t.detach()
+    total = t.untyped_storage().nbytes() // t.element_size() - t.storage_offset()
+    return torch.as_strided(t, (total,), (1,)).numpy()
+
+
+def fake_gemm_b3(ctx, mode, m, n, k, alpha, A, offa, lda, sA, pA, B, offb, ldb, sB, pB, beta, C, offc=0, ldc=0,
+                 sC=0, pC=0, *, batch=1, flags=0, tri_a=0, tri_b=0, row0=0, col0=0, pbase=0, pdiag=1.0):
+    """NumPy stand-in for cfmi_gemm_b3 on float64 'planes'; asserts that the
+    K-trims skip only zeros."""
+    if m <= 0 or n <= 0:
+        return
+    a_flat, b_flat, c_flat = _flat(A), _flat(B), _flat(C)
+    assert lda >= k and ldb >= k
+    for z in range(batch):
+        Am = _rows(a_flat, offa + z * sA, lda, pA, m, k)
+        Bm = _rows(b_flat, offb + z * sB, ldb, pB, n, k)
+        if flags & 1:
+            assert np.all(Am[np.arange(k)[None, :] > np.arange(m)[:, None] + tri_a] == 0), "A trim skips nonzeros"
+        if flags & 4:
+            assert np.all(Bm[np.arange(k)[None, :] > np.arange(n)[:, None] + tri_b] == 0), "B trim skips nonzeros"
+        prod = alpha * (Am @ Bm.T)
+        if mode == 0:
+            for j in range(n):
+                base = offc + z * sC + j * ldc
+                c_flat[base:base + m] = prod[:, j] + (beta * c_flat[base:base + m] if beta else 0.0)
+        elif mode == 1:
+            assert ldc >= n and pC >= m * ldc
+            for r in range(m):
+                base = offc + z * sC + r * ldc
+                for p in range(3):
+                    c_flat[base + p * pC: base + p * pC + n] = FRAC[p] * prod[r]
+        else:
+            for r in range(m):
+                i = row0 + r
+                for cidx in range(n):
+                    c = col0 + cidx
+                    if c > i:
+                        continue
+                    v = prod[r, cidx] * (pdiag if i == c else 1.0)
+                    q = offc + z * sC + i * (i + 1) // 2 + c - pbase
+                    c_flat[q] = v + (beta * c_flat[q] if beta else 0.0)
+
+
+def _as_planes(a):
+    return torch.tensor(np.stack([f * a for f in FRAC]))
+
+
+@pytest.mark.parametrize("bands,zcap", [([(0, 12)], None), ([(0, 5), (5, 12)], None),
+                                        ([(0, 3), (3, 4), (4, 9), (9, 12)], None), ([(0, 5), (5, 12)], 40)])
+def test_whiten_band_b3_matches_direct(monkeypatch, bands, zcap):
+    """Same bands as the FP64 layout test, on the three-piece operand
+    layout: row-major W planes (padded pitch), (3, nbins, mb, ma) stacks,
+    their transposes for the a != b blocks, three-piece Z."""
+    rng = np.random.default_rng(5)
+    n, W, blocks = _problem(rng, 4, 3)
+    monkeypatch.setattr(engine, "_gemm_b3", fake_gemm_b3)
+    monkeypatch.setattr(engine, "_plane_buffer", lambda ctx, k: torch.full((k,), np.nan, dtype=torch.float64))
+    ldw = engine.planes_ld(n)
+    Wpad = np.zeros((n, ldw))
+    Wpad[:, :n] = W
+    Wp = _as_planes(Wpad)                                 # (3, n, ldw)
+    for ra, ma, rb, mb, D in blocks:
+        mirror = _as_planes(np.ascontiguousarray(np.transpose(D, (0, 2, 1))))
+        blk = engine.Block(ra, ma, rb, mb, _as_planes(D), None if (ra == rb and ma == mb) else mirror)
+        got = []
+        for r0, r1 in bands:
+            X = torch.zeros((D.shape[0], engine.band_size(r0, r1)), dtype=torch.float64)
+            engine.whiten_band_b3(FakeCtx(), Wp, n, blk, X, r0, r1, zcap=zcap)
+            got.append(X.numpy())
+        got = np.concatenate(got, axis=1)
+        for b in range(D.shape[0]):
+            want = _direct(n, W, ra, ma, rb, mb, D[b])
+            np.testing.assert_allclose(got[b], want, rtol=1e-12, atol=1e-12)
+
+
+def test_whiten_band_b3_needs_mirror_for_partial_bands(monkeypatch):
+    rng = np.random.default_rng(1)
+    n, W, blocks = _problem(rng, 4, 2)
+    monkeypatch.setattr(engine, "_gemm_b3", fake_gemm_b3)
+    monkeypatch.setattr(engine, "_plane_buffer", lambda ctx, k: torch.zeros(k, dtype=torch.float64))
+    Wp = _as_planes(np.pad(W, ((0, 0), (0, engine.planes_ld(n) - n))))
+    ra, ma, rb, mb, D = blocks[2]
+    blk = engine.Block(ra, ma, rb, mb, _as_planes(D))
+    X = torch.zeros((2, engine.band_size(5, 12)), dtype=torch.float64)
+    with pytest.raises(ValueError, match="transposed stack"):
+        engine.whiten_band_b3(FakeCtx(), Wp, n, blk, X, 5, 12)
+    X = torch.zeros((2, engine.band_size(0, 12)), dtype=torch.float64)
+    engine.whiten_band_b3(FakeCtx(), Wp, n, blk, X, 0, 12)   # one full band reuses W D: no mirror needed
