@@ -380,11 +380,14 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     const bool small = kAcc64 || (M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096);
     const int tbm = small ? BSMALL : BM, tbn = small ? BSMALL : BN;
     int64_t tm = (M + tbm - 1) / tbm, tn = (N + tbn - 1) / tbn;
-    // Split K when the output grid alone cannot fill the 256 CUs.
+    // Split K when the output grid alone cannot fill the 256 CUs: aim at
+    // 2048 working blocks (four rounds of two per CU, so the last round's
+    // imbalance stays small); a symmetric result only works its lower tiles.
     int64_t nz = 1;
     int64_t tiles = tm * tn * batch;
+    const int64_t active = (tri & 64) && tm == tn ? tm * (tm + 1) / 2 * batch : tiles;
     if (K > 4 * BK && tiles < 512 && batch == 1) {
-        nz = std::min<int64_t>((1024 + tiles - 1) / tiles, (K + 255) / 256);
+        nz = std::min<int64_t>((2048 + active - 1) / active, (K + 255) / 256);
         nz = std::max<int64_t>(nz, 1);
         // Bound the partial buffer to 1 GiB.
         while (nz > 1 && (size_t)nz * M * N * sizeof(TE) > ((size_t)1 << 30)) nz = (nz + 1) / 2;
@@ -555,11 +558,38 @@ __global__ void fill_kernel(int64_t m, int64_t n, T* A, int64_t lda, T v, int mo
     }
 }
 
+// Contiguous fill: 16-byte vector stores (the 16-byte-aligned body), scalar
+// head and tail.
+template <typename T>
+__global__ void fill_flat_kernel(int64_t count, T* A, T v)
+{
+    constexpr int PER = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(PER)));
+    const int64_t head = (int64_t)(((16 - (reinterpret_cast<uintptr_t>(A) & 15)) & 15) / sizeof(T));
+    const int64_t h = head < count ? head : count;
+    const int64_t nvec = (count - h) / PER;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+    vec* body = reinterpret_cast<vec*>(A + h);
+    vec w;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) w[e] = v;
+    for (int64_t i = tid; i < nvec; i += step) body[i] = w;
+    const int64_t rest = h + nvec * PER;
+    if (tid < h) A[tid] = v;
+    if (tid < count - rest) A[rest + tid] = v;
+}
+
 template <typename T>
 int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, int mode)
 {
     if (m <= 0 || n <= 0) return CFMI_OK;
     int64_t tot = m * n;
+    if (mode == 0 && (lda == m || n == 1)) {
+        hipLaunchKernelGGL((fill_flat_kernel<T>), elem_grid(tot / (16 / sizeof(T)) + 1), dim3(256), 0, c->stream,
+                           tot, (T*)A, (T)v);
+        CFMI_LAUNCH_CHECK(c);
+        return CFMI_OK;
+    }
     hipLaunchKernelGGL((fill_kernel<T>), elem_grid(tot), dim3(256), 0, c->stream,
                        m, n, (T*)A, lda, (T)v, mode);
     CFMI_LAUNCH_CHECK(c);

@@ -31,8 +31,17 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 128, TN = 128, TK = 32, PADK = 8, NTH = 256;
+constexpr int TK = 32, PADK = 8;
 constexpr int LROW = TK + PADK;  // LDS row pitch in bf16 (80 bytes)
+
+// Block configurations: BT x BT output tiles, 2 BT threads, waves of 64 x BT/2
+// (BT / 64 along M, 2 along N).  BT = 256 halves the operand bytes per flop
+// and gives every wave 4 x 8 MFMA tiles (36 LDS reads per 192 MFMAs); its
+// 120 KB of LDS allow one block per CU (2 waves per SIMD).  BT = 128 (two
+// blocks per CU) serves grids too small to fill the CUs with 256-tiles.
+template <int BT> struct Cfg {
+    static constexpr int NT = 2 * BT, HALF = BT / 2, WMC = BT / 64, FN = BT / 32, MINB = BT == 128 ? 2 : 1;
+};
 
 __device__ __forceinline__ bf16 f2bf(float f)
 {
@@ -100,47 +109,47 @@ __device__ __forceinline__ u4 load16_guarded(const bf16* src, bool row_ok, int64
     return u4{w[0], w[1], w[2], w[3]};
 }
 
-// One 32-deep slab of a 128-row operand tile (3 planes x 128 rows x 4
-// vectors of 8): thread t holds k chunk 8 (t % 4) of rows t / 4 and
-// t / 4 + 64 in each plane.  Six named registers rather than an array, so
-// they never live in scratch memory.
+// One 32-deep slab of a BT-row operand tile (3 planes x BT rows x 4 vectors
+// of 8): thread t holds k chunk 8 (t % 4) of rows t / 4 and t / 4 + BT / 2 in
+// each plane.  Six named registers rather than an array, so they never live
+// in scratch memory.
 #define CFMI_B3_EACH(X) X(0, v0) X(1, v1) X(2, v2) X(3, v3) X(4, v4) X(5, v5)
+template <int BT>
 struct Stage {
-    u4 v0, v1, v2, v3, v4, v5;   // vector u: plane u / 2, row t / 4 + 64 (u % 2)
+    static constexpr int H = Cfg<BT>::HALF;
+    u4 v0, v1, v2, v3, v4, v5;   // vector u: plane u / 2, row t / 4 + H (u % 2)
 
-    __device__ __forceinline__ void load_fast(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
-                                              int64_t k0, int tid)
+    // Whole vectors.  The tile base (block row, k0, plane) is uniform; each
+    // lane adds a 32-bit byte offset for its two rows, which lets every load
+    // use a scalar base address.  Rows beyond ``rows`` are clamped to the last
+    // row: they feed only output rows / columns beyond M / N, which are never
+    // stored, so edge tiles need no predication.
+    __device__ __forceinline__ void load_vec(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
+                                             int64_t rows, int64_t k0, int tid)
     {
-        const bf16* base = P + (r0 + (tid >> 2)) * ld + k0 + (tid & 3) * 8;
-#define CFMI_B3_LF(u, vv) vv = *reinterpret_cast<const u4*>(base + (u / 2) * ps + (u % 2) * 64 * ld);
-        CFMI_B3_EACH(CFMI_B3_LF)
-#undef CFMI_B3_LF
-    }
-    // Whole vectors, rows beyond ``rows`` read as zeros (the slab is inside K).
-    __device__ __forceinline__ void load_rows(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
-                                              int64_t rows, int64_t k0, int tid)
-    {
-        const int64_t gr = r0 + (tid >> 2);
-        const bf16* base = P + gr * ld + k0 + (tid & 3) * 8;
-        const u4 z = {0u, 0u, 0u, 0u};
-#define CFMI_B3_LR(u, vv) \
-    vv = (gr + (u % 2) * 64 < rows) ? *reinterpret_cast<const u4*>(base + (u / 2) * ps + (u % 2) * 64 * ld) : z;
-        CFMI_B3_EACH(CFMI_B3_LR)
-#undef CFMI_B3_LR
+        const char* base = reinterpret_cast<const char*>(P + r0 * ld + k0);
+        const int64_t last = rows - 1 - r0;
+        const int64_t ra = (tid >> 2) < last ? (tid >> 2) : last;
+        const int64_t rb = (tid >> 2) + H < last ? (tid >> 2) + H : last;
+        const unsigned o0 = (unsigned)((ra * ld + (tid & 3) * 8) * 2);
+        const unsigned o1 = (unsigned)((rb * ld + (tid & 3) * 8) * 2);
+#define CFMI_B3_LV(u, vv) vv = *reinterpret_cast<const u4*>(base + (u / 2) * ps * 2 + ((u % 2) ? o1 : o0));
+        CFMI_B3_EACH(CFMI_B3_LV)
+#undef CFMI_B3_LV
     }
     __device__ __forceinline__ void load_guarded(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
                                                  int64_t rows, int64_t k0, int64_t kend, int tid)
     {
         const int64_t gr = r0 + (tid >> 2), gk = k0 + (tid & 3) * 8;
         const bf16* base = P + gr * ld + gk;
-#define CFMI_B3_LG(u, vv) vv = load16_guarded(base + (u / 2) * ps + (u % 2) * 64 * ld, gr + (u % 2) * 64 < rows, gk, kend);
+#define CFMI_B3_LG(u, vv) vv = load16_guarded(base + (u / 2) * ps + (u % 2) * H * ld, gr + (u % 2) * H < rows, gk, kend);
         CFMI_B3_EACH(CFMI_B3_LG)
 #undef CFMI_B3_LG
     }
-    __device__ __forceinline__ void store(bf16 (*S)[TM][LROW], int tid) const
+    __device__ __forceinline__ void store(bf16 (*S)[BT][LROW], int tid) const
     {
         const int row = tid >> 2, kc = (tid & 3) * 8;
-#define CFMI_B3_ST(u, vv) *reinterpret_cast<u4*>(&S[u / 2][row + (u % 2) * 64][kc]) = vv;
+#define CFMI_B3_ST(u, vv) *reinterpret_cast<u4*>(&S[u / 2][row + (u % 2) * H][kc]) = vv;
         CFMI_B3_EACH(CFMI_B3_ST)
 #undef CFMI_B3_ST
     }
@@ -149,8 +158,9 @@ struct Stage {
 
 // 6 MFMAs per 16 x 16 tile and 32-deep slab, smallest terms first; the four
 // row tiles of a column are interleaved so consecutive MFMAs are independent.
-__device__ __forceinline__ void slab_mfma(const bf16 (*As)[TM][LROW], const bf16 (*Bs)[TN][LROW], f4 (&acc)[4][4],
-                                          int wm, int wn, int lm, int lk)
+template <int BT>
+__device__ __forceinline__ void slab_mfma(const bf16 (*As)[BT][LROW], const bf16 (*Bs)[BT][LROW],
+                                          f4 (&acc)[4][Cfg<BT>::FN], int wm, int wn, int lm, int lk)
 {
     b8 a[4][3];
 #pragma unroll
@@ -158,7 +168,7 @@ __device__ __forceinline__ void slab_mfma(const bf16 (*As)[TM][LROW], const bf16
 #pragma unroll
         for (int p = 0; p < 3; ++p) a[i][p] = *reinterpret_cast<const b8*>(&As[p][wm + 16 * i + lm][8 * lk]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < Cfg<BT>::FN; ++j) {
         b8 b[3];
 #pragma unroll
         for (int p = 0; p < 3; ++p) b[p] = *reinterpret_cast<const b8*>(&Bs[p][wn + 16 * j + lm][8 * lk]);
@@ -171,20 +181,17 @@ __device__ __forceinline__ void slab_mfma(const bf16 (*As)[TM][LROW], const bf16
     }
 }
 
-// Staging variants: FAST (whole tile in range), ROWS (whole vectors, rows
-// predicated), ELEM (element-wise: partial last slab or unaligned operands).
-enum { FAST = 0, ROWS = 1, ELEM = 2 };
+// Staging variants: VEC (whole 16-byte vectors, slabs inside K) and ELEM
+// (element-wise: partial last slab or unaligned operands).
+enum { VEC = 0, ELEM = 1 };
 
-template <int LD>
+template <int BT, int LD>
 __device__ __forceinline__ void fetch(const B3Args& g, const bf16* A, const bf16* B, int64_t m0, int64_t n0,
-                                      int64_t k0, int64_t kend, Stage& sa, Stage& sb, int tid)
+                                      int64_t k0, int64_t kend, Stage<BT>& sa, Stage<BT>& sb, int tid)
 {
-    if constexpr (LD == FAST) {
-        sa.load_fast(A, g.lda, g.pA, m0, k0, tid);
-        sb.load_fast(B, g.ldb, g.pB, n0, k0, tid);
-    } else if constexpr (LD == ROWS) {
-        sa.load_rows(A, g.lda, g.pA, m0, g.M, k0, tid);
-        sb.load_rows(B, g.ldb, g.pB, n0, g.N, k0, tid);
+    if constexpr (LD == VEC) {
+        sa.load_vec(A, g.lda, g.pA, m0, g.M, k0, tid);
+        sb.load_vec(B, g.ldb, g.pB, n0, g.N, k0, tid);
     } else {
         sa.load_guarded(A, g.lda, g.pA, m0, g.M, k0, kend, tid);
         sb.load_guarded(B, g.ldb, g.pB, n0, g.N, k0, kend, tid);
@@ -193,61 +200,59 @@ __device__ __forceinline__ void fetch(const B3Args& g, const bf16* A, const bf16
 
 // K range [kbeg, kend) in 32-deep slabs: the next slab's global loads are in
 // flight (registers) while the current one is multiplied from LDS.
-template <int LD>
+template <int BT, int LD>
 __device__ __forceinline__ void k_loop(const B3Args& g, const bf16* A, const bf16* B, int64_t m0, int64_t n0,
-                                       int64_t kbeg, int64_t kend, bf16 (*As)[TM][LROW], bf16 (*Bs)[TN][LROW],
-                                       f4 (&acc)[4][4], int tid, int wm, int wn, int lm, int lk)
+                                       int64_t kbeg, int64_t kend, bf16 (*As)[BT][LROW], bf16 (*Bs)[BT][LROW],
+                                       f4 (&acc)[4][Cfg<BT>::FN], int tid, int wm, int wn, int lm, int lk)
 {
-    Stage sa, sb;
-    fetch<LD>(g, A, B, m0, n0, kbeg, kend, sa, sb, tid);
+    Stage<BT> sa, sb;
+    fetch<BT, LD>(g, A, B, m0, n0, kbeg, kend, sa, sb, tid);
     int64_t k0 = kbeg;
     for (; k0 + TK < kend; k0 += TK) {
         __syncthreads();
         sa.store(As, tid);
         sb.store(Bs, tid);
         __syncthreads();
-        fetch<LD>(g, A, B, m0, n0, k0 + TK, kend, sa, sb, tid);
-        slab_mfma(As, Bs, acc, wm, wn, lm, lk);
+        fetch<BT, LD>(g, A, B, m0, n0, k0 + TK, kend, sa, sb, tid);
+        slab_mfma<BT>(As, Bs, acc, wm, wn, lm, lk);
     }
     __syncthreads();
     sa.store(As, tid);
     sb.store(Bs, tid);
     __syncthreads();
-    slab_mfma(As, Bs, acc, wm, wn, lm, lk);
+    slab_mfma<BT>(As, Bs, acc, wm, wn, lm, lk);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(NTH, 2) void gemm_b3_kernel(B3Args g)
+template <int MODE, int BT>
+__global__ __launch_bounds__(Cfg<BT>::NT, Cfg<BT>::MINB) void gemm_b3_kernel(B3Args g)
 {
-    __shared__ __attribute__((aligned(16))) bf16 As[3][TM][LROW];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[3][TN][LROW];
+    constexpr int FN = Cfg<BT>::FN;
+    __shared__ __attribute__((aligned(16))) bf16 As[3][BT][LROW];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[3][BT][LROW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t z = blockIdx.z;
     int64_t ti, tj;
     tile_order(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
-    const int64_t m0 = ti * TM, n0 = tj * TN;
-    if (MODE == 2 && g.col0 + n0 > g.row0 + m0 + TM - 1) return;  // tile above the diagonal
+    const int64_t m0 = ti * BT, n0 = tj * BT;
+    if (MODE == 2 && g.col0 + n0 > g.row0 + m0 + BT - 1) return;  // tile above the diagonal
     const bf16* A = g.A + z * g.sA;
     const bf16* B = g.B + z * g.sB;
     int64_t kend = g.K;
-    if (g.tri & 1) kend = kend < m0 + TM + g.offA ? kend : m0 + TM + g.offA;
-    if (g.tri & 4) kend = kend < n0 + TN + g.offB ? kend : n0 + TN + g.offB;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    if (g.tri & 1) kend = kend < m0 + BT + g.offA ? kend : m0 + BT + g.offA;
+    if (g.tri & 4) kend = kend < n0 + BT + g.offB ? kend : n0 + BT + g.offB;
+    const int wm = (wave % Cfg<BT>::WMC) * 64, wn = (wave / Cfg<BT>::WMC) * Cfg<BT>::HALF;
     const int lm = lane & 15, lk = lane >> 4;
 
-    f4 acc[4][4];
+    f4 acc[4][FN];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
     const bool vec = g.vecA && g.vecB;
     const int64_t kf = vec ? kend / TK * TK : 0;
-    if (kf > 0) {
-        if (m0 + TM <= g.M && n0 + TN <= g.N) k_loop<FAST>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
-        else k_loop<ROWS>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
-    }
-    if (kf < kend) k_loop<ELEM>(g, A, B, m0, n0, kf, kend, As, Bs, acc, tid, wm, wn, lm, lk);
+    if (kf > 0) k_loop<BT, VEC>(g, A, B, m0, n0, 0, kf, As, Bs, acc, tid, wm, wn, lm, lk);
+    if (kf < kend) k_loop<BT, ELEM>(g, A, B, m0, n0, kf, kend, As, Bs, acc, tid, wm, wn, lm, lk);
 
     // Output lane layout (measured on gfx950): row 4 (lane / 16) + r, column lane % 16.
     float* Cf = (float*)g.C;
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(NTH, 2) void gemm_b3_kernel(B3Args g)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t gm = m0 + wm + 16 * i + 4 * lk + r;
@@ -375,14 +380,33 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.C = C; g.ldc = ldc; g.sC = sC; g.pC = pC;
     g.tri = flags & 5; g.offA = off_a; g.offB = off_b;
     g.row0 = row0; g.col0 = col0; g.pbase = pbase; g.pdiag = (float)pdiag;
-    g.tm = (m + TM - 1) / TM; g.tn = (n + TN - 1) / TN;
+    // 256-tiles for plain products with enough of them to keep every CU busy
+    // (a CU holds one), and an A-trim over less than half the rows.  Under a
+    // B-trim or a lower-triangle output the staircase of 256-cubes computes
+    // far more zeros (31% of the work of a 3968-row triangle against 10% for
+    // 128-cubes), which was measured to outweigh their higher MFMA rate.
+    // flags 256 / 512 force the 128 / 256 configuration.
+    const int64_t t256 = ((m + 255) / 256) * ((n + 255) / 256) * batch;
+    const bool flat = !(flags & 4) && out_mode != 2 && (!(flags & 1) || m >= 2 * (k - off_a));
+    const bool big = (flags & 512) ? true : (flags & 256) ? false : (flat && t256 >= 512);
+    const int64_t bt = big ? 256 : 128;
+    g.tm = (m + bt - 1) / bt; g.tn = (n + bt - 1) / bt;
     g.vecA = aligned16(A, lda, pA) && (sA % 8 == 0);
     g.vecB = aligned16(B, ldb, pB) && (sB % 8 == 0);
     if (g.tm * g.tn > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
     dim3 grid((unsigned)(g.tm * g.tn), 1, (unsigned)batch);
-    if (out_mode == 0) hipLaunchKernelGGL((gemm_b3_kernel<0>), grid, dim3(NTH), 0, c->stream, g);
-    else if (out_mode == 1) hipLaunchKernelGGL((gemm_b3_kernel<1>), grid, dim3(NTH), 0, c->stream, g);
-    else hipLaunchKernelGGL((gemm_b3_kernel<2>), grid, dim3(NTH), 0, c->stream, g);
+#define CFMI_B3_GO(MODE_, BT_) \
+    hipLaunchKernelGGL((gemm_b3_kernel<MODE_, BT_>), grid, dim3(Cfg<BT_>::NT), 0, c->stream, g)
+    if (big) {
+        if (out_mode == 0) CFMI_B3_GO(0, 256);
+        else if (out_mode == 1) CFMI_B3_GO(1, 256);
+        else CFMI_B3_GO(2, 256);
+    } else {
+        if (out_mode == 0) CFMI_B3_GO(0, 128);
+        else if (out_mode == 1) CFMI_B3_GO(1, 128);
+        else CFMI_B3_GO(2, 128);
+    }
+#undef CFMI_B3_GO
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }

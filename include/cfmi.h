@@ -191,9 +191,15 @@ int cfmi_gemm_ex(cfmi_ctx *ctx, int dtype, int out_dtype, int transa, int transb
  * with op(A)[m][k] = A[m lda + k] and op(B)[k][n] = B[n ldb + k] (planes pA,
  * pB elements apart; batches sA, sB, sC apart), six piece products per
  * term accumulated in FP32.  flags bit 0: A[m][k] = 0 for k > m + off_a;
- * bit 2: B[n][k] = 0 for k > n + off_b.  out_mode 0: FP32 C[m + n ldc]
- * (+ beta C); 1: three BF16 planes C[m ldc + n] (planes pC apart); 2: FP32
- * packed lower row band as cfmi_gemm_ex mode 2 (+ beta C).               */
+ * bit 2: B[n][k] = 0 for k > n + off_b (the operands must hold those
+ * zeros; the bits only shorten K); bits 8 / 9 force 128 x 128 / 256 x 256
+ * block tiles (default: 256 for untrimmed or mostly full-K products with
+ * at least 512 such tiles, outside out_mode 2).
+ * out_mode 0: FP32 C[m + n ldc] (+ beta C); 1: three BF16 planes
+ * C[m ldc + n] (planes pC apart); 2: FP32 packed lower row band as
+ * cfmi_gemm_ex mode 2 (+ beta C).  Replaces the FP32 products of the
+ * whitening in fisher.py:506-603 (no reference counterpart of the
+ * arithmetic itself).                                                     */
 int cfmi_gemm_b3(cfmi_ctx *ctx, int out_mode, int64_t m, int64_t n, int64_t k, double alpha,
                  const void *A, int64_t lda, int64_t sA, int64_t pA, int64_t off_a, const void *B,
                  int64_t ldb, int64_t sB, int64_t pB, int64_t off_b, double beta, void *C, int64_t ldc,

@@ -7,7 +7,8 @@ few units of FP32 rounding (2^-24 = 6e-8) times sqrt(K) and leaves a wide
 margin below a dropped cross term (a0 b1 alone is ~2^-9 relative).
 Covers the three outputs (FP32 column-major with beta, BF16 planes, packed
 lower row band with diagonal scale), ragged sizes, unaligned leading
-dimensions (element-wise staging), triangular K-trims and batches.
+dimensions (element-wise staging), triangular K-trims and batches, each with
+both block configurations (flags 256: 128 x 128 tiles, 512: 256 x 256).
 """
 
 import numpy as np
@@ -60,9 +61,13 @@ def test_split_join_round_trip(ctx):
     assert np.max(np.abs(got - X) / np.abs(X)) < 2.0 ** -25
 
 
+TILES = [256, 512]
+
+
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("M,N,K,lda,ldb", [(300, 190, 333, 336, 336), (128, 128, 256, 256, 256),
-                                           (77, 45, 1000, 1001, 1003), (1, 3, 5, 5, 5)])
-def test_gemm_b3_fp32_out(ctx, M, N, K, lda, ldb):
+                                           (77, 45, 1000, 1001, 1003), (1, 3, 5, 5, 5), (520, 600, 96, 96, 104)])
+def test_gemm_b3_fp32_out(ctx, M, N, K, lda, ldb, tile):
     import torch
     from cosmoforge_amd.engine import P
 
@@ -74,13 +79,14 @@ def test_gemm_b3_fp32_out(ctx, M, N, K, lda, ldb):
     Bp, pB = _planes(ctx, B, ldb)
     C = ctx.upload_cm(C0, dtype=torch.float32)
     ctx.check(ctx.lib.cfmi_gemm_b3(ctx.h, 0, M, N, K, 0.75, P(Ap), lda, 0, pA, 0, P(Bp), ldb, 0, pB, 0, 0.5,
-                                   P(C), M, 0, 0, 1, 0, 0, 0, 0, 1.0), "gemm_b3")
+                                   P(C), M, 0, 0, 1, tile, 0, 0, 0, 1.0), "gemm_b3")
     got = C.cpu().double().numpy().reshape(N, M).T
     want = 0.75 * A @ B.T + 0.5 * C0.astype(np.float32)
     assert np.max(np.abs(got - want)) <= TOL * (_scale(A, B) + np.max(np.abs(C0)))
 
 
-def test_gemm_b3_planes_out_batched_and_trimmed(ctx):
+@pytest.mark.parametrize("tile", TILES)
+def test_gemm_b3_planes_out_batched_and_trimmed(ctx, tile):
     """Mode 1 (BF16 planes of the product), batch of 2, A lower trapezoid
     with the K-trim flag: the trim must change nothing but the work done."""
     import torch
@@ -98,7 +104,7 @@ def test_gemm_b3_planes_out_batched_and_trimmed(ctx):
     for flags in (0, 1):
         Cp = torch.zeros(3 * pC, dtype=torch.int16, device=ctx.device)
         ctx.check(ctx.lib.cfmi_gemm_b3(ctx.h, 1, M, N, K, 1.0, P(Ap), lda, M * lda, pA, off, P(Bp), ldb, N * ldb, pB,
-                                       0, 0.0, P(Cp), ldc, M * ldc, pC, 2, flags, 0, 0, 0, 1.0), "gemm_b3")
+                                       0, 0.0, P(Cp), ldc, M * ldc, pC, 2, flags | tile, 0, 0, 0, 1.0), "gemm_b3")
         got = _join(ctx, Cp, 2 * M * ldc, pC).reshape(2, M, ldc)
         for b in range(2):
             want = As[b] @ Bs[b].T
@@ -106,7 +112,8 @@ def test_gemm_b3_planes_out_batched_and_trimmed(ctx):
             assert np.max(np.abs(got[b] - want)) <= TOL * _scale(As[b], Bs[b])
 
 
-def test_gemm_b3_packed_band(ctx):
+@pytest.mark.parametrize("tile", TILES)
+def test_gemm_b3_packed_band(ctx, tile):
     """Mode 2: rows r0..r1 of the lower triangle of alpha Z W^T, row-major
     packed from pbase, diagonal times pdiag; B upper-trimmed (W lower)."""
     import torch
@@ -123,7 +130,7 @@ def test_gemm_b3_packed_band(ctx):
     size = r1 * (r1 + 1) // 2 - pb
     X = torch.zeros(size, dtype=torch.float32, device=ctx.device)
     ctx.check(ctx.lib.cfmi_gemm_b3(ctx.h, 2, m, r1, K, 1.0, P(Zp), K, 0, pZ, 0, P(Wp), K, 0, pW, 0, 0.0, P(X), 0, 0,
-                                   0, 1, 4, r0, 0, pb, 0.5), "gemm_b3")
+                                   0, 1, 4 | tile, r0, 0, pb, 0.5), "gemm_b3")
     Pm = Z @ W.T
     want = np.concatenate([Pm[i - r0, :i + 1] * np.r_[np.ones(i), 0.5] for i in range(r0, r1)])
     got = X.cpu().double().numpy()
