@@ -28,10 +28,13 @@ sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 # HBM bytes per whitening launch, measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE on this workload (each scaled by what the counter reads for
-# a 1 GiB copy: x2.0 and x1.0); not measured live.  Keyed by (config, dtype).
-WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 113.6e9, (1, "f32"): 57.6e9}
+# a 1 GiB copy: x2.0 and x1.0); not measured live.  Keyed by (config, kernel).
+WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 113.6e9, (1, "f32"): 57.6e9, (1, "bf16x3"): 102.6e9}
+# The three-piece kernel issues six BF16 MFMA products per FP32-accurate one.
+B3_PRODUCTS = 6
 
 
 def make_map(ctx, cfg, seed):
@@ -231,6 +234,8 @@ def main():
     ap.add_argument("--precisions", default="fp64,mixed", help="comma list of fp64 / mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-params", type=int, default=2)
+    ap.add_argument("--mixed-gemm", default="bf16x3", choices=["bf16x3", "fp32"],
+                    help="whitening arithmetic of the mixed precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -273,7 +278,8 @@ def main():
 
     def step(prec, timer=None, lt=None):
         f = Fisher(cfg.params, basis=False, mask=cfg.mask, noise_cov1=cfg.noise, cls_data=cfg.cls,
-                   beam=cfg.beam, context=ctx, timer=timer, launch_timer=lt, precision=prec)
+                   beam=cfg.beam, context=ctx, timer=timer, launch_timer=lt, precision=prec,
+                   mixed_gemm=args.mixed_gemm)
         f.run()
         s = Spectra(fisher=f, maps1=maps)
         s.run()
@@ -305,8 +311,13 @@ def main():
     white_ms = lt.total_ms() / args.steps
     launches = len(lt.launches) / args.steps
     achieved = timer.flops / (white_ms * 1e-3) / 1e12
-    peak = FP32_MFMA_PEAK_TFLOPS if main_prec == "mixed" else FP64_MFMA_PEAK_TFLOPS
-    dt = "f32" if main_prec == "mixed" else "f64"
+    b3 = main_prec == "mixed" and args.mixed_gemm == "bf16x3"
+    dt = "bf16x3" if b3 else ("f32" if main_prec == "mixed" else "f64")
+    peak = {"bf16x3": BF16_MFMA_PEAK_TFLOPS, "f32": FP32_MFMA_PEAK_TFLOPS, "f64": FP64_MFMA_PEAK_TFLOPS}[dt]
+    kernel = {"bf16x3": "gemm_b3_kernel (three-piece BF16 operands, six BF16 MFMA products per FP32-accurate "
+                        "product; whitened derivatives W dC_b W^T, batched over bins)",
+              "f32": "gemm_kernel<float> (whitened derivatives W dC_b W^T, batched over bins)",
+              "f64": "gemm_kernel<double> (whitened derivatives W dC_b W^T, batched over bins)"}[dt]
     F = r["res"][0]
     out = {
         "metric": "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error",
@@ -319,11 +330,12 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 MFMA (FP64 recurrences, factorisation, Gram accumulation, QML)" if main_prec == "mixed"
-                 else "f64",
+        "dtype": ("bf16x3 MFMA at f32 accuracy" if b3 else "f32 MFMA") +
+                 " (FP64 recurrences, factorisation, Gram accumulation, QML)" if main_prec == "mixed" else "f64",
         "data": "synthetic (seeded map of S+N; reference-recipe spectra, mask, beam, noise)",
         "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(F.shape[0]),
                    "lmax_signal": cfg.params["lmax_signal"], "nsims": 1, "precision": main_prec,
+                   "mixed_gemm": args.mixed_gemm if main_prec == "mixed" else None,
                    "row_bands": len(timer.bands)},
         "stages_ms": {"deriv_stacks": round(timer.elapsed_ms("deriv0", "white0") / args.steps, 2),
                       "whitened_derivatives": round(timer.elapsed_ms("white0", "white1") / args.steps, 2),
@@ -331,11 +343,15 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "traffic": WHITENING_TRAFFIC_BYTES_PER_LAUNCH.get((args.config, dt)),
-                     "kernel": f"gemm_kernel<{'float' if dt == 'f32' else 'double'}> (whitened derivatives "
-                               "W dC_b W^T, batched over bins)",
+                     "kernel": kernel,
                      "launches_per_step": launches, "flops_per_launch": timer.flops / max(launches, 1),
                      "ms_per_launch": round(white_ms / max(launches, 1), 3)},
     }
+    if b3:
+        # achieved counts the FP32-equivalent (algorithmic) flops; the matrix
+        # cores execute six BF16 products for each of them.
+        out["roofline"]["mfma_products_per_flop"] = B3_PRODUCTS
+        out["roofline"]["mfma_issue_frac"] = round(B3_PRODUCTS * achieved / peak, 4)
     for prec, rr in runs.items():
         out[f"{prec}_s"] = round(rr["s"], 4)
     if "fp64" in runs and "mixed" in runs:
