@@ -55,6 +55,7 @@ SIGNATURES = {
     "cfmi_gemm_b3": (ci, [vp, ci, i64, i64, i64, cd, vp, i64, i64, i64, i64, vp, i64, i64, i64, i64, cd, vp, i64,
                           i64, i64, i64, ci, i64, i64, i64, cd]),
     "cfmi_split3": (ci, [vp, ci, i64, i64, vp, i64, vp, i64, i64]),
+    "cfmi_dot_batched": (ci, [vp, i64, i64, vp, i64, i64, vp, i64, vp, i64]),
     "cfmi_join3": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),

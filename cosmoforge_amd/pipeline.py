@@ -494,7 +494,8 @@ def qml(cov: Covariance, src: DerivativeSource, maps: np.ndarray, remove_nb: boo
     q_b + nb_b = 1/2 y^T dC_b y with y = C^-1 x, and nb_b = 1/2 Tr[G dC_b]
     with G = C^-1 N C^-1 (``spectra.py:1124-1186``).  With dC_b = e_a D_b e_b^T
     (+ transpose) both are <M_ab, D_b> (x2 when a != b) for M = y y^T and G:
-    one FP64-accumulated GEMM per spectrum over the flattened D_b.
+    one HBM-bound FP64 pass per spectrum over the flattened D_b
+    (``cfmi_dot_batched``).
 
     The contractions sum n^2 terms into results about n times smaller than
     the terms, so FP32 rounding of G or D would leave ~n * 6e-8 relative
@@ -530,7 +531,7 @@ def qml(cov: Covariance, src: DerivativeSource, maps: np.ndarray, remove_nb: boo
         M = cache[key]
         R = torch.empty((ns + 1, k), dtype=torch.float64, device=ctx.device)
         K = blk.ma * blk.mb
-        engine.gemm_ex(ctx, True, False, k, ns + 1, K, 1.0, D, 0, K, M, 0, K, 0.0, R, 0, k)
+        ctx.check(ctx.lib.cfmi_dot_batched(ctx.h, K, k, P(D), K, ns + 1, P(M), K, P(R), k), "dot_batched")
         mult = 0.5 * (1.0 if blk.sym else 2.0)
         out[:, s * nb_bins + b0: s * nb_bins + b0 + k] = mult * R.cpu().numpy()
         del blk, D

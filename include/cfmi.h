@@ -256,6 +256,14 @@ int cfmi_fill(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, void *A, int64_t l
 int cfmi_gram(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t K, const void *X,
               int64_t ldx, const void *Y, int64_t ldy, double *G, int64_t ldg);
 
+/* R[a + b ldr] = sum_k A[k + a lda] B[k + b ldb] for a < na, b < nb, k < K
+ * (FP64; lda, ldb >= K when na, nb > 1): an HBM-bound pass with fixed-order
+ * partial sums.  Replaces the contractions of the QML estimate and noise
+ * bias, 1/2 y^T dC_b y and 1/2 Tr[C^-1 N C^-1 dC_b] (spectra.py:1124-1186),
+ * as <D_b, M_q> over the flattened derivative blocks.                     */
+int cfmi_dot_batched(cfmi_ctx *ctx, int64_t K, int64_t na, const double *A, int64_t lda, int64_t nb,
+                     const double *B, int64_t ldb, double *R, int64_t ldr);
+
 /* ---- A18 / A21: batched exact likelihood -------------------------------- */
 /* A batch of njobs covariance matrices C_j = sum_k coef[j][k] M_{p(j),k},
  * p(j) = job_patch[j] (host array), from per-patch FP64 component matrices

@@ -354,3 +354,29 @@ def test_deriv_stacks_bf16x3_and_mirrors(ctx):
                 assert b.mirror.shape == (3, nb, ma, mb)
                 assert torch.equal(b.mirror, b.stack.transpose(2, 3)), name
         assert nmirror == sum(1 for g in engine.block_geometry(inp.layout) if not g.sym) > 0
+
+
+@pytest.mark.parametrize("K,na,nb", [(1_000_003, 37, 3), (4096, 1, 1), (5000, 5, 6), (17, 3, 2), (0, 2, 2)])
+def test_dot_batched(ctx, K, na, nb):
+    """cfmi_dot_batched against NumPy FP64 (ragged K, more than four B
+    vectors, K = 0), and bit-for-bit repeatable."""
+    import torch
+    from cosmoforge_amd.engine import P
+
+    rng = np.random.default_rng(K + na)
+    A = rng.standard_normal((na, K))
+    B = rng.standard_normal((nb, K))
+    Ad = torch.tensor(A, device=ctx.device)
+    Bd = torch.tensor(B, device=ctx.device)
+    outs = []
+    for _ in range(2):
+        R = torch.full((nb, na), np.nan, dtype=torch.float64, device=ctx.device)
+        ctx.check(ctx.lib.cfmi_dot_batched(ctx.h, K, na, P(Ad), K, nb, P(Bd), K, P(R), na), "dot_batched")
+        outs.append(R.cpu().numpy())
+    want = B @ A.T
+    if K == 0:
+        assert np.array_equal(outs[0], np.zeros((nb, na)))
+        return
+    scale = np.sqrt(K) * max(1.0, np.max(np.abs(A)) * np.max(np.abs(B)))
+    assert np.max(np.abs(outs[0] - want)) <= 1e-13 * scale
+    assert np.array_equal(outs[0], outs[1])
