@@ -145,20 +145,27 @@ struct Slab {
         else { n = (idx % (TBN / 2)) * 2; k = idx / (TBN / 2); }
     }
 
+    // Pairs along k (TA = 1 for A, TB = 0 for B) take rows beyond M / N
+    // clamped to the last row: they feed only outputs that are never stored,
+    // so edge tiles keep the vector loads; pairs along m / n need interior
+    // rows.
     __device__ __forceinline__ void load_fast(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
-                                              int64_t ldb, int64_t m0, int64_t n0, int64_t k0, int tid)
+                                              int64_t ldb, int64_t m0, int64_t n0, int64_t k0, int64_t M,
+                                              int64_t N, int tid)
     {
         typedef typename Vec2<T>::type V2;
 #pragma unroll
         for (int r = 0; r < PA; ++r) {
             int idx = tid + NT * r, m, k, n;
             amap(idx, m, k);
-            const T* pa = TA == 0 ? A + (m0 + m) + (k0 + k) * lda : A + (k0 + k) + (m0 + m) * lda;
+            const int64_t mr = TA == 0 ? m0 + m : (m0 + m < M ? m0 + m : M - 1);
+            const T* pa = TA == 0 ? A + mr + (k0 + k) * lda : A + (k0 + k) + mr * lda;
             V2 va = *reinterpret_cast<const V2*>(pa);
             ra[2 * r] = va.x;
             ra[2 * r + 1] = va.y;
             bmap(idx, k, n);
-            const T* pb = TB == 0 ? B + (k0 + k) + (n0 + n) * ldb : B + (n0 + n) + (k0 + k) * ldb;
+            const int64_t nr = TB == 0 ? (n0 + n < N ? n0 + n : N - 1) : n0 + n;
+            const T* pb = TB == 0 ? B + (k0 + k) + nr * ldb : B + nr + (k0 + k) * ldb;
             V2 vb = *reinterpret_cast<const V2*>(pb);
             rb[2 * r] = vb.x;
             rb[2 * r + 1] = vb.y;
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
     // Vector loads need 2-element alignment of every column start.
     const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % (2 * sizeof(T)) == 0)
                          && (lda % 2 == 0) && (ldb % 2 == 0) && (kbeg % 2 == 0);
-    const bool interior_mn = (m0 + BM <= M) && (n0 + BN <= N);
+    const bool rows_ok = (TA == 1 || m0 + BM <= M) && (TB == 0 || n0 + BN <= N);
 
     v4 acc[FM][FN];
 #pragma unroll
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
 
     Slab<T, TA, TB, BM, BN> sl;
     auto fetch = [&](int64_t k0) {
-        if (aligned && interior_mn && k0 + BK <= kend) sl.load_fast(A, lda, B, ldb, m0, n0, k0, tid);
+        if (aligned && rows_ok && k0 + BK <= kend) sl.load_fast(A, lda, B, ldb, m0, n0, k0, M, N, tid);
         else sl.load_guarded(A, lda, B, ldb, m0, n0, k0, M, N, kend, tid);
     };
 
