@@ -9,7 +9,8 @@ Mirrors the reference's ``Core`` seams for in-memory inputs
   (``core.py:378-401``);
 * ``mask`` (npix,) or (npix, ncomp), active = > 0.5, Q/U columns must agree
   (``core.py:403-472``);
-* ``noise_cov1``: reduced (n, n) float64 in active-row order, used as-is;
+* ``noise_cov1``: reduced (n, n) float64 in active-row order, used as-is
+  (a host array, or a device tensor that stays where it is);
 * ``cls_data``: {label: physical C_l}, l-indexed, every spectrum label
   required (``spectra_io.py:257-326``);
 * ``beam``: (>= 3, >= lmax_signal + 1) T/E/B windows; wins over
@@ -207,13 +208,28 @@ def resolve_cls(layout: Layout, cls_data, lmax: int) -> dict:
     return out
 
 
-def resolve_noise(n: int, noise_cov) -> np.ndarray:
+def resolve_noise(n: int, noise_cov):
+    """The (n, n) noise covariance: a host array, or a torch tensor already on
+    the device (kept there, so no host copy or PCIe transfer per solve)."""
     if noise_cov is None:
         raise ValueError("noise_cov1= is required (file inputs are not part of this build)")
+    if _is_device_tensor(noise_cov):
+        N = noise_cov
+        if tuple(N.shape) != (n, n):
+            raise ValueError(f"noise_cov1 has shape {tuple(N.shape)}, expected ({n}, {n}) for the active-pixel count")
+        return N
     N = np.asarray(noise_cov)
     if N.shape != (n, n):
         raise ValueError(f"noise_cov1 has shape {N.shape}, expected ({n}, {n}) for the active-pixel count")
     return N
+
+
+def _is_device_tensor(x) -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda
 
 
 def resolve_maps(n: int, nsims, maps) -> np.ndarray:

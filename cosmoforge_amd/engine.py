@@ -91,8 +91,11 @@ class Context:
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.device)
 
     def upload_cm(self, a: np.ndarray, dtype=torch.float64) -> torch.Tensor:
-        """Host (m, n) array -> column-major device matrix."""
+        """Host (m, n) array -> column-major device matrix (the transpose is
+        made on the device, not by a host copy)."""
         a = np.asarray(a)
+        if a.ndim == 2 and a.flags.c_contiguous:
+            return torch.as_tensor(a).to(self.device, dtype=dtype).T.contiguous()
         return torch.as_tensor(np.ascontiguousarray(a.T), dtype=dtype).to(self.device)
 
 

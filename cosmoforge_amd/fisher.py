@@ -67,7 +67,8 @@ class Fisher:
 
     @property
     def reduced_noise_cov1(self) -> np.ndarray:
-        return self.inputs.noise
+        N = self.inputs.noise
+        return N.detach().cpu().numpy() if hasattr(N, "detach") else N
 
     def set_binning(self, bins):
         """bins: list of (lmin_b, lmax_b) or an object with lmins/lmaxs."""

@@ -34,7 +34,12 @@ class Covariance:
         nest = str(p.ordering).upper() == "NESTED"
         self.vecs = [engine.pointings(ctx, p.nside, f.pixels, nest) for f in inputs.layout.fields]
         self.n = inputs.n
-        self.N = ctx.upload_cm(inputs.noise)
+        N = inputs.noise
+        if isinstance(N, torch.Tensor):
+            # (n, n) row-major device tensor -> column-major copy on the device
+            self.N = N.to(device=ctx.device, dtype=torch.float64).T.contiguous()
+        else:
+            self.N = ctx.upload_cm(N)
         self.S = None
         self.factor = None
         self._w32 = None

@@ -241,3 +241,21 @@ def test_mixed_whitening_engines_and_bands():
     assert errs["bands_vs_one"] < 1e-6
     with pytest.raises(ValueError, match="mixed_gemm"):
         pipeline.fisher(f.cov, bins, conv, precision="mixed", mixed_gemm="fp16")
+
+
+def test_device_resident_noise_matches_host():
+    """noise_cov1 given as a device tensor (kept in HBM) gives the same F,
+    bit for bit, as the host array, and the getter returns host data."""
+    import torch
+    from cosmoforge_amd import Fisher
+
+    case = cases.case_fisher_nside4("TQU")
+    f1 = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                beam=case.beam)
+    f1.run()
+    Nd = torch.as_tensor(np.ascontiguousarray(case.noise)).to("cuda")
+    f2 = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=Nd, cls_data=case.cls, beam=case.beam,
+                context=f1.ctx)
+    f2.run()
+    assert np.array_equal(f1.get_fisher_matrix(), f2.get_fisher_matrix())
+    assert np.array_equal(f2.reduced_noise_cov1, np.asarray(case.noise))
