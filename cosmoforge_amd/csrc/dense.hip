@@ -470,61 +470,65 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
 constexpr int NB = 64;
 
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info,
-                                                         int64_t koff, T* linv, T* U, int64_t ldu)
+__global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
+                                                        T* linv, T* U, int64_t ldu)
 {
-    __shared__ T s[NB][NB + 1];
-    __shared__ T si[NB][NB + 1];
-    __shared__ int bad;
+    // One wave: lane r holds row r of the block in registers.  Step q takes
+    // the pivot from lane q, scales column q, publishes it through LDS and
+    // every lane updates its row (right-looking, no block-wide sweeps).  The
+    // inverse is then solved column by column, lane c owning column c, with
+    // L read from LDS as broadcasts.
+    __shared__ T col[NB];
+    __shared__ T Ls[NB][NB + 1];
     if (*info != 0) return;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int r = idx & (NB - 1), q = idx >> 6;
-        s[r][q] = (r < kb && q < kb && r >= q) ? A[r + q * lda] : T(0);
-        si[r][q] = T(0);
-    }
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int q = 0; q < kb; ++q) {
-        if (tid == 0) {
-            T d = s[q][q];
-            if (!(d > T(0))) bad = q + 1;
-            else s[q][q] = sqrt(d);
+    const int r = threadIdx.x;
+    T row[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) row[c] = (r < kb && c <= r) ? A[r + c * lda] : T(0);
+    int bad = 0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        if (q < kb && !bad) {
+            const T d = __shfl(row[q], q, 64);
+            if (!(d > T(0))) {
+                bad = q + 1;
+            } else {
+                const T piv = sqrt(d);
+                const T l = r > q ? row[q] / piv : (r == q ? piv : T(0));
+                row[q] = l;
+                col[r] = l;
+                __syncthreads();
+#pragma unroll
+                for (int p = q + 1; p < NB; ++p) row[p] -= l * col[p];
+                __syncthreads();
+            }
         }
-        __syncthreads();
-        if (bad) break;
-        T piv = s[q][q];
-        for (int r = q + 1 + tid; r < kb; r += 256) s[r][q] = s[r][q] / piv;
-        __syncthreads();
-        for (int idx = tid; idx < NB * NB; idx += 256) {
-            int r = idx & (NB - 1), p = idx >> 6;
-            if (p > q && r >= p && r < kb) s[r][p] -= s[r][q] * s[p][q];
-        }
-        __syncthreads();
     }
     if (bad) {
-        if (tid == 0) atomicCAS(info, 0, (int)(koff + bad));
+        if (r == 0) atomicCAS(info, 0, (int)(koff + bad));
         return;
     }
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int r = idx & (NB - 1), q = idx >> 6;
-        if (r < kb && q < kb) A[r + q * lda] = r >= q ? s[r][q] : T(0);
-    }
-    // Inverse of the lower-triangular block, one column per thread.
-    if (tid < kb) {
-        int j = tid;
-        for (int i = j; i < kb; ++i) {
-            T sum = (i == j) ? T(1) : T(0);
-            for (int k = j; k < i; ++k) sum -= s[i][k] * si[k][j];
-            si[i][j] = sum / s[i][i];
-        }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const T v = c <= r ? row[c] : T(0);
+        Ls[r][c] = v;
+        if (r < kb && c < kb) A[r + c * lda] = v;
     }
     __syncthreads();
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int r = idx & (NB - 1), q = idx >> 6;
-        T v = (r < kb && q < kb && r >= q) ? si[r][q] : T(0);
-        linv[r + q * NB] = v;
-        if (U && r < kb && q < kb) U[r + q * ldu] = v;
+    // U = L^-1: lane c solves L u = e_c (u[k] = 0 for k < c).
+    const int c = r;
+    T u[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        T sum = (i == c) ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) sum -= Ls[i][k] * u[k];
+        u[i] = (i < c || i >= kb) ? T(0) : sum / Ls[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        linv[i + c * NB] = u[i];
+        if (U && i < kb && c < kb) U[i + c * ldu] = u[i];
     }
 }
 
@@ -616,7 +620,7 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
     }
     for (int64_t k = 0; k < n; k += NB) {
         int kb = (int)std::min<int64_t>(NB, n - k);
-        hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(256), 0, c->stream, A + k + k * lda,
+        hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k + k * lda,
                            lda, kb, c->d_info, k, linv, U ? U + k + k * ldu : (T*)nullptr, ldu);
         CFMI_LAUNCH_CHECK(c);
         int64_t m = n - k - kb;
