@@ -31,8 +31,15 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
-constexpr int TK = 32, PADK = 8;
-constexpr int LROW = TK + PADK;  // LDS row pitch in bf16 (80 bytes)
+// LDS operand rows are 32 bf16 (64 bytes, unpadded); the 16-byte chunk c of
+// row r sits at chunk c ^ swz(r).  With swz(r) = ((r >> 1) ^ (r >> 2)) & 3
+// the staging stores (8 lanes: 2 rows x 4 chunks, 128 contiguous bytes) and
+// the MFMA operand reads (16 lanes: 16 rows at one chunk) both hit distinct
+// banks, for 32 and for 64 banks of 4 bytes.  swz(r) depends on r mod 16
+// only.
+constexpr int TK = 32;
+constexpr int LROW = TK;
+__device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
 // Block configurations: BT x BT output tiles, 2 BT threads, waves of 64 x BT/2
 // (BT / 64 along M, 2 along N).  BT = 256 halves the operand bytes per flop
@@ -110,14 +117,19 @@ __device__ __forceinline__ u4 load16_guarded(const bf16* src, bool row_ok, int64
 }
 
 // One 32-deep slab of a BT-row operand tile (3 planes x BT rows x 4 vectors
-// of 8): thread t holds k chunk 8 (t % 4) of rows t / 4 and t / 4 + BT / 2 in
+// of 8): thread t holds k chunk skc(t) of rows srow(t) and srow(t) + BT / 2 in
 // each plane.  Six named registers rather than an array, so they never live
 // in scratch memory.
+// Thread t stages row srow(t) = t / 4 and k chunk t % 4 (elements
+// skc(t) .. + 7): four consecutive lanes read one row's 64 contiguous bytes.
+__device__ __forceinline__ int srow(int t) { return t >> 2; }
+__device__ __forceinline__ int skc(int t) { return (t & 3) * 8; }
+
 #define CFMI_B3_EACH(X) X(0, v0) X(1, v1) X(2, v2) X(3, v3) X(4, v4) X(5, v5)
 template <int BT>
 struct Stage {
     static constexpr int H = Cfg<BT>::HALF;
-    u4 v0, v1, v2, v3, v4, v5;   // vector u: plane u / 2, row t / 4 + H (u % 2)
+    u4 v0, v1, v2, v3, v4, v5;   // vector u: plane u / 2, row srow(t) + H (u % 2)
 
     // Whole vectors.  The tile base (block row, k0, plane) is uniform; each
     // lane adds a 32-bit byte offset for its two rows, which lets every load
@@ -129,10 +141,11 @@ struct Stage {
     {
         const char* base = reinterpret_cast<const char*>(P + r0 * ld + k0);
         const int64_t last = rows - 1 - r0;
-        const int64_t ra = (tid >> 2) < last ? (tid >> 2) : last;
-        const int64_t rb = (tid >> 2) + H < last ? (tid >> 2) + H : last;
-        const unsigned o0 = (unsigned)((ra * ld + (tid & 3) * 8) * 2);
-        const unsigned o1 = (unsigned)((rb * ld + (tid & 3) * 8) * 2);
+        const int sr = srow(tid);
+        const int64_t ra = sr < last ? sr : last;
+        const int64_t rb = sr + H < last ? sr + H : last;
+        const unsigned o0 = (unsigned)((ra * ld + skc(tid)) * 2);
+        const unsigned o1 = (unsigned)((rb * ld + skc(tid)) * 2);
 #define CFMI_B3_LV(u, vv) vv = *reinterpret_cast<const u4*>(base + (u / 2) * ps * 2 + ((u % 2) ? o1 : o0));
         CFMI_B3_EACH(CFMI_B3_LV)
 #undef CFMI_B3_LV
@@ -140,7 +153,7 @@ struct Stage {
     __device__ __forceinline__ void load_guarded(const bf16* __restrict__ P, int64_t ld, int64_t ps, int64_t r0,
                                                  int64_t rows, int64_t k0, int64_t kend, int tid)
     {
-        const int64_t gr = r0 + (tid >> 2), gk = k0 + (tid & 3) * 8;
+        const int64_t gr = r0 + srow(tid), gk = k0 + skc(tid);
         const bf16* base = P + gr * ld + gk;
 #define CFMI_B3_LG(u, vv) vv = load16_guarded(base + (u / 2) * ps + (u % 2) * H * ld, gr + (u % 2) * H < rows, gk, kend);
         CFMI_B3_EACH(CFMI_B3_LG)
@@ -148,7 +161,7 @@ struct Stage {
     }
     __device__ __forceinline__ void store(bf16 (*S)[BT][LROW], int tid) const
     {
-        const int row = tid >> 2, kc = (tid & 3) * 8;
+        const int row = srow(tid), kc = ((tid & 3) ^ swz(row)) * 8;   // swz(row + H) = swz(row)
 #define CFMI_B3_ST(u, vv) *reinterpret_cast<u4*>(&S[u / 2][row + (u % 2) * H][kc]) = vv;
         CFMI_B3_EACH(CFMI_B3_ST)
 #undef CFMI_B3_ST
@@ -162,16 +175,17 @@ template <int BT>
 __device__ __forceinline__ void slab_mfma(const bf16 (*As)[BT][LROW], const bf16 (*Bs)[BT][LROW],
                                           f4 (&acc)[4][Cfg<BT>::FN], int wm, int wn, int lm, int lk)
 {
+    const int kk = 8 * (lk ^ swz(lm));   // wm, wn, 16 i, 16 j are multiples of 16
     b8 a[4][3];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int p = 0; p < 3; ++p) a[i][p] = *reinterpret_cast<const b8*>(&As[p][wm + 16 * i + lm][8 * lk]);
+        for (int p = 0; p < 3; ++p) a[i][p] = *reinterpret_cast<const b8*>(&As[p][wm + 16 * i + lm][kk]);
 #pragma unroll
     for (int j = 0; j < Cfg<BT>::FN; ++j) {
         b8 b[3];
 #pragma unroll
-        for (int p = 0; p < 3; ++p) b[p] = *reinterpret_cast<const b8*>(&Bs[p][wn + 16 * j + lm][8 * lk]);
+        for (int p = 0; p < 3; ++p) b[p] = *reinterpret_cast<const b8*>(&Bs[p][wn + 16 * j + lm][kk]);
 #define CFMI_B3_TERM(pa, pb)                                                                            \
     _Pragma("unroll") for (int i = 0; i < 4; ++i)                                                       \
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][pa], b[pb], acc[i][j], 0, 0, 0);
