@@ -394,15 +394,18 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.C = C; g.ldc = ldc; g.sC = sC; g.pC = pC;
     g.tri = flags & 5; g.offA = off_a; g.offB = off_b;
     g.row0 = row0; g.col0 = col0; g.pbase = pbase; g.pdiag = (float)pdiag;
-    // 256-tiles for plain products with enough of them to keep every CU busy
-    // (a CU holds one), and an A-trim over less than half the rows.  Under a
-    // B-trim or a lower-triangle output the staircase of 256-cubes computes
-    // far more zeros (31% of the work of a 3968-row triangle against 10% for
-    // 128-cubes), which was measured to outweigh their higher MFMA rate.
-    // flags 256 / 512 force the 128 / 256 configuration.
+    // 256-tiles (a CU holds one) when there are enough of them to keep every
+    // CU busy and either the product is plain (no B-trim, no lower-triangle
+    // output, an A-trim over less than half the rows) or N and K are large
+    // (the row count of a band may be small).  Measured: on 3968-row triangles (trims, band outputs) the
+    // 256-cube staircase computes 31% zeros against 10% for 128-cubes and the
+    // 256 configuration is 10-70% slower; on the 8064 / 16128-row products of
+    // an nside-32 whitening it is 1.55x faster.  flags 256 / 512 force the
+    // 128 / 256 configuration.
     const int64_t t256 = ((m + 255) / 256) * ((n + 255) / 256) * batch;
     const bool flat = !(flags & 4) && out_mode != 2 && (!(flags & 1) || m >= 2 * (k - off_a));
-    const bool big = (flags & 512) ? true : (flags & 256) ? false : (flat && t256 >= 512);
+    const bool large = n >= 6144 && k >= 6144;
+    const bool big = (flags & 512) ? true : (flags & 256) ? false : ((flat || large) && t256 >= 512);
     const int64_t bt = big ? 256 : 128;
     g.tm = (m + bt - 1) / bt; g.tn = (n + bt - 1) / bt;
     g.vecA = aligned16(A, lda, pA) && (sA % 8 == 0);

@@ -558,6 +558,11 @@ def _gemm_b3(*a, **k):
     return gemm_b3(*a, **k)
 
 
+# Extra cfmi_gemm_b3 flags for the whitening products (256 / 512 force the
+# 128- / 256-tile configuration; 0 keeps the library's choice).
+B3_WHITEN_FLAGS = 0
+
+
 def _plane_buffer(ctx: Context, k: int) -> torch.Tensor:
     """Flat storage for k three-piece plane entries (int16 BF16 bits)."""
     return torch.empty(k, dtype=torch.int16, device=ctx.device)
@@ -594,6 +599,7 @@ def whiten_band_b3(ctx: Context, Wp: torch.Tensor, n: int, blk: Block, X: torch.
     def g(*a, **k):
         if counter is not None:
             counter.append(1)
+        k["flags"] = k.get("flags", 0) | B3_WHITEN_FLAGS
         _gemm_b3(ctx, *a, **k)
 
     def need(k):
