@@ -11,10 +11,10 @@
 //
 // Operands are "k-contiguous": A[m][k] at A + m lda + k, B[n][k] at
 // B + n ldb + k (one plane; planes pA / pB elements apart), so a 32-deep
-// K-slab of a 128-row tile is a 16-byte-vector copy into LDS ([row][k]) and
-// each MFMA operand (8 consecutive k) one 16-byte LDS read.  Tiles 128 x 128,
-// 4 waves of 64 x 64 (4 x 4 MFMA tiles), next slab prefetched into registers
-// during the MFMAs.
+// K-slab of a tile is a 16-byte-vector copy into LDS ([row][k]) and each
+// MFMA operand (8 consecutive k) one 16-byte LDS read.  Tiles 128 x 128
+// (4 waves of 64 x 64) or 256 x 256 (8 waves of 64 x 128); the next slab is
+// prefetched into registers during the MFMAs.
 //
 // Outputs: FP32 column-major (mode 0), three BF16 planes row-major
 // C[m ldc + n] (mode 1, the next product's operand), or the FP32 packed
@@ -44,7 +44,7 @@ __device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 // Block configurations: BT x BT output tiles, 2 BT threads, waves of 64 x BT/2
 // (BT / 64 along M, 2 along N).  BT = 256 halves the operand bytes per flop
 // and gives every wave 4 x 8 MFMA tiles (36 LDS reads per 192 MFMAs); its
-// 120 KB of LDS allow one block per CU (2 waves per SIMD).  BT = 128 (two
+// 96 KB of LDS allow one block per CU (2 waves per SIMD).  BT = 128 (two
 // blocks per CU) serves grids too small to fill the CUs with 256-tiles.
 template <int BT> struct Cfg {
     static constexpr int NT = 2 * BT, HALF = BT / 2, WMC = BT / 64, FN = BT / 32, MINB = BT == 128 ? 2 : 1;
