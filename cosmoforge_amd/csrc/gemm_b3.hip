@@ -87,20 +87,29 @@ struct B3Args {
     int64_t offA, offB;
     int64_t row0, col0, pbase;
     float pdiag;
-    int64_t tm, tn;
+    int64_t tm, tn, tmp;      // row / column tiles; tmp: enumerated rows (tm, or tm + 1)
     int vecA, vecB;           // 16-byte vector loads allowed
 };
 
 // Tile order: column tiles of B in turn, the row tiles of A in each; with a
 // triangular K-trim the heaviest tiles (longest K) are issued first.
-// (Grouping tiles into patches per XCD was measured no faster: L2 reuse of
-// the operand tiles is not what bounds this kernel.)
-__device__ __forceinline__ void tile_order(int64_t b, int64_t tm, int64_t tn, int tri, int64_t& ti, int64_t& tj)
+// Workgroups go to the 8 XCDs round-robin in launch order, so with an even
+// row count every XCD would own a fixed subset of row tiles; under an A-trim
+// or a lower-triangle output the work of a row tile grows with its index and
+// that subset decides the XCD's share (measured: 2.4x more work on one XCD
+// than on another for 16 rows of 256-tiles, and the whole launch as slow as
+// the untrimmed product).  The enumeration then counts tmp = tm + 1 rows (the
+// extra one exits at once), so the row-to-XCD assignment rotates from column
+// to column and evens out.
+__device__ __forceinline__ bool tile_order(int64_t b, int64_t tm, int64_t tmp, int64_t tn, int tri, int64_t& ti,
+                                           int64_t& tj)
 {
-    ti = b % tm;
-    tj = b / tm;
+    ti = b % tmp;
+    tj = b / tmp;
+    if (ti >= tm) return false;
     if (tri & 1) ti = tm - 1 - ti;
     if (tri & 4) tj = tn - 1 - tj;
+    return true;
 }
 
 // 16 bytes (8 bf16 at k .. k + 7 of one row), zero outside rows / [.., kend).
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(Cfg<BT>::NT, Cfg<BT>::MINB) void gemm_b3_kernel(B3A
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t z = blockIdx.z;
     int64_t ti, tj;
-    tile_order(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
+    if (!tile_order(blockIdx.x, g.tm, g.tmp, g.tn, g.tri, ti, tj)) return;
     const int64_t m0 = ti * BT, n0 = tj * BT;
     if (MODE == 2 && g.col0 + n0 > g.row0 + m0 + BT - 1) return;  // tile above the diagonal
     const bf16* A = g.A + z * g.sA;
@@ -395,23 +404,25 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.tri = flags & 5; g.offA = off_a; g.offB = off_b;
     g.row0 = row0; g.col0 = col0; g.pbase = pbase; g.pdiag = (float)pdiag;
     // 256-tiles (a CU holds one) when there are enough of them to keep every
-    // CU busy and either the product is plain (no B-trim, no lower-triangle
-    // output, an A-trim over less than half the rows) or N and K are large
-    // (the row count of a band may be small).  Measured: on 3968-row triangles (trims, band outputs) the
-    // 256-cube staircase computes 31% zeros against 10% for 128-cubes and the
-    // 256 configuration is 10-70% slower; on the 8064 / 16128-row products of
-    // an nside-32 whitening it is 1.55x faster.  flags 256 / 512 force the
-    // 128 / 256 configuration.
+    // CU busy and either the product has no B-trim and a full output (any
+    // A-trim: its rows are spread over the XCDs, see tile_order), or N and K
+    // are large (the row count of a band may be small).  Measured: on
+    // 3968-row lower-triangle band outputs with a B-trim the 256-cube
+    // staircase computes 31% zeros against 10% for 128-cubes and the 256
+    // configuration takes 2x as long; on the triangular 3968^3 products it is
+    // 1.3x faster, and on the 8064 / 16128-row products of an nside-32
+    // whitening 1.55x.  flags 256 / 512 force the 128 / 256 configuration.
     const int64_t t256 = ((m + 255) / 256) * ((n + 255) / 256) * batch;
-    const bool flat = !(flags & 4) && out_mode != 2 && (!(flags & 1) || m >= 2 * (k - off_a));
+    const bool plain = !(flags & 4) && out_mode != 2;
     const bool large = n >= 6144 && k >= 6144;
-    const bool big = (flags & 512) ? true : (flags & 256) ? false : ((flat || large) && t256 >= 512);
+    const bool big = (flags & 512) ? true : (flags & 256) ? false : ((plain || large) && t256 >= 512);
     const int64_t bt = big ? 256 : 128;
     g.tm = (m + bt - 1) / bt; g.tn = (n + bt - 1) / bt;
     g.vecA = aligned16(A, lda, pA) && (sA % 8 == 0);
     g.vecB = aligned16(B, ldb, pB) && (sB % 8 == 0);
-    if (g.tm * g.tn > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
-    dim3 grid((unsigned)(g.tm * g.tn), 1, (unsigned)batch);
+    g.tmp = ((flags & 1) || out_mode == 2) && g.tm % 2 == 0 ? g.tm + 1 : g.tm;
+    if (g.tmp * g.tn > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
+    dim3 grid((unsigned)(g.tmp * g.tn), 1, (unsigned)batch);
 #define CFMI_B3_GO(MODE_, BT_) \
     hipLaunchKernelGGL((gemm_b3_kernel<MODE_, BT_>), grid, dim3(Cfg<BT_>::NT), 0, c->stream, g)
     if (big) {
