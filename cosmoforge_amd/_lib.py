@@ -56,6 +56,7 @@ SIGNATURES = {
                           i64, i64, i64, ci, i64, i64, i64, cd]),
     "cfmi_split3": (ci, [vp, ci, i64, i64, vp, i64, vp, i64, i64]),
     "cfmi_dot_batched": (ci, [vp, i64, i64, vp, i64, i64, vp, i64, vp, i64]),
+    "cfmi_offdiag_nnz": (ci, [vp, i64, vp, i64, _c.POINTER(i64)]),
     "cfmi_join3": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),

@@ -78,9 +78,42 @@ __global__ __launch_bounds__(CT) void dot_reduce_kernel(int64_t nparts, int64_t 
     if (threadIdx.x == 0) R[a + (b0 + q) * ldr] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Off-diagonal nonzeros of an n x n column-major matrix: column j = blockIdx.y,
+// rows from blockIdx.x; one atomic add per wave.
+__global__ __launch_bounds__(CT) void offdiag_nnz_kernel(int64_t n, const double* A, int64_t lda,
+                                                         unsigned long long* count)
+{
+    const int64_t j = blockIdx.y, i = (int64_t)blockIdx.x * CT + threadIdx.x;
+    const unsigned long long v = (i < n && i != j && A[i + j * lda] != 0.0) ? 1ull : 0ull;
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(count, s);
+}
+
 }  // namespace
 
 extern "C" {
+
+int cfmi_offdiag_nnz(cfmi_ctx* c, int64_t n, const double* A, int64_t lda, int64_t* out)
+{
+    if (!c || !out) return CFMI_EINVAL;
+    *out = 0;
+    if (n <= 1) return CFMI_OK;
+    if (lda < n) return cfmi_fail(c, CFMI_EINVAL, "offdiag_nnz: lda < n");
+    if (n > 65535) return cfmi_fail(c, CFMI_EINVAL, "offdiag_nnz: n > 65535 (one grid row per column)");
+    // slot 8200 of the reduction staging (potrf uses [0, 4096), logdet 8192)
+    unsigned long long* dev = reinterpret_cast<unsigned long long*>(c->d_red + 8200);
+    CFMI_HIP(c, hipMemsetAsync(dev, 0, sizeof(unsigned long long), c->stream));
+    dim3 grid((unsigned)((n + CT - 1) / CT), (unsigned)n);
+    hipLaunchKernelGGL(offdiag_nnz_kernel, grid, dim3(CT), 0, c->stream, n, A, lda, dev);
+    CFMI_LAUNCH_CHECK(c);
+    unsigned long long h = 0;
+    CFMI_HIP(c, hipMemcpyAsync(&h, dev, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CFMI_HIP(c, hipStreamSynchronize(c->stream));
+    *out = (int64_t)h;
+    return CFMI_OK;
+}
 
 int cfmi_dot_batched(cfmi_ctx* c, int64_t K, int64_t na, const double* A, int64_t lda, int64_t nb,
                      const double* B, int64_t ldb, double* R, int64_t ldr)

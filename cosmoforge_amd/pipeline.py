@@ -15,6 +15,8 @@ buffers and walks the spectra / bins in the reference's order:
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -44,6 +46,17 @@ class Covariance:
         self.factor = None
         self._w32 = None
         self._wp = None
+        self._noise_diag = None
+
+    def noise_is_diagonal(self) -> bool:
+        """True when N has no nonzero off-diagonal entry (counted once, on
+        the device)."""
+        if self._noise_diag is None:
+            cnt = ctypes.c_int64()
+            self.ctx.check(self.ctx.lib.cfmi_offdiag_nnz(self.ctx.h, self.n, P(self.N), self.n, ctypes.byref(cnt)),
+                           "offdiag_nnz")
+            self._noise_diag = cnt.value == 0
+        return self._noise_diag
 
     def signal(self, cls=None) -> torch.Tensor:
         ctx = self.ctx
@@ -479,6 +492,17 @@ def noise_weighted_inverse(cov: Covariance, precision: str = "fp64") -> torch.Te
     W = whitener(cov, precision)
     Ci = torch.empty((n, n), dtype=T, device=ctx.device)
     engine.gemm(ctx, True, False, n, n, n, 1.0, W, 0, n, W, 0, n, 0.0, Ci, 0, n, 2)
+    if T == torch.float64 and cov.noise_is_diagonal():
+        # G = (C^-1 N) C^-1 with N diagonal: scale the columns of a copy of
+        # C^-1 by diag(N), then one product, lower tiles only (G symmetric).
+        d = torch.empty(n, dtype=T, device=ctx.device)
+        ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, 1, n, P(cov.N), n + 1, P(d), 1), "copy")
+        T1 = torch.empty_like(Ci)
+        ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, n, n, P(Ci), n, P(T1), n), "copy")
+        ctx.check(ctx.lib.cfmi_scale(ctx.h, 0, n, n, P(T1), n, P(d)), "scale")
+        G = torch.empty_like(Ci)
+        engine.gemm(ctx, False, False, n, n, n, 1.0, T1, 0, n, Ci, 0, n, 0.0, G, 0, n, 64)
+        return G
     if T == torch.float64:
         Nt = cov.N
     else:

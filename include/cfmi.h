@@ -264,6 +264,11 @@ int cfmi_gram(cfmi_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t K, const v
 int cfmi_dot_batched(cfmi_ctx *ctx, int64_t K, int64_t na, const double *A, int64_t lda, int64_t nb,
                      const double *B, int64_t ldb, double *R, int64_t ldr);
 
+/* *out = number of nonzero off-diagonal entries of an n x n FP64 matrix
+ * (n <= 65535).  Lets the QML's G = C^-1 N C^-1 use the structure of a
+ * diagonal N (the common white-noise case of spectra.py:1124-1186).      */
+int cfmi_offdiag_nnz(cfmi_ctx *ctx, int64_t n, const double *A, int64_t lda, int64_t *out);
+
 /* ---- A18 / A21: batched exact likelihood -------------------------------- */
 /* A batch of njobs covariance matrices C_j = sum_k coef[j][k] M_{p(j),k},
  * p(j) = job_patch[j] (host array), from per-patch FP64 component matrices

@@ -380,3 +380,37 @@ def test_dot_batched(ctx, K, na, nb):
     scale = np.sqrt(K) * max(1.0, np.max(np.abs(A)) * np.max(np.abs(B)))
     assert np.max(np.abs(outs[0] - want)) <= 1e-13 * scale
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_offdiag_nnz_and_diagonal_noise_G(ctx):
+    """cfmi_offdiag_nnz counts off-diagonal nonzeros; the QML's G for a
+    diagonal N (one symmetric product) equals the general three-product
+    form to FP64 rounding."""
+    import ctypes
+
+    import torch
+    from cosmoforge_amd import pipeline
+    from cosmoforge_amd.core import Inputs
+    from cosmoforge_amd.engine import P
+    from tests import cases
+
+    rng = np.random.default_rng(2)
+    A = np.diag(rng.random(300) + 1.0)
+    A[5, 7] = 1e-300
+    A[299, 0] = -2.0
+    Ad = ctx.upload_cm(A)
+    cnt = ctypes.c_int64()
+    ctx.check(ctx.lib.cfmi_offdiag_nnz(ctx.h, 300, P(Ad), 300, ctypes.byref(cnt)), "offdiag_nnz")
+    assert cnt.value == 2
+
+    case = cases.case_fisher_nside4("TQU")
+    p = dict(nside=case.nside, spins=case.spins, labels=case.labels, lmin=case.lmin, lmax=case.lmax,
+             lmax_signal=case.lmax_signal, ordering="NESTED" if case.nest else "RING")
+    cov = pipeline.Covariance(ctx, Inputs(p, case.mask, case.noise, case.cls, case.beam))
+    cov.build()
+    assert cov.noise_is_diagonal()
+    G1 = pipeline.noise_weighted_inverse(cov, "fp64").cpu().numpy()
+    cov._noise_diag = False                       # force the general three-product form
+    G3 = pipeline.noise_weighted_inverse(cov, "fp64").cpu().numpy()
+    assert np.max(np.abs(G1 - G3)) <= 1e-12 * np.max(np.abs(G3))
+    assert np.array_equal(G1, G1.T)

@@ -259,3 +259,31 @@ def test_device_resident_noise_matches_host():
     f2.run()
     assert np.array_equal(f1.get_fisher_matrix(), f2.get_fisher_matrix())
     assert np.array_equal(f2.reduced_noise_cov1, np.asarray(case.noise))
+
+
+def test_qml_correlated_noise_vs_oracle():
+    """A non-diagonal (correlated) noise covariance takes the general
+    G = C^-1 N C^-1 path: Fisher, QML estimates and noise bias against the
+    oracle."""
+    from cosmoforge_amd import Fisher, Spectra
+    from oracle import pipeline as OP
+
+    case = cases.case_fisher_nside4("TQU")
+    rng = np.random.default_rng(11)
+    lay = R.layout(case)
+    Q = rng.standard_normal((lay.n, 20))
+    case.noise = np.asarray(case.noise) + 0.05 * (Q @ Q.T) / 20.0
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    Fo, Cinv, S, = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, prm)
+    maps = np.linalg.cholesky(S + case.noise) @ rng.standard_normal((lay.n, 1))
+    q, nb, cho, nbho = OP.qml(lay, Cinv, case.noise, maps, Fo, prm)
+    f = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam)
+    f.run()
+    assert not f.cov.noise_is_diagonal()
+    assert _rel(f.get_fisher_matrix(), Fo) < 1e-10
+    sp = Spectra(fisher=f, maps1=maps)
+    sp.run()
+    assert _rel(sp.qml_results, q) < 1e-10
+    assert _rel(sp.qml_noise_bias, nb) < 1e-10
