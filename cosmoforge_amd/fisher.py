@@ -101,6 +101,7 @@ class Fisher:
         self.n_params = len(self.spectra_list) * len(self.bins)
         self.ctx.sync()
         t1 = time.perf_counter()
+        self._bandpower_window = None
         self.fisher, self.derivs = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
                                                    timer=self.timer, precision=self.precision,
                                                    launch_timer=self.launch_timer, mixed_gemm=self.mixed_gemm)
@@ -158,3 +159,54 @@ class Fisher:
         """The beam-smoothed Fisher matrix, as the reference defines the
         window of the convolved mode (``fisher.py:1000-1049``)."""
         return self.get_fisher_matrix()
+
+    def get_bandpower_window_function(self, per_ell_fisher: np.ndarray | None = None) -> np.ndarray:
+        """W = F_b^-1 Q F_perell (``fisher.py:1051-1184``): maps a per-l
+        (unbeamed, output-convention) theory to the expected binned
+        deconvolved bandpowers, <C_hat_b> = W C_l.  Q sums the l of each bin
+        (block-diagonal over spectra); F_perell is the delta_ell = 1 Fisher of
+        the same covariance, computed on the device when not given, and the
+        result is cached.  Shape (nspectra nbins, nspectra n_ell), blocks
+        spectrum-major as ``get_bandpower_slices``."""
+        import scipy.linalg
+
+        if self.bins is None or self.fisher is None:
+            raise ValueError("Bandpower window requires binning. Call set_binning() before run().")
+        cached = getattr(self, "_bandpower_window", None)
+        if per_ell_fisher is None and cached is not None:
+            return cached
+        lmin, lmax = self.params.lmin, self.params.lmax
+        n_ell = lmax - lmin + 1
+        if per_ell_fisher is None:
+            per_ell = [(ell, ell) for ell in range(lmin, lmax + 1)]
+            per_ell_fisher, _ = pipeline.fisher(self.cov, per_ell, self.params.output_convention,
+                                                precision=self.precision, keep_stacks=False,
+                                                mixed_gemm=self.mixed_gemm)
+        per_ell_fisher = np.asarray(per_ell_fisher, dtype=np.float64)
+        nspec = len(self.spectra_list)
+        if per_ell_fisher.shape != (nspec * n_ell, nspec * n_ell):
+            raise ValueError(f"per_ell_fisher has shape {per_ell_fisher.shape}, expected "
+                             f"({nspec * n_ell}, {nspec * n_ell})")
+        Qb = np.zeros((len(self.bins), n_ell))
+        for b, (lo, hi) in enumerate(self.bins):
+            Qb[b, lo - lmin: hi - lmin + 1] = 1.0
+        Q = np.kron(np.eye(nspec), Qb)
+        W = scipy.linalg.cho_solve(scipy.linalg.cho_factor(self.fisher, lower=True), Q @ per_ell_fisher)
+        self._bandpower_window = W
+        return W
+
+    def get_effective_ells(self) -> np.ndarray:
+        """Per bin, the window-weighted centroid sum_l W[b, l] l / sum_l W[b, l]
+        over the bin's own spectrum block (``fisher.py:1186-1232``)."""
+        if self.fisher is None:
+            raise ValueError("get_effective_ells requires a completed Fisher run. Call run() first.")
+        W = self.get_bandpower_window_function()
+        lmin = self.params.lmin
+        n_ell = self.params.lmax - lmin + 1
+        nb = len(self.bins)
+        ells = np.arange(lmin, lmin + n_ell, dtype=np.float64)
+        eff = np.empty(len(self.spectra_list) * nb)
+        for s in range(len(self.spectra_list)):
+            Wb = W[s * nb:(s + 1) * nb, s * n_ell:(s + 1) * n_ell]
+            eff[s * nb:(s + 1) * nb] = (Wb @ ells) / Wb.sum(axis=1)
+        return eff

@@ -141,7 +141,9 @@ class Spectra:
         return self._as_dict(err) if as_dict else err
 
     def get_effective_ells(self, use_midpoint: bool = False):
+        """Bin midpoints (``use_midpoint``), else the window centroids of
+        ``Fisher.get_effective_ells`` (``spectra.py:1686-1711``)."""
         bins = self.fisher_instance.bins
         if use_midpoint:
             return np.array([0.5 * (a + b) for a, b in bins])
-        raise NotImplementedError("window-centroid effective ells are not part of this build")
+        return self.fisher_instance.get_effective_ells()

@@ -287,3 +287,53 @@ def test_qml_correlated_noise_vs_oracle():
     sp.run()
     assert _rel(sp.qml_results, q) < 1e-10
     assert _rel(sp.qml_noise_bias, nb) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["T", "TQU"])
+def test_bandpower_window_and_effective_ells_vs_oracle(name):
+    """Fisher.get_bandpower_window_function with delta_ell = 2 bins on
+    nside-4 T and TQU: W = F_b^-1 Q F_perell against the oracle's F_b and
+    F_perell (tolerance scaled by cond(F_b));
+    each spectrum's rows sum to 1 over its own l (W Q^T = I); effective
+    multipoles as the oracle's window gives; Spectra delegates to Fisher."""
+    import scipy.linalg
+    from cosmoforge_amd import Fisher, Spectra
+    from oracle import pipeline as OP
+
+    case = cases.case_fisher_nside4(name)
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    lmin, lmax = case.lmin, case.lmax
+    bins2 = [(lo, min(lo + 1, lmax)) for lo in range(lmin, lmax + 1, 2)]
+    per_ell = [(ell, ell) for ell in range(lmin, lmax + 1)]
+    pw = lambda b: OP.parameter_weights(lay, beams, lmin, lmax, case.lmax_signal, bins=b,  # noqa: E731
+                                        convention=case.convention)
+    Fb, _, S = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, pw(bins2))
+    Fl, _, _ = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, pw(per_ell), S=S)
+    nspec, n_ell = len(lay.spectra_list()), lmax - lmin + 1
+    Qb = np.zeros((len(bins2), n_ell))
+    for b, (lo, hi) in enumerate(bins2):
+        Qb[b, lo - lmin: hi - lmin + 1] = 1.0
+    Q = np.kron(np.eye(nspec), Qb)
+    Wo = scipy.linalg.cho_solve(scipy.linalg.cho_factor(Fb, lower=True), Q @ Fl)
+
+    f = Fisher(_params(case), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam)
+    f.set_binning(bins2)
+    f.run()
+    W = f.get_bandpower_window_function()
+    tol = 1e-10 * np.linalg.cond(Fb)
+    assert np.max(np.abs(W - Wo)) <= tol * np.max(np.abs(Wo))
+    if np.linalg.cond(Fb) < 1e8:       # W Q^T = F_b^-1 (Q F_perell Q^T) = I up to conditioning
+        assert np.allclose(W @ Q.T, np.eye(W.shape[0]), atol=1e-8)
+    eff = f.get_effective_ells()
+    nb = len(bins2)
+    ells = np.arange(lmin, lmax + 1, dtype=np.float64)
+    eff_o = np.concatenate([(Wo[s * nb:(s + 1) * nb, s * n_ell:(s + 1) * n_ell] @ ells)
+                            / Wo[s * nb:(s + 1) * nb, s * n_ell:(s + 1) * n_ell].sum(axis=1) for s in range(nspec)])
+    assert np.max(np.abs(eff - eff_o)) <= tol * np.max(np.abs(eff_o))
+    sp = Spectra(fisher=f, maps1=np.random.default_rng(3).standard_normal((f.cov.n, 1)))
+    assert np.array_equal(sp.get_effective_ells(), eff)
+    assert f.get_bandpower_window_function() is W          # cached
+    with pytest.raises(ValueError, match="per_ell_fisher has shape"):
+        f.get_bandpower_window_function(per_ell_fisher=np.eye(3))
