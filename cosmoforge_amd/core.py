@@ -4,7 +4,7 @@ Mirrors the reference's ``Core`` seams for in-memory inputs
 (``src/cosmoforge.cosmocore/cosmocore/core.py:224-766``, ADR-0017):
 
 * ``params``: dict, YAML path or an object with attributes; defaults follow
-  ``settings.py:178-253``; the multipole chain
+  ``settings.py:178-253`` (file paths included); the multipole chain
   ``max(lmin_signal) <= lmin <= lmax <= lmax_signal`` clamps ``lmax``
   (``core.py:378-401``);
 * ``mask`` (npix,) or (npix, ncomp), active = > 0.5, Q/U columns must agree
@@ -15,6 +15,12 @@ Mirrors the reference's ``Core`` seams for in-memory inputs
   required (``spectra_io.py:257-326``);
 * ``beam``: (>= 3, >= lmax_signal + 1) T/E/B windows; wins over
   ``smoothing_type`` (``beam.py:128-221``).
+
+An input given as an array wins; without one the file named by ``params``
+is read (``maskfile``, ``covmatfile1`` with ``load_reduced``,
+``inputclfile`` with ``input_convention``, ``beam_file`` when
+``smoothing_type == "file"``, ``inputmapfile1``) through ``in_out``, as the
+reference's file adapters do (``core.py:403-472,638-714``, ``fields.py:915-940``).
 """
 
 from __future__ import annotations
@@ -25,6 +31,7 @@ from types import SimpleNamespace
 
 import numpy as np
 
+from . import in_out
 from .layout import Layout, build_layout
 
 DEFAULTS = dict(
@@ -33,6 +40,10 @@ DEFAULTS = dict(
     delta_ell=1, bin_lmins=None, bin_lmaxs=None, ordering="RING", nsims=None, remove_nb=True,
     input_convention="Cl", output_convention="Cl", lmin_signal=2, lmax_signal=None, lmin=2,
     parameters={}, fiducialfile=None,
+    maskfile="inputs/mask.fits", covmatfile1="inputs/NCVM1.bin", inputclfile="inputs/cls.dat",
+    beam_file="inputs/beam.fits", inputmapfile1="", load_reduced=False, output_geometry_file=None,
+    outnoisecovmat1=None, outinvcovmatfile1=None, outfilefisher=None, outcovmatfile=None, outerrfile=None,
+    root_dir="inputs", root_filename="theory_spectra",
 )
 
 
@@ -56,6 +67,10 @@ def read_params(params) -> SimpleNamespace:
     p = SimpleNamespace(**values)
     if isinstance(p.labels, str):
         p.labels = list(p.labels)
+    if p.physical_labels is None:
+        p.physical_labels = list(p.labels)
+    elif isinstance(p.physical_labels, str):
+        p.physical_labels = list(p.physical_labels)
     p.nfields = sum(1 if s == 0 else 2 for s in p.spins)
     ls = p.lmin_signal
     ls = [ls] * len(p.spins) if isinstance(ls, int) else list(ls)
@@ -100,7 +115,8 @@ def resolve_basis(basis):
 
 def resolve_mask(p, mask) -> np.ndarray:
     if mask is None:
-        raise ValueError("mask= is required (file inputs are not part of this build)")
+        nest = str(p.ordering).upper() == "NESTED"
+        mask = in_out.read_mask(p.maskfile, np.empty((p.nfields, 0)), nest=nest)
     mask = np.asarray(mask, dtype=np.float64)
     if mask.ndim == 1:
         mask = mask[:, None]
@@ -163,7 +179,12 @@ def resolve_beam(p, beam, lmax: int) -> np.ndarray:
         b = b[: lmax + 1]
         return np.vstack([b, b, b])
     if st == "file":
-        raise ValueError("smoothing_type='file' needs the beam= array (file inputs are not part of this build)")
+        bls = in_out.read_cl_fits(p.beam_file)
+        if bls.shape[0] < 3:
+            raise ValueError(f"Beam file must have at least 3 columns (T, E, B), got {bls.shape[0]}")
+        if bls.shape[1] < lmax + 1:
+            raise ValueError(f"Beam shape mismatch: expected (3, {lmax + 1}), got (3, {bls.shape[1]})")
+        return bls[:3, : lmax + 1].copy()
     raise ValueError(f"Unknown smoothtype='{st}'")
 
 
@@ -190,9 +211,11 @@ def smoothing_factors(layout: Layout, beams: dict, lmax: int) -> dict:
     return out
 
 
-def resolve_cls(layout: Layout, cls_data, lmax: int) -> dict:
+def resolve_cls(layout: Layout, cls_data, lmax: int, p=None) -> dict:
     if cls_data is None:
-        raise ValueError("cls_data= is required (file inputs are not part of this build)")
+        if p is None:
+            raise ValueError("cls_data= is required without params")
+        cls_data = in_out.readcl(p.inputclfile, p, lmax=lmax)
     if isinstance(cls_data, np.ndarray):
         if cls_data.ndim != 2 or cls_data.shape[1] != len(layout.labels):
             raise ValueError(f"Expected {len(layout.labels)} spectra columns, got {cls_data.shape}")
@@ -208,11 +231,20 @@ def resolve_cls(layout: Layout, cls_data, lmax: int) -> dict:
     return out
 
 
-def resolve_noise(n: int, noise_cov):
+def resolve_noise(n: int, noise_cov, p=None, layout: Layout | None = None):
     """The (n, n) noise covariance: a host array, or a torch tensor already on
-    the device (kept there, so no host copy or PCIe transfer per solve)."""
+    the device (kept there, so no host copy or PCIe transfer per solve).
+    Without one, ``p.covmatfile1`` is read: already reduced when
+    ``p.load_reduced``, else reduced from the full-sky file
+    (``core.py:638-680``)."""
     if noise_cov is None:
-        raise ValueError("noise_cov1= is required (file inputs are not part of this build)")
+        if p is None or layout is None:
+            raise ValueError("noise_cov1= is required without params")
+        C = np.empty((n, n), dtype=np.float64)
+        if p.load_reduced:
+            return in_out.read_covmat_reduced(p.covmatfile1, C)
+        npix = 12 * p.nside**2
+        return in_out.read_covmat(p.covmatfile1, npix, p.nfields, layout.active_index(npix), C)
     if _is_device_tensor(noise_cov):
         N = noise_cov
         if tuple(N.shape) != (n, n):
@@ -232,9 +264,17 @@ def _is_device_tensor(x) -> bool:
     return isinstance(x, torch.Tensor) and x.is_cuda
 
 
-def resolve_maps(n: int, nsims, maps) -> np.ndarray:
+def resolve_maps(n: int, nsims, maps, p=None, layout: Layout | None = None) -> np.ndarray:
+    """(n, nsims) maps: the array given, or ``p.inputmapfile1`` read for the
+    per-component active pixels and ``p.physical_labels`` (``core.py:682-714``)."""
     if maps is None:
-        raise ValueError("maps1= is required (file inputs are not part of this build)")
+        if p is None or layout is None:
+            raise ValueError("maps1= is required without params")
+        if not nsims:
+            raise ValueError("nsims must be set to read maps from inputmapfile1")
+        m = np.empty((n, int(nsims)), dtype=np.float64)
+        pixact = [f.pixels for f in layout.fields for _ in range(2 if f.spin == 2 else 1)]
+        return in_out.read_maps(m, p.inputmapfile1, pixact, p.physical_labels)
     m = np.asarray(maps, dtype=np.float64)
     if m.ndim == 1:
         m = m[:, None]
@@ -278,8 +318,9 @@ class Inputs:
         self.mask = resolve_mask(p, mask)
         self.layout = build_layout(self.mask, p.spins, p.labels)
         self.n = self.layout.n
-        self.noise = resolve_noise(self.n, noise_cov1)
-        self.cls = resolve_cls(self.layout, cls_data, self.lmax_signal)
+        self.noise = resolve_noise(self.n, noise_cov1, p, self.layout)
+        in_out.write_covmat_reduced(p.outnoisecovmat1, self.noise)
+        self.cls = resolve_cls(self.layout, cls_data, self.lmax_signal, p)
         self.beam3 = resolve_beam(p, beam, self.lmax_signal)
         self.beams = beam_by_label(self.layout, self.beam3)
         self.smoothing = smoothing_factors(self.layout, self.beams, self.lmax_signal)

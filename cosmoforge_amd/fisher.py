@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import engine, pipeline
+from . import engine, in_out, pipeline
 from .core import Inputs, bins_from_params, resolve_basis
 
 
@@ -85,6 +85,8 @@ class Fisher:
         if self.cov is None:
             self.cov = pipeline.Covariance(self.ctx, self.inputs)
         self.cov.build()
+        if not in_out._falsy(self.params.outinvcovmatfile1):
+            in_out.write_covmat_reduced(self.params.outinvcovmatfile1, engine.to_numpy_cm(self.cov.inverse()))
 
     def run(self) -> None:
         import time
@@ -107,6 +109,7 @@ class Fisher:
                                                    launch_timer=self.launch_timer, mixed_gemm=self.mixed_gemm)
         self.ctx.sync()
         self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
+        in_out.write_out_matrix(self.params.outfilefisher, self.fisher)
 
     compute = run
 

@@ -18,26 +18,67 @@ from copy import deepcopy
 
 import numpy as np
 
-from . import engine, pipeline
+from . import engine, in_out, pipeline
 from .core import Inputs, resolve_basis, resolve_maps
 
 
+def _blend(param: dict, fid: dict, lmin: int, lmax: int) -> dict:
+    """Fiducial spectrum outside [lmin, lmax], the point's inside
+    (``parameter_grid.py:225-260``)."""
+    out = {}
+    for key, arr in param.items():
+        if key not in fid:
+            out[key] = arr.copy()
+            continue
+        n = min(len(arr), len(fid[key]))
+        res = fid[key][:n].copy()
+        lo, hi = max(0, lmin), min(n - 1, lmax)
+        if lo <= hi:
+            res[lo: hi + 1] = arr[lo: hi + 1]
+        out[key] = res
+    return out
+
+
 class ParameterGrid:
+    """Grid points (Cartesian product of ``parameter_ranges``) and their
+    spectra: given in memory, or read per point from
+    ``{root_dir}/{root_filename}_{name}{value:.6g}[_...].txt`` and blended
+    with ``core_params.fiducialfile`` outside [lmin, lmax]
+    (``parameter_grid.py:85-262``)."""
+
     def __init__(self, core_params, parameter_ranges: dict, theoretical_spectra: dict | None = None,
                  root_dir=None, root_filename=None, lmax_signal=None):
-        if theoretical_spectra is None:
-            raise NotImplementedError("file-backed theory spectra are not part of this build; pass "
-                                      "theoretical_spectra={point: {label: C_l}}")
         self.core_params = core_params
         self.lmax_signal = lmax_signal if lmax_signal is not None else 4 * core_params.nside
         self.parameter_names = list(parameter_ranges.keys())
         self.parameter_ranges = deepcopy(parameter_ranges)
         vals = [self.parameter_ranges[k] for k in self.parameter_names]
         self.grid_points = list(itertools.product(*vals)) if vals else []
-        self.theoretical_spectra = deepcopy(theoretical_spectra)
+        if theoretical_spectra is not None:
+            self.theoretical_spectra = deepcopy(theoretical_spectra)
+        else:
+            if root_dir is None or root_filename is None:
+                raise ValueError("Either theoretical_spectra or both root_dir and root_filename must be provided")
+            self.root_dir, self.root_filename = root_dir, root_filename
+            self.read_theoretical_spectra()
         missing = [p for p in self.grid_points if p not in self.theoretical_spectra]
         if missing:
-            raise ValueError(f"no theoretical spectra for grid points {missing[:3]}")
+            raise ValueError(f"Missing theoretical spectra for {len(missing)} grid points")
+
+    def spectrum_file(self, point) -> str:
+        tag = "_".join(f"{self.parameter_names[i]}{v:.6g}" for i, v in enumerate(point))
+        return f"{self.root_dir}/{self.root_filename}_{tag}.txt"
+
+    def read_theoretical_spectra(self) -> None:
+        p = self.core_params
+        lmin, lmax = getattr(p, "lmin", None), getattr(p, "lmax", None)
+        fid = None
+        if getattr(p, "fiducialfile", None) is not None and lmin is not None and lmax is not None:
+            fid = in_out.readcl(p.fiducialfile, p, lmax=self.lmax_signal)
+        self.theoretical_spectra = {}
+        for point in self.grid_points:
+            spec = in_out.readcl(self.spectrum_file(point), p, lmax=self.lmax_signal)
+            self.theoretical_spectra[point] = spec if fid is None else _blend(spec, fid, lmin, lmax)
 
     def get_spectrum(self, point):
         return self.theoretical_spectra[point]
@@ -82,7 +123,7 @@ class PICSLike:
             cls_data = next(iter(theoretical_spectra.values()))
         self.inputs = Inputs(params, mask, noise_cov1, cls_data if cls_data is not None else fiducial_cls, beam)
         self.params = self.inputs.params
-        self.maps1 = resolve_maps(self.inputs.n, self.params.nsims, maps1)
+        self.maps1 = resolve_maps(self.inputs.n, self.params.nsims, maps1, self.params, self.inputs.layout)
         self.params.nsims = self.maps1.shape[1]
         if parameter_grid is None and theoretical_spectra is not None:
             pts = list(theoretical_spectra)
@@ -109,9 +150,21 @@ class PICSLike:
         chi2, logdet = pipeline.likelihood(self.cov, self.maps1, cls)
         return chi2, -0.5 * (chi2 + logdet)
 
+    def setup_parameter_grid(self) -> None:
+        """Grid from ``params.parameters`` ({name: [min, max, n]} -> linspace)
+        with spectra read from ``root_dir`` / ``root_filename``
+        (``picslike.py:409-431``)."""
+        names = list(self.params.parameters.keys())
+        ranges = {k: np.linspace(*self.params.parameters[k]) for k in names}
+        self.parameter_grid = ParameterGrid(self.params, ranges, root_dir=self.params.root_dir,
+                                            root_filename=self.params.root_filename,
+                                            lmax_signal=self.lmax_signal)
+
     def compute(self):
+        if self.parameter_grid is None and self.params.parameters:
+            self.setup_parameter_grid()
         if self.parameter_grid is None:
-            raise ValueError("PICSLike needs parameter_grid= or theoretical_spectra=")
+            raise ValueError("PICSLike needs parameter_grid=, theoretical_spectra= or params.parameters")
         pts = self.parameter_grid.grid_points
         ns = self.maps1.shape[1]
         chi2 = np.zeros((len(pts), ns))

@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import engine
+from . import engine, in_out
 from .core import Inputs, shape_weights
 from . import _lib
 from .engine import P
@@ -35,6 +35,10 @@ class Covariance:
         p = inputs.params
         nest = str(p.ordering).upper() == "NESTED"
         self.vecs = [engine.pointings(ctx, p.nside, f.pixels, nest) for f in inputs.layout.fields]
+        if not in_out._falsy(p.output_geometry_file):
+            fields = inputs.layout.fields
+            in_out.output_geometry(p.output_geometry_file, [f.pixels.size for f in fields],
+                                   [v.cpu().numpy() for v in self.vecs], [f.pixels for f in fields])
         self.n = inputs.n
         N = inputs.noise
         if isinstance(N, torch.Tensor):

@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import engine, pipeline
+from . import engine, in_out, pipeline
 from .core import resolve_maps
 from .fisher import Fisher
 
@@ -45,7 +45,7 @@ class Spectra:
             from .core import read_params
 
             extra = read_params(params)
-            for key in ("nsims", "remove_nb"):
+            for key in ("nsims", "remove_nb", "inputmapfile1", "outcovmatfile", "outerrfile"):
                 setattr(self.params, key, getattr(extra, key))
         self._maps_in = maps1
         self.maps1 = None
@@ -60,7 +60,8 @@ class Spectra:
         return self.fisher_instance.ctx
 
     def setup_maps(self):
-        self.maps1 = resolve_maps(self.fisher_instance.inputs.n, self.params.nsims, self._maps_in)
+        inp = self.fisher_instance.inputs
+        self.maps1 = resolve_maps(inp.n, self.params.nsims, self._maps_in, self.params, inp.layout)
 
     def setup_fisher_inversion(self):
         F = self.fisher_instance.get_fisher_matrix()
@@ -68,6 +69,12 @@ class Spectra:
             raise ValueError("Fisher matrix not available")
         self.fisher_normalized = F.copy()
         self.invfisher = pipeline.inverse_symm(self.ctx, F)
+        # F^-1 and the error bars as (nbins, nspectra) columns (spectra.py:651-661)
+        in_out.write_out_matrix(self.params.outcovmatfile, self.invfisher)
+        if not in_out._falsy(self.params.outerrfile):
+            err = np.sqrt(np.diag(self.invfisher))
+            nb = len(self.fisher_instance.bins)
+            in_out.writecl(self.params.outerrfile, err.reshape(-1, nb).T)
 
     def run(self):
         f = self.fisher_instance

@@ -18,6 +18,15 @@ Fixtures written (all float64, ``np.savez_compressed``):
   (``qube/tests/test_fisher.py:83-129``).
 * ``picslike_nside8_B.npz`` -- ``picslike/tests/data/nside8/B/fortran_reference``
   (``picslike/tests/test_fortran_validation.py:162-426``).
+* ``files/`` -- data files copied byte for byte (text files cut to their first
+  ``CL_ROWS`` multipole rows, the multi-simulation FITS file to its first two
+  ``SIM_XXX`` units) for the file-reader tests:
+  ``nside4_TQU_mask.fits`` and ``nside4_TQU_geometry.dat``
+  (``picslike/tests/data/nside4/TQU``: the reference's ``output_geometry`` of
+  that full-sky RING geometry), ``nside4_all_sims_TQU_2.fits``
+  (``nside4/maps/all_sims_TQU.fits``), and from ``nside8/B/fortran_reference``
+  ``B_mask.fits``, ``B_beam.fits``, ``B_map.fits``, ``B_input_dls.txt``,
+  ``B_fiducial_dls.txt`` and ``B_theory_r{0,1.25e-05,2.5e-05}.txt``.
 """
 
 from __future__ import annotations
@@ -49,8 +58,60 @@ def read_cl_text(path):
     return np.array(hdr), np.loadtxt(path)
 
 
+CL_ROWS = 63            # multipoles 2..64 of a C_l text file
+
+
+def copy_files(out: str) -> None:
+    import shutil
+
+    dst = os.path.join(out, "files")
+    os.makedirs(dst, exist_ok=True)
+    t4 = os.path.join(PL, "nside4/TQU")
+    fr = os.path.join(PL, "nside8/B/fortran_reference")
+    for src, name in ((os.path.join(t4, "mask.fits"), "nside4_TQU_mask.fits"),
+                      (os.path.join(t4, "geometry.dat"), "nside4_TQU_geometry.dat"),
+                      (os.path.join(fr, "mask.fits"), "B_mask.fits"),
+                      (os.path.join(fr, "beam.fits"), "B_beam.fits"),
+                      (os.path.join(fr, "map.fits"), "B_map.fits")):
+        shutil.copyfile(src, os.path.join(dst, name))
+    texts = [(os.path.join(fr, "input_dls.txt"), "B_input_dls.txt"),
+             (os.path.join(fr, "fiducial_dls.txt"), "B_fiducial_dls.txt")]
+    for tag in ("0", "1.25e-05", "2.5e-05"):
+        texts.append((os.path.join(fr, f"theory_spectra/fine_dls_r_likelihood_r{tag}.txt"), f"B_theory_r{tag}.txt"))
+    for src, name in texts:
+        with open(src) as fh:
+            lines = fh.readlines()[: 1 + CL_ROWS]
+        with open(os.path.join(dst, name), "w") as fh:
+            fh.writelines(lines)
+    # primary unit + SIM_000 + SIM_001: each unit is a whole number of 2880-byte records
+    raw = open(os.path.join(PL, "nside4/maps/all_sims_TQU.fits"), "rb").read()
+    off, units = 0, 0
+    while units < 3:
+        nax, bitpix, dims = 0, 8, {}
+        while True:
+            rec = raw[off: off + 2880].decode("ascii")
+            off += 2880
+            cards = [rec[i: i + 80] for i in range(0, 2880, 80)]
+            for c in cards:
+                key = c[:8].strip()
+                if key == "BITPIX":
+                    bitpix = int(c[10:].split("/")[0])
+                elif key == "NAXIS":
+                    nax = int(c[10:].split("/")[0])
+                elif key.startswith("NAXIS"):
+                    dims[key] = int(c[10:].split("/")[0])
+            if any(c[:8].strip() == "END" for c in cards):
+                break
+        nbytes = abs(bitpix) // 8 * int(np.prod([dims[f"NAXIS{a}"] for a in range(1, nax + 1)])) if nax else 0
+        off += -(-nbytes // 2880) * 2880
+        units += 1
+    with open(os.path.join(dst, "nside4_all_sims_TQU_2.fits"), "wb") as fh:
+        fh.write(raw[:off])
+
+
 def main():
     out = HERE
+    copy_files(out)
     a = np.loadtxt(os.path.join(CC, "ref_TQU_signal.dat"))
     np.savez_compressed(os.path.join(out, "signal_tqu_nside4_nest.npz"), n=a.shape[0], tril=tril(a))
     a = np.loadtxt(os.path.join(CC, "ref_T1T2_signal.dat"))

@@ -337,3 +337,89 @@ def test_bandpower_window_and_effective_ells_vs_oracle(name):
     assert f.get_bandpower_window_function() is W          # cached
     with pytest.raises(ValueError, match="per_ell_fisher has shape"):
         f.get_bandpower_window_function(per_ell_fisher=np.eye(3))
+
+
+def _b_config(tmp_path):
+    """The reference's nside-8 B-only PICSLike configuration
+    (``picslike/tests/data/nside8/B/fortran_reference/config.yaml``) over the
+    committed copies of its data files; the full-sky noise binary and the
+    theory-spectra directory are rebuilt from the fixtures."""
+    import os
+    import shutil
+
+    files = os.path.join(cases.GOLDEN, "files")
+    d = cases.load("picslike_nside8_B.npz")
+    cases.untril(768, d["ncov_tril"]).tofile(str(tmp_path / "ncov_B_only.bin"))
+    th = tmp_path / "theory_spectra"
+    th.mkdir()
+    for tag in ("0", "1.25e-05", "2.5e-05"):
+        shutil.copyfile(os.path.join(files, f"B_theory_r{tag}.txt"), str(th / f"fine_dls_r_likelihood_r{tag}.txt"))
+    return dict(
+        nside=8, spins=[0], labels=["B"], do_cross=False, ordering="RING",
+        maskfile=os.path.join(files, "B_mask.fits"), output_geometry_file=str(tmp_path / "out/geometry.dat"),
+        inputclfile=os.path.join(files, "B_input_dls.txt"), input_convention="Dl",
+        covmatfile1=str(tmp_path / "ncov_B_only.bin"), lmax_signal=32, lmin=2, lmax=16,
+        fiducialfile=os.path.join(files, "B_fiducial_dls.txt"), smoothing_type="file",
+        beam_file=os.path.join(files, "B_beam.fits"), apply_pixwin=False,
+        outnoisecovmat1=str(tmp_path / "out/output_ncov.bin"), outinvcovmatfile1=str(tmp_path / "out/invcov.bin"),
+        outfilefisher=str(tmp_path / "out/fisher.dat"), nsims=1,
+        inputmapfile1=os.path.join(files, "B_map.fits"),
+        outcovmatfile=str(tmp_path / "out/cov_matrix.dat"), outerrfile=str(tmp_path / "out/errors.dat"),
+        remove_nb=True, root_dir=str(th), root_filename="fine_dls_r_likelihood",
+        parameters={"r": [0.0, 2.5e-05, 3]})
+
+
+def test_picslike_from_yaml_files_matches_fortran_results(tmp_path):
+    """PICSLike run from a YAML file of paths alone (mask, beam, noise, map,
+    C_l and per-point theory files): the same chi2 / lnL as the in-memory
+    run and the reference's Fortran results."""
+    import yaml
+
+    from cosmoforge_amd import PICSLike
+
+    cfg = _b_config(tmp_path)
+    path = tmp_path / "config.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    print("picslike from files", path, flush=True)
+    pl = PICSLike(str(path), basis=False)
+    pl.compute()
+    _, _, res = cases.case_picslike_b()
+    chi2, lnl = pl.get_chi_squared(), pl.get_log_likelihood()
+    assert np.max(np.abs(chi2 / res[:, 3] - 1)) < 1e-9
+    assert np.max(np.abs(lnl - res[:, 2])) / np.abs(res[:, 2]).mean() < 1e-9
+    assert pl.get_best_fit() == {"r": pl.parameter_grid.grid_points[int(np.argmax(lnl))][0]}
+    g = np.loadtxt(cfg["output_geometry_file"], skiprows=1)
+    ref = cases.case_fortran_b().expect["geometry"]
+    assert np.array_equal(g[:, 0], ref[:, 0])
+    assert np.max(np.abs(g[:, 1:] - ref[:, 1:])) < 1e-15
+    red = np.fromfile(cfg["outnoisecovmat1"]).reshape(768, 768)
+    assert np.array_equal(red, pl.inputs.noise)
+
+
+def test_fisher_spectra_from_files_match_in_memory(tmp_path):
+    """Fisher / Spectra driven by file paths give bit-identical F and C_l to
+    the same inputs passed in memory, and write F, F^-1, the error bars and
+    C^-1 where the configuration names them."""
+    from cosmoforge_amd import Fisher, Spectra
+
+    cfg = _b_config(tmp_path)
+    print("fisher from files", flush=True)
+    f = Fisher(cfg, basis=False)
+    f.run()
+    sp = Spectra(fisher=f)
+    sp.run()
+    case, _, _ = cases.case_picslike_b()
+    f2 = Fisher(_params(case, nsims=1), basis=False, mask=case.mask, noise_cov1=case.noise,
+                cls_data=f.inputs.cls, beam=case.beam)
+    f2.run()
+    sp2 = Spectra(fisher=f2, maps1=case.maps)
+    sp2.run()
+    assert np.array_equal(f.get_fisher_matrix(), f2.get_fisher_matrix())
+    assert np.array_equal(sp.get_power_spectra(), sp2.get_power_spectra())
+    F = f.get_fisher_matrix()
+    assert np.array_equal(np.loadtxt(cfg["outfilefisher"]), F)
+    assert np.array_equal(np.loadtxt(cfg["outcovmatfile"]), sp.get_covariance())
+    err = np.loadtxt(cfg["outerrfile"])
+    assert np.array_equal(err.reshape(-1), sp.get_error_bars())
+    Ci = np.fromfile(cfg["outinvcovmatfile1"]).reshape(768, 768)
+    assert np.array_equal(Ci, f.inverse_covariance)
