@@ -40,9 +40,10 @@ DEFAULTS = dict(
     delta_ell=1, bin_lmins=None, bin_lmaxs=None, ordering="RING", nsims=None, remove_nb=True,
     input_convention="Cl", output_convention="Cl", lmin_signal=2, lmax_signal=None, lmin=2,
     parameters={}, fiducialfile=None,
-    maskfile="inputs/mask.fits", covmatfile1="inputs/NCVM1.bin", inputclfile="inputs/cls.dat",
-    beam_file="inputs/beam.fits", inputmapfile1="", load_reduced=False, output_geometry_file=None,
-    outnoisecovmat1=None, outinvcovmatfile1=None, outfilefisher=None, outcovmatfile=None, outerrfile=None,
+    maskfile="inputs/mask.fits", covmatfile1="inputs/NCVM1.bin", covmatfile2="inputs/NCVM2.bin",
+    inputclfile="inputs/cls.dat", beam_file="inputs/beam.fits", inputmapfile1="", inputmapfile2="",
+    load_reduced=False, output_geometry_file=None, outnoisecovmat1=None, outnoisecovmat2=None,
+    outinvcovmatfile1=None, outinvcovmatfile2=None, outfilefisher=None, outcovmatfile=None, outerrfile=None,
     root_dir="inputs", root_filename="theory_spectra",
 )
 
@@ -88,8 +89,7 @@ def read_params(params) -> SimpleNamespace:
         p.lmax = lmax_signal
     if p.lmax is not None and p.lmax < p.lmin:
         raise ValueError(f"lmax={p.lmax} < lmin={p.lmin}")
-    if p.do_cross:
-        raise NotImplementedError("do_cross (two noise covariances) is not part of this build")
+    p.do_cross = bool(p.do_cross)
     return p
 
 
@@ -231,28 +231,30 @@ def resolve_cls(layout: Layout, cls_data, lmax: int, p=None) -> dict:
     return out
 
 
-def resolve_noise(n: int, noise_cov, p=None, layout: Layout | None = None):
+def resolve_noise(n: int, noise_cov, p=None, layout: Layout | None = None, which: int = 1):
     """The (n, n) noise covariance: a host array, or a torch tensor already on
     the device (kept there, so no host copy or PCIe transfer per solve).
-    Without one, ``p.covmatfile1`` is read: already reduced when
+    Without one, ``p.covmatfile{which}`` is read: already reduced when
     ``p.load_reduced``, else reduced from the full-sky file
     (``core.py:638-680``)."""
+    name = f"noise_cov{which}"
     if noise_cov is None:
         if p is None or layout is None:
-            raise ValueError("noise_cov1= is required without params")
+            raise ValueError(f"{name}= is required without params")
+        path = getattr(p, f"covmatfile{which}")
         C = np.empty((n, n), dtype=np.float64)
         if p.load_reduced:
-            return in_out.read_covmat_reduced(p.covmatfile1, C)
+            return in_out.read_covmat_reduced(path, C)
         npix = 12 * p.nside**2
-        return in_out.read_covmat(p.covmatfile1, npix, p.nfields, layout.active_index(npix), C)
+        return in_out.read_covmat(path, npix, p.nfields, layout.active_index(npix), C)
     if _is_device_tensor(noise_cov):
         N = noise_cov
         if tuple(N.shape) != (n, n):
-            raise ValueError(f"noise_cov1 has shape {tuple(N.shape)}, expected ({n}, {n}) for the active-pixel count")
+            raise ValueError(f"{name} has shape {tuple(N.shape)}, expected ({n}, {n}) for the active-pixel count")
         return N
     N = np.asarray(noise_cov)
     if N.shape != (n, n):
-        raise ValueError(f"noise_cov1 has shape {N.shape}, expected ({n}, {n}) for the active-pixel count")
+        raise ValueError(f"{name} has shape {N.shape}, expected ({n}, {n}) for the active-pixel count")
     return N
 
 
@@ -264,17 +266,18 @@ def _is_device_tensor(x) -> bool:
     return isinstance(x, torch.Tensor) and x.is_cuda
 
 
-def resolve_maps(n: int, nsims, maps, p=None, layout: Layout | None = None) -> np.ndarray:
-    """(n, nsims) maps: the array given, or ``p.inputmapfile1`` read for the
-    per-component active pixels and ``p.physical_labels`` (``core.py:682-714``)."""
+def resolve_maps(n: int, nsims, maps, p=None, layout: Layout | None = None, which: int = 1) -> np.ndarray:
+    """(n, nsims) maps: the array given, or ``p.inputmapfile{which}`` read for
+    the per-component active pixels and ``p.physical_labels``
+    (``core.py:682-714``)."""
     if maps is None:
         if p is None or layout is None:
-            raise ValueError("maps1= is required without params")
+            raise ValueError(f"maps{which}= is required without params")
         if not nsims:
-            raise ValueError("nsims must be set to read maps from inputmapfile1")
+            raise ValueError(f"nsims must be set to read maps from inputmapfile{which}")
         m = np.empty((n, int(nsims)), dtype=np.float64)
         pixact = [f.pixels for f in layout.fields for _ in range(2 if f.spin == 2 else 1)]
-        return in_out.read_maps(m, p.inputmapfile1, pixact, p.physical_labels)
+        return in_out.read_maps(m, getattr(p, f"inputmapfile{which}"), pixact, p.physical_labels)
     m = np.asarray(maps, dtype=np.float64)
     if m.ndim == 1:
         m = m[:, None]
@@ -311,7 +314,7 @@ def shape_weights(lmax: int, convention: str) -> np.ndarray:
 class Inputs:
     """Resolved in-memory inputs shared by the three estimators."""
 
-    def __init__(self, params, mask, noise_cov1, cls_data, beam):
+    def __init__(self, params, mask, noise_cov1, cls_data, beam, noise_cov2=None):
         self.params = read_params(params)
         p = self.params
         self.lmax_signal = p.lmax_signal if p.lmax_signal is not None else 4 * p.nside
@@ -320,6 +323,11 @@ class Inputs:
         self.n = self.layout.n
         self.noise = resolve_noise(self.n, noise_cov1, p, self.layout)
         in_out.write_covmat_reduced(p.outnoisecovmat1, self.noise)
+        # the second noise covariance is read only for a cross analysis (core.py:614-622)
+        self.noise2 = None
+        if p.do_cross:
+            self.noise2 = resolve_noise(self.n, noise_cov2, p, self.layout, which=2)
+            in_out.write_covmat_reduced(p.outnoisecovmat2, self.noise2)
         self.cls = resolve_cls(self.layout, cls_data, self.lmax_signal, p)
         self.beam3 = resolve_beam(p, beam, self.lmax_signal)
         self.beams = beam_by_label(self.layout, self.beam3)

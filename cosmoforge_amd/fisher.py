@@ -26,8 +26,6 @@ class Fisher:
                  precision: str = "fp64", **kwargs):
         if params is None and params_file is not None:
             params = params_file
-        if noise_cov2 is not None:
-            raise NotImplementedError("noise_cov2 / do_cross is not part of this build")
         if kwargs.get("pixel_filter") is not None:
             raise NotImplementedError("pixel_filter is not part of this build")
         if symmetry_mode is not None:
@@ -40,11 +38,12 @@ class Fisher:
                                  "['SYMMETRIC', 'DIRECTIONAL']")
         self.basis = resolve_basis(basis)
         self._cache_derivatives = cache_derivatives
-        self.inputs = Inputs(params, mask, noise_cov1, cls_data, beam)
+        self.inputs = Inputs(params, mask, noise_cov1, cls_data, beam, noise_cov2)
         self.params = self.inputs.params
         self.fiducial_cls = fiducial_cls  # inert on the pixel-space path, as in the reference
         self.ctx = context if context is not None else engine.Context(device)
         self.cov: pipeline.Covariance | None = None
+        self.cov2: pipeline.Covariance | None = None   # C2 = S + N2 of a cross analysis
         self.fisher = None
         self.beam_smoothing = None
         self.spectra_list = None
@@ -52,6 +51,8 @@ class Fisher:
         if precision not in pipeline.PRECISIONS:
             raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
         self.precision = precision
+        if self.params.do_cross and precision != "fp64":
+            raise ValueError("precision='mixed' covers the auto-spectrum path; do_cross runs in fp64")
         self.derivs = None  # device derivative stacks (reused by Spectra when resident)
         self.timings = {}
         self.timer = kwargs.get("timer")
@@ -68,6 +69,11 @@ class Fisher:
     @property
     def reduced_noise_cov1(self) -> np.ndarray:
         N = self.inputs.noise
+        return N.detach().cpu().numpy() if hasattr(N, "detach") else N
+
+    @property
+    def reduced_noise_cov2(self) -> np.ndarray | None:
+        N = self.inputs.noise2
         return N.detach().cpu().numpy() if hasattr(N, "detach") else N
 
     def set_binning(self, bins):
@@ -87,6 +93,13 @@ class Fisher:
         self.cov.build()
         if not in_out._falsy(self.params.outinvcovmatfile1):
             in_out.write_covmat_reduced(self.params.outinvcovmatfile1, engine.to_numpy_cm(self.cov.inverse()))
+        if self.params.do_cross:
+            if self.cov2 is None:
+                self.cov2 = pipeline.Covariance(self.ctx, self.inputs, noise=self.inputs.noise2, vecs=self.cov.vecs)
+            self.cov2.build()
+            if not in_out._falsy(self.params.outinvcovmatfile2):
+                in_out.write_covmat_reduced(self.params.outinvcovmatfile2,
+                                            engine.to_numpy_cm(self.cov2.inverse()))
 
     def run(self) -> None:
         import time
@@ -104,9 +117,13 @@ class Fisher:
         self.ctx.sync()
         t1 = time.perf_counter()
         self._bandpower_window = None
-        self.fisher, self.derivs = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
-                                                   timer=self.timer, precision=self.precision,
-                                                   launch_timer=self.launch_timer, mixed_gemm=self.mixed_gemm)
+        if self.params.do_cross:
+            self.fisher, self.derivs = pipeline.fisher_cross(self.cov, self.cov2, self.bins,
+                                                             self.params.output_convention)
+        else:
+            self.fisher, self.derivs = pipeline.fisher(self.cov, self.bins, self.params.output_convention,
+                                                       timer=self.timer, precision=self.precision,
+                                                       launch_timer=self.launch_timer, mixed_gemm=self.mixed_gemm)
         self.ctx.sync()
         self.timings = {"covariance_s": t1 - t0, "fisher_s": time.perf_counter() - t1}
         in_out.write_out_matrix(self.params.outfilefisher, self.fisher)
@@ -131,6 +148,11 @@ class Fisher:
     def inverse_covariance(self) -> np.ndarray:
         """C^-1 (the reference keeps it in ``noise_cov1`` after ``run``)."""
         return engine.to_numpy_cm(self.cov.inverse())
+
+    @property
+    def inverse_covariance2(self) -> np.ndarray | None:
+        """C2^-1 = (S + N2)^-1 of a cross analysis (``noise_cov2`` after ``run``)."""
+        return None if self.cov2 is None else engine.to_numpy_cm(self.cov2.inverse())
 
     def get_error_bars(self, *, as_dict: bool = False):
         if self.fisher is None:

@@ -29,18 +29,22 @@ from .engine import P
 class Covariance:
     """Device-resident geometry, noise and (on demand) S, C and its factor."""
 
-    def __init__(self, ctx: engine.Context, inputs: Inputs):
+    def __init__(self, ctx: engine.Context, inputs: Inputs, noise=None, vecs=None):
+        """``noise``: another noise covariance over the same pixels (the
+        second one of a cross analysis), ``vecs``: pointings to share."""
         self.ctx = ctx
         self.inp = inputs
         p = inputs.params
-        nest = str(p.ordering).upper() == "NESTED"
-        self.vecs = [engine.pointings(ctx, p.nside, f.pixels, nest) for f in inputs.layout.fields]
-        if not in_out._falsy(p.output_geometry_file):
-            fields = inputs.layout.fields
-            in_out.output_geometry(p.output_geometry_file, [f.pixels.size for f in fields],
-                                   [v.cpu().numpy() for v in self.vecs], [f.pixels for f in fields])
+        if vecs is None:
+            nest = str(p.ordering).upper() == "NESTED"
+            vecs = [engine.pointings(ctx, p.nside, f.pixels, nest) for f in inputs.layout.fields]
+            if not in_out._falsy(p.output_geometry_file):
+                fields = inputs.layout.fields
+                in_out.output_geometry(p.output_geometry_file, [f.pixels.size for f in fields],
+                                       [v.cpu().numpy() for v in vecs], [f.pixels for f in fields])
+        self.vecs = vecs
         self.n = inputs.n
-        N = inputs.noise
+        N = inputs.noise if noise is None else noise
         if isinstance(N, torch.Tensor):
             # (n, n) row-major device tensor -> column-major copy on the device
             self.N = N.to(device=ctx.device, dtype=torch.float64).T.contiguous()
@@ -437,6 +441,113 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     Gh = engine.to_numpy_cm(G)
     F = 0.5 * (Gh + Gh.T)
     return F, src
+
+
+def fisher_cross(cov1: Covariance, cov2: Covariance, bins, convention: str = "Cl",
+                 mem_budget: int | None = None, mem_fraction: float = 0.8):
+    """(F, derivative source) of a cross analysis (``do_cross``), FP64.
+
+    F_ij = 1/2 Tr[C1^-1 dC_i C2^-1 dC_j] (``fisher.py:522-531,584-590``).  With
+    C_k^-1 = W_k^T W_k and E_i = W1 dC_i W2^T this is 1/2 <E_i, E_j>, a Gram
+    matrix of full (not symmetric) n x n products.  E_i is formed per
+    spectrum and all bins at once: T = D W2[:, b]^T, E = W1[:, a] T, plus the
+    transposed block's W1[:, b] (D^T W2[:, a]^T) when a != b.  The E of as
+    many spectra as fit twice in the budget are kept; spectrum groups that do
+    not fit together are regenerated per Gram block."""
+    ctx, n = cov1.ctx, cov1.n
+    src = DerivativeSource(cov1, bins, convention, torch.float64)
+    nb = len(bins)
+    geo = src.geometry
+    npar = len(geo) * nb
+    free = int(mem_budget) if mem_budget is not None else int(_free_bytes(ctx) * mem_fraction)
+    tmax = nb * max(max(g.ma, g.mb) for g in geo) * n * 8
+    if src.total_bytes + tmax > free // 2:
+        raise ValueError(f"cross Fisher needs its derivative stacks resident: {src.total_bytes / 1e9:.1f} GB "
+                         f"plus {tmax / 1e9:.1f} GB of staging exceed half the {free / 1e9:.1f} GB budget")
+    blocks = src.make_resident()
+    per = nb * n * n * 8                     # E of one spectrum
+    g = max(1, min(len(geo), (free - src.total_bytes - tmax) // (2 * per)))
+    groups = [list(range(s0, min(len(geo), s0 + g))) for s0 in range(0, len(geo), g)]
+    W1, W2 = cov1.factor.W, cov2.factor.W
+    T = torch.empty(tmax // 8, dtype=torch.float64, device=ctx.device)
+
+    def whitened(specs, out):
+        for k, s in enumerate(specs):
+            blk = blocks[s]
+            D, ma, mb = blk.stack, blk.ma, blk.mb
+            E = out[k * nb: (k + 1) * nb]
+            # T = D W2[:, rb:rb+mb]^T (ma x n per bin), E = W1[:, ra:ra+ma] T
+            engine.gemm_batched(ctx, False, True, ma, n, mb, 1.0, D, 0, ma, ma * mb, W2, blk.rb * n, n, 0,
+                                0.0, T, 0, ma, ma * n, nb)
+            engine.gemm_batched(ctx, False, False, n, n, ma, 1.0, W1, blk.ra * n, n, 0, T, 0, ma, ma * n,
+                                0.0, E, 0, n, n * n, nb)
+            if not blk.sym:
+                engine.gemm_batched(ctx, True, True, mb, n, ma, 1.0, D, 0, ma, ma * mb, W2, blk.ra * n, n, 0,
+                                    0.0, T, 0, mb, mb * n, nb)
+                engine.gemm_batched(ctx, False, False, n, n, mb, 1.0, W1, blk.rb * n, n, 0, T, 0, mb, mb * n,
+                                    1.0, E, 0, n, n * n, nb)
+        return out[: len(specs) * nb]
+
+    F = np.zeros((npar, npar))
+    bufA = torch.empty((g * nb, n * n), dtype=torch.float64, device=ctx.device)
+    bufB = torch.empty((g * nb, n * n), dtype=torch.float64, device=ctx.device) if len(groups) > 1 else None
+    for gi, I in enumerate(groups):
+        EI = whitened(I, bufA)
+        pi = slice(I[0] * nb, (I[-1] + 1) * nb)
+        F[pi, pi] = engine.gram(ctx, EI.shape[0], EI.shape[0], n * n, EI, EI)
+        for J in groups[gi + 1:]:
+            EJ = whitened(J, bufB)
+            pj = slice(J[0] * nb, (J[-1] + 1) * nb)
+            Gij = engine.gram(ctx, EI.shape[0], EJ.shape[0], n * n, EI, EJ)      # (|I|, |J|)
+            F[pi, pj] = Gij
+            F[pj, pi] = Gij.T
+    del T, bufA, bufB
+    F = 0.25 * (F + F.T)              # 1/2 G, symmetrised
+    return F, src
+
+
+def qml_cross(cov1: Covariance, cov2: Covariance, src: DerivativeSource, maps1: np.ndarray, maps2: np.ndarray,
+              chunk_budget: int | None = None):
+    """(q, nb) of a cross analysis: q_b = 1/2 y2^T dC_b y1 with y_k = C_k^-1 x_k
+    and no noise bias (``spectra.py:1146-1166``).  For dC_b = e_a D e_b^T
+    (+ transpose) the contraction is <D, y2_a y1_b^T + y1_a y2_b^T>
+    (the second term when a != b), one ``cfmi_dot_batched`` pass per spectrum."""
+    ctx, n = cov1.ctx, cov1.n
+    if maps1.shape != maps2.shape:
+        raise ValueError(f"maps1 {maps1.shape} and maps2 {maps2.shape} differ")
+    ns = maps1.shape[1]
+    Y1 = solve(cov1, maps1, "fp64")
+    Y2 = solve(cov2, maps2, "fp64")
+    if src.dtype != torch.float64:
+        src = DerivativeSource(cov1, src.bins, src.convention, torch.float64)
+    nbb = len(src.bins)
+    npar = len(src.geometry) * nbb
+    out = np.zeros((ns, npar))
+    if chunk_budget is None:
+        chunk_budget = int(_free_bytes(ctx) * 0.25)
+    cache = {}
+    for s, b0, blk in src.chunks(chunk_budget):
+        D = blk.stack
+        k = D.shape[0]
+        key = (blk.ra, blk.ma, blk.rb, blk.mb)
+        if key not in cache:
+            cache.clear()
+            ma, mb = blk.ma, blk.mb
+            M = torch.empty((ns, mb, ma), dtype=torch.float64, device=ctx.device)
+            for q in range(ns):
+                engine.gemm_ex(ctx, False, True, ma, mb, 1, 1.0, Y2, blk.ra + q * n, ma, Y1, blk.rb + q * n, mb,
+                               0.0, M, q * ma * mb, ma)
+                if not blk.sym:
+                    engine.gemm_ex(ctx, False, True, ma, mb, 1, 1.0, Y1, blk.ra + q * n, ma, Y2, blk.rb + q * n,
+                                   mb, 1.0, M, q * ma * mb, ma)
+            cache[key] = M
+        M = cache[key]
+        R = torch.empty((ns, k), dtype=torch.float64, device=ctx.device)
+        K = blk.ma * blk.mb
+        ctx.check(ctx.lib.cfmi_dot_batched(ctx.h, K, k, P(D), K, ns, P(M), K, P(R), k), "dot_batched")
+        out[:, s * nbb + b0: s * nbb + b0 + k] = 0.5 * R.cpu().numpy()
+        del blk, D
+    return out, np.zeros(npar)
 
 
 def whitened_maps(cov: Covariance, maps: np.ndarray) -> torch.Tensor:

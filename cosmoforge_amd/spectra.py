@@ -26,8 +26,6 @@ class Spectra:
                  params_file=None, device: int = 0, **kwargs):
         if params is None and params_file is not None:
             params = params_file
-        if maps2 is not None:
-            raise NotImplementedError("maps2 / do_cross is not part of this build")
         if fisher is not None:
             clash = [k for k, v in dict(mask=mask, noise_cov1=noise_cov1, noise_cov2=noise_cov2,
                                          cls_data=cls_data, fiducial_cls=fiducial_cls, beam=beam).items()
@@ -45,10 +43,12 @@ class Spectra:
             from .core import read_params
 
             extra = read_params(params)
-            for key in ("nsims", "remove_nb", "inputmapfile1", "outcovmatfile", "outerrfile"):
+            for key in ("nsims", "remove_nb", "inputmapfile1", "inputmapfile2", "outcovmatfile", "outerrfile"):
                 setattr(self.params, key, getattr(extra, key))
         self._maps_in = maps1
+        self._maps2_in = maps2
         self.maps1 = None
+        self.maps2 = None
         self.qml_results = None
         self.qml_noise_bias = None
         self.invfisher = None
@@ -62,6 +62,8 @@ class Spectra:
     def setup_maps(self):
         inp = self.fisher_instance.inputs
         self.maps1 = resolve_maps(inp.n, self.params.nsims, self._maps_in, self.params, inp.layout)
+        if self.params.do_cross:
+            self.maps2 = resolve_maps(inp.n, self.params.nsims, self._maps2_in, self.params, inp.layout, which=2)
 
     def setup_fisher_inversion(self):
         F = self.fisher_instance.get_fisher_matrix()
@@ -86,8 +88,11 @@ class Spectra:
                                             pipeline._dtype(f.precision))
         self.setup_maps()
         self.setup_fisher_inversion()
-        q, nb = pipeline.qml(f.cov, src, self.maps1, remove_nb=bool(self.params.remove_nb),
-                             precision=f.precision)
+        if self.params.do_cross:
+            q, nb = pipeline.qml_cross(f.cov, f.cov2, src, self.maps1, self.maps2)
+        else:
+            q, nb = pipeline.qml(f.cov, src, self.maps1, remove_nb=bool(self.params.remove_nb),
+                                 precision=f.precision)
         self.qml_results = q
         self.qml_noise_bias = nb
 

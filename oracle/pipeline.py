@@ -12,7 +12,10 @@ Restates the reference's traditional pixel-space path:
   ``:607-661`` (F^-1 via the Jacobi-scaled inverse), ``:1340-1356`` and
   ``:1713-1733`` (C_hat = q F^-1, noise bias = nb F^-1);
 * PICSLike -- ``src/cosmoforge.picslike/picslike/picslike.py:264-319,643-664``
-  (chi2 = d^T C^-1 d, logdet by a second Cholesky, lnL = -(chi2+logdet)/2).
+  (chi2 = d^T C^-1 d, logdet by a second Cholesky, lnL = -(chi2+logdet)/2);
+* cross analysis (``do_cross``) -- ``fisher.py:522-531,584-590``
+  (A_i = C2^-1 dC_i C1^-1, F_ij = 1/2 Tr(dC_j A_i)) and ``spectra.py:1146-1166``
+  (q_b = 1/2 sum y2 * (dC_b y1), no noise bias).
 """
 
 from __future__ import annotations
@@ -190,3 +193,37 @@ def picslike_point(layout: K.Layout, cls_s: dict, lmax_signal: int, N: np.ndarra
     chi2 = np.einsum("in,ij,jn->n", maps, Cinv, maps)
     logdet = matrix_slogdet_symm(C)[1]
     return chi2, logdet, -0.5 * (chi2 + logdet)
+
+
+def fisher_cross(layout: K.Layout, cls_s: dict, lmax_signal: int, N1: np.ndarray, N2: np.ndarray, params,
+                 S=None):
+    """(F, C1^-1, C2^-1) of a cross analysis in the reference formulation."""
+    if S is None:
+        S = K.signal_matrix(layout, cls_s, lmax_signal)
+    Cinv1 = matrix_inverse_symm(N1 + S)
+    Cinv2 = matrix_inverse_symm(N2 + S)
+    dCs, A = [], []
+    for fi, fj, mode, _, w in params:
+        dC = K.binned_derivative(layout, fi, fj, mode, w)
+        dCs.append(dC)
+        A.append(Cinv2 @ (dC @ Cinv1))
+    npar = len(A)
+    F = np.zeros((npar, npar))
+    for i in range(npar):
+        for j in range(i, npar):
+            v = 0.5 * K.matrix_trace(dCs[j], A[i])
+            F[i, j] = v
+            F[j, i] = v
+    return F, Cinv1, Cinv2
+
+
+def qml_cross(layout: K.Layout, Cinv1: np.ndarray, Cinv2: np.ndarray, maps1: np.ndarray, maps2: np.ndarray,
+              F: np.ndarray, params):
+    """(q, C_hat) of a cross analysis; maps are (n, nsims)."""
+    y1 = Cinv1 @ np.asarray(maps1, dtype=np.float64)
+    y2 = Cinv2 @ np.asarray(maps2, dtype=np.float64)
+    q = np.zeros((y1.shape[1], len(params)))
+    for p, (fi, fj, mode, _, w) in enumerate(params):
+        dC = K.binned_derivative(layout, fi, fj, mode, w)
+        q[:, p] = 0.5 * np.sum(y2 * (dC @ y1), axis=0)
+    return q, q @ matrix_inverse_symm(F)

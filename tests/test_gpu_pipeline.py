@@ -423,3 +423,47 @@ def test_fisher_spectra_from_files_match_in_memory(tmp_path):
     assert np.array_equal(err.reshape(-1), sp.get_error_bars())
     Ci = np.fromfile(cfg["outinvcovmatfile1"]).reshape(768, 768)
     assert np.array_equal(Ci, f.inverse_covariance)
+
+
+@pytest.mark.parametrize("name", ["T", "TQU"])
+def test_cross_fisher_and_qml_vs_oracle(name):
+    """do_cross: two noise covariances (the second correlated), two map sets
+    drawn from C1 and C2; F, the raw q and the deconvolved C_l against the
+    oracle's restatement of the reference at 1e-10; the Fisher computed in
+    several spectrum groups equals the one-group result."""
+    from cosmoforge_amd import Fisher, Spectra, pipeline
+    from oracle import kernels as OK
+    from oracle import pipeline as OP
+
+    case = cases.case_fisher_nside4(name)
+    n = case.noise.shape[0]
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((n, n)) / np.sqrt(n)
+    N2 = np.diag(np.diag(case.noise)) * 1.7 + 0.05 * np.mean(np.diag(case.noise)) * (A @ A.T)
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    S = OK.signal_matrix(lay, cls_s, case.lmax_signal)
+    x1 = np.linalg.cholesky(S + case.noise) @ rng.standard_normal((n, 2))
+    x2 = np.linalg.cholesky(S + N2) @ rng.standard_normal((n, 2))
+    Fo, C1, C2 = OP.fisher_cross(lay, cls_s, case.lmax_signal, case.noise, N2, prm, S=S)
+    qo, cho = OP.qml_cross(lay, C1, C2, x1, x2, Fo, prm)
+    print("cross", name, n, flush=True)
+    f = Fisher(_params(case, do_cross=True, nsims=2), basis=False, mask=case.mask, noise_cov1=case.noise,
+               noise_cov2=N2, cls_data=case.cls, beam=case.beam)
+    f.run()
+    sp = Spectra(fisher=f, maps1=x1, maps2=x2)
+    sp.run()
+    F = f.get_fisher_matrix()
+    assert _rel(F, Fo) < 1e-10
+    assert np.max(np.abs(np.diag(F) / np.diag(Fo) - 1)) < 1e-10
+    assert _rel(sp.qml_results, qo) < 1e-10
+    assert _rel(sp.get_power_spectra(), cho) < 1e-10
+    assert np.all(sp.qml_noise_bias == 0)
+    assert np.max(np.abs(f.inverse_covariance2 - C2)) / np.abs(C2).max() < 1e-10
+    ngeo = len(f.derivs.geometry)
+    if ngeo > 1:
+        per = len(f.bins) * n * n * 8
+        Fg, _ = pipeline.fisher_cross(f.cov, f.cov2, f.bins, f.params.output_convention,
+                                      mem_budget=2 * (f.derivs.total_bytes + 4 * per))
+        assert _rel(Fg, F) < 1e-12

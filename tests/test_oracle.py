@@ -188,3 +188,35 @@ def test_inverse_and_errors():
         LA.matrix_inverse_symm(M)
     with pytest.raises(np.linalg.LinAlgError):
         LA.cholesky_factor(M)
+
+
+@pytest.mark.parametrize("name", ["T", "TQU"])
+def test_cross_oracle_pins(name):
+    """The cross restatement against what the mathematics fixes: with N2 = N1
+    and maps2 = maps1 it is the auto Fisher and the auto q (noise bias kept);
+    with S = 0, N1 = a I and N2 = b I, F_ij = Tr(dC_i dC_j) / (2ab) and
+    q_b = x2^T dC_b x1 / (2ab), so exchanging or repeating C1 and C2 fails."""
+    c = cases.case_fisher_nside4(name)
+    c.lmax = 5
+    lay = R.layout(c)
+    cls_s, beams = R.smoothed(c, lay)
+    prm = R.params(c, lay, beams)
+    n = c.noise.shape[0]
+    rng = np.random.default_rng(11)
+    F, Cinv, S = P.fisher(lay, cls_s, c.lmax_signal, c.noise, prm)
+    Fx, C1, C2 = P.fisher_cross(lay, cls_s, c.lmax_signal, c.noise, c.noise, prm, S=S)
+    np.testing.assert_allclose(Fx, F, rtol=1e-11, atol=1e-11 * np.abs(F).max())
+    x = rng.standard_normal((n, 2))
+    q, _, _, _ = P.qml(lay, Cinv, c.noise, x, F, prm, remove_nb=False)
+    qx, _ = P.qml_cross(lay, C1, C2, x, x, Fx, prm)
+    np.testing.assert_allclose(qx, q, rtol=1e-11, atol=1e-11 * np.abs(q).max())
+    a, b = 0.7, 2.9
+    Z = np.zeros((n, n))
+    Fz, C1, C2 = P.fisher_cross(lay, cls_s, c.lmax_signal, a * np.eye(n), b * np.eye(n), prm, S=Z)
+    dCs = [K.binned_derivative(lay, fi, fj, mode, w) for fi, fj, mode, _, w in prm]
+    Fe = np.array([[np.sum(di * dj) for dj in dCs] for di in dCs]) / (2 * a * b)
+    np.testing.assert_allclose(Fz, Fe, rtol=1e-12, atol=1e-12 * np.abs(Fe).max())
+    x1, x2 = rng.standard_normal((n, 1)), rng.standard_normal((n, 1))
+    qz, _ = P.qml_cross(lay, C1, C2, x1, x2, Fz, prm)
+    qe = np.array([float(x2[:, 0] @ d @ x1[:, 0]) for d in dCs]) / (2 * a * b)
+    np.testing.assert_allclose(qz[0], qe, rtol=1e-12, atol=1e-12 * np.abs(qe).max())
