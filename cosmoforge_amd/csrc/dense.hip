@@ -359,9 +359,11 @@ __global__ void splitk_reduce_kernel(GemmArgs<T, TO, TE> g, int nz)
     }
 }
 
+// Upper triangle of each batch's n x n C (blockIdx.z, sC apart) from its lower.
 template <typename T>
-__global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc)
+__global__ void mirror_lower_kernel(int64_t n, T* C, int64_t ldc, int64_t sC)
 {
+    C += (int64_t)blockIdx.z * sC;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
          idx += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = idx % n, j = idx / n;
@@ -444,8 +446,9 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
         CFMI_LAUNCH_CHECK(c);
     }
     if ((tri & 64) && M == N && sh.om.mode == 0) {
-        hipLaunchKernelGGL((mirror_lower_kernel<TO>), elem_grid(M * M), dim3(256), 0,
-                           c->stream, M, (TO*)C, ldc);
+        dim3 mg = elem_grid(M * M);
+        mg.z = (unsigned)batch;
+        hipLaunchKernelGGL((mirror_lower_kernel<TO>), mg, dim3(256), 0, c->stream, M, (TO*)C, ldc, sh.sC);
         CFMI_LAUNCH_CHECK(c);
     }
     return CFMI_OK;

@@ -19,6 +19,18 @@ from . import engine, in_out, pipeline
 from .core import Inputs, bins_from_params, resolve_basis
 
 
+def _check_filter(f, inputs) -> None:
+    """The filter against this run's active pixels (``core.py:1090-1111``)."""
+    from .filters import mask_fingerprint
+
+    if f.n_pixels != inputs.n:
+        raise ValueError(f"pixel_filter was built for {f.n_pixels} pixels but this run has {inputs.n} active "
+                         "pixels; rebuild it against this mask")
+    if f.fingerprint is not None and f.fingerprint != mask_fingerprint(inputs.mask):
+        raise ValueError("pixel_filter was built against a different active-pixel ordering than this run's "
+                         "mask; its rows would be permuted. Rebuild it with active_pixel_index(mask) of this mask")
+
+
 class Fisher:
     def __init__(self, params=None, basis=None, cache_derivatives: bool = False, symmetry_mode=None,
                  mask=None, noise_cov1=None, noise_cov2=None, cls_data=None, fiducial_cls=None, beam=None,
@@ -26,8 +38,6 @@ class Fisher:
                  precision: str = "fp64", **kwargs):
         if params is None and params_file is not None:
             params = params_file
-        if kwargs.get("pixel_filter") is not None:
-            raise NotImplementedError("pixel_filter is not part of this build")
         if symmetry_mode is not None:
             name = getattr(symmetry_mode, "name", symmetry_mode)
             if not isinstance(name, str):
@@ -57,6 +67,15 @@ class Fisher:
         self.timings = {}
         self.timer = kwargs.get("timer")
         self.launch_timer = kwargs.get("launch_timer")
+        # pixel filter (ADR-0020): the analysis runs in the filter's range
+        self.pixel_filter = kwargs.get("pixel_filter")
+        self.maps_prefiltered = bool(kwargs.get("maps_prefiltered", False))
+        self.noise_prefiltered = bool(kwargs.get("noise_prefiltered", False))
+        self._device_filter = None
+        if self.pixel_filter is not None:
+            _check_filter(self.pixel_filter, self.inputs)
+            if precision != "fp64":
+                raise ValueError("precision='mixed' covers the unfiltered path; a pixel filter runs in fp64")
         self.mixed_gemm = kwargs.get("mixed_gemm", "bf16x3")
         if self.mixed_gemm not in pipeline.MIXED_GEMMS:
             raise ValueError(f"mixed_gemm must be one of {pipeline.MIXED_GEMMS}; got {self.mixed_gemm!r}")
@@ -87,15 +106,25 @@ class Fisher:
             raise ValueError("binning cannot extend above the inference ceiling.")
         self.bins = bins
 
+    def device_filter(self):
+        """The pixel filter's device copy (None without a filter)."""
+        if self.pixel_filter is not None and self._device_filter is None:
+            from .filters import DeviceFilter
+
+            self._device_filter = DeviceFilter(self.ctx, self.pixel_filter, self.noise_prefiltered,
+                                               self.maps_prefiltered)
+        return self._device_filter
+
     def prepare_covariance_matrices(self):
         if self.cov is None:
-            self.cov = pipeline.Covariance(self.ctx, self.inputs)
+            self.cov = pipeline.Covariance(self.ctx, self.inputs, pixel_filter=self.device_filter())
         self.cov.build()
         if not in_out._falsy(self.params.outinvcovmatfile1):
             in_out.write_covmat_reduced(self.params.outinvcovmatfile1, engine.to_numpy_cm(self.cov.inverse()))
         if self.params.do_cross:
             if self.cov2 is None:
-                self.cov2 = pipeline.Covariance(self.ctx, self.inputs, noise=self.inputs.noise2, vecs=self.cov.vecs)
+                self.cov2 = pipeline.Covariance(self.ctx, self.inputs, noise=self.inputs.noise2, vecs=self.cov.vecs,
+                                                pixel_filter=self.device_filter())
             self.cov2.build()
             if not in_out._falsy(self.params.outinvcovmatfile2):
                 in_out.write_covmat_reduced(self.params.outinvcovmatfile2,

@@ -134,7 +134,18 @@ class PICSLike:
                                            lmax_signal=self.inputs.lmax_signal)
         self.parameter_grid = parameter_grid
         self.ctx = context if context is not None else engine.Context(device)
-        self.cov = pipeline.Covariance(self.ctx, self.inputs)
+        # pixel filter (ADR-0020): N, S and the maps restricted to its range
+        self.pixel_filter = kwargs.get("pixel_filter")
+        flt = None
+        if self.pixel_filter is not None:
+            from .fisher import _check_filter
+            from .filters import DeviceFilter
+
+            _check_filter(self.pixel_filter, self.inputs)
+            flt = DeviceFilter(self.ctx, self.pixel_filter, kwargs.get("noise_prefiltered", False),
+                               kwargs.get("maps_prefiltered", False))
+            self.maps1 = flt.restrict_data(self.maps1, prefiltered=flt.maps_prefiltered)
+        self.cov = pipeline.Covariance(self.ctx, self.inputs, pixel_filter=flt)
         self.likelihood_result = None
         self.simulation_results = None
 

@@ -29,9 +29,12 @@ from .engine import P
 class Covariance:
     """Device-resident geometry, noise and (on demand) S, C and its factor."""
 
-    def __init__(self, ctx: engine.Context, inputs: Inputs, noise=None, vecs=None):
+    def __init__(self, ctx: engine.Context, inputs: Inputs, noise=None, vecs=None, pixel_filter=None):
         """``noise``: another noise covariance over the same pixels (the
-        second one of a cross analysis), ``vecs``: pointings to share."""
+        second one of a cross analysis), ``vecs``: pointings to share,
+        ``pixel_filter``: a ``filters.DeviceFilter``; N, S and C then live in
+        its r-dimensional range (``core.py:628-634``, ``signal_kernels.py``'s
+        ``pixel_filter=`` seam) and ``n`` is r."""
         self.ctx = ctx
         self.inp = inputs
         p = inputs.params
@@ -50,6 +53,13 @@ class Covariance:
             self.N = N.to(device=ctx.device, dtype=torch.float64).T.contiguous()
         else:
             self.N = ctx.upload_cm(N)
+        self.pixel_filter = pixel_filter
+        if pixel_filter is not None:
+            if pixel_filter.n != self.n:
+                raise ValueError(f"pixel_filter was built for {pixel_filter.n} pixels but this run has "
+                                 f"{self.n} active pixels; rebuild it against this mask")
+            self.N = pixel_filter.restrict_matrix(self.N, prefiltered=pixel_filter.noise_prefiltered)
+            self.n = pixel_filter.r
         self.S = None
         self.factor = None
         self._w32 = None
@@ -70,6 +80,10 @@ class Covariance:
         ctx = self.ctx
         cs = self.inp.smoothed_cls(cls)
         dev = {k: ctx.upload(v) for k, v in cs.items()}
+        if self.pixel_filter is not None:
+            full = engine.signal_matrix(ctx, self.inp.layout, self.vecs, dev, self.inp.lmax_signal)
+            self.S = self.pixel_filter.restrict_matrix(full)
+            return self.S
         self.S = engine.signal_matrix(ctx, self.inp.layout, self.vecs, dev, self.inp.lmax_signal, out=self.S)
         return self.S
 
@@ -87,6 +101,14 @@ class Covariance:
 
     def inverse(self) -> torch.Tensor:
         return engine.inverse_from_whitener(self.ctx, self.factor.W, self.n)
+
+
+def make_source(cov: Covariance, bins, convention: str, dtype, budget: int | None = None):
+    """The derivative source of ``cov``: pixel-space block stacks, or their
+    restrictions to the filter's range when ``cov`` carries a pixel filter."""
+    if getattr(cov, "pixel_filter", None) is not None:
+        return RestrictedSource(cov, bins, convention, dtype, budget)
+    return DerivativeSource(cov, bins, convention, dtype, budget)
 
 
 def parameter_weights(inp: Inputs, bins, convention: str) -> dict:
@@ -303,6 +325,61 @@ class DerivativeSource:
         del pool
 
 
+class RestrictedSource:
+    """Binned derivatives restricted to a pixel filter's range,
+    Sigma W^T dC_b W Sigma (``core.py:1455-1469``): dense symmetric r x r
+    blocks (one ``Block(0, r, 0, r)`` per spectrum), made from the
+    pixel-space block stacks chunk by chunk (``DeviceFilter.restrict_stack``)."""
+
+    def __init__(self, cov: Covariance, bins, convention: str, dtype, budget: int | None = None):
+        if dtype != torch.float64:
+            raise ValueError("a pixel filter runs in fp64")
+        self.cov, self.bins, self.dtype, self.convention = cov, bins, dtype, convention
+        self.base = DerivativeSource(cov, bins, convention, torch.float64, budget)
+        r = cov.n
+        self.geometry = [engine.Block(0, r, 0, r) for _ in self.base.geometry]
+        self.weights = self.base.weights
+        self.mirrors = False
+        self.bytes_per_bin = [r * r * 8 for _ in self.geometry]
+        self.total_bytes = sum(self.bytes_per_bin) * len(bins)
+        self.resident = None
+        self.budget = budget
+
+    def set_mirrors(self, on: bool):
+        self.mirrors = False
+
+    def fits(self, budget: int) -> bool:
+        return self.total_bytes <= budget
+
+    def _restricted(self, chunk_budget: int):
+        r = self.cov.n
+        grow = max(r * r / max(g.ma * g.mb, 1) for g in self.base.geometry)
+        for s, b0, blk in self.base.chunks(max(1, int(chunk_budget / (1.0 + grow)))):
+            R = self.cov.pixel_filter.restrict_stack(blk)
+            yield s, b0, engine.Block(0, r, 0, r, stack=R)
+            del blk
+
+    def make_resident(self):
+        if self.resident is None:
+            r, nb = self.cov.n, len(self.bins)
+            out = [engine.Block(0, r, 0, r, stack=torch.empty((nb, r, r), dtype=torch.float64,
+                                                             device=self.cov.ctx.device))
+                   for _ in self.geometry]
+            for s, b0, blk in self._restricted(int(_free_bytes(self.cov.ctx) * 0.25)):
+                k = blk.stack.shape[0]
+                out[s].stack[b0: b0 + k].copy_(blk.stack)
+                del blk
+            self.resident = out
+        return self.resident
+
+    def chunks(self, chunk_budget: int):
+        if self.resident is not None:
+            for s, blk in enumerate(self.resident):
+                yield s, 0, blk
+            return
+        yield from self._restricted(chunk_budget)
+
+
 def _free_bytes(ctx) -> int:
     """Device memory available to new buffers: free memory plus what torch's
     caching allocator holds but does not use."""
@@ -383,7 +460,7 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     W = whitener_planes(cov) if b3 else whitener(cov, precision)
     sdt = engine.BF16X3 if b3 else T
     es_z = _stack_bytes(sdt)
-    src = DerivativeSource(cov, bins, convention, sdt)
+    src = make_source(cov, bins, convention, sdt)
     nb = len(bins)
     npar = len(src.geometry) * nb
     # Whitening products (K <= a few 10^4, terms of both signs) accumulate in
@@ -455,7 +532,7 @@ def fisher_cross(cov1: Covariance, cov2: Covariance, bins, convention: str = "Cl
     many spectra as fit twice in the budget are kept; spectrum groups that do
     not fit together are regenerated per Gram block."""
     ctx, n = cov1.ctx, cov1.n
-    src = DerivativeSource(cov1, bins, convention, torch.float64)
+    src = make_source(cov1, bins, convention, torch.float64)
     nb = len(bins)
     geo = src.geometry
     npar = len(geo) * nb
@@ -519,7 +596,7 @@ def qml_cross(cov1: Covariance, cov2: Covariance, src: DerivativeSource, maps1: 
     Y1 = solve(cov1, maps1, "fp64")
     Y2 = solve(cov2, maps2, "fp64")
     if src.dtype != torch.float64:
-        src = DerivativeSource(cov1, src.bins, src.convention, torch.float64)
+        src = make_source(cov1, src.bins, src.convention, torch.float64)
     nbb = len(src.bins)
     npar = len(src.geometry) * nbb
     out = np.zeros((ns, npar))
@@ -649,7 +726,7 @@ def qml(cov: Covariance, src: DerivativeSource, maps: np.ndarray, remove_nb: boo
     ns = maps.shape[1]
     Y = solve(cov, maps, precision)
     if src.dtype != torch.float64:
-        src = DerivativeSource(cov, src.bins, src.convention, torch.float64)
+        src = make_source(cov, src.bins, src.convention, torch.float64)
     G = noise_weighted_inverse(cov, "fp64")
     nb_bins = len(src.bins)
     npar = len(src.geometry) * nb_bins

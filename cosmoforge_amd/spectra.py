@@ -64,6 +64,12 @@ class Spectra:
         self.maps1 = resolve_maps(inp.n, self.params.nsims, self._maps_in, self.params, inp.layout)
         if self.params.do_cross:
             self.maps2 = resolve_maps(inp.n, self.params.nsims, self._maps2_in, self.params, inp.layout, which=2)
+        # filter seam for d (ADR-0020): restricted to the filter's range
+        flt = self.fisher_instance.device_filter()
+        if flt is not None:
+            self.maps1 = flt.restrict_data(self.maps1, prefiltered=flt.maps_prefiltered)
+            if self.maps2 is not None:
+                self.maps2 = flt.restrict_data(self.maps2, prefiltered=flt.maps_prefiltered)
 
     def setup_fisher_inversion(self):
         F = self.fisher_instance.get_fisher_matrix()
@@ -84,7 +90,7 @@ class Spectra:
             f.run()
         src = f.derivs
         if src is None:
-            src = pipeline.DerivativeSource(f.cov, f.bins, self.params.output_convention,
+            src = pipeline.make_source(f.cov, f.bins, self.params.output_convention,
                                             pipeline._dtype(f.precision))
         self.setup_maps()
         self.setup_fisher_inversion()

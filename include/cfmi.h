@@ -138,7 +138,9 @@ int cfmi_deriv_stack_02(cfmi_ctx *ctx, int dtype, const double *w_te, const doub
 /* C = alpha op(A) op(B) + beta C; dtype CFMI_F64 or CFMI_F32; trans* != 0
  * selects the transpose.  flags: bit 0 op(A) lower-triangular, bit 1 op(A)
  * upper, bit 2 op(B) upper, bit 3 op(B) lower (zero parts are skipped, they
- * must hold zeros), bit 4 write only the lower triangle of C.             */
+ * must hold zeros), bit 4 write only the lower triangle of C, bit 6 (64)
+ * the product is symmetric (m == n): only its lower tiles are computed and
+ * every batch's upper triangle is mirrored from them.                     */
 int cfmi_gemm(cfmi_ctx *ctx, int dtype, int transa, int transb, int64_t m, int64_t n, int64_t k,
               double alpha, const void *A, int64_t lda, const void *B, int64_t ldb, double beta,
               void *C, int64_t ldc, int flags);

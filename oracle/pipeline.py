@@ -13,6 +13,9 @@ Restates the reference's traditional pixel-space path:
   ``:1713-1733`` (C_hat = q F^-1, noise bias = nb F^-1);
 * PICSLike -- ``src/cosmoforge.picslike/picslike/picslike.py:264-319,643-664``
   (chi2 = d^T C^-1 d, logdet by a second Cholesky, lnL = -(chi2+logdet)/2);
+* pixel filters -- ``src/cosmoforge.cosmocore/cosmocore/basics/linalg.py:477-528``
+  (Sigma W^T X W Sigma, Sigma W^T d) at the seams of ``core.py:1070-1140,1455-1469``
+  (N, S, every binned derivative and the maps restricted, the estimator run in r dimensions);
 * cross analysis (``do_cross``) -- ``fisher.py:522-531,584-590``
   (A_i = C2^-1 dC_i C1^-1, F_ij = 1/2 Tr(dC_j A_i)) and ``spectra.py:1146-1166``
   (q_b = 1/2 sum y2 * (dC_b y1), no noise bias).
@@ -227,3 +230,52 @@ def qml_cross(layout: K.Layout, Cinv1: np.ndarray, Cinv2: np.ndarray, maps1: np.
         dC = K.binned_derivative(layout, fi, fj, mode, w)
         q[:, p] = 0.5 * np.sum(y2 * (dC @ y1), axis=0)
     return q, q @ matrix_inverse_symm(F)
+
+
+def restrict(X: np.ndarray, W: np.ndarray, sigma=None) -> np.ndarray:
+    """Sigma W^T X W Sigma, symmetrised (``linalg.py:477-503``)."""
+    M = W.T @ X @ W
+    if sigma is not None:
+        M = M * sigma[:, None] * sigma[None, :]
+    return 0.5 * (M + M.T)
+
+
+def restrict_data(d: np.ndarray, W: np.ndarray, sigma=None) -> np.ndarray:
+    """Sigma W^T d (``linalg.py:506-528``)."""
+    out = W.T @ d
+    if sigma is not None:
+        out = out * (sigma if out.ndim == 1 else sigma[:, None])
+    return out
+
+
+def fisher_filtered(layout: K.Layout, cls_s: dict, lmax_signal: int, N: np.ndarray, params, W: np.ndarray,
+                    sigma=None, S=None):
+    """(F, C'^-1, N') of the restricted problem: C' = S' + N', A_i = C'^-1 dC'_i,
+    F_ij = 1/2 Tr(A_i A_j)."""
+    if S is None:
+        S = K.signal_matrix(layout, cls_s, lmax_signal)
+    Nr = restrict(N, W, sigma)
+    Cinv = matrix_inverse_symm(restrict(S, W, sigma) + Nr)
+    A = []
+    for fi, fj, mode, _, w in params:
+        A.append(Cinv @ restrict(K.binned_derivative(layout, fi, fj, mode, w), W, sigma))
+    npar = len(A)
+    F = np.zeros((npar, npar))
+    for i in range(npar):
+        for j in range(i, npar):
+            F[i, j] = F[j, i] = 0.5 * K.matrix_trace(A[i], A[j])
+    return F, Cinv, Nr
+
+
+def qml_filtered(layout: K.Layout, Cinv: np.ndarray, Nr: np.ndarray, maps: np.ndarray, F: np.ndarray, params,
+                 W: np.ndarray, sigma=None, remove_nb: bool = True):
+    """(q, nb, C_hat) of the restricted problem with d' = Sigma W^T d."""
+    y = Cinv @ restrict_data(np.asarray(maps, dtype=np.float64), W, sigma)
+    G = Cinv @ (Nr @ Cinv)
+    q = np.zeros((y.shape[1], len(params)))
+    nb = np.zeros(len(params))
+    for p, (fi, fj, mode, _, w) in enumerate(params):
+        dC = restrict(K.binned_derivative(layout, fi, fj, mode, w), W, sigma)
+        nb[p] = 0.5 * K.matrix_trace(G, dC)
+        q[:, p] = 0.5 * np.sum(y * (dC @ y), axis=0) - (nb[p] if remove_nb else 0.0)
+    return q, nb, q @ matrix_inverse_symm(F)

@@ -467,3 +467,106 @@ def test_cross_fisher_and_qml_vs_oracle(name):
         Fg, _ = pipeline.fisher_cross(f.cov, f.cov2, f.bins, f.params.output_convention,
                                       mem_budget=2 * (f.derivs.total_bytes + 4 * per))
         assert _rel(Fg, F) < 1e-12
+
+
+def _filter_case(name):
+    case = cases.case_fisher_nside4(name)
+    case.lmax = 6
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    return case, lay, cls_s, prm
+
+
+@pytest.mark.parametrize("kind", ["deprojection_TQU", "graded_T"])
+def test_filtered_fisher_qml_picslike_vs_oracle(kind):
+    """A pixel filter (ADR-0020): N, S, the binned derivatives and the maps
+    restricted to its range on the device; F, q, the noise bias, C_hat and
+    the PICSLike chi2 / ln|C'| against the oracle's restricted problem at
+    1e-10."""
+    from cosmoforge_amd import Fisher, PICSLike, Spectra
+    from cosmoforge_amd.filters import Filter
+    from oracle import healpix
+    from oracle import kernels as OK
+    from oracle import pipeline as OP
+    from oracle.linalg import matrix_slogdet_symm
+
+    name = kind.split("_")[1]
+    case, lay, cls_s, prm = _filter_case(name)
+    n = case.noise.shape[0]
+    rng = np.random.default_rng(13)
+    if kind.startswith("deprojection"):
+        # monopole and dipole of T, and a Q/U template, removed
+        act = np.nonzero(case.mask[:, 0] > 0.5)[0]
+        vec = healpix.pointings(4, act, nest=False)
+        T = np.zeros((n, 5))
+        nt = act.size
+        T[:nt, 0] = 1.0
+        T[:nt, 1:4] = vec
+        T[nt:, 4] = rng.standard_normal(n - nt)
+        flt = Filter.from_deprojection(T, mask=case.mask)
+        W, sig = flt.W, None
+    else:
+        Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        s = np.r_[np.linspace(2.0, 0.5, n - 20), np.full(20, 1e-12)]
+        flt = Filter.from_operator(Q1 @ np.diag(s) @ Q2.T, range_epsilon=1e-6)
+        W, sig = flt.W, flt.sigma
+    S = OK.signal_matrix(lay, cls_s, case.lmax_signal)
+    x = np.linalg.cholesky(S + case.noise) @ rng.standard_normal((n, 2))
+    Fo, Cf, Nr = OP.fisher_filtered(lay, cls_s, case.lmax_signal, case.noise, prm, W, sig, S=S)
+    qo, nbo, cho = OP.qml_filtered(lay, Cf, Nr, x, Fo, prm, W, sig)
+    print("filter", kind, n, flt.rank, flush=True)
+    f = Fisher(_params(case, nsims=2), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam, pixel_filter=flt)
+    f.run()
+    sp = Spectra(fisher=f, maps1=x)
+    sp.run()
+    F = f.get_fisher_matrix()
+    assert _rel(F, Fo) < 1e-10
+    assert np.max(np.abs(np.diag(F) / np.diag(Fo) - 1)) < 1e-10
+    # q: cond(C) is 1.1e8 here, and two FP64 formulations of the oracle
+    # itself (explicit C^-1 against solves) differ by 2.3e-10 in q and
+    # 1.2e-10 in q + nb, 2e-11 in nb: q and C_hat are held to 1e-9
+    assert _rel(sp.qml_noise_bias, nbo) < 1e-10
+    assert _rel(sp.qml_results + sp.qml_noise_bias, qo + nbo) < 1e-9
+    assert _rel(sp.qml_results, qo) < 1e-9
+    assert _rel(sp.get_power_spectra(), cho) < 1e-9
+    pl = PICSLike(_params(case, nsims=2), basis=False, mask=case.mask, noise_cov1=case.noise, beam=case.beam,
+                  maps1=x, theoretical_spectra={(0,): case.cls}, pixel_filter=flt)
+    pl.compute()
+    Cr = OP.restrict(S, W, sig) + Nr
+    xr = OP.restrict_data(x, W, sig)
+    chi2 = np.einsum("in,ij,jn->n", xr, np.linalg.inv(Cr), xr)
+    lnl = -0.5 * (chi2 + matrix_slogdet_symm(Cr)[1])
+    assert np.max(np.abs(pl.simulation_results[0].chi_squared_values / chi2[0] - 1)) < 1e-9
+    assert np.max(np.abs(pl.get_log_likelihood() - lnl.mean())) < 1e-9 * np.abs(lnl).max()
+
+
+def test_invertible_filter_leaves_the_estimator_unchanged():
+    """Full-rank graded filter: F, q and the noise bias equal the unfiltered
+    device results (Gaussian Fisher invariance), to the conditioning the
+    filter adds."""
+    import warnings
+
+    from cosmoforge_amd import Fisher, Spectra
+    from cosmoforge_amd.filters import Filter
+
+    case, lay, cls_s, prm = _filter_case("TQU")
+    n = case.noise.shape[0]
+    rng = np.random.default_rng(3)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        flt = Filter.from_operator(Q1 @ np.diag(np.linspace(1.5, 0.7, n)), range_rank=n)
+    x = rng.standard_normal((n, 1)) * np.sqrt(np.diag(case.noise))[:, None]
+    out = []
+    for pf in (None, flt):
+        f = Fisher(_params(case, nsims=1), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                   beam=case.beam, pixel_filter=pf)
+        f.run()
+        sp = Spectra(fisher=f, maps1=x)
+        sp.run()
+        out.append((f.get_fisher_matrix(), sp.qml_results, sp.qml_noise_bias))
+    for a, b in zip(out[0], out[1]):
+        assert _rel(b, a) < 1e-9

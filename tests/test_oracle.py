@@ -220,3 +220,49 @@ def test_cross_oracle_pins(name):
     qz, _ = P.qml_cross(lay, C1, C2, x1, x2, Fz, prm)
     qe = np.array([float(x2[:, 0] @ d @ x1[:, 0]) for d in dCs]) / (2 * a * b)
     np.testing.assert_allclose(qz[0], qe, rtol=1e-12, atol=1e-12 * np.abs(qe).max())
+
+
+def test_filtered_oracle_pins():
+    """The filtered restatement against what the mathematics fixes:
+    * an invertible filter (full rank, graded) leaves F, q and the noise bias
+      unchanged (Tegmark, Taylor & Heavens 1997, eq. 24);
+    * the projector onto a subset of pixels gives the unfiltered analysis of
+      the mask holding only those pixels;
+    * deprojecting templates can only lose information: F - F_filtered is
+      positive semi-definite."""
+    c = cases.case_fisher_nside4("T")
+    c.lmax = 6
+    lay = R.layout(c)
+    cls_s, beams = R.smoothed(c, lay)
+    prm = R.params(c, lay, beams)
+    n = c.noise.shape[0]
+    rng = np.random.default_rng(5)
+    F, Cinv, S = P.fisher(lay, cls_s, c.lmax_signal, c.noise, prm)
+    x = np.linalg.cholesky(S + c.noise) @ rng.standard_normal((n, 2))
+    q, nb, _, _ = P.qml(lay, Cinv, c.noise, x, F, prm)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sig = np.linspace(2.0, 0.5, n)
+    Ff, Cf, Nr = P.fisher_filtered(lay, cls_s, c.lmax_signal, c.noise, prm, Q1, sig, S=S)
+    np.testing.assert_allclose(Ff, F, rtol=1e-9, atol=1e-9 * np.abs(F).max())
+    qf, nbf, _ = P.qml_filtered(lay, Cf, Nr, x, Ff, prm, Q1, sig)
+    np.testing.assert_allclose(qf, q, rtol=1e-8, atol=1e-9 * np.abs(q).max())
+    np.testing.assert_allclose(nbf, nb, rtol=1e-9, atol=1e-9 * np.abs(nb).max())
+    # pixel selection = the smaller mask
+    keep = np.sort(rng.choice(n, n - 37, replace=False))
+    Wsel = np.eye(n)[:, keep]
+    Fs, _, _ = P.fisher_filtered(lay, cls_s, c.lmax_signal, c.noise, prm, Wsel, None, S=S)
+    mask2 = c.mask.copy()
+    act = np.nonzero(c.mask[:, 0] > 0.5)[0]
+    mask2[np.setdiff1d(act, act[keep]), 0] = 0.0
+    c2 = cases.case_fisher_nside4("T")
+    c2.lmax, c2.mask = 6, mask2
+    lay2 = R.layout(c2)
+    F2, _, _ = P.fisher(lay2, cls_s, c.lmax_signal, c.noise[np.ix_(keep, keep)], prm)
+    np.testing.assert_allclose(Fs, F2, rtol=1e-11, atol=1e-11 * np.abs(F2).max())
+    # deprojection of the monopole and dipole loses information
+    vec = healpix.pointings(4, act, nest=False)
+    T = np.column_stack([np.ones(n), vec])
+    Wd = np.linalg.svd(T, full_matrices=True)[0][:, 4:]
+    Fd, _, _ = P.fisher_filtered(lay, cls_s, c.lmax_signal, c.noise, prm, Wd, None, S=S)
+    assert np.linalg.eigvalsh(F - Fd).min() > -1e-9 * np.abs(F).max()
+    assert np.abs(F - Fd).max() > 1e-6 * np.abs(F).max()
