@@ -94,9 +94,10 @@ def read_params(params) -> SimpleNamespace:
 
 
 def resolve_basis(basis):
-    """``basis=`` semantics (ADR-0018).  Only the pixel-space estimator is
-    built here: ``None``/``"auto"`` resolve to it (the reference's auto picks
-    the harmonic basis, an out-of-scope algebra for the same estimator)."""
+    """``basis=`` semantics (ADR-0018).  ``None``/``"auto"``/``False``/
+    ``"pixel"`` select the pixel-space estimator; ``"harmonic"`` the same
+    estimator with its traces contracted over harmonic modes
+    (``harmonic.HarmonicContraction``), uncompressed."""
     if basis is None or basis is False:
         return "pixel"
     if isinstance(basis, str):
@@ -104,7 +105,7 @@ def resolve_basis(basis):
         if b in ("auto", "pixel"):
             return "pixel"
         if b == "harmonic":
-            raise NotImplementedError("the harmonic basis is not part of this build; use basis=False")
+            return "harmonic"
         raise ValueError(f"unknown basis {basis!r}")
     if isinstance(basis, dict):
         if basis.get("epsilon") or basis.get("mode_fraction") or basis.get("compression_target"):

@@ -61,6 +61,9 @@ class Fisher:
         if precision not in pipeline.PRECISIONS:
             raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
         self.precision = precision
+        if self.basis == "harmonic" and (precision != "fp64" or self.params.do_cross
+                                         or kwargs.get("pixel_filter") is not None):
+            raise NotImplementedError("basis='harmonic' runs the fp64 auto-spectrum path without a pixel filter")
         if self.params.do_cross and precision != "fp64":
             raise ValueError("precision='mixed' covers the auto-spectrum path; do_cross runs in fp64")
         self.derivs = None  # device derivative stacks (reused by Spectra when resident)
@@ -146,7 +149,12 @@ class Fisher:
         self.ctx.sync()
         t1 = time.perf_counter()
         self._bandpower_window = None
-        if self.params.do_cross:
+        if self.basis == "harmonic":
+            from .harmonic import HarmonicContraction
+
+            self.derivs = HarmonicContraction(self.cov, self.bins, self.params.output_convention)
+            self.fisher = self.derivs.fisher()
+        elif self.params.do_cross:
             self.fisher, self.derivs = pipeline.fisher_cross(self.cov, self.cov2, self.bins,
                                                              self.params.output_convention)
         else:

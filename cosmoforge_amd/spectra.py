@@ -89,6 +89,15 @@ class Spectra:
         if f.fisher is None:
             f.run()
         src = f.derivs
+        if f.basis == "harmonic":
+            if src is None:
+                from .harmonic import HarmonicContraction
+
+                src = f.derivs = HarmonicContraction(f.cov, f.bins, self.params.output_convention)
+            self.setup_maps()
+            self.setup_fisher_inversion()
+            self.qml_results, self.qml_noise_bias = src.qml(self.maps1, remove_nb=bool(self.params.remove_nb))
+            return
         if src is None:
             src = pipeline.make_source(f.cov, f.bins, self.params.output_convention,
                                             pipeline._dtype(f.precision))

@@ -302,6 +302,43 @@ int cfmi_batch_capture(cfmi_batch *b, const double *coef, double *chi2, double *
 int cfmi_batch_replay(cfmi_batch *b);
 int cfmi_batch_destroy(cfmi_batch *b);
 
+/* ---- harmonic-mode contractions (no reference counterpart in the pixel
+ * path; the algebra of src/cosmoforge.qube/qube/fisher.py:416-500 and
+ * cosmocore/basis/harmonic_basis.py:19-117) ------------------------------
+ * Modes are l-major over l in [lmin, lmax]: m = 0, then the cos and sin
+ * modes of m = 1..l; (lmax+1)^2 - lmin^2 modes per group. */
+
+/* Real harmonic modes at the pixel centres vec (npix x 3, row-major unit
+ * vectors) into the column-major V (ldv rows, ncols columns; the other
+ * entries are left as they are).  spin 0: rows row_a.., columns col_a +
+ * mode.  spin 2 (lmin >= 2): E columns col_a + mode and B columns col_b +
+ * mode over the Q rows row_a.. and U rows row_b.., signed so that
+ * sum_m x_lm y_lm^T is the pixel-space dS/dC_l^XY of the T, E, B blocks. */
+int cfmi_harmonic_modes(cfmi_ctx *ctx, int spin, int64_t npix, const double *vec, int lmin, int lmax,
+                        double *V, int64_t ldv, int64_t ncols, int64_t row_a, int64_t row_b,
+                        int64_t col_a, int64_t col_b);
+
+/* F[p][q] (npar x npar, npar = nspec nbins, parameter p = spectrum p/nbins,
+ * bin p%nbins) = 1/2 Tr[C^-1 dC_p C^-1 dC_q] from the symmetric
+ * M = V^T C^-1 V (ldm): with dC_p = sum_{l in bin} w[s][l] sum_m sum over
+ * the spectrum's (u, v) group pairs of u_lm v_lm^T.  Device tables: bin_lo,
+ * bin_hi (nbins), pairs (nspec x 4 group ids u0 v0 u1 v1, u1 = -1 for an
+ * auto spectrum), goff (first column of each group in M), w (nspec x lw).
+ * The caller keeps every index inside M and w.                           */
+int cfmi_harmonic_fisher(cfmi_ctx *ctx, const double *M, int64_t ldm, int lmin, int nbins,
+                         const int *bin_lo, const int *bin_hi, int nspec, const int *pairs,
+                         const int64_t *goff, const double *w, int lw, double *F);
+
+/* out[p + j ldo] = alpha sum_{l in bin(p)} w[s][l] sum_m sum_{(u,v)}
+ * sum_{i < ni} X[i + (goff[u] + k) ldx + j sxj] Y[i + (goff[v] + k) ldy + j syj]
+ * (k the mode index), for j < ncols: the QML forms u^T y v^T y (ni = 1)
+ * and u^T G v (ni = n).                                                 */
+int cfmi_harmonic_pairsum(cfmi_ctx *ctx, int64_t ni, const double *X, int64_t ldx, int64_t sxj,
+                          const double *Y, int64_t ldy, int64_t syj, int64_t ncols, int lmin, int nbins,
+                          const int *bin_lo, const int *bin_hi, int nspec, const int *pairs,
+                          const int64_t *goff, const double *w, int lw, double alpha, double *out,
+                          int64_t ldo);
+
 /* ---- A21: ensemble MLP surrogate of -2 ln L ---------------------------- */
 /* E networks 3 -> 256 -> 256 -> 256 -> 1 (SiLU).  Parameters per member
  * (cfmi_mlp_param_count() values, layout W1[256][3], b1, W2[256][256], b2,

@@ -414,3 +414,47 @@ def test_offdiag_nnz_and_diagonal_noise_G(ctx):
     G3 = pipeline.noise_weighted_inverse(cov, "fp64").cpu().numpy()
     assert np.max(np.abs(G1 - G3)) <= 1e-12 * np.max(np.abs(G3))
     assert np.array_equal(G1, G1.T)
+
+
+@pytest.mark.parametrize("lmin,lmax", [(2, 12), (5, 40)])
+def test_harmonic_modes_vs_oracle(ctx, lmin, lmax):
+    """cfmi_harmonic_modes (Wigner recurrences on the device) against the
+    oracle's closed-form Wigner sums for a T + QU layout, NESTED pixels in
+    both hemispheres; out-of-range arguments are refused."""
+    import torch
+
+    from cosmoforge_amd import engine
+    from oracle import harmonic as OH
+    from oracle import healpix
+    from oracle import kernels as OK
+
+    nside = 8
+    rng = np.random.default_rng(1)
+    mask = np.zeros((768, 3))
+    mask[np.sort(rng.choice(768, 300, replace=False)), 0] = 1
+    qu = np.sort(rng.choice(768, 250, replace=False))
+    mask[qu, 1:] = 1
+    lay = OK.layout_from_mask(mask, [0, 2], ["T", "E", "B"], nside, nest=True)
+    groups = OH.mode_groups(lay, lmin, lmax)
+    ng = OH.n_modes(lmin, lmax)
+    n = lay.n
+    V = torch.zeros((3 * ng, n), dtype=torch.float64, device=ctx.device)
+    offs = lay.offsets()
+    for fi, f in enumerate(lay.fields):
+        vec = ctx.upload(np.ascontiguousarray(healpix.pointings(nside, f.pixels, nest=True)))
+        if f.spin == 0:
+            rc = ctx.lib.cfmi_harmonic_modes(ctx.h, 0, f.n, engine.P(vec), lmin, lmax, engine.P(V), n, 3 * ng,
+                                             offs[fi], 0, 0, 0)
+        else:
+            rc = ctx.lib.cfmi_harmonic_modes(ctx.h, 2, f.n, engine.P(vec), lmin, lmax, engine.P(V), n, 3 * ng,
+                                             offs[fi], offs[fi] + f.n, ng, 2 * ng)
+        ctx.check(rc, "harmonic_modes")
+    Vh = V.cpu().numpy().T            # (n, 3 ng)
+    for g, lab in enumerate(["T", "E", "B"]):
+        ref = groups[lab]
+        got = Vh[:, g * ng:(g + 1) * ng]
+        assert np.abs(got - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), lab
+    vec = ctx.upload(np.zeros((4, 3)))
+    assert ctx.lib.cfmi_harmonic_modes(ctx.h, 2, 4, engine.P(vec), 1, 8, engine.P(V), n, 3 * ng, 0, 0, 0, 0) < 0
+    assert ctx.lib.cfmi_harmonic_modes(ctx.h, 0, 4, engine.P(vec), lmin, lmax, engine.P(V), n, ng - 1, 0, 0, 0,
+                                       0) < 0

@@ -570,3 +570,51 @@ def test_invertible_filter_leaves_the_estimator_unchanged():
         out.append((f.get_fisher_matrix(), sp.qml_results, sp.qml_noise_bias))
     for a, b in zip(out[0], out[1]):
         assert _rel(b, a) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["T", "TQU", "TEB"])
+def test_harmonic_contraction_vs_oracle_and_pixel_path(name):
+    """basis="harmonic": modes built on the device, M = (W V)^T (W V), F and
+    the QML forms reduced over the modes; against the oracle's mode
+    restatement at 1e-10 and against the pixel path on the device (1e-10 for
+    temperature, 1e-8 with polarisation: the pixel path's acos rotation
+    angles)."""
+    from cosmoforge_amd import Fisher, Spectra
+    from oracle import harmonic as OH
+    from oracle import kernels as OK
+    from oracle import pipeline as OP
+
+    case = cases.case_fisher_nside4(name)
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    S = OK.signal_matrix(lay, cls_s, case.lmax_signal)
+    n = case.noise.shape[0]
+    x = np.linalg.cholesky(S + case.noise) @ np.random.default_rng(9).standard_normal((n, 2))
+    Cinv = OP.matrix_inverse_symm(S + case.noise)
+    Fo = OH.fisher_modes(lay, Cinv, prm, case.lmin, case.lmax)
+    qo, nbo = OH.qml_modes(lay, Cinv, case.noise, x, prm, case.lmin, case.lmax)
+    print("harmonic", name, n, flush=True)
+    out = {}
+    for basis in ("harmonic", False):
+        f = Fisher(_params(case, nsims=2), basis=basis, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                   beam=case.beam)
+        f.run()
+        sp = Spectra(fisher=f, maps1=x)
+        sp.run()
+        out[basis] = (f.get_fisher_matrix(), sp.qml_results, sp.qml_noise_bias, sp.get_power_spectra())
+    F, q, nb, ch = out["harmonic"]
+    assert _rel(F, Fo) < 1e-10
+    assert np.max(np.abs(np.diag(F) / np.diag(Fo) - 1)) < 1e-10
+    # nb = 1/2 <G, dC_b> is a cancelling sum (terms ~1e3 x nb): 1e-8
+    assert _rel(nb, nbo) < 1e-8
+    assert _rel(q + nb, qo + nbo) < 1e-9
+    cho = (qo) @ OP.matrix_inverse_symm(Fo)
+    assert _rel(ch, cho) < 1e-9
+    # against the pixel route: F, nb and q + nb; C_hat = q F^-1 would carry
+    # the routes' 1e-8 difference in F times cond(F)
+    tol = 1e-10 if name == "T" else 1e-8
+    Fp, qp, nbp, chp = out[False]
+    assert _rel(F, Fp) < tol
+    assert _rel(nb, nbp) < 1e-8
+    assert _rel(q + nb, qp + nbp) < 10 * tol
