@@ -233,6 +233,7 @@ def main():
     ap.add_argument("--no-picslike", action="store_true", help="skip the batched-likelihood leg")
     ap.add_argument("--precisions", default="fp64,mixed", help="comma list of fp64 / mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-harmonic", action="store_true", help="skip the harmonic-mode (basis='harmonic') leg")
     ap.add_argument("--cpu-params", type=int, default=2)
     ap.add_argument("--mixed-gemm", default="bf16x3", choices=["bf16x3", "fp32"],
                     help="whitening arithmetic of the mixed precision")
@@ -281,8 +282,8 @@ def main():
     # per solve, ~35 ms at n = 5952).
     noise_dev = torch.as_tensor(cfg.noise).to(ctx.device)
 
-    def step(prec, timer=None, lt=None):
-        f = Fisher(cfg.params, basis=False, mask=cfg.mask, noise_cov1=noise_dev, cls_data=cfg.cls,
+    def step(prec, timer=None, lt=None, basis=False):
+        f = Fisher(cfg.params, basis=basis, mask=cfg.mask, noise_cov1=noise_dev, cls_data=cfg.cls,
                    beam=cfg.beam, context=ctx, timer=timer, launch_timer=lt, precision=prec,
                    mixed_gemm=args.mixed_gemm)
         f.run()
@@ -290,6 +291,8 @@ def main():
         s.run()
         out = (f.get_fisher_matrix(), s.get_power_spectra(), s.qml_noise_bias, s.qml_results,
                f.get_error_bars())
+        if basis == "harmonic":
+            out = out + (f.derivs.nt, f.derivs.flops)
         f.release()
         del f, s
         return out
@@ -366,6 +369,28 @@ def main():
             "F_maxrel": rel(Fm, F), "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.abs(np.diag(F)))),
             "Cl_maxrel": rel(Cm, C), "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
             "noise_bias_maxrel": rel(nbm, nb), "qml_raw_maxrel": rel(qm, q), "cond_F": float(np.linalg.cond(F))}
+    if not args.no_harmonic and args.config < 3:
+        # the same estimator with its traces contracted over harmonic modes
+        # (FP64): M = (W V)^T (W V) and mode reductions instead of the
+        # per-parameter whitening products
+        step("fp64", basis="harmonic")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            hres = step("fp64", basis="harmonic")
+        torch.cuda.synchronize()
+        hs = (time.perf_counter() - t0) / args.steps
+        print("[bench] harmonic steps done", file=sys.stderr, flush=True)
+        Fh, Ch, nbh, qh, _, nmodes, hflops = hres
+        hleg = {"s": round(hs, 4), "modes": int(nmodes), "gemm_flops_per_solve": hflops,
+                "speedup_vs_pixel": {p: round(rr["s"] / hs, 2) for p, rr in runs.items()}}
+        if "fp64" in runs:
+            F, C, nb, q, sig = runs["fp64"]["res"]
+            hleg["error_vs_pixel_fp64"] = {
+                "F_maxrel": rel(Fh, F),
+                "F_diag_rel_max": float(np.max(np.abs(np.diag(Fh) - np.diag(F)) / np.abs(np.diag(F)))),
+                "Cl_over_sigma_max": float(np.max(np.abs(Ch - C) / sig)), "noise_bias_maxrel": rel(nbh, nb)}
+        out["harmonic_fp64"] = hleg
     if not args.no_picslike:
         out["picslike"] = picslike_leg(ctx, log)[0]
     if not args.no_cpu and args.config < 2:

@@ -8,11 +8,12 @@ dC_b = sum_{l in b} w_l sum_m (x_lm y_lm^T [+ y_lm x_lm^T]), so that
 * F_pq = 1/2 sum_{modes of p, q} w w' sum_pairs M[v, u'] M[u, v'] with
   M = V^T C^-1 V = (W V)^T (W V), W the whitener of C = S + N;
 * q_p = 1/2 sum w sum_pairs (u^T y)(v^T y) with y = C^-1 x, and
-  nb_p = 1/2 sum w sum_pairs u^T G v with G = C^-1 N C^-1.
+  nb_p = 1/2 sum w sum_pairs u^T G v with G = C^-1 N C^-1, both from
+  R = C^-1 V = W^T Z (u^T y = R_u^T x, u^T G v = R_u^T N R_v).
 
-C, its factor and G are the pixel path's (S from the pixel recurrences).  The
-device work is two FP64 GEMMs (Z = W V, M = Z^T Z) and HBM-bound mode
-reductions (``cfmi_harmonic_fisher``, ``cfmi_harmonic_pairsum``), instead of
+C and its factor are the pixel path's (S from the pixel recurrences).  The
+device work is three FP64 GEMMs (Z = W V, M = Z^T Z, R = W^T Z) and
+HBM-bound mode reductions (``cfmi_harmonic_fisher``, ``cfmi_harmonic_pairsum``), instead of
 the pixel path's n^2 x n products per parameter: about 30x less work at
 nside 16 and 60x at nside 32.
 
@@ -85,51 +86,72 @@ class HarmonicContraction:
         self._w = ctx.upload(np.ascontiguousarray(w, dtype=np.float64))
         self._lo = ctx.upload(np.array([a for a, _ in self.bins], dtype=np.int32), dtype=torch.int32)
         self._hi = ctx.upload(np.array([b for _, b in self.bins], dtype=np.int32), dtype=torch.int32)
-        self.M = None
+        self._Z = None
 
     @property
     def flops(self) -> float:
-        """FP64 flops of Z = W V (W lower triangular) and M = Z^T Z (one half)."""
+        """FP64 flops of Z = W V and R = W^T Z (W triangular) and M = Z^T Z
+        (its lower half)."""
         n, nt = self.cov.n, self.nt
-        return float(n) * n * nt + float(n) * nt * (nt + 1)
+        return 2.0 * n * n * nt + float(n) * nt * (nt + 1)
 
     def _tables(self):
         return (self.lmin, self.nbins, P(self._lo), P(self._hi), self.nspec, P(self._pairs), P(self._goff),
                 P(self._w), self.lw)
 
+    def _z(self) -> torch.Tensor:
+        """Z = W V (n x N, column-major), kept for the QML forms."""
+        if self._Z is None:
+            ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
+            Z = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
+            engine.gemm(ctx, False, False, n, nt, n, 1.0, self.cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
+            self._Z = Z
+        return self._Z
+
     def fisher(self) -> np.ndarray:
         ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
-        W = self.cov.factor.W
-        Z = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-        engine.gemm(ctx, False, False, n, nt, n, 1.0, W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
+        Z = self._z()
         M = torch.empty((nt, nt), dtype=torch.float64, device=ctx.device)
         engine.gram(ctx, nt, nt, n, Z, Z, out=M)
-        del Z
         F = torch.empty((self.npar, self.npar), dtype=torch.float64, device=ctx.device)
         ctx.check(ctx.lib.cfmi_harmonic_fisher(ctx.h, P(M), nt, *self._tables(), P(F)), "harmonic_fisher")
-        self.M = M
+        del M
         return engine.to_numpy_cm(F)
 
     def qml(self, maps: np.ndarray, remove_nb: bool = True):
-        """(q (nsims, n_p), nb (n_p,)) as ``pipeline.qml``."""
+        """(q (nsims, n_p), nb (n_p,)) as ``pipeline.qml``.  With R = C^-1 V =
+        W^T Z: u^T y = R_u^T x and u^T G v = R_u^T N R_v, so neither y nor G is
+        formed; a diagonal N scales the rows of R."""
         ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
         maps = np.asarray(maps, dtype=np.float64)
         if maps.ndim == 1:
             maps = maps[:, None]
         ns = maps.shape[1]
-        Y = pipeline.solve(self.cov, maps, "fp64")
-        Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)        # column-major ns x N
-        engine.gemm(ctx, True, False, ns, nt, n, 1.0, Y, 0, n, self.V, 0, n, 0.0, Ut, 0, ns)
-        q = torch.empty((ns, self.npar), dtype=torch.float64, device=ctx.device)   # column-major npar x ns
+        if maps.shape[0] != n:
+            raise ValueError(f"maps have {maps.shape[0]} rows, expected {n}")
+        Z = self._z()
+        R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)         # C^-1 V, column-major n x N
+        engine.gemm(ctx, True, False, n, nt, n, 1.0, self.cov.factor.W, 0, n, Z, 0, n, 0.0, R, 0, n, 2)
+        self._Z = None
+        del Z
+        X = ctx.upload_cm(maps)
+        Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)       # column-major ns x N
+        engine.gemm(ctx, True, False, ns, nt, n, 1.0, X, 0, n, R, 0, n, 0.0, Ut, 0, ns)
+        q = torch.empty((ns, self.npar), dtype=torch.float64, device=ctx.device)  # column-major npar x ns
         ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, 1, P(Ut), ns, 1, P(Ut), ns, 1, ns, *self._tables(), 0.5,
                                                 P(q), self.npar), "harmonic_pairsum")
-        G = pipeline.noise_weighted_inverse(self.cov, "fp64")
-        GV = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-        engine.gemm(ctx, False, False, n, nt, n, 1.0, G, 0, n, self.V, 0, n, 0.0, GV, 0, n)
-        del G
+        NR = torch.empty_like(R)
+        if self.cov.noise_is_diagonal():
+            d = torch.empty(n, dtype=torch.float64, device=ctx.device)
+            ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, 1, n, P(self.cov.N), n + 1, P(d), 1), "copy")
+            ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, n, nt, P(R), n, P(NR), n), "copy")
+            ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(NR), n, P(d)), "scale")
+        else:
+            engine.gemm(ctx, False, False, n, nt, n, 1.0, self.cov.N, 0, n, R, 0, n, 0.0, NR, 0, n)
         nb = torch.empty(self.npar, dtype=torch.float64, device=ctx.device)
-        ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, n, P(GV), n, 0, P(self.V), n, 0, 1, *self._tables(), 0.5,
+        ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, n, P(NR), n, 0, P(R), n, 0, 1, *self._tables(), 0.5,
                                                 P(nb), self.npar), "harmonic_pairsum")
+        del R, NR
         qh = q.cpu().numpy()              # (ns, npar): row s holds simulation s
         nbh = nb.cpu().numpy()
         if remove_nb:
