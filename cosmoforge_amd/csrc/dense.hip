@@ -119,6 +119,24 @@ __device__ __forceinline__ void tile_of(int64_t b, int64_t tm, int64_t tn, int t
     if (tri & 4) tj = tn - 1 - tj;
 }
 
+// Symmetric result (bit 6, tm == tn): the grid enumerates the lower tiles
+// only, row-major (t -> (ti, tj <= ti)), each XCD taking a contiguous,
+// equal-count range of them (round-robin dispatch sends block b to XCD b % 8).
+__device__ __forceinline__ void tri_tile_of(int64_t b, int64_t tm, int64_t& ti, int64_t& tj)
+{
+    const int64_t G = tm * (tm + 1) / 2;
+    int64_t t = b;
+    if (G >= 64) {
+        const int64_t q = G / 8, r = G % 8, x = b % 8, loc = b / 8;
+        t = x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+    }
+    int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    while (i * (i + 1) / 2 > t) --i;
+    ti = i;
+    tj = t - i * (i + 1) / 2;
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
@@ -228,7 +246,8 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
     __shared__ T Bs[BK][BN + PAD];
     const int tid = threadIdx.x;
     int64_t ti, tj;
-    tile_of(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
+    if ((g.tri & 64) && g.tm == g.tn) tri_tile_of(blockIdx.x, g.tm, ti, tj);
+    else tile_of(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
     const int64_t m0 = ti * BM;
     const int64_t n0 = tj * BN;
     const int z = blockIdx.y;
@@ -420,7 +439,8 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     g.tri = tri; g.offA = sh.offA; g.offB = sh.offB;
     g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = sh.om;
     g.sA = sh.sA; g.sB = sh.sB; g.sC = sh.sC;
-    dim3 grid((unsigned)(tm * tn), (unsigned)nz, (unsigned)batch);
+    const bool sym_tiles = (tri & 64) && tm == tn;
+    dim3 grid((unsigned)(sym_tiles ? tm * (tm + 1) / 2 : tm * tn), (unsigned)nz, (unsigned)batch);
 #define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                        \
     do {                                                                                                  \
         if constexpr (kAcc64) {                                                                           \
@@ -963,8 +983,8 @@ int cfmi_gemm_ex(cfmi_ctx* c, int dtype, int out_dtype, int transa, int transb, 
     sh.om.mode = out_mode; sh.om.row0 = row0; sh.om.col0 = col0; sh.om.pn = out_mode == 1 ? ldc : 0;
     sh.om.pbase = pbase; sh.om.pdiag = pdiag; sh.om.poff = poff;
     sh.batch = batch; sh.sA = sA; sh.sB = sB; sh.sC = sC;
-    int tri = flags & 31;
-    if (out_mode != 0) tri |= 16;
+    int tri = flags & (31 | 64);
+    if (out_mode != 0) tri = (tri & 31) | 16;
     if (out_mode != 0) ldc = 1;
     const bool acc64 = (flags & CFMI_GEMM_ACC64) != 0;
     if (dtype == CFMI_F64 && out_dtype == CFMI_F64)
