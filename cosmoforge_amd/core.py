@@ -291,13 +291,33 @@ def resolve_maps(n: int, nsims, maps, p=None, layout: Layout | None = None, whic
 
 
 def bins_from_params(p):
+    """[(l_min, l_max)] of the bins (``bins.py:80-140``): explicit
+    ``bin_lmins``/``bin_lmaxs`` (bins below l = 2 dropped, sorted, no
+    overlap), else uniform ``delta_ell`` bins from ``lmin`` (the floor)."""
     if p.bin_lmins is not None and p.bin_lmaxs is not None:
+        if len(p.bin_lmins) != len(p.bin_lmaxs):
+            raise ValueError(f"lmins and lmaxs must have the same length, got {len(p.bin_lmins)} and "
+                             f"{len(p.bin_lmaxs)}")
         lo, hi = np.asarray(p.bin_lmins, int), np.asarray(p.bin_lmaxs, int)
-        keep = (hi >= 2) & (lo >= 2)
-        return list(zip(lo[keep].tolist(), hi[keep].tolist()))
-    nb = (p.lmax - p.lmin + 1) // p.delta_ell
-    lo = p.lmin + np.arange(nb) * p.delta_ell
-    return list(zip(lo.tolist(), (lo + p.delta_ell - 1).tolist()))
+        floor = 2
+    else:
+        nb = (p.lmax - p.lmin + 1) // p.delta_ell
+        lo = p.lmin + np.arange(nb) * p.delta_ell
+        hi = lo + p.delta_ell - 1
+        floor = p.lmin
+    keep = (hi >= floor) & (lo >= floor)
+    lo, hi = lo[keep], hi[keep]
+    if lo.size == 0:
+        raise ValueError(f"No valid bins (all bins below lmin_floor={floor})")
+    for i, (a, b) in enumerate(zip(lo, hi)):
+        if a > b:
+            raise ValueError(f"Bin {i}: lmin={a} > lmax={b}")
+    order = np.argsort(lo, kind="stable")
+    lo, hi = lo[order], hi[order]
+    for i in range(lo.size - 1):
+        if lo[i + 1] <= hi[i]:
+            raise ValueError(f"Bins {i} and {i + 1} overlap: [{lo[i]}, {hi[i]}] and [{lo[i + 1]}, {hi[i + 1]}]")
+    return list(zip(lo.tolist(), hi.tolist()))
 
 
 def shape_weights(lmax: int, convention: str) -> np.ndarray:

@@ -273,3 +273,30 @@ def test_whiten_band_b3_needs_mirror_for_partial_bands(monkeypatch):
         engine.whiten_band_b3(FakeCtx(), Wp, n, blk, X, 5, 12)
     X = torch.zeros((2, engine.band_size(0, 12)), dtype=torch.float64)
     engine.whiten_band_b3(FakeCtx(), Wp, n, blk, X, 0, 12)   # one full band reuses W D: no mirror needed
+
+
+def test_bins_from_params_rules():
+    """Bin construction as ``cosmocore/bins.py:80-140``: uniform delta_ell
+    bins from lmin (remainder dropped), explicit bins sorted with l < 2
+    dropped, and the overlap / order / empty errors."""
+    from types import SimpleNamespace as NS
+
+    from cosmoforge_amd.core import bins_from_params
+
+    def P(**k):
+        base = dict(bin_lmins=None, bin_lmaxs=None, lmin=2, lmax=10, delta_ell=3)
+        base.update(k)
+        return NS(**base)
+
+    assert bins_from_params(P()) == [(2, 4), (5, 7), (8, 10)]
+    assert bins_from_params(P(lmax=11)) == [(2, 4), (5, 7), (8, 10)]
+    assert bins_from_params(P(delta_ell=1, lmax=4)) == [(2, 2), (3, 3), (4, 4)]
+    assert bins_from_params(P(bin_lmins=[10, 0, 2], bin_lmaxs=[12, 1, 9])) == [(2, 9), (10, 12)]
+    with pytest.raises(ValueError, match="overlap"):
+        bins_from_params(P(bin_lmins=[2, 5], bin_lmaxs=[6, 9]))
+    with pytest.raises(ValueError, match="lmin=7 > lmax=5"):
+        bins_from_params(P(bin_lmins=[7], bin_lmaxs=[5]))
+    with pytest.raises(ValueError, match="No valid bins"):
+        bins_from_params(P(bin_lmins=[0], bin_lmaxs=[1]))
+    with pytest.raises(ValueError, match="same length"):
+        bins_from_params(P(bin_lmins=[2, 3], bin_lmaxs=[4]))
