@@ -618,3 +618,30 @@ def test_harmonic_contraction_vs_oracle_and_pixel_path(name):
     assert _rel(F, Fp) < tol
     assert _rel(nb, nbp) < 1e-8
     assert _rel(q + nb, qp + nbp) < 10 * tol
+
+
+def test_harmonic_contraction_with_wide_and_gapped_bins():
+    """basis="harmonic" with delta_ell = 2 bins from lmin = 3 and with
+    explicit bins leaving a gap: the same F and noise bias as the pixel route
+    on the device (TQU, 1e-8), and the mode range follows the bins."""
+    from cosmoforge_amd import Fisher, Spectra
+
+    case = cases.case_fisher_nside4("TQU")
+    n = case.noise.shape[0]
+    x = np.random.default_rng(2).standard_normal((n, 1)) * np.sqrt(np.diag(case.noise))[:, None]
+    for extra in (dict(lmin=3, lmax=8, delta_ell=2), dict(bin_lmins=[2, 6, 9], bin_lmaxs=[3, 7, 12], lmax=12)):
+        print("harmonic bins", extra, flush=True)
+        out = []
+        for basis in ("harmonic", False):
+            p = _params(case, nsims=1, **extra)
+            f = Fisher(p, basis=basis, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls, beam=case.beam)
+            f.run()
+            sp = Spectra(fisher=f, maps1=x)
+            sp.run()
+            out.append((f.get_fisher_matrix(), sp.qml_noise_bias, f))
+        (F, nb, fh), (Fp, nbp, _) = out
+        assert F.shape == Fp.shape
+        assert _rel(F, Fp) < 1e-8
+        assert _rel(nb, nbp) < 1e-8
+        lo, hi = min(a for a, _ in fh.bins), max(b for _, b in fh.bins)
+        assert fh.derivs is not None and (fh.derivs.lmin, fh.derivs.lmax) == (lo, hi)
