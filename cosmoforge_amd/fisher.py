@@ -230,8 +230,6 @@ class Fisher:
         the same covariance, computed on the device when not given, and the
         result is cached.  Shape (nspectra nbins, nspectra n_ell), blocks
         spectrum-major as ``get_bandpower_slices``."""
-        import scipy.linalg
-
         if self.bins is None or self.fisher is None:
             raise ValueError("Bandpower window requires binning. Call set_binning() before run().")
         cached = getattr(self, "_bandpower_window", None)
@@ -241,9 +239,14 @@ class Fisher:
         n_ell = lmax - lmin + 1
         if per_ell_fisher is None:
             per_ell = [(ell, ell) for ell in range(lmin, lmax + 1)]
-            per_ell_fisher, _ = pipeline.fisher(self.cov, per_ell, self.params.output_convention,
-                                                precision=self.precision, keep_stacks=False,
-                                                mixed_gemm=self.mixed_gemm)
+            if self.basis == "harmonic":
+                from .harmonic import HarmonicContraction
+
+                per_ell_fisher = HarmonicContraction(self.cov, per_ell, self.params.output_convention).fisher()
+            else:
+                per_ell_fisher, _ = pipeline.fisher(self.cov, per_ell, self.params.output_convention,
+                                                    precision=self.precision, keep_stacks=False,
+                                                    mixed_gemm=self.mixed_gemm)
         per_ell_fisher = np.asarray(per_ell_fisher, dtype=np.float64)
         nspec = len(self.spectra_list)
         if per_ell_fisher.shape != (nspec * n_ell, nspec * n_ell):
@@ -253,7 +256,9 @@ class Fisher:
         for b, (lo, hi) in enumerate(self.bins):
             Qb[b, lo - lmin: hi - lmin + 1] = 1.0
         Q = np.kron(np.eye(nspec), Qb)
-        W = scipy.linalg.cho_solve(scipy.linalg.cho_factor(self.fisher, lower=True), Q @ per_ell_fisher)
+        # F_b^-1 from the device's Jacobi-scaled Cholesky inverse, then two
+        # parameter-sized products
+        W = pipeline.inverse_symm(self.ctx, self.fisher) @ (Q @ per_ell_fisher)
         self._bandpower_window = W
         return W
 
