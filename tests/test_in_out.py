@@ -349,3 +349,26 @@ def test_inputs_from_files_match_in_memory_inputs(tmp_path):
     # load_reduced reads the reduced file back
     p2 = dict(params, covmatfile1=params["outnoisecovmat1"], load_reduced=True, outnoisecovmat1=None)
     assert np.array_equal(Inputs(p2, None, None, None, None).noise, case.noise)
+
+
+def test_cross_inputs_from_files(tmp_path):
+    """do_cross: the second noise covariance from covmatfile2 (reduced to the
+    active pixels, written back to outnoisecovmat2) and the second map set
+    from inputmapfile2, as the first ones."""
+    from cosmoforge_amd.core import resolve_maps
+
+    params = _b_files(tmp_path)
+    d = load("picslike_nside8_B.npz")
+    n2 = 2.0 * untril(768, d["ncov_tril"])
+    n2.tofile(str(tmp_path / "ncov2.bin"))
+    params.update(do_cross=True, covmatfile2=str(tmp_path / "ncov2.bin"), inputmapfile2=f("B_map.fits"),
+                  outnoisecovmat2=str(tmp_path / "out" / "red2.bin"))
+    inp = Inputs(params, None, None, None, None)
+    case, _, _ = case_picslike_b()
+    assert np.array_equal(inp.noise2, 2.0 * case.noise)
+    assert np.array_equal(np.fromfile(params["outnoisecovmat2"]).reshape(inp.n, inp.n), 2.0 * case.noise)
+    m2 = resolve_maps(inp.n, 1, None, inp.params, inp.layout, which=2)
+    assert np.array_equal(m2, case.maps)
+    # without do_cross the second covariance is not read
+    params.update(do_cross=False, covmatfile2=str(tmp_path / "missing.bin"))
+    assert Inputs(params, None, None, None, None).noise2 is None
