@@ -645,3 +645,35 @@ def test_harmonic_contraction_with_wide_and_gapped_bins():
         assert _rel(nb, nbp) < 1e-8
         lo, hi = min(a for a, _ in fh.bins), max(b for _, b in fh.bins)
         assert fh.derivs is not None and (fh.derivs.lmin, fh.derivs.lmax) == (lo, hi)
+
+
+def test_prefiltered_inputs_match_raw_inputs():
+    """noise_prefiltered / maps_prefiltered: a noise covariance F N F^T and
+    maps F d that already carry a graded filter F = U Sigma W^T, restricted
+    with U^T, give the F, q and noise bias of the raw N and d restricted with
+    Sigma W^T (exactly so for the rank-r operator)."""
+    from cosmoforge_amd import Fisher, Spectra
+    from cosmoforge_amd.filters import Filter
+
+    case, lay, cls_s, prm = _filter_case("T")
+    n = case.noise.shape[0]
+    rng = np.random.default_rng(21)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.r_[np.linspace(1.8, 0.6, n - 12), np.zeros(12)]
+    Fop = Q1 @ np.diag(s) @ Q2.T
+    flt = Filter.from_operator(Fop, range_rank=n - 12)
+    Fr = flt.U @ np.diag(flt.sigma) @ flt.W.T
+    x = rng.standard_normal((n, 1)) * np.sqrt(np.diag(case.noise))[:, None]
+    out = []
+    for pref in (False, True):
+        N = Fr @ case.noise @ Fr.T if pref else case.noise
+        d = Fr @ x if pref else x
+        f = Fisher(_params(case, nsims=1), basis=False, mask=case.mask, noise_cov1=N, cls_data=case.cls,
+                   beam=case.beam, pixel_filter=flt, noise_prefiltered=pref, maps_prefiltered=pref)
+        f.run()
+        sp = Spectra(fisher=f, maps1=d)
+        sp.run()
+        out.append((f.get_fisher_matrix(), sp.qml_results + sp.qml_noise_bias, sp.qml_noise_bias))
+    for a, b in zip(out[0], out[1]):
+        assert _rel(b, a) < 1e-9
