@@ -69,7 +69,7 @@ SIGNATURES = {
     "cfmi_fill": (ci, [vp, ci, i64, i64, vp, i64, cd]),
     "cfmi_gram": (ci, [vp, ci, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
     "cfmi_harmonic_modes": (ci, [vp, ci, i64, vp, ci, ci, vp, i64, i64, i64, i64, i64, i64]),
-    "cfmi_harmonic_fisher": (ci, [vp, vp, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci, vp]),
+    "cfmi_harmonic_fisher": (ci, [vp, vp, vp, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci, vp]),
     "cfmi_harmonic_pairsum": (ci, [vp, i64, vp, i64, i64, vp, i64, i64, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci,
                                    cd, vp, i64]),
     "cfmi_batch_create": (ci, [vp, ci, ci, vp, ci, vp, vp, vp, vp, ci, vp, ci, ci, ci, _c.POINTER(vp)]),

@@ -182,7 +182,10 @@ __device__ double block_sum(double v, double* red)
 }
 
 // One block per lower-triangle entry (p1 >= p2) of F.
-__global__ __launch_bounds__(RT) void harm_fisher_kernel(const double* __restrict__ M, int64_t ldm, HarmTables t,
+// M2: V^T C2^-1 V of a cross analysis (M itself otherwise):
+// Tr[C1^-1 u' v'^T C2^-1 u v^T] = M1[v, u'] M2[u, v'].
+__global__ __launch_bounds__(RT) void harm_fisher_kernel(const double* __restrict__ M,
+                                                         const double* __restrict__ M2, int64_t ldm, HarmTables t,
                                                          int npar, double* __restrict__ F)
 {
     const int64_t idx = blockIdx.x;
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(RT) void harm_fisher_kernel(const double* __restric
             const int64_t u = t.goff[P1[2 * e]] + k1, v = t.goff[P1[2 * e + 1]] + k1;
             for (int f = 0; f < n2; ++f) {
                 const int64_t u2 = t.goff[P2[2 * f]] + k2, v2 = t.goff[P2[2 * f + 1]] + k2;
-                term += M[v + u2 * ldm] * M[u + v2 * ldm];
+                term += M[v + u2 * ldm] * M2[u + v2 * ldm];
             }
         }
         acc += a * term;
@@ -287,9 +290,9 @@ int cfmi_harmonic_modes(cfmi_ctx* c, int spin, int64_t npix, const double* vec, 
     return CFMI_OK;
 }
 
-int cfmi_harmonic_fisher(cfmi_ctx* c, const double* M, int64_t ldm, int lmin, int nbins, const int* bin_lo,
-                         const int* bin_hi, int nspec, const int* pairs, const int64_t* goff, const double* w,
-                         int lw, double* F)
+int cfmi_harmonic_fisher(cfmi_ctx* c, const double* M, const double* M2, int64_t ldm, int lmin, int nbins,
+                         const int* bin_lo, const int* bin_hi, int nspec, const int* pairs, const int64_t* goff,
+                         const double* w, int lw, double* F)
 {
     if (!c) return CFMI_EINVAL;
     int rc = check_tables(c, lmin, nbins, bin_lo, bin_hi, nspec, pairs, goff, w, lw);
@@ -299,7 +302,8 @@ int cfmi_harmonic_fisher(cfmi_ctx* c, const double* M, int64_t ldm, int lmin, in
     const int64_t nent = npar * (npar + 1) / 2;
     if (nent > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "harmonic_fisher: too many entries");
     HarmTables t{bin_lo, bin_hi, pairs, goff, w, lw, nbins, lmin};
-    hipLaunchKernelGGL(harm_fisher_kernel, dim3((unsigned)nent), dim3(RT), 0, c->stream, M, ldm, t, (int)npar, F);
+    hipLaunchKernelGGL(harm_fisher_kernel, dim3((unsigned)nent), dim3(RT), 0, c->stream, M, M2 ? M2 : M, ldm, t,
+                       (int)npar, F);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }

@@ -61,9 +61,8 @@ class Fisher:
         if precision not in pipeline.PRECISIONS:
             raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
         self.precision = precision
-        if self.basis == "harmonic" and (precision != "fp64" or self.params.do_cross
-                                         or kwargs.get("pixel_filter") is not None):
-            raise NotImplementedError("basis='harmonic' runs the fp64 auto-spectrum path without a pixel filter")
+        if self.basis == "harmonic" and (precision != "fp64" or kwargs.get("pixel_filter") is not None):
+            raise NotImplementedError("basis='harmonic' runs in fp64 without a pixel filter")
         if self.params.do_cross and precision != "fp64":
             raise ValueError("precision='mixed' covers the auto-spectrum path; do_cross runs in fp64")
         self.derivs = None  # device derivative stacks (reused by Spectra when resident)
@@ -152,7 +151,8 @@ class Fisher:
         if self.basis == "harmonic":
             from .harmonic import HarmonicContraction
 
-            self.derivs = HarmonicContraction(self.cov, self.bins, self.params.output_convention)
+            self.derivs = HarmonicContraction(self.cov, self.bins, self.params.output_convention,
+                                              cov2=self.cov2 if self.params.do_cross else None)
             self.fisher = self.derivs.fisher()
         elif self.params.do_cross:
             self.fisher, self.derivs = pipeline.fisher_cross(self.cov, self.cov2, self.bins,
@@ -242,7 +242,8 @@ class Fisher:
             if self.basis == "harmonic":
                 from .harmonic import HarmonicContraction
 
-                per_ell_fisher = HarmonicContraction(self.cov, per_ell, self.params.output_convention).fisher()
+                per_ell_fisher = HarmonicContraction(self.cov, per_ell, self.params.output_convention,
+                                                     cov2=self.cov2 if self.params.do_cross else None).fisher()
             else:
                 per_ell_fisher, _ = pipeline.fisher(self.cov, per_ell, self.params.output_convention,
                                                     precision=self.precision, keep_stacks=False,

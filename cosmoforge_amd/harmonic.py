@@ -36,10 +36,13 @@ class HarmonicContraction:
     """Mode operator V (n x N_modes, column-major, in HBM) and the tables of
     one (Covariance, bins) pair."""
 
-    def __init__(self, cov: pipeline.Covariance, bins, convention: str = "Cl"):
+    def __init__(self, cov: pipeline.Covariance, bins, convention: str = "Cl",
+                 cov2: pipeline.Covariance | None = None):
+        """``cov2``: C2 = S + N2 of a cross analysis (``do_cross``)."""
         if getattr(cov, "pixel_filter", None) is not None:
             raise NotImplementedError("the harmonic contraction runs without a pixel filter")
         self.cov, self.bins = cov, [(int(a), int(b)) for a, b in bins]
+        self.cov2 = cov2
         ctx, inp = cov.ctx, cov.inp
         lay = inp.layout
         self.lmin = min(a for a, _ in self.bins)
@@ -86,7 +89,7 @@ class HarmonicContraction:
         self._w = ctx.upload(np.ascontiguousarray(w, dtype=np.float64))
         self._lo = ctx.upload(np.array([a for a, _ in self.bins], dtype=np.int32), dtype=torch.int32)
         self._hi = ctx.upload(np.array([b for _, b in self.bins], dtype=np.int32), dtype=torch.int32)
-        self._Z = None
+        self._Z = {}
 
     @property
     def flops(self) -> float:
@@ -99,45 +102,84 @@ class HarmonicContraction:
         return (self.lmin, self.nbins, P(self._lo), P(self._hi), self.nspec, P(self._pairs), P(self._goff),
                 P(self._w), self.lw)
 
-    def _z(self) -> torch.Tensor:
-        """Z = W V (n x N, column-major), kept for the QML forms."""
-        if self._Z is None:
-            ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
+    def _z(self, cov) -> torch.Tensor:
+        """Z = W V (n x N, column-major) of ``cov``, kept for the QML forms."""
+        key = id(cov)
+        if key not in self._Z:
+            ctx, n, nt = cov.ctx, cov.n, self.nt
             Z = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-            engine.gemm(ctx, False, False, n, nt, n, 1.0, self.cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
-            self._Z = Z
-        return self._Z
+            engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
+            self._Z[key] = Z
+        return self._Z[key]
 
-    def fisher(self) -> np.ndarray:
-        ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
-        Z = self._z()
+    def _m(self, cov) -> torch.Tensor:
+        ctx, n, nt = cov.ctx, cov.n, self.nt
+        Z = self._z(cov)
         M = torch.empty((nt, nt), dtype=torch.float64, device=ctx.device)
         engine.gram(ctx, nt, nt, n, Z, Z, out=M)
+        return M
+
+    def _r(self, cov) -> torch.Tensor:
+        """R = C^-1 V = W^T Z (column-major n x N); releases Z."""
+        ctx, n, nt = cov.ctx, cov.n, self.nt
+        Z = self._z(cov)
+        R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
+        engine.gemm(ctx, True, False, n, nt, n, 1.0, cov.factor.W, 0, n, Z, 0, n, 0.0, R, 0, n, 2)
+        del self._Z[id(cov)]
+        return R
+
+    def _ut(self, R: torch.Tensor, maps: np.ndarray) -> torch.Tensor:
+        """u^T C^-1 x = R^T x for every mode and simulation (column-major ns x N)."""
+        ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
+        ns = maps.shape[1]
+        X = ctx.upload_cm(maps)
+        Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)
+        engine.gemm(ctx, True, False, ns, nt, n, 1.0, X, 0, n, R, 0, n, 0.0, Ut, 0, ns)
+        return Ut
+
+    def fisher(self) -> np.ndarray:
+        """F = 1/2 Tr[C^-1 dC_p C^-1 dC_q], or 1/2 Tr[C1^-1 dC_p C2^-1 dC_q]
+        with ``cov2`` (``fisher.py:522-531,584-590``)."""
+        ctx = self.cov.ctx
+        M = self._m(self.cov)
+        M2 = self._m(self.cov2) if self.cov2 is not None else None
         F = torch.empty((self.npar, self.npar), dtype=torch.float64, device=ctx.device)
-        ctx.check(ctx.lib.cfmi_harmonic_fisher(ctx.h, P(M), nt, *self._tables(), P(F)), "harmonic_fisher")
-        del M
+        ctx.check(ctx.lib.cfmi_harmonic_fisher(ctx.h, P(M), P(M2) if M2 is not None else None, self.nt,
+                                               *self._tables(), P(F)), "harmonic_fisher")
+        del M, M2
         return engine.to_numpy_cm(F)
 
-    def qml(self, maps: np.ndarray, remove_nb: bool = True):
-        """(q (nsims, n_p), nb (n_p,)) as ``pipeline.qml``.  With R = C^-1 V =
-        W^T Z: u^T y = R_u^T x and u^T G v = R_u^T N R_v, so neither y nor G is
-        formed; a diagonal N scales the rows of R."""
-        ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
+    @staticmethod
+    def _maps(maps, n):
         maps = np.asarray(maps, dtype=np.float64)
         if maps.ndim == 1:
             maps = maps[:, None]
-        ns = maps.shape[1]
         if maps.shape[0] != n:
             raise ValueError(f"maps have {maps.shape[0]} rows, expected {n}")
-        Z = self._z()
-        R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)         # C^-1 V, column-major n x N
-        engine.gemm(ctx, True, False, n, nt, n, 1.0, self.cov.factor.W, 0, n, Z, 0, n, 0.0, R, 0, n, 2)
-        self._Z = None
-        del Z
-        X = ctx.upload_cm(maps)
-        Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)       # column-major ns x N
-        engine.gemm(ctx, True, False, ns, nt, n, 1.0, X, 0, n, R, 0, n, 0.0, Ut, 0, ns)
+        return maps
+
+    def qml(self, maps: np.ndarray, remove_nb: bool = True, maps2: np.ndarray | None = None):
+        """(q (nsims, n_p), nb (n_p,)) as ``pipeline.qml``.  With R = C^-1 V =
+        W^T Z: u^T y = R_u^T x and u^T G v = R_u^T N R_v, so neither y nor G is
+        formed; a diagonal N scales the rows of R.  With ``cov2`` and
+        ``maps2``: q = 1/2 y2^T dC y1 and no noise bias (``spectra.py:1146-1166``)."""
+        ctx, n, nt = self.cov.ctx, self.cov.n, self.nt
+        maps = self._maps(maps, n)
+        ns = maps.shape[1]
         q = torch.empty((ns, self.npar), dtype=torch.float64, device=ctx.device)  # column-major npar x ns
+        if self.cov2 is not None:
+            if maps2 is None:
+                raise ValueError("a cross analysis needs maps2")
+            maps2 = self._maps(maps2, n)
+            if maps2.shape != maps.shape:
+                raise ValueError(f"maps1 {maps.shape} and maps2 {maps2.shape} differ")
+            Ut1 = self._ut(self._r(self.cov), maps)
+            Ut2 = self._ut(self._r(self.cov2), maps2)
+            ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, 1, P(Ut2), ns, 1, P(Ut1), ns, 1, ns, *self._tables(),
+                                                    0.5, P(q), self.npar), "harmonic_pairsum")
+            return q.cpu().numpy(), np.zeros(self.npar)
+        R = self._r(self.cov)
+        Ut = self._ut(R, maps)
         ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, 1, P(Ut), ns, 1, P(Ut), ns, 1, ns, *self._tables(), 0.5,
                                                 P(q), self.npar), "harmonic_pairsum")
         NR = torch.empty_like(R)

@@ -93,10 +93,12 @@ class Spectra:
             if src is None:
                 from .harmonic import HarmonicContraction
 
-                src = f.derivs = HarmonicContraction(f.cov, f.bins, self.params.output_convention)
+                src = f.derivs = HarmonicContraction(f.cov, f.bins, self.params.output_convention,
+                                                     cov2=f.cov2 if self.params.do_cross else None)
             self.setup_maps()
             self.setup_fisher_inversion()
-            self.qml_results, self.qml_noise_bias = src.qml(self.maps1, remove_nb=bool(self.params.remove_nb))
+            self.qml_results, self.qml_noise_bias = src.qml(self.maps1, remove_nb=bool(self.params.remove_nb),
+                                                            maps2=self.maps2)
             return
         if src is None:
             src = pipeline.make_source(f.cov, f.bins, self.params.output_convention,

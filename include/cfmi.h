@@ -320,12 +320,13 @@ int cfmi_harmonic_modes(cfmi_ctx *ctx, int spin, int64_t npix, const double *vec
 
 /* F[p][q] (npar x npar, npar = nspec nbins, parameter p = spectrum p/nbins,
  * bin p%nbins) = 1/2 Tr[C^-1 dC_p C^-1 dC_q] from the symmetric
- * M = V^T C^-1 V (ldm): with dC_p = sum_{l in bin} w[s][l] sum_m sum over
+ * M = V^T C^-1 V (ldm); with M2 = V^T C2^-1 V (non-NULL) the cross-analysis
+ * 1/2 Tr[C1^-1 dC_p C2^-1 dC_q].  dC_p = sum_{l in bin} w[s][l] sum_m sum over
  * the spectrum's (u, v) group pairs of u_lm v_lm^T.  Device tables: bin_lo,
  * bin_hi (nbins), pairs (nspec x 4 group ids u0 v0 u1 v1, u1 = -1 for an
  * auto spectrum), goff (first column of each group in M), w (nspec x lw).
  * The caller keeps every index inside M and w.                           */
-int cfmi_harmonic_fisher(cfmi_ctx *ctx, const double *M, int64_t ldm, int lmin, int nbins,
+int cfmi_harmonic_fisher(cfmi_ctx *ctx, const double *M, const double *M2, int64_t ldm, int lmin, int nbins,
                          const int *bin_lo, const int *bin_hi, int nspec, const int *pairs,
                          const int64_t *goff, const double *w, int lw, double *F);
 
