@@ -250,6 +250,17 @@ class DeviceFilter:
             ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, r, r, P(M), r, P(sig)), "scale")
         return M
 
+    def restrict_columns(self, X: torch.Tensor, k: int) -> torch.Tensor:
+        """Sigma W^T X for an n x k column-major device X (r x k, column-major):
+        pixel-space mode vectors into the range (``basis/base.py``'s
+        V' = V W Sigma)."""
+        ctx, n, r = self.ctx, self.n, self.r
+        Y = torch.empty((k, r), dtype=torch.float64, device=ctx.device)
+        engine.gemm(ctx, True, False, r, k, n, 1.0, self.W, 0, n, X, 0, n, 0.0, Y, 0, r)
+        if self.sigma is not None:
+            ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, r, k, P(Y), r, P(self.sigma)), "scale")
+        return Y
+
     def restrict_data(self, maps: np.ndarray, prefiltered: bool = False) -> np.ndarray:
         """Sigma B^T d for (n, nsims) maps (host in, host out)."""
         ctx, n, r = self.ctx, self.n, self.r

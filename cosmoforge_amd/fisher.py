@@ -61,8 +61,8 @@ class Fisher:
         if precision not in pipeline.PRECISIONS:
             raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
         self.precision = precision
-        if self.basis == "harmonic" and (precision != "fp64" or kwargs.get("pixel_filter") is not None):
-            raise NotImplementedError("basis='harmonic' runs in fp64 without a pixel filter")
+        if self.basis == "harmonic" and precision != "fp64":
+            raise NotImplementedError("basis='harmonic' runs in fp64")
         if self.params.do_cross and precision != "fp64":
             raise ValueError("precision='mixed' covers the auto-spectrum path; do_cross runs in fp64")
         self.derivs = None  # device derivative stacks (reused by Spectra when resident)

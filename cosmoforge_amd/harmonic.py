@@ -39,8 +39,6 @@ class HarmonicContraction:
     def __init__(self, cov: pipeline.Covariance, bins, convention: str = "Cl",
                  cov2: pipeline.Covariance | None = None):
         """``cov2``: C2 = S + N2 of a cross analysis (``do_cross``)."""
-        if getattr(cov, "pixel_filter", None) is not None:
-            raise NotImplementedError("the harmonic contraction runs without a pixel filter")
         self.cov, self.bins = cov, [(int(a), int(b)) for a, b in bins]
         self.cov2 = cov2
         ctx, inp = cov.ctx, cov.inp
@@ -58,7 +56,10 @@ class HarmonicContraction:
             for lab in f.labels:
                 self.group[lab.upper()] = len(self.group)
         self.nt = self.ng * len(self.group)
-        n = cov.n
+        # modes on the active pixels; under a pixel filter they are then
+        # restricted to its range, V' = Sigma W^T V (r x N)
+        flt = getattr(cov, "pixel_filter", None)
+        n = inp.n
         self.V = torch.zeros((self.nt, n), dtype=torch.float64, device=ctx.device)   # column-major n x N
         for fi, f in enumerate(lay.fields):
             vec = cov.vecs[fi]
@@ -71,6 +72,8 @@ class HarmonicContraction:
                 rc = ctx.lib.cfmi_harmonic_modes(ctx.h, 2, f.npix, P(vec), self.lmin, self.lmax, P(self.V), n,
                                                  self.nt, f.offset, f.offset + f.npix, ge * self.ng, gb * self.ng)
             ctx.check(rc, "harmonic_modes")
+        if flt is not None:
+            self.V = flt.restrict_columns(self.V, self.nt)
         specs = lay.spectra_list()
         pairs = np.full((len(specs), 4), -1, dtype=np.int32)
         for k, sp in enumerate(specs):

@@ -712,3 +712,30 @@ def test_harmonic_cross_analysis_vs_oracle(name):
     assert _rel(f.get_fisher_matrix(), Fo) < tol
     assert _rel(sp.qml_results, qo) < 10 * tol
     assert np.all(sp.qml_noise_bias == 0)
+
+
+@pytest.mark.parametrize("name", ["T", "TQU"])
+def test_harmonic_contraction_under_pixel_filter(name):
+    """basis="harmonic" with a pixel filter: the modes restricted to the
+    filter's range (V' = Sigma W^T V) give the filtered pixel route's F, q + nb
+    and nb (1e-10 for temperature, 1e-8 with polarisation)."""
+    from cosmoforge_amd import Fisher, Spectra
+    from cosmoforge_amd.filters import Filter
+
+    case, lay, cls_s, prm = _filter_case(name)
+    n = case.noise.shape[0]
+    rng = np.random.default_rng(29)
+    T = rng.standard_normal((n, 3))
+    flt = Filter.from_deprojection(T, mask=case.mask)
+    x = rng.standard_normal((n, 1)) * np.sqrt(np.diag(case.noise))[:, None]
+    out = []
+    for basis in ("harmonic", False):
+        f = Fisher(_params(case, nsims=1), basis=basis, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+                   beam=case.beam, pixel_filter=flt)
+        f.run()
+        sp = Spectra(fisher=f, maps1=x)
+        sp.run()
+        out.append((f.get_fisher_matrix(), sp.qml_results + sp.qml_noise_bias, sp.qml_noise_bias))
+    tol = 1e-10 if name == "T" else 1e-8
+    for k, (a, b) in enumerate(zip(out[0], out[1])):
+        assert _rel(a, b) < (tol if k == 0 else 100 * tol), k
