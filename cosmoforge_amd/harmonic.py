@@ -56,6 +56,14 @@ class HarmonicContraction:
             for lab in f.labels:
                 self.group[lab.upper()] = len(self.group)
         self.nt = self.ng * len(self.group)
+        # HBM: V (n x N, twice while a filter restricts it), Z and M (N x N),
+        # then R and N R; C's factor and W are already resident
+        need = 8 * (3 * inp.n * self.nt + self.nt * self.nt) * (2 if cov2 is not None else 1)
+        free = pipeline._free_bytes(ctx)
+        if need > free:
+            raise MemoryError(f"the harmonic contraction needs {need / 1e9:.1f} GB of device memory for "
+                              f"{self.nt} modes over {inp.n} rows, {free / 1e9:.1f} GB are free; use basis=False "
+                              "(the pixel route bands its products)")
         # modes on the active pixels; under a pixel filter they are then
         # restricted to its range, V' = Sigma W^T V (r x N)
         flt = getattr(cov, "pixel_filter", None)
