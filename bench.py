@@ -198,7 +198,7 @@ def surrogate_leg(ctx, patches, tun, log, n_train=20000, n_cand=65536):
     rep = screen(model, cand, lambda th: like(th, graph=True), spread_threshold=0.5, keep_delta=25.0)
     torch.cuda.synchronize()
     t_screen = time.perf_counter() - t0
-    # the surrogate's error, judged by the exact likelihood on a sample of the rejected candidates
+    # the surrogate's error, measured with the exact likelihood on a sample of the rejected candidates
     rng = np.random.default_rng(3)
     rej = rep.rejected
     chk = rng.choice(rej, size=min(2048, rej.size), replace=False) if rej.size else rej
