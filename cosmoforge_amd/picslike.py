@@ -47,7 +47,7 @@ class ParameterGrid:
     (``parameter_grid.py:85-262``)."""
 
     def __init__(self, core_params, parameter_ranges: dict, theoretical_spectra: dict | None = None,
-                 root_dir=None, root_filename=None, lmax_signal=None):
+                 root_dir=None, root_filename=None, lmax_signal=None, _require_spectra: bool = True):
         self.core_params = core_params
         self.lmax_signal = lmax_signal if lmax_signal is not None else 4 * core_params.nside
         self.parameter_names = list(parameter_ranges.keys())
@@ -62,7 +62,7 @@ class ParameterGrid:
             self.root_dir, self.root_filename = root_dir, root_filename
             self.read_theoretical_spectra()
         missing = [p for p in self.grid_points if p not in self.theoretical_spectra]
-        if missing:
+        if missing and _require_spectra:
             raise ValueError(f"Missing theoretical spectra for {len(missing)} grid points")
 
     def spectrum_file(self, point) -> str:
@@ -81,25 +81,60 @@ class ParameterGrid:
             self.theoretical_spectra[point] = spec if fid is None else _blend(spec, fid, lmin, lmax)
 
     def get_spectrum(self, point):
-        return self.theoretical_spectra[point]
+        try:
+            return self.theoretical_spectra[point]
+        except KeyError as exc:
+            raise KeyError(f"No theoretical spectrum found for parameters: {point}") from exc
 
     def get_parameter_dict(self, point):
+        if len(point) != len(self.parameter_names):
+            raise ValueError(f"Parameter point has {len(point)} values, expected {len(self.parameter_names)}")
         return dict(zip(self.parameter_names, point))
+
+    def get_total_points(self) -> int:
+        return len(self.grid_points)
 
     @property
     def n_points(self):
         return len(self.grid_points)
 
+    def get_grid_shape(self) -> tuple:
+        return tuple(len(self.parameter_ranges[k]) for k in self.parameter_names)
+
+    def get_parameter_index(self, point) -> int:
+        try:
+            return self.grid_points.index(point)
+        except ValueError as exc:
+            raise ValueError(f"Parameter point {point} not found in grid") from exc
+
+    def add_spectrum(self, point, spectrum) -> None:
+        self.theoretical_spectra[point] = deepcopy(spectrum)
+
+    def __len__(self) -> int:
+        return len(self.grid_points)
+
+    def __iter__(self):
+        return iter(self.grid_points)
+
+    def __contains__(self, point) -> bool:
+        return point in self.grid_points
+
 
 class LikelihoodResult:
-    def __init__(self, parameter_grid: ParameterGrid, chi_squared_values, log_likelihood_values):
-        self.parameter_grid = parameter_grid
-        self.chi_squared_values = np.asarray(chi_squared_values)
-        self.log_likelihood_values = np.asarray(log_likelihood_values)
+    """chi2 and ln L over a parameter grid (``likelihood_result.py:33-420``)."""
 
-    @property
-    def likelihood_values(self):
-        return np.exp(self.log_likelihood_values - np.max(self.log_likelihood_values))
+    def __init__(self, parameter_grid: ParameterGrid, chi_squared_values, log_likelihood_values):
+        npts = len(parameter_grid.grid_points)
+        if len(chi_squared_values) != npts:
+            raise ValueError(f"Chi-squared array length {len(chi_squared_values)} doesn't match grid size {npts}")
+        if len(log_likelihood_values) != npts:
+            raise ValueError(f"Log-likelihood array length {len(log_likelihood_values)} doesn't match grid size "
+                             f"{npts}")
+        self.parameter_grid = parameter_grid
+        self.chi_squared_values = np.array(chi_squared_values, dtype=np.float64)
+        self.log_likelihood_values = np.array(log_likelihood_values, dtype=np.float64)
+        # exp(lnL - max lnL), the grid's relative likelihood
+        self.likelihood_values = np.exp(self.log_likelihood_values - np.max(self.log_likelihood_values))
 
     def get_best_fit(self):
         i = int(np.argmax(self.likelihood_values))
@@ -107,6 +142,81 @@ class LikelihoodResult:
 
     def get_chi_squared_minimum(self) -> float:
         return float(np.min(self.chi_squared_values))
+
+    def get_maximum_likelihood(self) -> float:
+        return float(np.max(self.likelihood_values))
+
+    def get_marginalized_likelihood(self, parameter_name: str) -> np.ndarray:
+        """Likelihood summed over the other parameters, normalised to unit sum."""
+        grid = self.parameter_grid
+        if parameter_name not in grid.parameter_names:
+            raise ValueError(f"Parameter '{parameter_name}' not found in grid")
+        k = grid.parameter_names.index(parameter_name)
+        values = np.asarray(grid.parameter_ranges[parameter_name])
+        col = np.array([pt[k] for pt in grid.grid_points])
+        marg = np.array([self.likelihood_values[col == v].sum() for v in values])
+        tot = marg.sum()
+        return marg / tot if tot > 0 else marg
+
+    @staticmethod
+    def _interval(values: np.ndarray, like: np.ndarray, level: float):
+        """Highest-density set of grid values holding ``level`` of the mass;
+        its hull (``likelihood_result.py:270-328``)."""
+        tot = like.sum()
+        if tot == 0:
+            return float(values[0]), float(values[-1])
+        order = np.argsort(like)[::-1]
+        mass = np.cumsum(like[order]) / tot
+        keep = values[order[: int(np.searchsorted(mass, level)) + 1]]
+        return float(keep.min()), float(keep.max())
+
+    def get_confidence_intervals(self, confidence_level: float = 0.68) -> dict:
+        grid = self.parameter_grid
+        return {k: self._interval(np.asarray(grid.parameter_ranges[k]), self.get_marginalized_likelihood(k),
+                                  confidence_level) for k in grid.parameter_names}
+
+    def get_summary_statistics(self) -> dict:
+        return {"best_fit_parameters": self.get_best_fit(), "chi_squared_minimum": self.get_chi_squared_minimum(),
+                "maximum_likelihood": self.get_maximum_likelihood(),
+                "confidence_intervals_68": self.get_confidence_intervals(0.68),
+                "confidence_intervals_95": self.get_confidence_intervals(0.95),
+                "total_parameter_points": len(self.parameter_grid),
+                "parameter_names": list(self.parameter_grid.parameter_names)}
+
+    def save(self, output_path) -> None:
+        """The grid and the results as a NumPy archive (no pickled objects;
+        the reference pickles the object)."""
+        from pathlib import Path
+
+        path = Path(output_path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        grid = self.parameter_grid
+        with open(path, "wb") as fh:
+            np.savez(fh, names=np.array(grid.parameter_names, dtype=str),
+                     points=np.array(grid.grid_points, dtype=np.float64).reshape(len(grid.grid_points), -1),
+                     chi2=self.chi_squared_values, lnl=self.log_likelihood_values,
+                     **{f"range_{i}": np.asarray(grid.parameter_ranges[k], dtype=np.float64)
+                        for i, k in enumerate(grid.parameter_names)})
+
+    @classmethod
+    def load(cls, input_path) -> "LikelihoodResult":
+        """A result written by ``save`` (the grid without its spectra)."""
+        from pathlib import Path
+        from types import SimpleNamespace
+
+        path = Path(input_path)
+        if not path.exists():
+            raise FileNotFoundError(f"Results file not found: {path}")
+        with np.load(path, allow_pickle=False) as d:
+            names = [str(x) for x in d["names"]]
+            ranges = {k: d[f"range_{i}"] for i, k in enumerate(names)}
+            grid = ParameterGrid(SimpleNamespace(nside=0), ranges, theoretical_spectra={}, lmax_signal=0,
+                                 _require_spectra=False)
+            return cls(grid, d["chi2"], d["lnl"])
+
+    def __repr__(self) -> str:
+        return (f"LikelihoodResult(parameters={len(self.parameter_grid.parameter_names)}, "
+                f"grid_points={len(self.parameter_grid)}, χ²_min={self.get_chi_squared_minimum():.3f})")
 
 
 class PICSLike:
@@ -148,6 +258,7 @@ class PICSLike:
         self.cov = pipeline.Covariance(self.ctx, self.inputs, pixel_filter=flt)
         self.likelihood_result = None
         self.simulation_results = None
+        self.simulation_index = 0
 
     @property
     def lmax_signal(self):
@@ -210,5 +321,26 @@ class PICSLike:
         return self.simulation_results
 
     def get_mean_likelihood_result(self):
-        self._need()
+        if self.likelihood_result is None:
+            raise RuntimeError("Likelihood not computed. Call compute() first.")
         return self.likelihood_result
+
+    def set_simulation_index(self, sim_idx: int) -> None:
+        """Select the simulation an analysis reports on (``picslike.py:377-407``)."""
+        if self.maps1 is not None:
+            top = self.maps1.shape[1] - 1
+            if sim_idx < 0 or sim_idx > top:
+                raise ValueError(f"Simulation index {sim_idx} out of range [0, {top}]")
+        self.simulation_index = sim_idx
+
+    def save_results(self, output_path: str) -> None:
+        """The mean result at ``output_path`` and every simulation's at
+        ``{stem}_sim_XX{suffix}`` beside it (``picslike.py:771-810``)."""
+        from pathlib import Path
+
+        if self.likelihood_result is None:
+            raise RuntimeError("No results to save. Run compute() first.")
+        self.likelihood_result.save(output_path)
+        base = Path(output_path)
+        for i, r in enumerate(self.simulation_results or []):
+            r.save(str(base.parent / f"{base.stem}_sim_{i:02d}{base.suffix}"))
