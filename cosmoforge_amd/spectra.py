@@ -116,43 +116,65 @@ class Spectra:
     compute = run
 
     # -- getters ------------------------------------------------------------
+    _MODES = ("deconvolved", "decorrelated", "convolved")
+
+    def _check_mode(self, mode):
+        if mode not in self._MODES:
+            raise ValueError(f"mode must be one of {self._MODES}, got '{mode}'")
+
     def _normalize_spectra(self, spectra: np.ndarray) -> np.ndarray:
         if self.invfisher is None:
             raise ValueError("Fisher inversion must be set up first.")
         return spectra @ self.invfisher
+
+    def _inv_fisher_sqrt(self) -> np.ndarray:
+        """F^-1/2 from the eigendecomposition, eigenvalues at or below 1e-12 of
+        the largest dropped (``spectra.py:663-740``)."""
+        if self.inv_fisher_sqrt is None:
+            w, v = np.linalg.eigh(self.fisher_normalized)
+            inv = np.zeros_like(w)
+            ok = w > w.max() * 1e-12
+            inv[ok] = 1.0 / np.sqrt(w[ok])
+            self.inv_fisher_sqrt = (v * inv) @ v.T
+        return self.inv_fisher_sqrt
 
     def _as_dict(self, arr):
         f = self.fisher_instance
         nb = len(f.bins)
         return {lab: arr[..., k * nb:(k + 1) * nb] for k, lab in enumerate(f.spectra_list)}
 
+    def _make_convolve_theory(self, W: np.ndarray):
+        """Callable taking a flat (n_params,) binned theory or a label-keyed
+        dict of per-bin arrays; returns W @ theory (``spectra.py:1573-1614``)."""
+        labs = self.fisher_instance.spectra_list
+
+        def convolve_theory(cl_theory):
+            if isinstance(cl_theory, dict):
+                missing = [lab for lab in labs if lab not in cl_theory]
+                if missing:
+                    raise KeyError(f"convolve_theory dict missing entries for: {missing}. "
+                                   f"Expected keys: {labs}")
+                cl_theory = np.concatenate([np.asarray(cl_theory[lab]) for lab in labs])
+            return W @ np.asarray(cl_theory)
+
+        return convolve_theory
+
     def get_power_spectra(self, mode: str = "deconvolved", *, as_dict: bool = False):
+        """deconvolved y F^-1, decorrelated y F^-1/2, or convolved
+        (y, W, convolve_theory) (``spectra.py:1358-1553``)."""
+        self._check_mode(mode)
         if self.qml_results is None:
             return None
         if mode == "deconvolved":
             out = self._normalize_spectra(self.qml_results)
         elif mode == "decorrelated":
-            if self.inv_fisher_sqrt is None:
-                w, v = np.linalg.eigh(self.fisher_normalized)
-                self.inv_fisher_sqrt = (v / np.sqrt(w)) @ v.T
-            out = self.qml_results @ self.inv_fisher_sqrt
-        elif mode == "convolved":
-            W = self.fisher_instance.get_window_matrix()
-            y = self.qml_results
-
-            def convolve_theory(cl_theory):
-                if isinstance(cl_theory, dict):
-                    labs = self.fisher_instance.spectra_list
-                    missing = [lab for lab in labs if lab not in cl_theory]
-                    if missing:
-                        raise KeyError(f"convolve_theory dict missing entries for: {missing}. "
-                                       f"Expected keys: {labs}")
-                    cl_theory = np.concatenate([np.asarray(cl_theory[lab]) for lab in labs])
-                return W @ np.asarray(cl_theory)
-
-            return ((self._as_dict(y) if as_dict else y), W, convolve_theory)
+            out = self.qml_results @ self._inv_fisher_sqrt()
         else:
-            raise ValueError(f"unknown mode {mode!r}; expected deconvolved, decorrelated or convolved")
+            W = self.fisher_instance.get_window_matrix()
+            if W is None:
+                raise ValueError("Window matrix not available from Fisher instance.")
+            y = self.qml_results
+            return ((self._as_dict(y) if as_dict else y), W, self._make_convolve_theory(W))
         return self._as_dict(out) if as_dict else out
 
     def get_noise_bias(self):
@@ -160,14 +182,109 @@ class Spectra:
             return None
         return self._normalize_spectra(self.qml_noise_bias)
 
-    def get_covariance(self):
-        return self.invfisher
-
-    def get_error_bars(self, *, as_dict: bool = False):
+    def get_covariance(self, mode: str = "deconvolved"):
+        """F^-1, the identity, or F for the three modes (``spectra.py:1880-1971``)."""
+        self._check_mode(mode)
+        if mode == "convolved":
+            return None if self.fisher_normalized is None else self.fisher_normalized.copy()
         if self.invfisher is None:
             return None
-        err = np.sqrt(np.diag(self.invfisher))
+        return self.invfisher.copy() if mode == "deconvolved" else np.eye(self.invfisher.shape[0])
+
+    def get_error_bars(self, mode: str = "deconvolved", *, as_dict: bool = False):
+        cov = self.get_covariance(mode)
+        if cov is None:
+            return None
+        err = np.sqrt(np.diag(cov))
         return self._as_dict(err) if as_dict else err
+
+    def convolve_theory(self, cl_theory):
+        """W @ binned theory, the expected raw estimate (``spectra.py:2039-2097``)."""
+        W = self.fisher_instance.get_window_matrix()
+        return None if W is None else W @ np.asarray(cl_theory)
+
+    def convolve_theory_for_inference(self, cl_theory):
+        """Expected deconvolved bandpowers of a per-l theory through the
+        bandpower window (``spectra.py:1735-1878``): a flat spectrum-major
+        vector of nspectra n_ell values, a label-keyed dict, or for one
+        spectrum an (n_ell,) or (>= lmax+1,) l-indexed array."""
+        f = self.fisher_instance
+        if getattr(f, "fisher", None) is None:
+            raise ValueError("Fisher matrix has not been computed. Call fisher.run() "
+                             "(and spectra.run()) before convolve_theory_for_inference().")
+        lmin, lmax = self.params.lmin, self.params.lmax
+        n_ell = lmax - lmin + 1
+        labs = f.spectra_list
+
+        def perell(arr, what):
+            if arr.ndim != 1:
+                raise ValueError(f"{what} must be 1D, got shape {arr.shape}")
+            if arr.size == n_ell:
+                return arr
+            if arr.size >= lmax + 1:
+                return arr[lmin:lmax + 1]
+            raise ValueError(f"{what} length {arr.size} does not match expected "
+                             f"{n_ell} (ℓ=lmin..lmax) or {lmax + 1} (ℓ=0..lmax)")
+
+        if isinstance(cl_theory, dict):
+            missing = [lab for lab in labs if lab not in cl_theory]
+            if missing:
+                raise KeyError(f"cl_theory dict missing entries for: {missing}. Expected keys: {labs}")
+            vec = np.concatenate([perell(np.asarray(cl_theory[lab], dtype=np.float64), lab) for lab in labs])
+        else:
+            cl = np.asarray(cl_theory, dtype=np.float64)
+            if cl.ndim != 1:
+                raise ValueError(f"cl_theory must be 1D, got shape {cl.shape}")
+            if len(labs) > 1:
+                if cl.size != len(labs) * n_ell:
+                    raise ValueError(f"cl_theory length {cl.size} does not match expected "
+                                     f"{len(labs) * n_ell} (nspectra·n_ell, spectrum-major) for a "
+                                     f"{len(labs)}-spectrum run; or pass a label-keyed dict.")
+                vec = cl
+            else:
+                vec = perell(cl, "cl_theory")
+        return f.get_bandpower_window_function() @ vec
+
+    def write_power_spectra(self, mode: str = "deconvolved", filename: str | None = None,
+                            include_errors: bool = True) -> None:
+        """Mean over simulations as (nbins, nspectra) columns, with the error
+        bars in ``{base}_errors.{ext}``; convolved mode writes the mean raw y
+        and the window ``{base}_window.{ext}`` instead
+        (``spectra.py:2099-2239``).  The default name is
+        ``{outclfile base}_{mode}.{ext}``, else ``spectra_{mode}.dat``."""
+        self._check_mode(mode)
+        if filename is None:
+            out = getattr(self.params, "outclfile", None)
+            if not in_out._falsy(out):
+                base, _, ext = str(out).rpartition(".") if "." in str(out) else (str(out), "", "dat")
+                filename = f"{base}_{mode}.{ext}"
+            else:
+                filename = f"spectra_{mode}.dat"
+        filename = str(filename)
+        stem, dot, ext = filename.rpartition(".")
+
+        def sibling(tag):
+            return f"{stem}_{tag}.{ext}" if dot else f"{filename}_{tag}"
+
+        nb = len(self.fisher_instance.bins)
+        if mode == "convolved":
+            res = self.get_power_spectra(mode="convolved")
+            if res is None:
+                return
+            y, W, _ = res
+            in_out._make_parent(in_out._path(filename))
+            np.savetxt(filename, np.mean(y, axis=0), header="Raw QML estimates (y vector, mean across simulations)")
+            np.savetxt(sibling("window"), W, header="Window matrix W: <y> = W @ C_true")
+            return
+        spectra = self.get_power_spectra(mode=mode)
+        if spectra is None:
+            return
+        # spectrum-major vector -> (nbins, nspectra) columns (spectra_io.py:72-97)
+        in_out.writecl(filename, np.mean(spectra, axis=0).reshape(-1, nb).T)
+        if include_errors:
+            err = self.get_error_bars(mode=mode)
+            if err is not None:
+                in_out.writecl(sibling("errors"), err.reshape(-1, nb).T)
 
     def get_effective_ells(self, use_midpoint: bool = False):
         """Bin midpoints (``use_midpoint``), else the window centroids of

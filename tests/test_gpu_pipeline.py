@@ -423,6 +423,12 @@ def test_fisher_spectra_from_files_match_in_memory(tmp_path):
     assert np.array_equal(err.reshape(-1), sp.get_error_bars())
     Ci = np.fromfile(cfg["outinvcovmatfile1"]).reshape(768, 768)
     assert np.array_equal(Ci, f.inverse_covariance)
+    sp.params.outclfile = str(tmp_path / "out" / "cl.dat")
+    sp.write_power_spectra()
+    cl = np.loadtxt(tmp_path / "out" / "cl_deconvolved.dat").reshape(len(f.bins), -1)
+    np.testing.assert_allclose(cl.T.reshape(-1), sp.get_power_spectra().mean(0), rtol=1e-15)
+    sp.write_power_spectra(mode="convolved")
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "out" / "cl_convolved_window.dat"), F, rtol=1e-15)
 
 
 @pytest.mark.parametrize("name", ["T", "TQU"])
