@@ -89,6 +89,10 @@ struct B3Args {
     float pdiag;
     int64_t tm, tn, tmp;      // row / column tiles; tmp: enumerated rows (tm, or tm + 1)
     int vecA, vecB;           // 16-byte vector loads allowed
+    int grouped;              // 1: XCD-grouped tile order (group_tile), 0: tile_order over blockIdx.z
+    int64_t gm, gn;           // group of gm x gn tiles
+    int64_t gtm, gtmp, gtn;   // group rows (gtmp: enumerated, odd), group columns
+    int64_t ngroups;          // gtmp gtn batch
 };
 
 // Tile order: column tiles of B in turn, the row tiles of A in each; with a
@@ -109,6 +113,35 @@ __device__ __forceinline__ bool tile_order(int64_t b, int64_t tm, int64_t tmp, i
     if (ti >= tm) return false;
     if (tri & 1) ti = tm - 1 - ti;
     if (tri & 4) tj = tn - 1 - tj;
+    return true;
+}
+
+// XCD-grouped order.  Tiles are taken in groups of gm x gn (one group fills
+// the CUs of one XCD: 32 CUs x 1 block of 256-tiles, or x 2 blocks of
+// 128-tiles), so the blocks an XCD runs at once share gm row panels of A and
+// gn column panels of B in its L2 instead of bringing their own.  Workgroup x
+// runs on XCD x % 8 (round-robin dispatch); its slot x / 8 on that XCD picks
+// the XCD's (slot / G)-th group, global group gg = 8 (slot / G) + x % 8, and
+// the tile slot % G in it (rows fastest).  Groups are enumerated group rows
+// fastest over an odd count gtmp (a padding row exits at once), then group
+// columns, then the batch, so consecutive groups of a trimmed product, whose
+// work varies with the row, rotate over the XCDs.  Every (batch, tile) pair is
+// reached exactly once; slots beyond the grid of tiles exit.
+__device__ __forceinline__ bool group_tile(const B3Args& g, int64_t x, int64_t& z, int64_t& ti, int64_t& tj)
+{
+    const int64_t G = g.gm * g.gn;
+    const int64_t slot = x >> 3;
+    const int64_t gg = (slot / G) * 8 + (x & 7);
+    if (gg >= g.ngroups) return false;
+    const int64_t w = slot % G;
+    const int64_t gi = gg % g.gtmp, rest = gg / g.gtmp;
+    if (gi >= g.gtm) return false;
+    z = rest / g.gtn;
+    ti = gi * g.gm + w % g.gm;
+    tj = (rest % g.gtn) * g.gn + w / g.gm;
+    if (ti >= g.tm || tj >= g.tn) return false;
+    if (g.tri & 1) ti = g.tm - 1 - ti;
+    if (g.tri & 4) tj = g.tn - 1 - tj;
     return true;
 }
 
@@ -253,9 +286,12 @@ __global__ __launch_bounds__(Cfg<BT>::NT, Cfg<BT>::MINB) void gemm_b3_kernel(B3A
     __shared__ __attribute__((aligned(16))) bf16 As[3][BT][LROW];
     __shared__ __attribute__((aligned(16))) bf16 Bs[3][BT][LROW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t z = blockIdx.z;
-    int64_t ti, tj;
-    if (!tile_order(blockIdx.x, g.tm, g.tmp, g.tn, g.tri, ti, tj)) return;
+    int64_t z = blockIdx.z, ti, tj;
+    if (g.grouped) {
+        if (!group_tile(g, blockIdx.x, z, ti, tj)) return;
+    } else if (!tile_order(blockIdx.x, g.tm, g.tmp, g.tn, g.tri, ti, tj)) {
+        return;
+    }
     const int64_t m0 = ti * BT, n0 = tj * BT;
     if (MODE == 2 && g.col0 + n0 > g.row0 + m0 + BT - 1) return;  // tile above the diagonal
     const bf16* A = g.A + z * g.sA;
@@ -421,8 +457,17 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.vecA = aligned16(A, lda, pA) && (sA % 8 == 0);
     g.vecB = aligned16(B, ldb, pB) && (sB % 8 == 0);
     g.tmp = ((flags & 1) || out_mode == 2) && g.tm % 2 == 0 ? g.tm + 1 : g.tm;
-    if (g.tmp * g.tn > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
-    dim3 grid((unsigned)(g.tmp * g.tn), 1, (unsigned)batch);
+    // flag 1024: the column-by-column order of tile_order, batch over blockIdx.z
+    g.grouped = !(flags & 1024);
+    g.gm = big ? 4 : 8;
+    g.gn = 8;
+    g.gtm = (g.tm + g.gm - 1) / g.gm;
+    g.gtn = (g.tn + g.gn - 1) / g.gn;
+    g.gtmp = g.gtm | 1;
+    g.ngroups = g.gtmp * g.gtn * batch;
+    const int64_t nwg = g.grouped ? (g.ngroups + 7) / 8 * 8 * g.gm * g.gn : g.tmp * g.tn;
+    if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_b3: too many tiles");
+    dim3 grid((unsigned)nwg, 1, g.grouped ? 1u : (unsigned)batch);
 #define CFMI_B3_GO(MODE_, BT_) \
     hipLaunchKernelGGL((gemm_b3_kernel<MODE_, BT_>), grid, dim3(Cfg<BT_>::NT), 0, c->stream, g)
     if (big) {

@@ -137,6 +137,45 @@ def test_gemm_b3_packed_band(ctx, tile):
     assert np.max(np.abs(got - want)) <= TOL * _scale(Z, W)
 
 
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("mode,flags", [(1, 1), (2, 4), (0, 0)])
+def test_gemm_b3_tile_orders_agree_bitwise(ctx, tile, mode, flags):
+    """The XCD-grouped tile order (default) and the column-by-column order
+    (flag 1024) compute every tile with the same K loop, so a batched,
+    trimmed product over several groups of tiles is bit-identical in both,
+    and within tolerance of FP64."""
+    import torch
+    from cosmoforge_amd.engine import P
+
+    rng = np.random.default_rng(11 + mode)
+    nb, n, r0 = 3, 1100, 0
+    A = [np.tril(rng.standard_normal((n, n))) for _ in range(nb)]
+    B = [np.tril(rng.standard_normal((n, n))) for _ in range(nb)]
+    Ap, pA = _planes(ctx, np.vstack(A), n)
+    Bp, pB = _planes(ctx, np.vstack(B), n)
+    outs = []
+    for order in (0, 1024):
+        if mode == 1:
+            pC = nb * n * n
+            C = torch.zeros(3 * pC, dtype=torch.int16, device=ctx.device)
+            args = (C, n, n * n, pC)
+        elif mode == 2:
+            C = torch.zeros(nb * (n * (n + 1) // 2), dtype=torch.float32, device=ctx.device)
+            args = (C, 0, n * (n + 1) // 2, 0)
+        else:
+            C = torch.zeros(nb * n * n, dtype=torch.float32, device=ctx.device)
+            args = (C, n, n * n, 0)
+        ctx.check(ctx.lib.cfmi_gemm_b3(ctx.h, mode, n, n, n, 1.0, P(Ap), n, n * n, pA, 0, P(Bp), n, n * n, pB, 0,
+                                       0.0, P(args[0]), args[1], args[2], args[3], nb, flags | tile | order, r0, 0,
+                                       0, 1.0), "gemm_b3")
+        outs.append(C.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    if mode == 0:
+        got = outs[0].astype(np.float64).reshape(nb, n, n)
+        for b in range(nb):
+            assert np.max(np.abs(got[b].T - A[b] @ B[b].T)) <= TOL * _scale(A[b], B[b])
+
+
 def test_gemm_b3_rejects_bad_arguments(ctx):
     from cosmoforge_amd.engine import P
 
