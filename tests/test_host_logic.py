@@ -300,3 +300,56 @@ def test_bins_from_params_rules():
         bins_from_params(P(bin_lmins=[0], bin_lmaxs=[1]))
     with pytest.raises(ValueError, match="same length"):
         bins_from_params(P(bin_lmins=[2, 3], bin_lmaxs=[4]))
+
+
+def _group_tile_plan(m, n, batch, big, tri):
+    """Host mirror of cfmi_gemm_b3's grouped launch: sizes as set in
+    gemm_b3.hip's entry point, mapping as group_tile."""
+    bt = 256 if big else 128
+    tm, tn = -(-m // bt), -(-n // bt)
+    gm, gn = (4 if big else 8), 8
+    gtm, gtn = -(-tm // gm), -(-tn // gn)
+    gtmp = gtm | 1
+    ng = gtmp * gtn * batch
+    nwg = -(-ng // 8) * 8 * gm * gn
+
+    def tile(x):
+        G = gm * gn
+        slot = x >> 3
+        gg = (slot // G) * 8 + (x & 7)
+        if gg >= ng:
+            return None
+        w, gi, rest = slot % G, gg % gtmp, gg // gtmp
+        if gi >= gtm:
+            return None
+        ti, tj = gi * gm + w % gm, (rest % gtn) * gn + w // gm
+        if ti >= tm or tj >= tn:
+            return None
+        return rest // gtn, (tm - 1 - ti if tri & 1 else ti), (tn - 1 - tj if tri & 4 else tj)
+
+    return tm, tn, nwg, tile
+
+
+@pytest.mark.parametrize("m,n,batch", [(5952, 5952, 46), (3968, 3968, 46), (1984, 5952, 46), (16128, 16128, 3),
+                                       (1, 1, 1), (129, 7000, 5), (1100, 1100, 3), (8064, 257, 7)])
+def test_grouped_tile_order_reaches_every_tile_once(m, n, batch):
+    """The XCD-grouped order is a bijection onto (batch, row tile, column
+    tile) for both tile sizes and trims, and keeps the eight XCDs' shares of
+    the square row-trimmed whitening products (work ~ row index + 1) within
+    6% of each other."""
+    for big in (0, 1):
+        for tri in (0, 1, 4, 5):
+            tm, tn, nwg, tile = _group_tile_plan(m, n, batch, big, tri)
+            seen, load = set(), [0] * 8
+            for x in range(nwg):
+                t = tile(x)
+                if t is None:
+                    continue
+                z, ti, tj = t
+                assert 0 <= z < batch and 0 <= ti < tm and 0 <= tj < tn
+                assert t not in seen
+                seen.add(t)
+                load[x & 7] += ti + 1
+            assert len(seen) == tm * tn * batch
+            if tri & 1 and m == n and m >= 3968:     # the trimmed whitening products
+                assert max(load) / min(load) < 1.06
