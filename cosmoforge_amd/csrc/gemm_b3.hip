@@ -22,6 +22,7 @@
 // i (i + 1) / 2 + c - pbase, diagonal scaled by pdiag).
 #include "cfmi_internal.h"
 
+#include <algorithm>
 #include <cstring>
 
 namespace {
@@ -116,13 +117,13 @@ __device__ __forceinline__ bool tile_order(int64_t b, int64_t tm, int64_t tmp, i
     return true;
 }
 
-// XCD-grouped order.  Tiles are taken in groups of gm x gn (one group fills
-// the CUs of one XCD: 32 CUs x 1 block of 256-tiles, or x 2 blocks of
-// 128-tiles), so the blocks an XCD runs at once share gm row panels of A and
-// gn column panels of B in its L2 instead of bringing their own.  Workgroup x
-// runs on XCD x % 8 (round-robin dispatch); its slot x / 8 on that XCD picks
-// the XCD's (slot / G)-th group, global group gg = 8 (slot / G) + x % 8, and
-// the tile slot % G in it (rows fastest).  Groups are enumerated group rows
+// XCD-grouped order.  Tiles are taken in groups of gm x gn, each group run
+// by one XCD.  Workgroup x runs on XCD x % 8 (round-robin dispatch); its slot
+// x / 8 on that XCD picks the XCD's (slot / G)-th group, global group gg =
+// 8 (slot / G) + x % 8, and the tile slot % G in it (rows fastest).  The
+// blocks an XCD holds at once (32 or 64) span a few consecutive groups of
+// its own: the same gn column panels of B, and group rows 8 apart, so they
+// share the B panels in that XCD's L2.  Groups are enumerated group rows
 // fastest over an odd count gtmp (a padding row exits at once), then group
 // columns, then the batch, so consecutive groups of a trimmed product, whose
 // work varies with the row, rotate over the XCDs.  Every (batch, tile) pair is
@@ -459,8 +460,14 @@ int cfmi_gemm_b3(cfmi_ctx* c, int out_mode, int64_t m, int64_t n, int64_t k, dou
     g.tmp = ((flags & 1) || out_mode == 2) && g.tm % 2 == 0 ? g.tm + 1 : g.tm;
     // flag 1024: the column-by-column order of tile_order, batch over blockIdx.z
     g.grouped = !(flags & 1024);
-    g.gm = big ? 4 : 8;
-    g.gn = 8;
+    // group shape 2 x 8 tiles (both tile sizes); a value v > 0 in bits 12-15
+    // / 16-19 gives 2^(v - 1) group rows / columns instead (1 .. 64).
+    // Measured on the nside-16 TQU whitening: 157.7 ms at 2 x 8, within 1%
+    // of it for 1-2 rows x 8-64 columns, 167.8 ms at 4 x 8 / 4 x 32, 188 ms
+    // at 8 x 8 128-tiles with 4 x 8 256-tiles, 200 ms at 16 x 4.
+    const int lgm = (flags >> 12) & 15, lgn = (flags >> 16) & 15;
+    g.gm = lgm ? (1 << std::min(lgm - 1, 6)) : 2;
+    g.gn = lgn ? (1 << std::min(lgn - 1, 6)) : 8;
     g.gtm = (g.tm + g.gm - 1) / g.gm;
     g.gtn = (g.tn + g.gn - 1) / g.gn;
     g.gtmp = g.gtm | 1;

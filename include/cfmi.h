@@ -197,7 +197,8 @@ int cfmi_gemm_ex(cfmi_ctx *ctx, int dtype, int out_dtype, int transa, int transb
  * zeros; the bits only shorten K); bits 8 / 9 force 128 x 128 / 256 x 256
  * block tiles (default: 256 with at least 512 such tiles for products
  * without a B-trim outside out_mode 2, and for N, K >= 6144); bit 10
- * takes the tiles column by column instead of in XCD-sized groups.
+ * takes the tiles column by column instead of in groups of 2 x 8 per XCD
+ * (bits 12-15 / 16-19: v > 0 gives 2^(v - 1) group rows / columns).
  * out_mode 0: FP32 C[m + n ldc] (+ beta C); 1: three BF16 planes
  * C[m ldc + n] (planes pC apart); 2: FP32 packed lower row band as
  * cfmi_gemm_ex mode 2 (+ beta C).  Replaces the FP32 products of the

@@ -307,7 +307,7 @@ def _group_tile_plan(m, n, batch, big, tri):
     gemm_b3.hip's entry point, mapping as group_tile."""
     bt = 256 if big else 128
     tm, tn = -(-m // bt), -(-n // bt)
-    gm, gn = (4 if big else 8), 8
+    gm, gn = 2, 8
     gtm, gtn = -(-tm // gm), -(-tn // gn)
     gtmp = gtm | 1
     ng = gtmp * gtn * batch
