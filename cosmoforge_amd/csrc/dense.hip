@@ -78,6 +78,10 @@ struct GemmArgs {
     int64_t tm, tn;
     OutMap o;
     int64_t sA = 0, sB = 0, sC = 0;  // per-batch element strides (blockIdx.z)
+    // Batched launches without split-K: tiles in groups of gm x gn per XCD
+    // over a one-dimensional grid (group_tile); 0: tile_of / tri_tile_of.
+    int grouped = 0;
+    int64_t gm = 0, gn = 0, gtm = 0, gtmp = 0, gtn = 0, ngroups = 0;
 };
 
 __device__ __forceinline__ int64_t packed_index(int64_t i, int64_t j, int64_t n)
@@ -135,6 +139,32 @@ __device__ __forceinline__ void tri_tile_of(int64_t b, int64_t tm, int64_t& ti, 
     while (i * (i + 1) / 2 > t) --i;
     ti = i;
     tj = t - i * (i + 1) / 2;
+}
+
+// Batched products: workgroup x runs on XCD x % 8; its slot x / 8 there
+// picks group 8 (slot / G) + x % 8 of gm x gn tiles and tile slot % G in it
+// (rows fastest).  Groups run over an odd number gtmp of group rows (a
+// padding row exits), then group columns, then the batch, so consecutive
+// groups of a trimmed product rotate over the XCDs, and the blocks an XCD
+// holds at once share the column panels of B in its L2.  Every (batch,
+// tile) pair is reached once; trimmed tiles are reversed as in tile_of.
+template <typename G_>
+__device__ __forceinline__ bool group_tile(const G_& g, int64_t x, int64_t& z, int64_t& ti, int64_t& tj)
+{
+    const int64_t G = g.gm * g.gn;
+    const int64_t slot = x >> 3;
+    const int64_t gg = (slot / G) * 8 + (x & 7);
+    if (gg >= g.ngroups) return false;
+    const int64_t w = slot % G;
+    const int64_t gi = gg % g.gtmp, rest = gg / g.gtmp;
+    if (gi >= g.gtm) return false;
+    z = rest / g.gtn;
+    ti = gi * g.gm + w % g.gm;
+    tj = (rest % g.gtn) * g.gn + w / g.gm;
+    if (ti >= g.tm || tj >= g.tn) return false;
+    if (g.tri & 1) ti = g.tm - 1 - ti;
+    if (g.tri & 4) tj = g.tn - 1 - tj;
+    return true;
 }
 
 template <typename T> struct Vec2;
@@ -245,17 +275,22 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
     __shared__ T As[BK][BM + PAD];
     __shared__ T Bs[BK][BN + PAD];
     const int tid = threadIdx.x;
-    int64_t ti, tj;
-    if ((g.tri & 64) && g.tm == g.tn) tri_tile_of(blockIdx.x, g.tm, ti, tj);
-    else tile_of(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
+    int64_t ti, tj, zb = blockIdx.z;
+    if (g.grouped) {
+        if (!group_tile(g, blockIdx.x, zb, ti, tj)) return;
+    } else if ((g.tri & 64) && g.tm == g.tn) {
+        tri_tile_of(blockIdx.x, g.tm, ti, tj);
+    } else {
+        tile_of(blockIdx.x, g.tm, g.tn, g.tri, ti, tj);
+    }
     const int64_t m0 = ti * BM;
     const int64_t n0 = tj * BN;
     const int z = blockIdx.y;
     const int64_t M = g.M, N = g.N, K = g.K;
-    if (blockIdx.z) {
-        g.A += blockIdx.z * g.sA;
-        g.B += blockIdx.z * g.sB;
-        g.C += blockIdx.z * g.sC;
+    if (zb) {
+        g.A += zb * g.sA;
+        g.B += zb * g.sB;
+        g.C += zb * g.sC;
     }
     const bool lower_only = (g.tri & 16) != 0;
     if (lower_only && g.o.col0 + n0 > g.o.row0 + m0 + BM - 1) return;
@@ -440,7 +475,19 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = sh.om;
     g.sA = sh.sA; g.sB = sh.sB; g.sC = sh.sC;
     const bool sym_tiles = (tri & 64) && tm == tn;
+    // tri bit 256: batched tiles in the plain order (tile_of over blockIdx.z)
+    g.grouped = batch > 1 && nz == 1 && !sym_tiles && !(tri & 256);
     dim3 grid((unsigned)(sym_tiles ? tm * (tm + 1) / 2 : tm * tn), (unsigned)nz, (unsigned)batch);
+    if (g.grouped) {
+        g.gm = 2; g.gn = 8;
+        g.gtm = (tm + g.gm - 1) / g.gm;
+        g.gtn = (tn + g.gn - 1) / g.gn;
+        g.gtmp = g.gtm | 1;
+        g.ngroups = g.gtmp * g.gtn * batch;
+        const int64_t nwg = (g.ngroups + 7) / 8 * 8 * g.gm * g.gn;
+        if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm: too many tiles");
+        grid = dim3((unsigned)nwg, 1, 1);
+    }
 #define CFMI_GEMM_LAUNCH(TA_, TB_)                                                                        \
     do {                                                                                                  \
         if constexpr (kAcc64) {                                                                           \
@@ -1002,8 +1049,8 @@ int cfmi_gemm_ex(cfmi_ctx* c, int dtype, int out_dtype, int transa, int transb, 
     sh.om.mode = out_mode; sh.om.row0 = row0; sh.om.col0 = col0; sh.om.pn = out_mode == 1 ? ldc : 0;
     sh.om.pbase = pbase; sh.om.pdiag = pdiag; sh.om.poff = poff;
     sh.batch = batch; sh.sA = sA; sh.sB = sB; sh.sC = sC;
-    int tri = flags & (31 | 64);
-    if (out_mode != 0) tri = (tri & 31) | 16;
+    int tri = (flags & (31 | 64)) | ((flags & 1024) ? 256 : 0);
+    if (out_mode != 0) tri = (tri & (31 | 256)) | 16;
     if (out_mode != 0) ldc = 1;
     const bool acc64 = (flags & CFMI_GEMM_ACC64) != 0;
     if (dtype == CFMI_F64 && out_dtype == CFMI_F64)

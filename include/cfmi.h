@@ -174,7 +174,10 @@ int cfmi_gemm_packed_batched(cfmi_ctx *ctx, int dtype, int transa, int transb, i
  *     bit 2 k > n + off_b, bit 3 k < n + off_b;
  *     CFMI_GEMM_ACC64: FP32 products accumulated by the MFMA in FP32 over
  *     chunks of 256 terms and in FP64 across chunks (always on for
- *     out_dtype F64 with FP32 inputs).
+ *     out_dtype F64 with FP32 inputs);
+ *     bit 10 (1024): batches take their tiles in the plain order over the
+ *     batch grid dimension; by default a batched product (batch > 1) runs
+ *     its tiles in groups of 2 x 8 per XCD over a one-dimensional grid.
  *   out_mode 0: dense C (ldc); 1: packed lower triangle of an ldc x ldc
  *     matrix as cfmi_gemm_packed; 2: packed lower rows, row-major: element
  *     (i, j <= i) at C + i (i + 1) / 2 + j - pbase.  In modes 1 and 2,
