@@ -302,12 +302,12 @@ def test_bins_from_params_rules():
         bins_from_params(P(bin_lmins=[2, 3], bin_lmaxs=[4]))
 
 
-def _group_tile_plan(m, n, batch, big, tri):
-    """Host mirror of cfmi_gemm_b3's grouped launch: sizes as set in
-    gemm_b3.hip's entry point, mapping as group_tile."""
+def _group_tile_plan(m, n, batch, big, tri, gm=2, gn=8):
+    """Host mirror of the grouped launches of cfmi_gemm_b3 (2 x 8 groups)
+    and of batched cfmi_gemm_ex (1 x 16 in FP64): sizes as set in the entry
+    points, mapping as group_tile."""
     bt = 256 if big else 128
     tm, tn = -(-m // bt), -(-n // bt)
-    gm, gn = 2, 8
     gtm, gtn = -(-tm // gm), -(-tn // gn)
     gtmp = gtm | 1
     ng = gtmp * gtn * batch
@@ -332,14 +332,15 @@ def _group_tile_plan(m, n, batch, big, tri):
 
 @pytest.mark.parametrize("m,n,batch", [(5952, 5952, 46), (3968, 3968, 46), (1984, 5952, 46), (16128, 16128, 3),
                                        (1, 1, 1), (129, 7000, 5), (1100, 1100, 3), (8064, 257, 7)])
-def test_grouped_tile_order_reaches_every_tile_once(m, n, batch):
+@pytest.mark.parametrize("gm,gn", [(2, 8), (1, 16)])
+def test_grouped_tile_order_reaches_every_tile_once(m, n, batch, gm, gn):
     """The XCD-grouped order is a bijection onto (batch, row tile, column
     tile) for both tile sizes and trims, and keeps the eight XCDs' shares of
     the square row-trimmed whitening products (work ~ row index + 1) within
     6% of each other."""
     for big in (0, 1):
         for tri in (0, 1, 4, 5):
-            tm, tn, nwg, tile = _group_tile_plan(m, n, batch, big, tri)
+            tm, tn, nwg, tile = _group_tile_plan(m, n, batch, big, tri, gm, gn)
             seen, load = set(), [0] * 8
             for x in range(nwg):
                 t = tile(x)
