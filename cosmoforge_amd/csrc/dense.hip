@@ -480,11 +480,11 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     g.grouped = batch > 1 && nz == 1 && !sym_tiles && !(tri & 256);
     dim3 grid((unsigned)(sym_tiles ? tm * (tm + 1) / 2 : tm * tn), (unsigned)nz, (unsigned)batch);
     if (g.grouped) {
-        // default groups: 1 x 16 tiles in FP64, 2 x 8 otherwise (measured
-        // on the nside-16 whitening: FP64 444 ms at 1 x 16, 446 at 1 x 8,
-        // 458 at 2 x 8, 487 at 4 x 8, 535 at 8 x 8)
-        g.gm = sh.gm > 0 ? sh.gm : (sizeof(T) == 8 ? 1 : 2);
-        g.gn = sh.gn > 0 ? sh.gn : (sizeof(T) == 8 ? 16 : 8);
+        // default groups of 1 x 16 tiles (measured on the nside-16
+        // whitening: FP64 444 ms at 1 x 16, 446 at 1 x 8, 458 at 2 x 8, 487
+        // at 4 x 8, 535 at 8 x 8; FP32 243 at 1 x 16, 244 at 1 x 8, 251 at 2 x 8)
+        g.gm = sh.gm > 0 ? sh.gm : 1;
+        g.gn = sh.gn > 0 ? sh.gn : 16;
         g.gtm = (tm + g.gm - 1) / g.gm;
         g.gtn = (tn + g.gn - 1) / g.gn;
         g.gtmp = g.gtm | 1;

@@ -177,7 +177,7 @@ int cfmi_gemm_packed_batched(cfmi_ctx *ctx, int dtype, int transa, int transb, i
  *     out_dtype F64 with FP32 inputs);
  *     bit 10 (1024): batches take their tiles in the plain order over the
  *     batch grid dimension; by default a batched product (batch > 1) runs
- *     its tiles in groups per XCD (1 x 16 tiles in FP64, 2 x 8 in FP32)
+ *     its tiles in groups of 1 x 16 per XCD
  *     over a one-dimensional grid; bits 12-15 / 16-19: v > 0 gives
  *     2^(v - 1) group rows / columns instead.
  *   out_mode 0: dense C (ldc); 1: packed lower triangle of an ldc x ldc

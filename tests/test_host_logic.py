@@ -304,8 +304,8 @@ def test_bins_from_params_rules():
 
 def _group_tile_plan(m, n, batch, big, tri, gm=2, gn=8):
     """Host mirror of the grouped launches of cfmi_gemm_b3 (2 x 8 groups)
-    and of batched cfmi_gemm_ex (1 x 16 in FP64): sizes as set in the entry
-    points, mapping as group_tile."""
+    and of batched cfmi_gemm_ex (1 x 16): sizes as set in the entry points,
+    mapping as group_tile."""
     bt = 256 if big else 128
     tm, tn = -(-m // bt), -(-n // bt)
     gtm, gtn = -(-tm // gm), -(-tn // gn)
