@@ -53,3 +53,34 @@ void* cfmi_workspace(cfmi_ctx* c, size_t bytes);
 int cfmi_gemm_impl(cfmi_ctx* c, int dtype, int ta, int tb, int64_t M, int64_t N, int64_t K,
                    double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
                    double beta, void* C, int64_t ldc, int tri);
+
+// XCD-grouped tile order of the batched GEMMs (gemm_b3_kernel, and the
+// batched launches of gemm_kernel).  Workgroup x runs on XCD x % 8
+// (round-robin dispatch); its slot x / 8 there picks group 8 (slot / G) +
+// x % 8 of gm x gn tiles and tile slot % G in it (rows fastest).  Groups run
+// over an odd number gtmp of group rows (a padding row exits at once), then
+// group columns, then the batch, so consecutive groups of a trimmed product,
+// whose work varies with the row, rotate over the XCDs, and the blocks an XCD
+// holds at once share the column panels of B in its L2.  Every (batch, tile)
+// pair is reached exactly once; slots beyond the grid of tiles return false.
+// A-trimmed (tri & 1) / B-trimmed (tri & 4) products take rows / columns
+// in reverse, heaviest first.
+// G_ needs gm, gn, gtm, gtmp, gtn, ngroups, tm, tn and tri.
+template <typename G_>
+__device__ __forceinline__ bool group_tile(const G_& g, int64_t x, int64_t& z, int64_t& ti, int64_t& tj)
+{
+    const int64_t G = g.gm * g.gn;
+    const int64_t slot = x >> 3;
+    const int64_t gg = (slot / G) * 8 + (x & 7);
+    if (gg >= g.ngroups) return false;
+    const int64_t w = slot % G;
+    const int64_t gi = gg % g.gtmp, rest = gg / g.gtmp;
+    if (gi >= g.gtm) return false;
+    z = rest / g.gtn;
+    ti = gi * g.gm + w % g.gm;
+    tj = (rest % g.gtn) * g.gn + w / g.gm;
+    if (ti >= g.tm || tj >= g.tn) return false;
+    if (g.tri & 1) ti = g.tm - 1 - ti;
+    if (g.tri & 4) tj = g.tn - 1 - tj;
+    return true;
+}

@@ -141,32 +141,6 @@ __device__ __forceinline__ void tri_tile_of(int64_t b, int64_t tm, int64_t& ti, 
     tj = t - i * (i + 1) / 2;
 }
 
-// Batched products: workgroup x runs on XCD x % 8; its slot x / 8 there
-// picks group 8 (slot / G) + x % 8 of gm x gn tiles and tile slot % G in it
-// (rows fastest).  Groups run over an odd number gtmp of group rows (a
-// padding row exits), then group columns, then the batch, so consecutive
-// groups of a trimmed product rotate over the XCDs, and the blocks an XCD
-// holds at once share the column panels of B in its L2.  Every (batch,
-// tile) pair is reached once; trimmed tiles are reversed as in tile_of.
-template <typename G_>
-__device__ __forceinline__ bool group_tile(const G_& g, int64_t x, int64_t& z, int64_t& ti, int64_t& tj)
-{
-    const int64_t G = g.gm * g.gn;
-    const int64_t slot = x >> 3;
-    const int64_t gg = (slot / G) * 8 + (x & 7);
-    if (gg >= g.ngroups) return false;
-    const int64_t w = slot % G;
-    const int64_t gi = gg % g.gtmp, rest = gg / g.gtmp;
-    if (gi >= g.gtm) return false;
-    z = rest / g.gtn;
-    ti = gi * g.gm + w % g.gm;
-    tj = (rest % g.gtn) * g.gn + w / g.gm;
-    if (ti >= g.tm || tj >= g.tn) return false;
-    if (g.tri & 1) ti = g.tm - 1 - ti;
-    if (g.tri & 4) tj = g.tn - 1 - tj;
-    return true;
-}
-
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };

@@ -117,34 +117,7 @@ __device__ __forceinline__ bool tile_order(int64_t b, int64_t tm, int64_t tmp, i
     return true;
 }
 
-// XCD-grouped order.  Tiles are taken in groups of gm x gn, each group run
-// by one XCD.  Workgroup x runs on XCD x % 8 (round-robin dispatch); its slot
-// x / 8 on that XCD picks the XCD's (slot / G)-th group, global group gg =
-// 8 (slot / G) + x % 8, and the tile slot % G in it (rows fastest).  The
-// blocks an XCD holds at once (32 or 64) span a few consecutive groups of
-// its own: the same gn column panels of B, and group rows 8 apart, so they
-// share the B panels in that XCD's L2.  Groups are enumerated group rows
-// fastest over an odd count gtmp (a padding row exits at once), then group
-// columns, then the batch, so consecutive groups of a trimmed product, whose
-// work varies with the row, rotate over the XCDs.  Every (batch, tile) pair is
-// reached exactly once; slots beyond the grid of tiles exit.
-__device__ __forceinline__ bool group_tile(const B3Args& g, int64_t x, int64_t& z, int64_t& ti, int64_t& tj)
-{
-    const int64_t G = g.gm * g.gn;
-    const int64_t slot = x >> 3;
-    const int64_t gg = (slot / G) * 8 + (x & 7);
-    if (gg >= g.ngroups) return false;
-    const int64_t w = slot % G;
-    const int64_t gi = gg % g.gtmp, rest = gg / g.gtmp;
-    if (gi >= g.gtm) return false;
-    z = rest / g.gtn;
-    ti = gi * g.gm + w % g.gm;
-    tj = (rest % g.gtn) * g.gn + w / g.gm;
-    if (ti >= g.tm || tj >= g.tn) return false;
-    if (g.tri & 1) ti = g.tm - 1 - ti;
-    if (g.tri & 4) tj = g.tn - 1 - tj;
-    return true;
-}
+// The XCD-grouped order (group_tile) is shared with the other GEMMs: cfmi_internal.h.
 
 // 16 bytes (8 bf16 at k .. k + 7 of one row), zero outside rows / [.., kend).
 __device__ __forceinline__ u4 load16_guarded(const bf16* src, bool row_ok, int64_t k, int64_t kend)
