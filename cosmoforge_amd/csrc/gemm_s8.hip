@@ -1,0 +1,503 @@
+// FP64 products on the int8 matrix cores (gfx950 v_mfma_i32_16x16x64_i8),
+// exact integer accumulation, operands held as int8 slices.
+//
+// Slicing.  Row r of an operand (the k-contiguous rows of A, or of B^T) is
+// scaled by mu_r = max_k |x_rk| and split into S signed int8 slices:
+//     t = (x / mu) * 127,   s_1 = rint(t),   t <- (t - s_1) * 254,
+//     s_2 = rint(t),        t <- (t - s_2) * 254,  ...
+// so x = mu (s_1 / 127 + s_2 / (127 254) + ... ) up to |x - sum| <=
+// mu / (2 127 254^(S-1)); every |s_i| <= 127 (the remainders stay in
+// [-1/2, 1/2]).  S = 4 keeps 2^-32 of the row maximum, S = 5 2^-40.
+//
+// Product.  C_mn = mu_m nu_n / 127^2 sum_d 254^-d sum_{p+q=d} <A_p[m], B_q[n]>
+// over the levels d = 0 .. S-1 (S (S+1) / 2 int8 products; the dropped
+// levels d >= S are below the slicing error).  The inner sums are exact
+// int32 MFMA sums: level d has d + 1 products of |.| <= 127^2 per k, so
+// S K 127^2 < 2^31 bounds K per launch (the host splits longer K and adds
+// the chunks in FP64).  The epilogue combines the levels in FP64 (Horner,
+// no multiply-add contraction in this file), scales by the row factors
+// and applies alpha / beta, so the result is a deterministic function of
+// the slices: tests reproduce it bit for bit on the host.
+//
+// Tiles 128 x 128, 8 waves of 32 x 64 (2 x 4 MFMA tiles of 16 x 16, S
+// int32 accumulators each), one 64-deep K-slab of every slice in LDS per
+// step (64-byte rows, 16-byte chunks XOR-swizzled by row), the next slab
+// prefetched into registers during the MFMAs.  Operand lane layout
+// (measured on gfx950): lane l reads A[m = l % 16][k = 16 (l / 16) + j] and
+// B[k = 16 (l / 16) + j][n = l % 16], j = 0..15 (16 bytes); output row
+// 4 (l / 16) + r, column l % 16.
+#include "cfmi_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+// Every floating-point operation below is rounded on its own (no fused
+// multiply-add), so the host restatement can reproduce the results exactly.
+// (The header intrinsics __dmul_rn / __dadd_rn are plain operators compiled
+// with contraction on, so they are not used here.)
+#pragma clang fp contract(off)
+
+namespace {
+
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+
+constexpr int SK = 64;     // k per slab (one MFMA deep)
+constexpr int BT = 128;    // tile edge
+constexpr int NTH = 512;   // 8 waves: 4 along M x 2 along N, 32 x 64 each
+constexpr double kInv254 = 1.0 / 254.0;
+constexpr double kInv127sq = 1.0 / 16129.0;
+
+__device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+// Slices of 16 consecutive values of one row into S 16-byte vectors.
+template <int S>
+__device__ __forceinline__ void slice16(const double (&x)[16], double inv_mu, u4 (&out)[S])
+{
+    unsigned w[S][4];
+#pragma unroll
+    for (int p = 0; p < S; ++p)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[p][e] = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        double t = (x[j] * inv_mu) * 127.0;
+        if (!(fabs(t) <= 127.0)) t = 0.0;              // NaN / Inf rows: the scale carries the NaN
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            const double s = rint(t);
+            const unsigned b = (unsigned)(int)s & 0xffu;
+            w[p][j >> 2] |= b << (8 * (j & 3));
+            t = (t - s) * 254.0;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < S; ++p) out[p] = u4{w[p][0], w[p][1], w[p][2], w[p][3]};
+}
+
+// mu = max |x| over the row (NaN if any entry is NaN); 1 for an all-zero row.
+__device__ __forceinline__ double finish_mu(double m) { return m == 0.0 ? 1.0 : m; }
+__device__ __forceinline__ double absmax(double m, double v)
+{
+    const double a = fabs(v);
+    return (a > m || a != a) ? a : m;
+}
+
+// Layout 0: row r at src + r lds (k contiguous).  One wave per row: the
+// row maximum by a wave reduction, then 16-value chunks, zero beyond k.
+template <int S>
+__global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k, const double* __restrict__ src,
+                                                         int64_t lds, char* __restrict__ dst, int64_t ldk,
+                                                         int64_t ps, double* __restrict__ mu)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const double* row = src + r * lds;
+    double m = 0.0;
+    for (int64_t j = lane; j < k; j += 64) m = absmax(m, row[j]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = absmax(m, __shfl_xor(m, off, 64));
+    m = finish_mu(m);
+    if (lane == 0) mu[r] = m;
+    const double inv = 1.0 / m;
+    for (int64_t c = lane; c * 16 < ldk; c += 64) {
+        double x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t kk = c * 16 + j;
+            x[j] = kk < k ? row[kk] : 0.0;
+        }
+        u4 o[S];
+        slice16<S>(x, inv, o);
+#pragma unroll
+        for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + c * 16) = o[p];
+    }
+}
+
+// Layout 1: element (r, k) at src[r + k lds] (the rows of the operand are
+// the rows of a column-major matrix).  Pass 1: per-row maxima over k chunks
+// (thread per row, coalesced along r), merged with an integer atomicMax on
+// the bit patterns (non-negative doubles, NaN above +Inf, order as
+// integers).  Pass 2: 64 x 64 tiles transposed through LDS.
+__global__ __launch_bounds__(256) void rowmax_cm_kernel(int64_t rows, int64_t k, const double* __restrict__ src,
+                                                        int64_t lds, int64_t kchunk,
+                                                        unsigned long long* __restrict__ bits)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kchunk;
+    const int64_t k1 = k0 + kchunk < k ? k0 + kchunk : k;
+    double m = 0.0;
+    for (int64_t j = k0; j < k1; ++j) m = absmax(m, src[r + j * lds]);
+    unsigned long long b;
+    memcpy(&b, &m, 8);
+    if (b) atomicMax(bits + r, b);
+}
+
+__global__ void zero_kernel(int64_t count, double* p)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+__global__ void finish_mu_kernel(int64_t rows, double* mu)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
+        mu[r] = finish_mu(mu[r]);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void slice_cm_kernel(int64_t rows, int64_t k, const double* __restrict__ src,
+                                                       int64_t lds, char* __restrict__ dst, int64_t ldk, int64_t ps,
+                                                       const double* __restrict__ mu)
+{
+    __shared__ double tile[64][65];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e, rr = idx & 63, kk = idx >> 6;
+        const int64_t r = r0 + rr, kg = k0 + kk;
+        tile[kk][rr] = (r < rows && kg < k) ? src[r + kg * lds] : 0.0;
+    }
+    __syncthreads();
+    const int rr = tid >> 2, c = tid & 3;
+    const int64_t r = r0 + rr;
+    if (r >= rows || k0 + 16 * c >= ldk) return;
+    double x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = tile[16 * c + j][rr];
+    u4 o[S];
+    slice16<S>(x, 1.0 / mu[r], o);
+#pragma unroll
+    for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + k0 + 16 * c) = o[p];
+}
+
+// Slices -> FP64 (the value the slices represent), for tests.
+__global__ void unslice_kernel(int s, int64_t rows, int64_t k, const signed char* __restrict__ src, int64_t ldk,
+                               int64_t ps, const double* __restrict__ mu, double* __restrict__ dst, int64_t ldd)
+{
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < rows * k;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = idx / k, kk = idx % k;
+        double v = (double)src[(s - 1) * ps + r * ldk + kk];
+        for (int p = s - 2; p >= 0; --p) v = (double)src[p * ps + r * ldk + kk] + v * kInv254;
+        dst[r * ldd + kk] = (v * (1.0 / 127.0)) * mu[r];
+    }
+}
+
+struct S8Args {
+    int64_t M, N, K;           // K a multiple of SK, within both operands' row pitch
+    double alpha, beta;
+    const char* A;
+    int64_t lda, psA;
+    const double* muA;
+    const char* B;
+    int64_t ldb, psB;
+    const double* muB;
+    double* C;
+    int64_t ldc;
+    int lower;                 // compute the tiles on or below the diagonal, store m >= n only
+    int tri;                   // group_tile: no trims
+    int64_t tm, tn;
+    int64_t gm, gn, gtm, gtmp, gtn, ngroups;
+};
+
+// Staging: thread t holds 16-byte chunk t % 4 of row t / 4 of every slice
+// of the A and B tiles (one slab).  Named registers (no arrays), as arrays
+// of vectors indexed in unrolled loops have ended up in scratch memory.
+#define CFMI_S8_EACH(X) X(0) X(1) X(2) X(3) X(4)
+template <int S>
+struct Stage8 {
+    u4 a0, a1, a2, a3, a4, b0, b1, b2, b3, b4;
+
+    __device__ __forceinline__ void load(const S8Args& g, int64_t m0, int64_t n0, int64_t k0, int tid)
+    {
+        const int sr = tid >> 2, ck = (tid & 3) * 16;
+        const int64_t ra = m0 + sr < g.M ? sr : g.M - 1 - m0;
+        const int64_t rb = n0 + sr < g.N ? sr : g.N - 1 - n0;
+        const char* pa = g.A + m0 * g.lda + k0;
+        const char* pb = g.B + n0 * g.ldb + k0;
+        const unsigned oa = (unsigned)(ra * g.lda + ck), ob = (unsigned)(rb * g.ldb + ck);
+#define CFMI_S8_LD(p)                                                               \
+        if constexpr (S > p) {                                                      \
+            a##p = *reinterpret_cast<const u4*>(pa + p * g.psA + oa);               \
+            b##p = *reinterpret_cast<const u4*>(pb + p * g.psB + ob);               \
+        }
+        CFMI_S8_EACH(CFMI_S8_LD)
+#undef CFMI_S8_LD
+    }
+    __device__ __forceinline__ void store(char (*As)[BT][SK], char (*Bs)[BT][SK], int tid) const
+    {
+        const int row = tid >> 2, kc = ((tid & 3) ^ swz(row)) * 16;
+#define CFMI_S8_ST(p)                                                               \
+        if constexpr (S > p) {                                                      \
+            *reinterpret_cast<u4*>(&As[p][row][kc]) = a##p;                         \
+            *reinterpret_cast<u4*>(&Bs[p][row][kc]) = b##p;                         \
+        }
+        CFMI_S8_EACH(CFMI_S8_ST)
+#undef CFMI_S8_ST
+    }
+};
+#undef CFMI_S8_EACH
+
+template <int S>
+__device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (*Bs)[BT][SK], i4 (&acc)[S][2][4],
+                                          int wm, int wn, int lm, int lk)
+{
+    const int kk = 16 * (lk ^ swz(lm));   // wm, wn, 16 i, 16 j are multiples of 16
+    i4 a[2][S];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int p = 0; p < S; ++p) a[i][p] = *reinterpret_cast<const i4*>(&As[p][wm + 16 * i + lm][kk]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        i4 b[S];
+#pragma unroll
+        for (int p = 0; p < S; ++p) b[p] = *reinterpret_cast<const i4*>(&Bs[p][wn + 16 * j + lm][kk]);
+#pragma unroll
+        for (int d = 0; d < S; ++d)
+#pragma unroll
+            for (int p = 0; p <= d; ++p)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    acc[d][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][p], b[d - p], acc[d][i][j], 0, 0, 0);
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
+{
+    __shared__ __attribute__((aligned(16))) char As[S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char Bs[S][BT][SK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t z, ti, tj;
+    if (!group_tile(g, blockIdx.x, z, ti, tj)) return;
+    const int64_t m0 = ti * BT, n0 = tj * BT;
+    if (g.lower && n0 > m0 + BT - 1) return;
+    const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
+    const int lm = lane & 15, lk = lane >> 4;
+
+    i4 acc[S][2][4];
+#pragma unroll
+    for (int d = 0; d < S; ++d)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
+
+    Stage8<S> st;
+    st.load(g, m0, n0, 0, tid);
+    int64_t k0 = 0;
+    for (; k0 + SK < g.K; k0 += SK) {
+        __syncthreads();
+        st.store(As, Bs, tid);
+        __syncthreads();
+        st.load(g, m0, n0, k0 + SK, tid);
+        slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
+    }
+    __syncthreads();
+    st.store(As, Bs, tid);
+    __syncthreads();
+    slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
+
+    double sa[2][4], sb[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t gm = m0 + wm + 16 * i + 4 * lk + r;
+            sa[i][r] = gm < g.M ? g.alpha * g.muA[gm] : 0.0;
+        }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t gn = n0 + wn + 16 * j + lm;
+        sb[j] = gn < g.N ? g.muB[gn] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gm = m0 + wm + 16 * i + 4 * lk + r;
+                const int64_t gn = n0 + wn + 16 * j + lm;
+                if (gm >= g.M || gn >= g.N || (g.lower && gn > gm)) continue;
+                double v = (double)acc[S - 1][i][j][r];
+#pragma unroll
+                for (int d = S - 2; d >= 0; --d) v = (double)acc[d][i][j][r] + v * kInv254;
+                v = (v * kInv127sq) * (sa[i][r] * sb[j]);
+                double* p = g.C + gm + gn * g.ldc;
+                *p = g.beta == 0.0 ? v : v + g.beta * *p;
+            }
+}
+
+// C(n, m) = C(m, n) for m > n, 64 x 64 tiles through LDS (both sides coalesced).
+__global__ __launch_bounds__(256) void mirror_lower_s8_kernel(int64_t n, double* C, int64_t ldc)
+{
+    __shared__ double t[64][65];
+    const int64_t bi = blockIdx.x, bj = blockIdx.y;   // source tile (rows bi, columns bj), bi >= bj
+    if (bj > bi) return;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
+        const int64_t r = bi * 64 + rr, c = bj * 64 + cc;
+        t[cc][rr] = (r < n && c < n) ? C[r + c * ldc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
+        // destination (row bj*64 + rr, column bi*64 + cc) = source (bi*64 + cc, bj*64 + rr)
+        const int64_t r = bj * 64 + rr, c = bi * 64 + cc;
+        if (r < n && c < n && c > r) C[r + c * ldc] = t[rr][cc];
+    }
+}
+
+inline dim3 grid_of(int64_t tot)
+{
+    int64_t b = (tot + 255) / 256;
+    return dim3((unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536));
+}
+
+inline int64_t round64(int64_t k) { return (k + SK - 1) / SK * SK; }
+
+template <int S>
+int slice_t(cfmi_ctx* c, int layout, int64_t rows, int64_t k, const double* src, int64_t lds, char* dst,
+            int64_t ldk, int64_t ps, double* mu)
+{
+    if (layout == 0) {
+        hipLaunchKernelGGL((slice_rows_kernel<S>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, rows,
+                           k, src, lds, dst, ldk, ps, mu);
+        CFMI_LAUNCH_CHECK(c);
+        return CFMI_OK;
+    }
+    hipLaunchKernelGGL(zero_kernel, grid_of(rows), dim3(256), 0, c->stream, rows, mu);
+    CFMI_LAUNCH_CHECK(c);
+    const int64_t kchunk = 512;
+    const int64_t nkc = (k + kchunk - 1) / kchunk;
+    if (nkc > 0) {
+        hipLaunchKernelGGL(rowmax_cm_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nkc), dim3(256), 0,
+                           c->stream, rows, k, src, lds, kchunk, reinterpret_cast<unsigned long long*>(mu));
+        CFMI_LAUNCH_CHECK(c);
+    }
+    hipLaunchKernelGGL(finish_mu_kernel, grid_of(rows), dim3(256), 0, c->stream, rows, mu);
+    CFMI_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL((slice_cm_kernel<S>), dim3((unsigned)((rows + 63) / 64), (unsigned)(ldk / 64)), dim3(256), 0,
+                       c->stream, rows, k, src, lds, dst, ldk, ps, mu);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+template <int S>
+int gemm_s8_launch(cfmi_ctx* c, S8Args g)
+{
+    g.tm = (g.M + BT - 1) / BT;
+    g.tn = (g.N + BT - 1) / BT;
+    g.tri = 0;
+    g.gm = 2;
+    g.gn = 8;
+    g.gtm = (g.tm + g.gm - 1) / g.gm;
+    g.gtn = (g.tn + g.gn - 1) / g.gn;
+    g.gtmp = g.gtm | 1;
+    g.ngroups = g.gtmp * g.gtn;
+    const int64_t nwg = (g.ngroups + 7) / 8 * 8 * g.gm * g.gn;
+    if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: too many tiles");
+    hipLaunchKernelGGL((gemm_s8_kernel<S>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int64_t cfmi_s8_kmax(int s)
+{
+    if (s < 1 || s > 8) return 0;
+    return ((int64_t)0x7fffffff / (16129LL * s)) / SK * SK;
+}
+
+int cfmi_slice8(cfmi_ctx* c, int s, int layout, int64_t rows, int64_t k, const double* src, int64_t lds, void* dst,
+                int64_t ldk, int64_t ps, double* mu)
+{
+    if (!c) return CFMI_EINVAL;
+    if (rows < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "slice8: negative size");
+    if (rows == 0) return CFMI_OK;
+    if (s < 3 || s > 5) return cfmi_fail(c, CFMI_EINVAL, "slice8: %d slices (3 to 5 are built)", s);
+    if (layout != 0 && layout != 1) return cfmi_fail(c, CFMI_EINVAL, "slice8: layout must be 0 or 1");
+    if (ldk < round64(k) || ldk % SK != 0)
+        return cfmi_fail(c, CFMI_EINVAL, "slice8: row pitch %lld must be a multiple of 64 and >= %lld",
+                         (long long)ldk, (long long)round64(k));
+    if (ps < rows * ldk || ps % 16 != 0) return cfmi_fail(c, CFMI_EINVAL, "slice8: slice planes overlap");
+    if (!aligned16(dst)) return cfmi_fail(c, CFMI_EINVAL, "slice8: destination not 16-byte aligned");
+    if ((layout == 0 && lds < k) || (layout == 1 && lds < rows))
+        return cfmi_fail(c, CFMI_EINVAL, "slice8: source leading dimension too small");
+    char* d = (char*)dst;
+    switch (s) {
+        case 3: return slice_t<3>(c, layout, rows, k, src, lds, d, ldk, ps, mu);
+        case 4: return slice_t<4>(c, layout, rows, k, src, lds, d, ldk, ps, mu);
+        default: return slice_t<5>(c, layout, rows, k, src, lds, d, ldk, ps, mu);
+    }
+}
+
+int cfmi_unslice8(cfmi_ctx* c, int s, int64_t rows, int64_t k, const void* src, int64_t ldk, int64_t ps,
+                  const double* mu, double* dst, int64_t ldd)
+{
+    if (!c) return CFMI_EINVAL;
+    if (rows <= 0 || k <= 0) return CFMI_OK;
+    if (s < 1 || s > 8 || ldk < k || ldd < k) return cfmi_fail(c, CFMI_EINVAL, "unslice8: bad arguments");
+    hipLaunchKernelGGL(unslice_kernel, grid_of(rows * k), dim3(256), 0, c->stream, s, rows, k,
+                       (const signed char*)src, ldk, ps, mu, dst, ldd);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alpha, const void* A, int64_t lda,
+                 int64_t psA, const double* muA, const void* B, int64_t ldb, int64_t psB, const double* muB,
+                 double beta, double* C, int64_t ldc, int flags)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m < 0 || n < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: negative size");
+    if (m == 0 || n == 0) return CFMI_OK;
+    if (s < 3 || s > 5) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: %d slices (3 to 5 are built)", s);
+    const int64_t kp = round64(k);
+    if (lda < kp || ldb < kp || lda % 16 || ldb % 16)
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: slice row pitch must be a multiple of 16 and >= %lld",
+                         (long long)kp);
+    if (psA < m * lda || psB < n * ldb || psA % 16 || psB % 16)
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: slice planes overlap or are misaligned");
+    if (!aligned16(A) || !aligned16(B)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: operands not 16-byte aligned");
+    if (ldc < m) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: ldc < m");
+    const bool lower = flags & 1;
+    if (lower && m != n) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
+    if (k == 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: k must be positive");
+    // K chunks within the exact int32 range, added in FP64 (beta = 1 after the first)
+    const int64_t kmax = cfmi_s8_kmax(s);
+    double b = beta;
+    for (int64_t k0 = 0; k0 < kp; k0 += kmax) {
+        const int64_t kk = std::min<int64_t>(kmax, kp - k0);
+        S8Args g = {};
+        g.M = m; g.N = n; g.K = kk;
+        g.alpha = alpha; g.beta = b;
+        g.A = (const char*)A + k0; g.lda = lda; g.psA = psA; g.muA = muA;
+        g.B = (const char*)B + k0; g.ldb = ldb; g.psB = psB; g.muB = muB;
+        g.C = C; g.ldc = ldc; g.lower = lower ? 1 : 0;
+        int rc = s == 3 ? gemm_s8_launch<3>(c, g) : s == 4 ? gemm_s8_launch<4>(c, g) : gemm_s8_launch<5>(c, g);
+        if (rc) return rc;
+        b = 1.0;
+    }
+    if (lower) {
+        const unsigned t = (unsigned)((n + 63) / 64);
+        hipLaunchKernelGGL(mirror_lower_s8_kernel, dim3(t, t), dim3(256), 0, c->stream, n, C, ldc);
+        CFMI_LAUNCH_CHECK(c);
+    }
+    return CFMI_OK;
+}
+
+}  // extern "C"

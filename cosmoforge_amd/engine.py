@@ -357,13 +357,19 @@ class Factor:
     W: torch.Tensor      # whitener U D with C^-1 = W^T W (lower, col-major)
     d: torch.Tensor      # Jacobi scaling 1/sqrt|C_ii|
     logdet: float        # ln|C|
+    U: torch.Tensor | None = None   # inverses of L's diagonal nb-blocks (when W is not formed)
+    nb: int = 0
 
 
-def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True) -> Factor:
-    """Jacobi-scaled Cholesky of C (overwritten by L) and the whitener."""
+def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, diag_blocks: int = 0) -> Factor:
+    """Jacobi-scaled Cholesky of C (overwritten by L) and the whitener.
+
+    ``diag_blocks`` = nb > 0 (instead of the whitener): ``Factor.U`` holds
+    the inverses of L's nb x nb diagonal blocks at their places (the rest of
+    U is unused), the diagonal solves of the blocked ``trsm``."""
     d = torch.empty(n, dtype=torch.float64, device=ctx.device)
     ctx.check(ctx.lib.cfmi_jacobi_scale(ctx.h, n, P(C), n, P(d)), "jacobi_scale")
-    U = ctx.empty(n, n) if need_whitener else None
+    U = ctx.empty(n, n) if (need_whitener or diag_blocks) else None
     info = ctypes.c_int(0)
     rc = ctx.lib.cfmi_potrf(ctx.h, F64, n, P(C), n, P(U), n, ctypes.byref(info))
     ctx.check(rc, "potrf")
@@ -374,7 +380,18 @@ def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True) ->
     if need_whitener:
         ctx.check(ctx.lib.cfmi_trtri(ctx.h, F64, n, P(C), n, P(U), n), "trtri")
         ctx.check(ctx.lib.cfmi_scale(ctx.h, 0, n, n, P(U), n, P(d)), "scale")
-    return Factor(C, U, d, float(ld.value))
+        return Factor(C, U, d, float(ld.value))
+    if diag_blocks:
+        if diag_blocks % 64:
+            raise ValueError("diagonal blocks must be a multiple of 64 rows")
+        for j0 in range(0, n, diag_blocks):
+            w = min(diag_blocks, n - j0)
+            o = j0 + j0 * n
+            ctx.check(ctx.lib.cfmi_trtri(ctx.h, F64, w, P(C, o), n, P(U, o), n), "trtri")
+        f = Factor(C, None, d, float(ld.value))
+        f.U, f.nb = U, diag_blocks
+        return f
+    return Factor(C, None, d, float(ld.value))
 
 
 def gemm(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int,
@@ -651,3 +668,96 @@ def gram(ctx: Context, m: int, nn: int, K: int, X: torch.Tensor, Y: torch.Tensor
     sym = 64 if (X.data_ptr() == Y.data_ptr() and m == nn) else 0
     gemm_ex(ctx, True, False, m, nn, K, 1.0, X, 0, K, Y, 0, K, beta, G, 0, m, flags=sym | _lib.CFMI_GEMM_ACC64)
     return G if out is not None else to_numpy_cm(G)
+
+
+# ---------------------------------------------------------------------------
+# FP64 products on the int8 matrix cores: sliced operands (cfmi_gemm_s8)
+# ---------------------------------------------------------------------------
+
+class Slices:
+    """int8 slices of up to ``rows`` operand rows over ``k`` values:
+    ``data`` (s, rows, ldk) int8 (k contiguous, ldk a multiple of 64),
+    ``mu`` (rows,) FP64 row scales.  A buffer can be refilled with fewer
+    rows; the plane stride stays rows * ldk."""
+
+    def __init__(self, ctx: Context, s: int, rows: int, k: int):
+        self.s, self.rows, self.k = int(s), int(rows), int(k)
+        self.ldk = (self.k + 63) // 64 * 64
+        self.data = torch.empty((self.s, max(self.rows, 1), self.ldk), dtype=torch.int8, device=ctx.device)
+        self.mu = torch.empty(max(self.rows, 1), dtype=torch.float64, device=ctx.device)
+
+    @property
+    def ps(self) -> int:
+        return self.data.shape[1] * self.ldk
+
+
+def slice8(ctx: Context, s: int, layout: int, rows: int, k: int, src: torch.Tensor, off: int, lds: int,
+           out: Slices | None = None) -> Slices:
+    """Slice ``rows`` operand rows of ``k`` FP64 values: layout 0, row r at
+    src[off + r lds] (k contiguous); layout 1, element (r, k) at
+    src[off + r + k lds]."""
+    if out is None:
+        out = Slices(ctx, s, rows, k)
+    if out.s != s or rows > out.data.shape[1] or k > out.ldk:
+        raise ValueError("slice buffer too small")
+    ctx.check(ctx.lib.cfmi_slice8(ctx.h, s, layout, rows, k, P(src, off), lds, P(out.data), out.ldk, out.ps,
+                                  P(out.mu)), "slice8")
+    out.k = k
+    return out
+
+
+def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row0: int, B: Slices, b_row0: int,
+            beta: float, C: torch.Tensor, offc: int, ldc: int, flags: int = 0):
+    """C[m x n] (column-major FP64, at offc) = alpha A B^T + beta C over rows
+    a_row0.. of A and b_row0.. of B (``cfmi_gemm_s8``)."""
+    if m <= 0 or n <= 0:
+        return
+    if A.s != B.s:
+        raise ValueError("operands sliced with different slice counts")
+    ctx.check(ctx.lib.cfmi_gemm_s8(ctx.h, A.s, m, n, k, alpha, P(A.data, a_row0 * A.ldk), A.ldk, A.ps,
+                                   P(A.mu, a_row0), P(B.data, b_row0 * B.ldk), B.ldk, B.ps, P(B.mu, b_row0), beta,
+                                   P(C, offc), ldc, flags), "gemm_s8")
+
+
+def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs: int, transpose: bool = False,
+         slices: int = 0, bufs=None):
+    """X = L^-1 B (or L^-T B with ``transpose``) for the factor's L, n x nrhs
+    column-major with leading dimension n; B is consumed (it holds the
+    running right-hand side).  Blocked over L's nb-blocks: the diagonal
+    solve is an FP64 product with the block inverse (``factor(...,
+    diag_blocks=nb)``), the update of the remaining rows a GEMM, FP64 or on
+    the int8 matrix cores with ``slices`` = s (both operands sliced per
+    block: the rows of L's panel and the columns of the solved block)."""
+    nb = f.nb
+    if not nb or f.U is None:
+        raise ValueError("trsm needs the factor's diagonal-block inverses (factor(..., diag_blocks=nb))")
+    L, U = f.L, f.U
+    blocks = [(j0, min(n, j0 + nb)) for j0 in range(0, n, nb)]
+    if transpose:
+        blocks = blocks[::-1]
+    if slices and bufs is None:
+        bufs = (Slices(ctx, slices, n, nb), Slices(ctx, slices, nrhs, nb))
+    for j0, j1 in blocks:
+        w = j1 - j0
+        o = j0 + j0 * n
+        gemm(ctx, transpose, False, w, nrhs, w, 1.0, U, o, n, B, j0, n, 0.0, X, j0, n, 2 if transpose else 1)
+        if not transpose:
+            m, lo = n - j1, j1
+        else:
+            m, lo = j0, 0
+        if m == 0:
+            continue
+        if slices:
+            As, Bs = bufs
+            # A rows: L[j1:, j0:j1] rows (layout 1), or columns 0..j0-1 of L[j0:j1, :] (layout 0)
+            if not transpose:
+                slice8(ctx, slices, 1, m, w, L, j1 + j0 * n, n, As)
+            else:
+                slice8(ctx, slices, 0, m, w, L, j0, n, As)
+            slice8(ctx, slices, 0, nrhs, w, X, j0, n, Bs)
+            gemm_s8(ctx, m, nrhs, w, -1.0, As, 0, Bs, 0, 1.0, B, lo, n)
+        elif not transpose:
+            gemm(ctx, False, False, m, nrhs, w, -1.0, L, j1 + j0 * n, n, X, j0, n, 1.0, B, lo, n)
+        else:
+            gemm(ctx, True, False, m, nrhs, w, -1.0, L, j0, n, X, j0, n, 1.0, B, lo, n)
+    return X

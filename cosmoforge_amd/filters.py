@@ -26,9 +26,15 @@ import torch
 from . import engine
 from .engine import P
 
+# Cuts on singular values are relative to the largest one.  Below sqrt(eps)
+# the restricted covariance, which carries sigma^2, loses every digit.
 MIN_RANGE_EPSILON = float(np.sqrt(np.finfo(np.float64).eps))
-GAP_WARN_RATIO = 0.1          # sigma_{k+1}/sigma_k above this is "no spectral gap"
-_PROJECTOR_ATOL = 1e-10
+# A cut is "clean" when the first dropped singular value is at most this
+# fraction of the last kept one.
+GAP_WARN_RATIO = 0.1
+# Tolerance on |sigma - 1| and on the agreement of the two singular bases
+# below which a factored operator counts as an orthogonal projector.
+PROJECTOR_TOL = 1e-8
 
 
 def active_pixel_index(mask) -> np.ndarray:
@@ -46,169 +52,184 @@ def mask_fingerprint(mask) -> bytes:
     return blake2b(active_pixel_index(mask).tobytes(), digest_size=16).digest()
 
 
-def _complement(A: np.ndarray, epsilon: float):
-    """(kept, removed): orthonormal bases of the complement of range(A) and of
-    its directions above the relative cut ``epsilon`` (``filters.py:63-79``)."""
-    U, s, _ = np.linalg.svd(A, full_matrices=True)
-    nrem = int(np.count_nonzero(s > epsilon * s[0])) if s.size else 0
-    if nrem >= A.shape[0]:
-        raise ValueError(f"the {A.shape[1]} columns handed in span all {A.shape[0]} pixels above "
-                         f"epsilon={epsilon:g}, leaving a rank-0 filter; the geometry cannot support this "
-                         "many independent templates")
-    return np.ascontiguousarray(U[:, nrem:]), np.ascontiguousarray(U[:, :nrem])
+def _fingerprint_or_none(mask):
+    return None if mask is None else mask_fingerprint(mask)
+
+
+def _split_space(A: np.ndarray, rel_cut: float):
+    """Orthonormal bases (span, complement) of the column space of A: the left
+    singular vectors whose singular value exceeds ``rel_cut`` times the
+    largest, and the remaining ones of the full pixel space."""
+    n = A.shape[0]
+    left, sv, _ = np.linalg.svd(A, full_matrices=True)
+    keep = 0 if sv.size == 0 or sv[0] == 0.0 else int(np.sum(sv > rel_cut * sv[0]))
+    return np.ascontiguousarray(left[:, :keep]), np.ascontiguousarray(left[:, keep:n])
+
+
+def _pick_rank(sv: np.ndarray, range_epsilon, range_rank):
+    """(rank, effective epsilon) of a truncated SVD; raises ValueError on an
+    unusable knob.  ``range_epsilon`` keeps sigma_i > eps sigma_1 (eps clipped
+    up to MIN_RANGE_EPSILON with a warning), ``range_rank`` keeps the leading
+    singular values, which must stay above MIN_RANGE_EPSILON sigma_1."""
+    if (range_epsilon is None) == (range_rank is None):
+        raise ValueError("from_operator needs exactly one truncation knob: range_epsilon or range_rank "
+                         "(both were %s)" % ("missing" if range_epsilon is None else "given"))
+    top = sv[0] if sv.size else 0.0
+    if range_rank is not None:
+        k = int(range_rank)
+        if k < 1 or k > sv.size:
+            raise ValueError(f"range_rank={k} is not in 1..{sv.size}")
+        if sv[k - 1] <= MIN_RANGE_EPSILON * top:
+            raise ValueError(f"range_rank={k} reaches sigma_k / sigma_1 = {sv[k - 1] / top:.3g}, not above "
+                             f"MIN_RANGE_EPSILON = {MIN_RANGE_EPSILON:.3g}")
+        return k, None
+    eps = float(range_epsilon)
+    if eps < MIN_RANGE_EPSILON:
+        warnings.warn(f"range_epsilon={eps:g} raised (clipped) to MIN_RANGE_EPSILON = {MIN_RANGE_EPSILON:.3g}",
+                      stacklevel=3)
+        eps = MIN_RANGE_EPSILON
+    k = int(np.sum(sv > eps * top))
+    if k == 0:
+        raise ValueError(f"range_epsilon={eps:g} keeps no singular value")
+    return k, eps
+
+
+def _gap_warning(sv: np.ndarray, k: int) -> None:
+    """Warn when the cut at rank k is not at a spectral gap, or when nothing
+    was cut (an invertible filter leaves the estimator unchanged)."""
+    if k == sv.size:
+        warnings.warn(f"every one of the {k} singular values was kept: the filter is invertible "
+                      f"(restricted condition number {(sv[0] / sv[-1]) ** 2:.3g})", stacklevel=3)
+        return
+    ratio = sv[k] / sv[k - 1]
+    if ratio > GAP_WARN_RATIO:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            steps = sv[1:] / sv[:-1]
+        best = int(np.nanargmin(steps)) + 1
+        warnings.warn(f"rank-{k} truncation is not at a spectral gap (sigma_{k + 1} / sigma_{k} = {ratio:.3g}); "
+                      f"the sharpest drop is after rank {best} ({steps[best - 1]:.3g})", stacklevel=3)
 
 
 class Filter:
-    """A pixel-space filter as its truncated thin SVD ``F = U Sigma W^T``
-    (``filters.py:82-170``).  ``W``, ``U``: (n_pixels, rank) orthonormal;
-    ``sigma``: kept singular values; a projector has ``U is W``."""
+    """A pixel filter F = U diag(sigma) W^T, truncated at rank r (the public
+    record of ``filters.py:82-170``): ``W`` and ``U`` are (n_pixels, r) with
+    orthonormal columns, ``sigma`` the kept singular values.  For an
+    orthogonal projector U and W are one array and sigma is all ones;
+    ``Q_removed`` spans the removed directions when they were formed."""
 
-    __slots__ = ("W", "U", "sigma", "rank", "is_projector", "range_epsilon", "range_rank", "Q_removed",
-                 "fingerprint", "n_pixels")
+    _FIELDS = ("W", "U", "sigma", "rank", "is_projector", "range_epsilon", "range_rank", "Q_removed",
+               "fingerprint", "n_pixels")
+    __slots__ = _FIELDS
 
     def __init__(self, W, U, sigma, rank, is_projector, range_epsilon, range_rank, Q_removed, fingerprint,
                  n_pixels):
-        if W.ndim != 2:
-            raise ValueError(f"W must be 2-D, got shape {W.shape}")
-        if W.shape != (n_pixels, rank):
-            raise ValueError(f"W has shape {W.shape}, expected ({n_pixels}, {rank})")
-        if U.shape != W.shape:
-            raise ValueError(f"U has shape {U.shape}, expected {W.shape} (same as W)")
-        if sigma.shape != (rank,):
-            raise ValueError(f"sigma has shape {sigma.shape}, expected ({rank},)")
+        shape = (int(n_pixels), int(rank))
+        if np.ndim(W) != 2 or W.shape != shape:
+            raise ValueError(f"W must have shape {shape}; got {np.shape(W)}")
+        if np.shape(U) != shape:
+            raise ValueError(f"U must have shape {shape} like W; got {np.shape(U)}")
+        if np.shape(sigma) != (shape[1],):
+            raise ValueError(f"sigma must have shape ({shape[1]},); got {np.shape(sigma)}")
         if is_projector and U is not W:
-            raise ValueError("a projector's U must be the same object as W, so that raw and pre-filtered "
-                             "inputs land in identical coordinates")
-        for k, v in zip(self.__slots__, (W, U, sigma, rank, is_projector, range_epsilon, range_rank, Q_removed,
-                                         fingerprint, n_pixels)):
-            object.__setattr__(self, k, v)
+            raise ValueError("a projector keeps one basis: pass the same array as U and W")
+        for name, value in zip(self._FIELDS, (W, U, sigma, int(rank), bool(is_projector), range_epsilon,
+                                              range_rank, Q_removed, fingerprint, int(n_pixels))):
+            object.__setattr__(self, name, value)
 
     def __setattr__(self, key, value):
-        raise AttributeError("Filter is immutable")
+        raise AttributeError(f"cannot set {key!r}: Filter is immutable")
 
     def __repr__(self):
-        kind = "projector" if self.is_projector else "graded"
-        knob = (f"range_rank={self.range_rank}" if self.range_rank is not None
-                else f"range_epsilon={self.range_epsilon:g}")
-        return f"Filter({kind}, n_pixels={self.n_pixels}, rank={self.rank}, {knob})"
+        cut = f"range_epsilon={self.range_epsilon:g}" if self.range_epsilon is not None else \
+            f"range_rank={self.range_rank}"
+        return (f"Filter({'projector' if self.is_projector else 'graded'}, n_pixels={self.n_pixels}, "
+                f"rank={self.rank}, {cut})")
 
     @property
     def scaling(self):
-        """``sigma``, or None for a projector."""
+        """The kept singular values of a graded filter; None for a projector."""
         return None if self.is_projector else self.sigma
 
     @classmethod
+    def _projector(cls, basis, removed, range_rank, mask):
+        """The orthogonal projector onto span(basis)."""
+        r = basis.shape[1]
+        return cls(basis, basis, np.ones(r), r, True, None, range_rank, removed, _fingerprint_or_none(mask),
+                   basis.shape[0])
+
+    @classmethod
     def from_operator(cls, F, *, range_epsilon=None, range_rank=None, mask=None) -> "Filter":
-        """Thin SVD of a dense (n, n) operator, truncated by ``range_epsilon``
-        (sigma_i > eps sigma_1, clipped to MIN_RANGE_EPSILON) or ``range_rank``
-        (``filters.py:209-311``)."""
-        if range_epsilon is None and range_rank is None:
-            raise ValueError("a rank threshold is required: pass range_epsilon or range_rank. There is no "
-                             "default because the right cut depends on the operator's spectrum, and the "
-                             "restricted covariance carries Σ²")
-        if range_epsilon is not None and range_rank is not None:
-            raise ValueError("range_epsilon and range_rank are mutually exclusive; pass one")
+        """Factor a dense (n, n) operator by its SVD and truncate it
+        (``filters.py:209-311``).  An operator whose kept part is an
+        orthogonal projector (unit singular values, equal left and right
+        bases) is stored as one, with its removed directions."""
         F = np.asarray(F, dtype=np.float64)
         if F.ndim != 2 or F.shape[0] != F.shape[1]:
-            raise ValueError(f"F must be square (n_pixels, n_pixels), got shape {F.shape}")
-        n = F.shape[0]
-        Uf, s, Wt = np.linalg.svd(F)
-        eff = None
-        if range_rank is not None:
-            k = int(range_rank)
-            if not 1 <= k <= s.size:
-                raise ValueError(f"range_rank={k} outside 1..{s.size} for this operator")
-            if s[k - 1] <= MIN_RANGE_EPSILON * s[0]:
-                raise ValueError(f"range_rank={k} keeps σ_{k}/σ₁ = {s[k - 1] / s[0]:.3g}, at or below "
-                                 f"MIN_RANGE_EPSILON={MIN_RANGE_EPSILON:g}: the restricted covariance would be "
-                                 "singular. Lower range_rank")
-        else:
-            eps = float(range_epsilon)
-            if eps < MIN_RANGE_EPSILON:
-                warnings.warn(f"range_epsilon={eps:g} is below MIN_RANGE_EPSILON={MIN_RANGE_EPSILON:g} and has "
-                              "been clipped: the restricted covariance carries Σ², so a smaller cut puts it "
-                              "beyond double precision", stacklevel=2)
-                eps = MIN_RANGE_EPSILON
-            eff = eps
-            k = int(np.count_nonzero(s > eps * s[0]))
-            if k == 0:
-                raise ValueError(f"no singular value of F exceeds {eps:g} × σ₁; the filter would have rank 0")
-        _warn_on_truncation(s, k)
-        sigma = np.ascontiguousarray(s[:k])
-        W = np.ascontiguousarray(Wt[:k].T)
-        proj = bool(np.allclose(sigma, 1.0, rtol=0.0, atol=_PROJECTOR_ATOL)
-                    and np.allclose(Uf[:, :k], W, rtol=0.0, atol=1e-8))
-        if proj:
-            U, Qr = W, np.ascontiguousarray(Wt[k:].T)
-        else:
-            U, Qr = np.ascontiguousarray(Uf[:, :k]), None
-        return cls(W, U, sigma, k, proj, eff, None if range_rank is None else int(range_rank), Qr,
-                   None if mask is None else mask_fingerprint(mask), n)
+            raise ValueError(f"F must be a square (n_pixels, n_pixels) array; got shape {F.shape}")
+        left, sv, right_t = np.linalg.svd(F)
+        k, eff = _pick_rank(sv, range_epsilon, range_rank)
+        _gap_warning(sv, k)
+        W = np.ascontiguousarray(right_t[:k].T)
+        U = np.ascontiguousarray(left[:, :k])
+        rr = None if range_rank is None else int(range_rank)
+        fp = _fingerprint_or_none(mask)
+        # projector: F restricted to its kept range acts as the identity there
+        unit = np.max(np.abs(sv[:k] - 1.0)) <= PROJECTOR_TOL
+        if unit and np.max(np.abs(U.T @ W - np.eye(k))) <= PROJECTOR_TOL:
+            return cls(W, W, np.ones(k), k, True, eff, rr, np.ascontiguousarray(right_t[k:].T), fp, F.shape[0])
+        return cls(W, U, np.ascontiguousarray(sv[:k]), k, False, eff, rr, None, fp, F.shape[0])
 
     @classmethod
     def from_deprojection(cls, T, *, template_epsilon=1e-3, mask=None) -> "Filter":
-        """The projector annihilating the template stack T (n, n_templates)
-        (``filters.py:313-356``)."""
+        """The orthogonal projector that removes the span of the template
+        columns T (n, n_templates), counting only directions above
+        ``template_epsilon`` times the strongest (``filters.py:313-356``)."""
         T = np.asarray(T, dtype=np.float64)
         if T.ndim != 2:
-            raise ValueError(f"T must be 2-D (n_pixels, n_templates), got {T.shape}")
-        W, Qr = _complement(T, float(template_epsilon))
-        return cls(W, W, np.ones(W.shape[1]), W.shape[1], True, None, W.shape[1], Qr,
-                   None if mask is None else mask_fingerprint(mask), T.shape[0])
+            raise ValueError(f"T must be a 2-D (n_pixels, n_templates) array; got shape {T.shape}")
+        removed, kept = _split_space(T, float(template_epsilon))
+        if kept.shape[1] == 0:
+            raise ValueError(f"the templates span all {T.shape[0]} pixels: deprojecting them leaves a rank-0 "
+                             "filter")
+        return cls._projector(kept, removed, kept.shape[1], mask)
 
     @classmethod
     def from_subspace(cls, W, *, mask=None) -> "Filter":
-        """The projector onto an orthonormal (n, r) basis (``filters.py:358-400``)."""
+        """The orthogonal projector onto the span of an orthonormal (n, r)
+        basis (``filters.py:358-400``)."""
         W = np.ascontiguousarray(W, dtype=np.float64)
         if W.ndim != 2:
-            raise ValueError(f"W must be 2-D (n_pixels, rank), got {W.shape}")
-        if W.shape[1] > W.shape[0]:
-            raise ValueError(f"W has {W.shape[1]} columns for {W.shape[0]} pixels; a subspace cannot exceed "
-                             "the space")
-        norms = np.linalg.norm(W, axis=0)
-        if not np.allclose(norms, 1.0, atol=1e-8):
-            raise ValueError(f"W's columns must be orthonormal; column norms range over "
-                             f"[{norms.min():g}, {norms.max():g}]")
-        return cls(W, W, np.ones(W.shape[1]), W.shape[1], True, None, W.shape[1], None,
-                   None if mask is None else mask_fingerprint(mask), W.shape[0])
+            raise ValueError(f"W must be a 2-D (n_pixels, rank) array; got shape {W.shape}")
+        n, r = W.shape
+        if r > n:
+            raise ValueError(f"a subspace of {n} pixels cannot have {r} basis vectors")
+        if np.max(np.abs(W.T @ W - np.eye(r)), initial=0.0) > PROJECTOR_TOL:
+            raise ValueError("W must have orthonormal columns (W^T W = I)")
+        return cls._projector(W, None, r, mask)
 
     def __and__(self, other):
         return intersect(self, other)
 
 
-def _warn_on_truncation(s: np.ndarray, k: int) -> None:
-    if k < s.size:
-        gap = s[k] / s[k - 1]
-        if gap > GAP_WARN_RATIO:
-            with np.errstate(invalid="ignore", divide="ignore"):
-                largest = int(np.nanargmin(s[1:] / s[:-1])) + 1
-            warnings.warn(f"filter truncated at rank {k} without a spectral gap: σ_{k + 1}/σ_{k} = {gap:.3g}. "
-                          f"The largest gap in the spectrum is at rank {largest} "
-                          f"(ratio {s[largest] / s[largest - 1]:.3g})", stacklevel=3)
-    else:
-        cond = s[0] / s[-1]
-        warnings.warn(f"filter kept every singular value (rank {k}); it is invertible and restriction leaves "
-                      f"the estimator unchanged, but the restricted covariance carries condition {cond**2:.3g}",
-                      stacklevel=3)
-
-
 def intersect(f1: Filter, f2: Filter) -> Filter:
-    """Projector onto range(W1) ∩ range(W2), as the complement of the union of
-    the removed subspaces (``filters.py:431-490``)."""
+    """The projector onto range(f1) ∩ range(f2): the orthogonal complement of
+    the union of the two removed subspaces (``filters.py:431-490``)."""
     for f in (f1, f2):
         if not f.is_projector:
-            raise ValueError("intersection is defined for projectors only; a graded filter is a transfer "
-                             "function, not a subspace")
+            raise ValueError("only projectors can be intersected (a graded filter is not a subspace)")
         if f.Q_removed is None:
-            raise ValueError("intersection needs Q_removed, which from_subspace does not form; build the "
-                             "operands with from_deprojection")
+            raise ValueError("intersect needs each filter's removed directions (Q_removed); from_subspace "
+                             "does not form them, from_deprojection and from_operator do")
     if f1.n_pixels != f2.n_pixels:
-        raise ValueError(f"pixel-space dimensions differ: {f1.n_pixels} and {f2.n_pixels}")
-    if f1.fingerprint is not None and f2.fingerprint is not None and f1.fingerprint != f2.fingerprint:
-        raise ValueError("the two filters were built against different active-pixel orderings")
-    W, Qr = _complement(np.column_stack([f1.Q_removed, f2.Q_removed]), MIN_RANGE_EPSILON)
-    return Filter(W, W, np.ones(W.shape[1]), W.shape[1], True, None, W.shape[1], Qr,
-                  f1.fingerprint or f2.fingerprint, f1.n_pixels)
+        raise ValueError(f"the filters act on {f1.n_pixels} and {f2.n_pixels} pixels")
+    if None not in (f1.fingerprint, f2.fingerprint) and f1.fingerprint != f2.fingerprint:
+        raise ValueError("the filters were built for different active-pixel orderings")
+    removed, kept = _split_space(np.hstack([f1.Q_removed, f2.Q_removed]), MIN_RANGE_EPSILON)
+    if kept.shape[1] == 0:
+        raise ValueError("the two removed subspaces together span every pixel: the intersection is empty")
+    fp = f1.fingerprint if f1.fingerprint is not None else f2.fingerprint
+    r = kept.shape[1]
+    return Filter(kept, kept, np.ones(r), r, True, None, r, removed, fp, f1.n_pixels)
 
 
 class DeviceFilter:

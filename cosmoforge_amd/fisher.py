@@ -61,8 +61,10 @@ class Fisher:
         if precision not in pipeline.PRECISIONS:
             raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
         self.precision = precision
-        if self.basis == "harmonic" and precision != "fp64":
-            raise NotImplementedError("basis='harmonic' runs in fp64")
+        if self.basis == "harmonic" and precision != "fp64" and (self.params.do_cross or
+                                                                  kwargs.get("pixel_filter") is not None):
+            raise NotImplementedError("basis='harmonic' with precision='mixed' covers auto analyses without "
+                                      "a pixel filter")
         if self.params.do_cross and precision != "fp64":
             raise ValueError("precision='mixed' covers the auto-spectrum path; do_cross runs in fp64")
         self.derivs = None  # device derivative stacks (reused by Spectra when resident)
@@ -120,7 +122,12 @@ class Fisher:
     def prepare_covariance_matrices(self):
         if self.cov is None:
             self.cov = pipeline.Covariance(self.ctx, self.inputs, pixel_filter=self.device_filter())
-        self.cov.build()
+        if self.basis == "harmonic" and self.precision == "mixed":
+            from .harmonic import HarmonicContraction
+
+            self.cov.build(diag_blocks=HarmonicContraction.TRSM_BLOCK)
+        else:
+            self.cov.build()
         if not in_out._falsy(self.params.outinvcovmatfile1):
             in_out.write_covmat_reduced(self.params.outinvcovmatfile1, engine.to_numpy_cm(self.cov.inverse()))
         if self.params.do_cross:
@@ -152,7 +159,8 @@ class Fisher:
             from .harmonic import HarmonicContraction
 
             self.derivs = HarmonicContraction(self.cov, self.bins, self.params.output_convention,
-                                              cov2=self.cov2 if self.params.do_cross else None)
+                                              cov2=self.cov2 if self.params.do_cross else None,
+                                              precision=self.precision)
             self.fisher = self.derivs.fisher()
         elif self.params.do_cross:
             self.fisher, self.derivs = pipeline.fisher_cross(self.cov, self.cov2, self.bins,
@@ -243,7 +251,8 @@ class Fisher:
                 from .harmonic import HarmonicContraction
 
                 per_ell_fisher = HarmonicContraction(self.cov, per_ell, self.params.output_convention,
-                                                     cov2=self.cov2 if self.params.do_cross else None).fisher()
+                                                     cov2=self.cov2 if self.params.do_cross else None,
+                                                     precision=self.precision).fisher()
             else:
                 per_ell_fisher, _ = pipeline.fisher(self.cov, per_ell, self.params.output_convention,
                                                     precision=self.precision, keep_stacks=False,

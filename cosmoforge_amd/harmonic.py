@@ -17,6 +17,14 @@ HBM-bound mode reductions (``cfmi_harmonic_fisher``, ``cfmi_harmonic_pairsum``),
 the pixel path's n^2 x n products per parameter: about 30x less work at
 nside 16 and 60x at nside 32.
 
+``precision="mixed"``: the three products run on the int8 matrix cores with
+sliced operands (``engine.gemm_s8``: exact integer sums, operands held to
+2^-32 / 2^-40 of their row maxima), and without the whitener: Z = L^-1 D V
+and R = D L^-T Z as blocked triangular solves with L, the Cholesky factor of
+D C D (FP64, as are the diagonal-block solves, ln|C| and y = C^-1 x).  The
+factor's rows are flat where the whitener's are dominated by the diagonal,
+which is what lets few slices reach ~1e-9 on F.
+
 The modes span l in [first bin's l_min, last bin's l_max]; the derivative of
 the pixel path carries the reference's acos-based rotation angles, which
 agree with the exact spin-2 harmonics to ~1e-8 relative, so the two routes
@@ -36,11 +44,25 @@ class HarmonicContraction:
     """Mode operator V (n x N_modes, column-major, in HBM) and the tables of
     one (Covariance, bins) pair."""
 
+    # int8 slices of the mixed products: Z = L^-1 D V, M = Z^T Z, R = D L^-T Z
+    MIXED_SLICES = (4, 5, 4)
+    TRSM_BLOCK = 512
+
     def __init__(self, cov: pipeline.Covariance, bins, convention: str = "Cl",
-                 cov2: pipeline.Covariance | None = None):
-        """``cov2``: C2 = S + N2 of a cross analysis (``do_cross``)."""
+                 cov2: pipeline.Covariance | None = None, precision: str = "fp64", slices=None):
+        """``cov2``: C2 = S + N2 of a cross analysis (``do_cross``);
+        ``precision``: "fp64" or "mixed" (auto analyses without a filter)."""
         self.cov, self.bins = cov, [(int(a), int(b)) for a, b in bins]
         self.cov2 = cov2
+        if precision not in pipeline.PRECISIONS:
+            raise ValueError(f"precision must be one of {pipeline.PRECISIONS}; got {precision!r}")
+        if precision == "mixed" and (cov2 is not None or getattr(cov, "pixel_filter", None) is not None):
+            raise ValueError("the mixed harmonic contraction covers auto analyses without a pixel filter")
+        self.precision = precision
+        self.slices = tuple(slices) if slices is not None else self.MIXED_SLICES
+        if precision == "mixed" and getattr(cov.factor, "nb", 0) == 0:
+            raise ValueError("the mixed harmonic contraction needs the factor's diagonal-block inverses: "
+                             "build the covariance with diag_blocks")
         ctx, inp = cov.ctx, cov.inp
         lay = inp.layout
         self.lmin = min(a for a, _ in self.bins)
@@ -114,12 +136,19 @@ class HarmonicContraction:
                 P(self._w), self.lw)
 
     def _z(self, cov) -> torch.Tensor:
-        """Z = W V (n x N, column-major) of ``cov``, kept for the QML forms."""
+        """Z = W V = L^-1 D V (n x N, column-major) of ``cov``, kept for the
+        QML forms."""
         key = id(cov)
         if key not in self._Z:
             ctx, n, nt = cov.ctx, cov.n, self.nt
             Z = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-            engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
+            if self.precision == "mixed":
+                B = self.V.clone()
+                ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(B), n, P(cov.factor.d)), "scale")
+                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self.slices[0])
+                del B
+            else:
+                engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
             self._Z[key] = Z
         return self._Z[key]
 
@@ -127,17 +156,41 @@ class HarmonicContraction:
         ctx, n, nt = cov.ctx, cov.n, self.nt
         Z = self._z(cov)
         M = torch.empty((nt, nt), dtype=torch.float64, device=ctx.device)
-        engine.gram(ctx, nt, nt, n, Z, Z, out=M)
+        if self.precision == "mixed":
+            Zs = engine.slice8(ctx, self.slices[1], 0, nt, n, Z, 0, n)
+            engine.gemm_s8(ctx, nt, nt, n, 1.0, Zs, 0, Zs, 0, 0.0, M, 0, nt, flags=1)
+            del Zs
+        else:
+            engine.gram(ctx, nt, nt, n, Z, Z, out=M)
         return M
 
     def _r(self, cov) -> torch.Tensor:
-        """R = C^-1 V = W^T Z (column-major n x N); releases Z."""
+        """R = C^-1 V = W^T Z = D L^-T Z (column-major n x N); releases Z."""
         ctx, n, nt = cov.ctx, cov.n, self.nt
         Z = self._z(cov)
         R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
+        if self.precision == "mixed":
+            del self._Z[id(cov)]
+            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self.slices[2])
+            del Z
+            ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(R), n, P(cov.factor.d)), "scale")
+            return R
         engine.gemm(ctx, True, False, n, nt, n, 1.0, cov.factor.W, 0, n, Z, 0, n, 0.0, R, 0, n, 2)
         del self._Z[id(cov)]
         return R
+
+    def _solve(self, cov, maps: np.ndarray) -> torch.Tensor:
+        """y = C^-1 x = D L^-T L^-1 D x in FP64 (column-major n x nsims)."""
+        ctx, n = cov.ctx, cov.n
+        ns = maps.shape[1]
+        X = ctx.upload_cm(maps)
+        ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, ns, P(X), n, P(cov.factor.d)), "scale")
+        T = ctx.empty(n, ns)
+        engine.trsm(ctx, cov.factor, n, X, T, ns)
+        Y = ctx.empty(n, ns)
+        engine.trsm(ctx, cov.factor, n, T, Y, ns, transpose=True)
+        ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, ns, P(Y), n, P(cov.factor.d)), "scale")
+        return Y
 
     def _ut(self, R: torch.Tensor, maps: np.ndarray) -> torch.Tensor:
         """u^T C^-1 x = R^T x for every mode and simulation (column-major ns x N)."""
@@ -190,7 +243,14 @@ class HarmonicContraction:
                                                     0.5, P(q), self.npar), "harmonic_pairsum")
             return q.cpu().numpy(), np.zeros(self.npar)
         R = self._r(self.cov)
-        Ut = self._ut(R, maps)
+        if self.precision == "mixed":
+            # u^T y from y = C^-1 x solved in FP64, rather than from the sliced R
+            Y = self._solve(self.cov, maps)
+            Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)
+            engine.gemm(ctx, True, False, ns, nt, n, 1.0, Y, 0, n, self.V, 0, n, 0.0, Ut, 0, ns)
+            del Y
+        else:
+            Ut = self._ut(R, maps)
         ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, 1, P(Ut), ns, 1, P(Ut), ns, 1, ns, *self._tables(), 0.5,
                                                 P(q), self.npar), "harmonic_pairsum")
         NR = torch.empty_like(R)

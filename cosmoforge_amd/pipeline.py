@@ -87,20 +87,35 @@ class Covariance:
         self.S = engine.signal_matrix(ctx, self.inp.layout, self.vecs, dev, self.inp.lmax_signal, out=self.S)
         return self.S
 
-    def build(self, cls=None, need_whitener: bool = True) -> engine.Factor:
-        """S, then C = S + N factorised in a separate buffer (S is kept)."""
+    def build(self, cls=None, need_whitener: bool = True, diag_blocks: int = 0) -> engine.Factor:
+        """S, then C = S + N factorised in a separate buffer (S is kept).
+        ``diag_blocks``: keep the inverses of L's diagonal blocks (for
+        ``engine.trsm``) instead of forming the whitener."""
         ctx, n = self.ctx, self.n
         S = self.signal(cls)
         C = ctx.empty(n, n)
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(S), n, 0.0, P(C), n), "axpby")
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
-        self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener)
+        self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener and not diag_blocks,
+                                    diag_blocks=diag_blocks)
         self._w32 = None
         self._wp = None
         return self.factor
 
     def inverse(self) -> torch.Tensor:
-        return engine.inverse_from_whitener(self.ctx, self.factor.W, self.n)
+        f = self.factor
+        if f.W is not None:
+            return engine.inverse_from_whitener(self.ctx, f.W, self.n)
+        # C^-1 = D L^-T L^-1 D from the blocked solves (no whitener formed)
+        ctx, n = self.ctx, self.n
+        X = ctx.empty(n, n)
+        ctx.check(ctx.lib.cfmi_fill(ctx.h, engine.F64, n, n, P(X), n, 0.0), "fill")
+        ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, 1, n, P(f.d), 1, P(X), n + 1), "copy")
+        T = ctx.empty(n, n)
+        engine.trsm(ctx, f, n, X, T, n)
+        engine.trsm(ctx, f, n, T, X, n, transpose=True)
+        ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, n, P(X), n, P(f.d)), "scale")
+        return X
 
 
 def make_source(cov: Covariance, bins, convention: str, dtype, budget: int | None = None):

@@ -94,7 +94,8 @@ class Spectra:
                 from .harmonic import HarmonicContraction
 
                 src = f.derivs = HarmonicContraction(f.cov, f.bins, self.params.output_convention,
-                                                     cov2=f.cov2 if self.params.do_cross else None)
+                                                     cov2=f.cov2 if self.params.do_cross else None,
+                                                     precision=f.precision)
             self.setup_maps()
             self.setup_fisher_inversion()
             self.qml_results, self.qml_noise_bias = src.qml(self.maps1, remove_nb=bool(self.params.remove_nb),

@@ -221,6 +221,33 @@ int cfmi_split3(cfmi_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void
                 void *dst, int64_t ldd, int64_t ps);
 int cfmi_join3(cfmi_ctx *ctx, int64_t count, const void *src, int64_t ps, double *dst);
 
+/* ---- FP64 products on the int8 matrix cores (sliced operands) ---------- */
+/* An operand row x_r (k values) is held as s int8 slices and a scale
+ * mu_r = max |x_rk|: x = mu (s_1/127 + s_2/(127*254) + ...), each slice a
+ * plane of rows x ldk bytes (k contiguous, zero beyond k), planes ps bytes
+ * apart; s = 3, 4, 5 keep the row to 2^-24, 2^-32, 2^-40 of mu.
+ * cfmi_slice8 layout 0: row r of the source at src + r lds (k contiguous);
+ * layout 1: element (r, k) at src[r + k lds] (rows of a column-major
+ * matrix).  ldk a multiple of 64 >= k; dst 16-byte aligned.                */
+int cfmi_slice8(cfmi_ctx *ctx, int s, int layout, int64_t rows, int64_t k, const double *src, int64_t lds,
+                void *dst, int64_t ldk, int64_t ps, double *mu);
+/* The FP64 values the slices represent, dst[r ldd + k] (tests).            */
+int cfmi_unslice8(cfmi_ctx *ctx, int s, int64_t rows, int64_t k, const void *src, int64_t ldk, int64_t ps,
+                  const double *mu, double *dst, int64_t ldd);
+/* Largest k one launch accumulates exactly in int32 (s k 127^2 < 2^31).    */
+int64_t cfmi_s8_kmax(int s);
+/* C (m x n, column-major FP64) = alpha A B^T + beta C with A (m rows) and
+ * B (n rows) sliced over the same k: the s(s+1)/2 slice products with
+ * p + q < s on v_mfma_i32_16x16x64_i8 (exact int32 sums), levels combined
+ * in FP64; k beyond cfmi_s8_kmax is split and the parts added in FP64.
+ * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored.
+ * Serves the C^-1 V, V^T C^-1 V and C^-1 dC products that replace
+ * matrix_mult at qube/qube/fisher.py:506-531 and spectra.py:1144-1161 in
+ * the mixed precision (no reference counterpart of the arithmetic).       */
+int cfmi_gemm_s8(cfmi_ctx *ctx, int s, int64_t m, int64_t n, int64_t k, double alpha, const void *A, int64_t lda,
+                 int64_t psA, const double *muA, const void *B, int64_t ldb, int64_t psB, const double *muB,
+                 double beta, double *C, int64_t ldc, int flags);
+
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);
 

@@ -27,9 +27,9 @@ def test_from_operator_graded_and_rank_knobs():
     np.testing.assert_allclose(f.U @ np.diag(f.sigma) @ f.W.T, Q1[:, :30] @ np.diag(s[:30]) @ Q2[:, :30].T,
                                atol=1e-12)
     assert Filter.from_operator(F, range_rank=30).rank == 30
-    with pytest.raises(ValueError, match="rank threshold is required"):
+    with pytest.raises(ValueError, match="exactly one truncation knob"):
         Filter.from_operator(F)
-    with pytest.raises(ValueError, match="mutually exclusive"):
+    with pytest.raises(ValueError, match="exactly one truncation knob"):
         Filter.from_operator(F, range_epsilon=1e-3, range_rank=3)
     with pytest.raises(ValueError, match="MIN_RANGE_EPSILON"):
         Filter.from_operator(F, range_rank=35)
@@ -67,7 +67,7 @@ def test_from_subspace_and_intersection():
     assert f.rank == 7 and f.is_projector
     with pytest.raises(ValueError, match="orthonormal"):
         Filter.from_subspace(2 * W)
-    with pytest.raises(ValueError, match="cannot exceed"):
+    with pytest.raises(ValueError, match="cannot have"):
         Filter.from_subspace(rng.standard_normal((3, 5)))
     T1, T2 = rng.standard_normal((n, 2)), rng.standard_normal((n, 3))
     f1, f2 = Filter.from_deprojection(T1), Filter.from_deprojection(T2)

@@ -1,0 +1,167 @@
+"""The int8-sliced FP64 products (``cfmi_slice8`` / ``cfmi_gemm_s8``) and the
+blocked triangular solve built on them (``engine.trsm``).
+
+* Slicing and products equal the host restatement (``tests/s8_emul.py``) bit
+  for bit: same slices, same exact integer level sums, same FP64 epilogue,
+  for 3, 4 and 5 slices, both source layouts, ragged sizes, alpha / beta,
+  the symmetric (lower tiles + mirror) output and a K split beyond the exact
+  int32 range.
+* The products stay within the slicing bound of the FP64 product, which a
+  dropped level or a wrong scale would exceed.
+* The blocked solves equal SciPy's triangular solves to the bound of their
+  arithmetic (FP64: 1e-12; 4 or 5 slices: the slicing error times the growth
+  of the solve).
+"""
+
+import numpy as np
+import pytest
+
+from tests import s8_emul as E
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cosmoforge_amd import engine
+
+    return engine.Context(0)
+
+
+def _mat(rng, m, k, spread=3):
+    """Rows of different magnitudes, some entries zero, one zero row."""
+    X = rng.standard_normal((m, k)) * 10.0 ** rng.uniform(-spread, spread, (m, 1))
+    X[rng.random((m, k)) < 0.05] = 0.0
+    X[m // 3] = 0.0
+    return X
+
+
+def _dev_slices(ctx, X, s, layout):
+    """Slice the rows of host X on the device; layout 1 reads X column-major
+    (rows of a column-major matrix), layout 0 reads it as k-contiguous rows."""
+    import torch
+    from cosmoforge_amd import engine
+
+    m, k = X.shape
+    if layout == 1:
+        src = ctx.upload_cm(X)                     # element (r, k) at r + k m
+        return engine.slice8(ctx, s, 1, m, k, src, 0, m)
+    src = torch.as_tensor(np.ascontiguousarray(X)).to(ctx.device)   # row r at r k
+    return engine.slice8(ctx, s, 0, m, k, src, 0, k)
+
+
+@pytest.mark.parametrize("s", [3, 4, 5])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_slices_bit_exact(ctx, s, layout):
+    rng = np.random.default_rng(10 + s + 7 * layout)
+    X = _mat(rng, 203, 331)
+    sl = _dev_slices(ctx, X, s, layout)
+    mu, ref = E.slices(X, s)
+    got = sl.data.cpu().numpy()[:, :203, :]
+    np.testing.assert_array_equal(sl.mu.cpu().numpy()[:203], mu)
+    np.testing.assert_array_equal(got[:, :, :331].astype(np.int64), ref)
+    assert not got[:, :, 331:].any()                          # zero padding up to ldk
+    # the represented values stay within the slicing bound
+    err = np.abs(E.unslice(mu, ref) - X)
+    assert np.all(err <= (0.5 / (127.0 * 254.0 ** (s - 1)) + 4e-16) * mu[:, None])   # + FP64 rounding
+
+
+@pytest.mark.parametrize("s", [3, 4, 5])
+@pytest.mark.parametrize("shape", [(130, 97, 200), (257, 385, 1000), (64, 64, 64)])
+def test_gemm_s8_bit_exact_and_bound(ctx, s, shape):
+    from cosmoforge_amd import engine
+
+    m, n, k = shape
+    rng = np.random.default_rng(s * 100 + m)
+    A, B = _mat(rng, m, k), _mat(rng, n, k)
+    C0 = rng.standard_normal((m, n))
+    Sa, Sb = _dev_slices(ctx, A, s, 1), _dev_slices(ctx, B, s, 0)
+    C = ctx.upload_cm(C0)
+    engine.gemm_s8(ctx, m, n, k, -0.75, Sa, 0, Sb, 0, 0.5, C, 0, m)
+    got = engine.to_numpy_cm(C)
+    want = E.gemm(A, B, s, alpha=-0.75, beta=0.5, C=C0)
+    np.testing.assert_array_equal(got, want)
+    exact = -0.75 * (A @ B.T) + 0.5 * C0
+    assert np.all(np.abs(got - exact) <= 0.75 * E.bound(A, B, s) + 1e-15 * np.abs(C0))
+
+
+def test_gemm_s8_symmetric_and_offsets(ctx):
+    """Lower tiles + mirror equal the full product; row offsets into the
+    slice buffers address the right rows."""
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(5)
+    Z = rng.standard_normal((300, 777))
+    S = _dev_slices(ctx, Z, 4, 0)
+    M = torch.zeros((300, 300), dtype=torch.float64, device=ctx.device)
+    engine.gemm_s8(ctx, 300, 300, 777, 1.0, S, 0, S, 0, 0.0, M, 0, 300, flags=1)
+    got = engine.to_numpy_cm(M)
+    want = E.gemm(Z, Z, 4)
+    low = np.tril(want)
+    np.testing.assert_array_equal(np.tril(got), low)
+    np.testing.assert_array_equal(got, low + np.tril(low, -1).T)
+    # rows 40.. of A against rows 7.. of B
+    C = torch.zeros((50, 100), dtype=torch.float64, device=ctx.device)   # column-major 100 x 50
+    engine.gemm_s8(ctx, 100, 50, 777, 1.0, S, 40, S, 7, 0.0, C, 0, 100)
+    full = want
+    # the slices of a row do not depend on the other rows, so the block of
+    # the full product is the product of the blocks
+    np.testing.assert_array_equal(engine.to_numpy_cm(C), full[40:140, 7:57])
+
+
+def test_gemm_s8_k_split(ctx):
+    """k beyond the exact int32 range of one pass: chunks added in FP64."""
+    from cosmoforge_amd import engine
+
+    s = 5
+    k = E.kmax(s) + 1000
+    assert ctx.lib.cfmi_s8_kmax(s) == E.kmax(s)
+    rng = np.random.default_rng(9)
+    A, B = rng.standard_normal((70, k)), rng.standard_normal((33, k))
+    Sa, Sb = _dev_slices(ctx, A, s, 0), _dev_slices(ctx, B, s, 0)
+    C = ctx.empty(70, 33)
+    engine.gemm_s8(ctx, 70, 33, k, 1.0, Sa, 0, Sb, 0, 0.0, C, 0, 70)
+    got = engine.to_numpy_cm(C)
+    np.testing.assert_array_equal(got, E.gemm(A, B, s))
+    assert np.all(np.abs(got - A @ B.T) <= E.bound(A, B, s))
+
+
+def test_gemm_s8_rejects_bad_arguments(ctx):
+    from cosmoforge_amd import engine
+
+    S = engine.Slices(ctx, 4, 10, 64)
+    C = ctx.empty(10, 10)
+    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 6, 10, 10, 64, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu),
+                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0.0, engine.P(C), 10, 0)
+    assert bad < 0 and b"slices" in ctx.lib.cfmi_last_error(ctx.h)
+    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 4, 10, 10, 200, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu),
+                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0.0, engine.P(C), 10, 0)
+    assert bad < 0   # row pitch 64 below k = 200
+
+
+@pytest.mark.parametrize("slices", [0, 4, 5])
+def test_blocked_trsm(ctx, slices):
+    import torch
+    from scipy.linalg import solve_triangular
+
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(3 + slices)
+    n, nrhs = 900, 130
+    G = rng.standard_normal((n, n))
+    C = G @ G.T / n + np.diag(rng.uniform(0.5, 2.0, n))
+    dC = ctx.upload_cm(C)
+    f = engine.factor(ctx, dC, n, need_whitener=False, diag_blocks=128)
+    Lh = np.tril(engine.to_numpy_cm(f.L))
+    Bh = rng.standard_normal((n, nrhs))
+    X = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
+    engine.trsm(ctx, f, n, ctx.upload_cm(Bh), X, nrhs, slices=slices)
+    want = solve_triangular(Lh, Bh, lower=True)
+    err = np.max(np.abs(engine.to_numpy_cm(X) - want)) / np.max(np.abs(want))
+    Y = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
+    engine.trsm(ctx, f, n, X.clone(), Y, nrhs, transpose=True, slices=slices)
+    want2 = solve_triangular(Lh.T, want, lower=False)
+    err2 = np.max(np.abs(engine.to_numpy_cm(Y) - want2)) / np.max(np.abs(want2))
+    tol = {0: 1e-12, 4: 1e-7, 5: 1e-9}[slices]
+    assert err < tol and err2 < tol, (err, err2)
