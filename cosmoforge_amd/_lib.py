@@ -60,6 +60,7 @@ SIGNATURES = {
     "cfmi_join3": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_slice8": (ci, [vp, ci, ci, i64, i64, vp, i64, vp, i64, i64, vp]),
     "cfmi_unslice8": (ci, [vp, ci, i64, i64, vp, i64, i64, vp, vp, i64]),
+    "cfmi_transpose": (ci, [vp, i64, i64, vp, i64, vp, i64]),
     "cfmi_s8_kmax": (i64, [ci]),
     "cfmi_gemm_s8": (ci, [vp, ci, i64, i64, i64, cd, vp, i64, i64, vp, vp, i64, i64, vp, cd, vp, i64, ci]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),

@@ -518,6 +518,7 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
 // ---------------------------------------------------------------------------
 constexpr int NB = 64;
 constexpr int64_t OB = 512;   // Cholesky panel width (a multiple of NB)
+constexpr int64_t OB2 = 2048; // outer panel width (a multiple of OB)
 
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
@@ -668,37 +669,50 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
         int rc = fill_t<T>(c, n, n, U, ldu, 0.0, 0);
         if (rc) return rc;
     }
-    // Panels of OB columns: inside a panel, 64-column steps (one-wave
-    // diagonal factor, row-block triangular solve, update of the panel's
-    // remaining columns); then one rank-OB update of the trailing matrix,
-    // which carries almost all of the n^3/3 flops at a K that keeps the
-    // matrix cores busy (a rank-64 update is bound by re-reading C).
-    for (int64_t k0 = 0; k0 < n; k0 += OB) {
-        const int64_t ob = std::min<int64_t>(OB, n - k0);
-        for (int64_t k = k0; k < k0 + ob; k += NB) {
-            int kb = (int)std::min<int64_t>(NB, k0 + ob - k);
-            hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k + k * lda,
-                               lda, kb, c->d_info, k, linv, U ? U + k + k * ldu : (T*)nullptr, ldu);
-            CFMI_LAUNCH_CHECK(c);
-            int64_t m = n - k - kb;
-            if (m > 0) {
-                hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + NB - 1) / NB)), dim3(256), 0,
-                                   c->stream, A + (k + kb) + k * lda, lda, m, kb, (const T*)linv,
-                                   (const int*)c->d_info);
+    // Three levels of column panels.  Inside an OB-column panel, 64-column
+    // steps (one-wave diagonal factor, row-block triangular solve, update of
+    // the panel's remaining columns); after each OB panel, a rank-OB update
+    // of the columns left in its OB2-column panel; after each OB2 panel one
+    // rank-OB2 update of the trailing matrix, which carries almost all of
+    // the n^3/3 flops at a K that keeps the matrix cores busy (a rank-64 or
+    // rank-512 update of the whole trailing matrix is bound by re-reading C).
+    for (int64_t K0 = 0; K0 < n; K0 += OB2) {
+        const int64_t ob2 = std::min<int64_t>(OB2, n - K0);
+        for (int64_t k0 = K0; k0 < K0 + ob2; k0 += OB) {
+            const int64_t ob = std::min<int64_t>(OB, K0 + ob2 - k0);
+            for (int64_t k = k0; k < k0 + ob; k += NB) {
+                int kb = (int)std::min<int64_t>(NB, k0 + ob - k);
+                hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k + k * lda,
+                                   lda, kb, c->d_info, k, linv, U ? U + k + k * ldu : (T*)nullptr, ldu);
                 CFMI_LAUNCH_CHECK(c);
-                const int64_t w = k0 + ob - (k + kb);   // panel columns still to update
-                if (w > 0) {
-                    T* P = A + (k + kb) + k * lda;
-                    int rc = gemm_t<T>(c, 0, 1, m, w, kb, -1.0, P, lda, P, lda, 1.0,
-                                       A + (k + kb) + (k + kb) * lda, lda, 16);
-                    if (rc) return rc;
+                int64_t m = n - k - kb;
+                if (m > 0) {
+                    hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + NB - 1) / NB)), dim3(256), 0,
+                                       c->stream, A + (k + kb) + k * lda, lda, m, kb, (const T*)linv,
+                                       (const int*)c->d_info);
+                    CFMI_LAUNCH_CHECK(c);
+                    const int64_t w = k0 + ob - (k + kb);   // panel columns still to update
+                    if (w > 0) {
+                        T* P = A + (k + kb) + k * lda;
+                        int rc = gemm_t<T>(c, 0, 1, m, w, kb, -1.0, P, lda, P, lda, 1.0,
+                                           A + (k + kb) + (k + kb) * lda, lda, 16);
+                        if (rc) return rc;
+                    }
                 }
             }
+            const int64_t w2 = K0 + ob2 - (k0 + ob);       // OB2-panel columns still to update
+            const int64_t m1 = n - k0 - ob;
+            if (w2 > 0 && m1 > 0) {
+                T* P1 = A + (k0 + ob) + k0 * lda;
+                int rc = gemm_t<T>(c, 0, 1, m1, w2, ob, -1.0, P1, lda, P1, lda, 1.0,
+                                   A + (k0 + ob) + (k0 + ob) * lda, lda, 16);
+                if (rc) return rc;
+            }
         }
-        const int64_t m2 = n - k0 - ob;
+        const int64_t m2 = n - K0 - ob2;
         if (m2 > 0) {
-            T* P2 = A + (k0 + ob) + k0 * lda;
-            int rc = gemm_t<T>(c, 0, 1, m2, m2, ob, -1.0, P2, lda, P2, lda, 1.0, A + (k0 + ob) + (k0 + ob) * lda,
+            T* P2 = A + (K0 + ob2) + K0 * lda;
+            int rc = gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A + (K0 + ob2) + (K0 + ob2) * lda,
                                lda, 16);
             if (rc) return rc;
         }

@@ -267,11 +267,15 @@ __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (
     }
 }
 
-template <int S>
+// DB = 1: two LDS stages, one barrier per slab: the slab prefetched into
+// registers during the MFMAs of slab k is stored into the other stage at the
+// start of step k + 1, while the stage of slab k + 1 (stored during step k)
+// is multiplied.  DB = 0: one stage, stored between two barriers.
+template <int S, int DB>
 __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 {
-    __shared__ __attribute__((aligned(16))) char As[S][BT][SK];
-    __shared__ __attribute__((aligned(16))) char Bs[S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char As[DB + 1][S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char Bs[DB + 1][S][BT][SK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t z, ti, tj;
     if (!group_tile(g, blockIdx.x, z, ti, tj)) return;
@@ -290,18 +294,35 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 
     Stage8<S> st;
     st.load(g, m0, n0, 0, tid);
-    int64_t k0 = 0;
-    for (; k0 + SK < g.K; k0 += SK) {
+    if constexpr (DB) {
+        st.store(As[0], Bs[0], tid);
+        if (SK < g.K) st.load(g, m0, n0, SK, tid);
         __syncthreads();
-        st.store(As, Bs, tid);
+        for (int64_t k0 = 0;; k0 += SK) {
+            const int cur = (int)((k0 / SK) & 1);
+            const bool more = k0 + SK < g.K;
+            if (more) {
+                st.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+                if (k0 + 2 * SK < g.K) st.load(g, m0, n0, k0 + 2 * SK, tid);
+            }
+            slab_mfma<S>(As[cur], Bs[cur], acc, wm, wn, lm, lk);
+            if (!more) break;
+            __syncthreads();
+        }
+    } else {
+        int64_t k0 = 0;
+        for (; k0 + SK < g.K; k0 += SK) {
+            __syncthreads();
+            st.store(As[0], Bs[0], tid);
+            __syncthreads();
+            st.load(g, m0, n0, k0 + SK, tid);
+            slab_mfma<S>(As[0], Bs[0], acc, wm, wn, lm, lk);
+        }
         __syncthreads();
-        st.load(g, m0, n0, k0 + SK, tid);
-        slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
+        st.store(As[0], Bs[0], tid);
+        __syncthreads();
+        slab_mfma<S>(As[0], Bs[0], acc, wm, wn, lm, lk);
     }
-    __syncthreads();
-    st.store(As, Bs, tid);
-    __syncthreads();
-    slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
 
     double sa[2][4], sb[4];
 #pragma unroll
@@ -357,6 +378,29 @@ __global__ __launch_bounds__(256) void mirror_lower_s8_kernel(int64_t n, double*
     }
 }
 
+// dst (n x m, column-major) = src^T for src (m x n, column-major), 64 x 64
+// tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_kernel(int64_t m, int64_t n, const double* __restrict__ src,
+                                                        int64_t lds, double* __restrict__ dst, int64_t ldd)
+{
+    __shared__ double t[64][65];
+    const int64_t bi = blockIdx.x, bj = blockIdx.y;   // source rows bi*64.., columns bj*64..
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
+        const int64_t r = bi * 64 + rr, c = bj * 64 + cc;
+        t[cc][rr] = (r < m && c < n) ? src[r + c * lds] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
+        const int64_t r = bj * 64 + rr, c = bi * 64 + cc;   // dst (r, c) = src (c, r)
+        if (r < n && c < m) dst[r + c * ldd] = t[rr][cc];
+    }
+}
+
 inline dim3 grid_of(int64_t tot)
 {
     int64_t b = (tot + 255) / 256;
@@ -393,7 +437,7 @@ int slice_t(cfmi_ctx* c, int layout, int64_t rows, int64_t k, const double* src,
 }
 
 template <int S>
-int gemm_s8_launch(cfmi_ctx* c, S8Args g)
+int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
 {
     g.tm = (g.M + BT - 1) / BT;
     g.tn = (g.N + BT - 1) / BT;
@@ -406,7 +450,14 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g)
     g.ngroups = g.gtmp * g.gtn;
     const int64_t nwg = (g.ngroups + 7) / 8 * 8 * g.gm * g.gn;
     if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: too many tiles");
-    hipLaunchKernelGGL((gemm_s8_kernel<S>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    if constexpr (S <= 4) {
+        if (db) {
+            hipLaunchKernelGGL((gemm_s8_kernel<S, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+            CFMI_LAUNCH_CHECK(c);
+            return CFMI_OK;
+        }
+    }
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
@@ -444,6 +495,19 @@ int cfmi_slice8(cfmi_ctx* c, int s, int layout, int64_t rows, int64_t k, const d
         case 4: return slice_t<4>(c, layout, rows, k, src, lds, d, ldk, ps, mu);
         default: return slice_t<5>(c, layout, rows, k, src, lds, d, ldk, ps, mu);
     }
+}
+
+int cfmi_transpose(cfmi_ctx* c, int64_t m, int64_t n, const double* src, int64_t lds, double* dst, int64_t ldd)
+{
+    if (!c) return CFMI_EINVAL;
+    if (m <= 0 || n <= 0) return CFMI_OK;
+    if (lds < m || ldd < n) return cfmi_fail(c, CFMI_EINVAL, "transpose: leading dimension too small");
+    if ((m + 63) / 64 > 0x7fffffffLL || (n + 63) / 64 > 65535)
+        return cfmi_fail(c, CFMI_EINVAL, "transpose: matrix too large");
+    hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256), 0,
+                       c->stream, m, n, src, lds, dst, ldd);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
 }
 
 int cfmi_unslice8(cfmi_ctx* c, int s, int64_t rows, int64_t k, const void* src, int64_t ldk, int64_t ps,
@@ -488,7 +552,10 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
         g.A = (const char*)A + k0; g.lda = lda; g.psA = psA; g.muA = muA;
         g.B = (const char*)B + k0; g.ldb = ldb; g.psB = psB; g.muB = muB;
         g.C = C; g.ldc = ldc; g.lower = lower ? 1 : 0;
-        int rc = s == 3 ? gemm_s8_launch<3>(c, g) : s == 4 ? gemm_s8_launch<4>(c, g) : gemm_s8_launch<5>(c, g);
+        // two LDS stages for s <= 4 (s = 5 would spill registers); flag 2: one stage
+        const bool db = s <= 4 && !(flags & 2);
+        int rc = s == 3 ? gemm_s8_launch<3>(c, g, db) : s == 4 ? gemm_s8_launch<4>(c, g, db)
+                                                              : gemm_s8_launch<5>(c, g, false);
         if (rc) return rc;
         b = 1.0;
     }

@@ -240,14 +240,31 @@ __global__ __launch_bounds__(RT) void harm_pairsum_kernel(int64_t ni, const doub
     const double* Xj = X + j * sxj;
     const double* Yj = Y + j * syj;
     double acc = 0.0;
-    for (int64_t q = threadIdx.x; q < K * ni; q += RT) {
-        const int64_t i = q % ni, k = ka + q / ni;
-        const double a = t.w[s * t.lw + ell_of(k, t.lmin)];
-        if (a == 0.0) continue;
-        double term = 0.0;
-        for (int e = 0; e < np; ++e)
-            term += Xj[i + (t.goff[Pp[2 * e]] + k) * ldx] * Yj[i + (t.goff[Pp[2 * e + 1]] + k) * ldy];
-        acc += a * term;
+    if (ni == 1) {
+        // one value per mode: threads over the modes
+        for (int64_t kk = threadIdx.x; kk < K; kk += RT) {
+            const int64_t k = ka + kk;
+            const double a = t.w[s * t.lw + ell_of(k, t.lmin)];
+            if (a == 0.0) continue;
+            double term = 0.0;
+            for (int e = 0; e < np; ++e)
+                term += Xj[(t.goff[Pp[2 * e]] + k) * ldx] * Yj[(t.goff[Pp[2 * e + 1]] + k) * ldy];
+            acc += a * term;
+        }
+    } else {
+        // modes in turn (weight looked up once per mode), threads over the
+        // contiguous rows of the two columns of each pair
+        for (int64_t k = ka; k < ka + K; ++k) {
+            const double a = t.w[s * t.lw + ell_of(k, t.lmin)];
+            if (a == 0.0) continue;
+            double term = 0.0;
+            for (int e = 0; e < np; ++e) {
+                const double* xc = Xj + (t.goff[Pp[2 * e]] + k) * ldx;
+                const double* yc = Yj + (t.goff[Pp[2 * e + 1]] + k) * ldy;
+                for (int64_t i = threadIdx.x; i < ni; i += RT) term += xc[i] * yc[i];
+            }
+            acc += a * term;
+        }
     }
     __shared__ double red[RT / 64];
     const double r = block_sum(acc, red);

@@ -719,45 +719,80 @@ def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row
                                    P(C, offc), ldc, flags), "gemm_s8")
 
 
+def transpose(ctx: Context, A: torch.Tensor, m: int, n: int, lda: int) -> torch.Tensor:
+    """A^T (n x m, column-major) of a column-major m x n FP64 matrix."""
+    T = ctx.empty(n, m)
+    ctx.check(ctx.lib.cfmi_transpose(ctx.h, m, n, P(A), lda, P(T), n), "transpose")
+    return T
+
+
 def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs: int, transpose: bool = False,
-         slices: int = 0, bufs=None):
+         slices: int = 0, outer: int = 2048):
     """X = L^-1 B (or L^-T B with ``transpose``) for the factor's L, n x nrhs
     column-major with leading dimension n; B is consumed (it holds the
-    running right-hand side).  Blocked over L's nb-blocks: the diagonal
-    solve is an FP64 product with the block inverse (``factor(...,
-    diag_blocks=nb)``), the update of the remaining rows a GEMM, FP64 or on
-    the int8 matrix cores with ``slices`` = s (both operands sliced per
-    block: the rows of L's panel and the columns of the solved block)."""
+    running right-hand side).
+
+    Two-level blocking: inside an ``outer``-row block, nb-row steps (nb the
+    factor's diagonal-block size) whose diagonal solve is an FP64 product
+    with the block inverse (``factor(..., diag_blocks=nb)``) and whose
+    update reaches the rows of the same outer block; after each outer block
+    one update of every remaining row with K = ``outer``, so the right-hand
+    side is read and written n / outer times instead of n / nb.  Updates run
+    in FP64, or on the int8 matrix cores with ``slices`` = s: the rows of L's
+    panel (rows of L^T, kept once per factor) and the solved block's columns
+    are sliced per update, each row scaled over the panel's width."""
     nb = f.nb
     if not nb or f.U is None:
         raise ValueError("trsm needs the factor's diagonal-block inverses (factor(..., diag_blocks=nb))")
+    outer = max(nb, outer // nb * nb)
     L, U = f.L, f.U
-    blocks = [(j0, min(n, j0 + nb)) for j0 in range(0, n, nb)]
-    if transpose:
-        blocks = blocks[::-1]
-    if slices and bufs is None:
-        bufs = (Slices(ctx, slices, n, nb), Slices(ctx, slices, nrhs, nb))
-    for j0, j1 in blocks:
-        w = j1 - j0
-        o = j0 + j0 * n
-        gemm(ctx, transpose, False, w, nrhs, w, 1.0, U, o, n, B, j0, n, 0.0, X, j0, n, 2 if transpose else 1)
-        if not transpose:
-            m, lo = n - j1, j1
-        else:
-            m, lo = j0, 0
-        if m == 0:
-            continue
-        if slices:
-            As, Bs = bufs
-            # A rows: L[j1:, j0:j1] rows (layout 1), or columns 0..j0-1 of L[j0:j1, :] (layout 0)
+    if slices and not transpose and getattr(f, "Lt", None) is None:
+        f.Lt = transpose_tri(ctx, L, n)
+    As = Bs = None
+
+    def update(r0: int, r1: int, k0: int, k1: int):
+        """B[r0:r1] -= L[r0:r1, k0:k1] X[k0:k1] (forward) or
+        B[r0:r1] -= L[k0:k1, r0:r1]^T X[k0:k1] (transpose)."""
+        nonlocal As, Bs
+        m, w = r1 - r0, k1 - k0
+        if m <= 0:
+            return
+        if not slices:
             if not transpose:
-                slice8(ctx, slices, 1, m, w, L, j1 + j0 * n, n, As)
+                gemm(ctx, False, False, m, nrhs, w, -1.0, L, r0 + k0 * n, n, X, k0, n, 1.0, B, r0, n)
             else:
-                slice8(ctx, slices, 0, m, w, L, j0, n, As)
-            slice8(ctx, slices, 0, nrhs, w, X, j0, n, Bs)
-            gemm_s8(ctx, m, nrhs, w, -1.0, As, 0, Bs, 0, 1.0, B, lo, n)
-        elif not transpose:
-            gemm(ctx, False, False, m, nrhs, w, -1.0, L, j1 + j0 * n, n, X, j0, n, 1.0, B, lo, n)
+                gemm(ctx, True, False, m, nrhs, w, -1.0, L, k0 + r0 * n, n, X, k0, n, 1.0, B, r0, n)
+            return
+        if As is None:
+            As, Bs = Slices(ctx, slices, n, outer), Slices(ctx, slices, nrhs, outer)
+        # rows r of the A operand: L[r, k0:k1] is column r of L^T; L[k0:k1, r] column r of L
+        src = f.Lt if not transpose else L
+        slice8(ctx, slices, 0, m, w, src, k0 + r0 * n, n, As)
+        slice8(ctx, slices, 0, nrhs, w, X, k0, n, Bs)
+        gemm_s8(ctx, m, nrhs, w, -1.0, As, 0, Bs, 0, 1.0, B, r0, n)
+
+    outers = [(o0, min(n, o0 + outer)) for o0 in range(0, n, outer)]
+    if transpose:
+        outers = outers[::-1]
+    for o0, o1 in outers:
+        inner = [(j0, min(o1, j0 + nb)) for j0 in range(o0, o1, nb)]
+        if transpose:
+            inner = inner[::-1]
+        for j0, j1 in inner:
+            w = j1 - j0
+            o = j0 + j0 * n
+            gemm(ctx, transpose, False, w, nrhs, w, 1.0, U, o, n, B, j0, n, 0.0, X, j0, n, 2 if transpose else 1)
+            if not transpose:
+                update(j1, o1, j0, j1)
+            else:
+                update(o0, j0, j0, j1)
+        if not transpose:
+            update(o1, n, o0, o1)
         else:
-            gemm(ctx, True, False, m, nrhs, w, -1.0, L, j0, n, X, j0, n, 1.0, B, lo, n)
+            update(0, o0, o0, o1)
     return X
+
+
+def transpose_tri(ctx: Context, L: torch.Tensor, n: int) -> torch.Tensor:
+    """L^T of a column-major n x n matrix (its rows become contiguous)."""
+    return transpose(ctx, L, n, n, n)

@@ -80,6 +80,11 @@ class Fisher:
             _check_filter(self.pixel_filter, self.inputs)
             if precision != "fp64":
                 raise ValueError("precision='mixed' covers the unfiltered path; a pixel filter runs in fp64")
+        # harmonic route in FP64: "whitener" (products with W = L^-1 D) or
+        # "trsm" (blocked solves with L, as the mixed precision does)
+        self.harmonic_solver = kwargs.get("harmonic_solver", "whitener")
+        if self.harmonic_solver not in ("whitener", "trsm"):
+            raise ValueError(f"harmonic_solver must be 'whitener' or 'trsm'; got {self.harmonic_solver!r}")
         self.mixed_gemm = kwargs.get("mixed_gemm", "bf16x3")
         if self.mixed_gemm not in pipeline.MIXED_GEMMS:
             raise ValueError(f"mixed_gemm must be one of {pipeline.MIXED_GEMMS}; got {self.mixed_gemm!r}")
@@ -122,7 +127,7 @@ class Fisher:
     def prepare_covariance_matrices(self):
         if self.cov is None:
             self.cov = pipeline.Covariance(self.ctx, self.inputs, pixel_filter=self.device_filter())
-        if self.basis == "harmonic" and self.precision == "mixed":
+        if self.basis == "harmonic" and (self.precision == "mixed" or self.harmonic_solver == "trsm"):
             from .harmonic import HarmonicContraction
 
             self.cov.build(diag_blocks=HarmonicContraction.TRSM_BLOCK)

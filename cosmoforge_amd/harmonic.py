@@ -142,10 +142,10 @@ class HarmonicContraction:
         if key not in self._Z:
             ctx, n, nt = cov.ctx, cov.n, self.nt
             Z = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-            if self.precision == "mixed":
+            if self._blocked(cov):
                 B = self.V.clone()
                 ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(B), n, P(cov.factor.d)), "scale")
-                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self.slices[0])
+                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self._s(0))
                 del B
             else:
                 engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
@@ -169,15 +169,24 @@ class HarmonicContraction:
         ctx, n, nt = cov.ctx, cov.n, self.nt
         Z = self._z(cov)
         R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
-        if self.precision == "mixed":
+        if self._blocked(cov):
             del self._Z[id(cov)]
-            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self.slices[2])
+            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self._s(2))
             del Z
             ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(R), n, P(cov.factor.d)), "scale")
             return R
         engine.gemm(ctx, True, False, n, nt, n, 1.0, cov.factor.W, 0, n, Z, 0, n, 0.0, R, 0, n, 2)
         del self._Z[id(cov)]
         return R
+
+    def _blocked(self, cov) -> bool:
+        """Z and R by blocked solves with L (the factor keeps L's diagonal
+        blocks inverted and no whitener), else by products with W."""
+        return cov.factor.W is None and getattr(cov.factor, "nb", 0) > 0
+
+    def _s(self, i: int) -> int:
+        """Slices of product i (0 when the product runs in FP64)."""
+        return self.slices[i] if self.precision == "mixed" else 0
 
     def _solve(self, cov, maps: np.ndarray) -> torch.Tensor:
         """y = C^-1 x = D L^-T L^-1 D x in FP64 (column-major n x nsims)."""
@@ -243,8 +252,8 @@ class HarmonicContraction:
                                                     0.5, P(q), self.npar), "harmonic_pairsum")
             return q.cpu().numpy(), np.zeros(self.npar)
         R = self._r(self.cov)
-        if self.precision == "mixed":
-            # u^T y from y = C^-1 x solved in FP64, rather than from the sliced R
+        if self._blocked(self.cov):
+            # u^T y from y = C^-1 x solved in FP64 (rather than from a sliced R)
             Y = self._solve(self.cov, maps)
             Ut = torch.empty((nt, ns), dtype=torch.float64, device=ctx.device)
             engine.gemm(ctx, True, False, ns, nt, n, 1.0, Y, 0, n, self.V, 0, n, 0.0, Ut, 0, ns)

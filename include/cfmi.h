@@ -234,13 +234,17 @@ int cfmi_slice8(cfmi_ctx *ctx, int s, int layout, int64_t rows, int64_t k, const
 /* The FP64 values the slices represent, dst[r ldd + k] (tests).            */
 int cfmi_unslice8(cfmi_ctx *ctx, int s, int64_t rows, int64_t k, const void *src, int64_t ldk, int64_t ps,
                   const double *mu, double *dst, int64_t ldd);
+/* dst (n x m) = src^T for a column-major FP64 m x n src (tiled copy).     */
+int cfmi_transpose(cfmi_ctx *ctx, int64_t m, int64_t n, const double *src, int64_t lds, double *dst,
+                   int64_t ldd);
 /* Largest k one launch accumulates exactly in int32 (s k 127^2 < 2^31).    */
 int64_t cfmi_s8_kmax(int s);
 /* C (m x n, column-major FP64) = alpha A B^T + beta C with A (m rows) and
  * B (n rows) sliced over the same k: the s(s+1)/2 slice products with
  * p + q < s on v_mfma_i32_16x16x64_i8 (exact int32 sums), levels combined
  * in FP64; k beyond cfmi_s8_kmax is split and the parts added in FP64.
- * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored.
+ * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored;
+ * bit 1: one LDS stage instead of two (s <= 4; s = 5 always uses one).
  * Serves the C^-1 V, V^T C^-1 V and C^-1 dC products that replace
  * matrix_mult at qube/qube/fisher.py:506-531 and spectra.py:1144-1161 in
  * the mixed precision (no reference counterpart of the arithmetic).       */

@@ -8,9 +8,9 @@ blocked triangular solve built on them (``engine.trsm``).
   int32 range.
 * The products stay within the slicing bound of the FP64 product, which a
   dropped level or a wrong scale would exceed.
-* The blocked solves equal SciPy's triangular solves to the bound of their
-  arithmetic (FP64: 1e-12; 4 or 5 slices: the slicing error times the growth
-  of the solve).
+* The blocked solves (one and two levels of blocking) equal SciPy's
+  triangular solves to the bound of their arithmetic (FP64: 1e-12; 4 or 5
+  slices: the slicing error times the growth of the solve).
 """
 
 import numpy as np
@@ -140,8 +140,8 @@ def test_gemm_s8_rejects_bad_arguments(ctx):
     assert bad < 0   # row pitch 64 below k = 200
 
 
-@pytest.mark.parametrize("slices", [0, 4, 5])
-def test_blocked_trsm(ctx, slices):
+@pytest.mark.parametrize("slices,outer", [(0, 2048), (4, 2048), (5, 2048), (0, 384), (4, 384)])
+def test_blocked_trsm(ctx, slices, outer):
     import torch
     from scipy.linalg import solve_triangular
 
@@ -156,11 +156,11 @@ def test_blocked_trsm(ctx, slices):
     Lh = np.tril(engine.to_numpy_cm(f.L))
     Bh = rng.standard_normal((n, nrhs))
     X = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
-    engine.trsm(ctx, f, n, ctx.upload_cm(Bh), X, nrhs, slices=slices)
+    engine.trsm(ctx, f, n, ctx.upload_cm(Bh), X, nrhs, slices=slices, outer=outer)
     want = solve_triangular(Lh, Bh, lower=True)
     err = np.max(np.abs(engine.to_numpy_cm(X) - want)) / np.max(np.abs(want))
     Y = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
-    engine.trsm(ctx, f, n, X.clone(), Y, nrhs, transpose=True, slices=slices)
+    engine.trsm(ctx, f, n, X.clone(), Y, nrhs, transpose=True, slices=slices, outer=outer)
     want2 = solve_triangular(Lh.T, want, lower=False)
     err2 = np.max(np.abs(engine.to_numpy_cm(Y) - want2)) / np.max(np.abs(want2))
     tol = {0: 1e-12, 4: 1e-7, 5: 1e-9}[slices]
