@@ -1,22 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark: Fisher + QML time-to-solution on one MI355X, mixed vs FP64.
 
-One step = one complete Fisher + QML solve of the nside-16 TQU, lmax-47
-configuration (n = 5952 rows, 276 parameters): pointings, S, C = S + N and
-its Cholesky, the 6 derivative stacks, the whitened derivatives, the Fisher
-Gram matrix, the QML estimates, noise bias and their F^-1 normalisation.
-Each precision (FP64, mixed) is timed over K steps after W warm-ups; the
-line reports the mixed-precision time as ``value``, the FP64 time, the
-speedup and the mixed-vs-FP64 errors.  Inputs (mask, spectra, beam, noise,
-one seeded map) are resident before the timed region.
+Default workload: nside 32 TQU, lmax 95, |b| > 20 deg (n = 24192 rows, 564
+parameters), beam 4 deg, white noise 0.5 / 1 uK^2, one seeded map of S + N.
+One step = one complete Fisher + QML solve with the inputs resident in HBM:
+pointings, S, C = S + N and its Cholesky factor, the harmonic-mode
+contraction of the traces (F from M = V^T C^-1 V, q from y = C^-1 x, the
+noise bias from C^-1 V), and the F^-1 normalisation.  The mixed precision
+runs the three large products on the int8 matrix cores with sliced operands
+(exact integer sums; the Cholesky, diagonal solves and y in FP64); the FP64
+precision runs the same algorithm with FP64 MFMA products.  Both are timed
+over K steps after W warm-ups; ``value`` is the mixed seconds per solve.
 
-    python bench.py [--steps K] [--warmup W] [--config 0|1|2] [--precisions fp64,mixed] [--no-cpu]
+Outside the timed loops: the pixel-space route of the reference's
+formulation (per-parameter whitened derivatives, FP64) once, as the accuracy
+anchor, and its mixed variant once; the batched PICSLike likelihood of the
+disc workload; the learned surrogate (trained on exact evaluations of that
+likelihood, then screening); and the CPU oracle on a bounded sample of the
+same workload, extrapolated (labelled as such).
+
+    python bench.py [--steps K] [--warmup W] [--config 0|1|2|3|4] [--no-pixel] [--no-picslike]
+                    [--no-surrogate] [--no-cpu]
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -26,15 +37,18 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error"
+INT8_MFMA_PEAK_TOPS = 5000.0       # MI355X dense int8 MFMA
 FP64_MFMA_PEAK_TFLOPS = 78.6
-FP32_MFMA_PEAK_TFLOPS = 157.3
-BF16_MFMA_PEAK_TFLOPS = 2500.0
-# HBM bytes per whitening launch, measured with rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE on this workload (each scaled by what the counter reads for
-# a 1 GiB copy: x2.0 and x1.0); not measured live.  Keyed by (config, kernel).
-WHITENING_TRAFFIC_BYTES_PER_LAUNCH = {(1, "f64"): 87.5e9, (1, "f32"): 44.0e9, (1, "bf16x3"): 41.8e9}
-# The three-piece kernel issues six BF16 MFMA products per FP32-accurate one.
-B3_PRODUCTS = 6
+# HBM bytes per gemm_s8 launch of the config-2 mixed solve, from rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE over one solve (each scaled by what
+# the counter reads for a plain 4 GiB copy), divided by the launches; not
+# measured live.  Keyed by config.
+S8_TRAFFIC_BYTES_PER_LAUNCH = {}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def make_map(ctx, cfg, seed):
@@ -53,10 +67,36 @@ def make_map(ctx, cfg, seed):
     return engine.to_numpy_cm(x) / f.d.cpu().numpy()[:, None]
 
 
-def cpu_baseline(cfg, maps, k_params: int):
-    """The oracle (reference formulation, FP64) on the host cores, on a
-    bounded sample: S, C^-1 and k parameters' C^-1 dC_b products and traces,
-    extrapolated to every parameter of the configuration."""
+def errors(res, ref) -> dict:
+    """Per-quantity differences of one solve against another.
+
+    F: |dF_ij| / sqrt(F_ii F_jj) (the diagonal: plain relative error).
+    C_l: |dC| / max(|C|, sigma), sigma = sqrt((F^-1)_ll): relative where the
+    estimate exceeds its error bar, in units of the error bar where it does
+    not (EB, TB and noise-dominated bins scatter around zero).  Noise bias:
+    |d nb_p| / sqrt(F_pp), the shift in units of the standard deviation of
+    q_p.  q: |dq_p| / sqrt(F_pp)."""
+    F, C, nb, q, sig = ref[:5]
+    Fm, Cm, nbm, qm = res[:4]
+    d = np.sqrt(np.diag(F))
+    E = np.abs(Fm - F) / np.outer(d, d)
+    C, Cm = np.ravel(C), np.ravel(Cm)
+    return {"F_entry_max": float(E.max()),
+            "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.diag(F))),
+            "Cl_rel_max": float(np.max(np.abs(Cm - C) / np.maximum(np.abs(C), sig))),
+            "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
+            "noise_bias_over_sigma_q_max": float(np.max(np.abs(nbm - nb) / d)),
+            "q_over_sigma_q_max": float(np.max(np.abs(qm - q) / d[None, :]))}
+
+
+def cpu_baseline(cfg, stride: int, k_params: int = 2):
+    """The oracle in the reference formulation (FP64, LAPACK + OpenMP C
+    recurrences) on the host cores, on a bounded sample of the workload:
+    every ``stride``-th active pixel of the same mask, spectra, beam and
+    noise (n_s = n / stride rows), timing S, C^-1, k parameters' dC_b and
+    C^-1 dC_b, one trace, and the QML's G = C^-1 N C^-1, y and per-parameter
+    contractions; each term is scaled to n by its operation count (pairs
+    n^2, dense products n^3, traces n^2) and to every parameter."""
     from threadpoolctl import threadpool_info
 
     from oracle import kernels as K
@@ -65,12 +105,24 @@ def cpu_baseline(cfg, maps, k_params: int):
 
     threads = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    lay = K.layout_from_mask(cfg.mask, cfg.params["spins"], cfg.params["labels"], cfg.params["nside"])
-    L = cfg.params["lmax_signal"]
+    p = cfg.params
+    mask = np.array(cfg.mask, dtype=np.float64)
+    if stride > 1:
+        act = np.nonzero(mask[:, 0] > 0.5)[0]
+        keep = np.zeros(mask.shape[0], dtype=bool)
+        keep[act[::stride]] = True
+        mask[~keep] = 0.0
+    lay = K.layout_from_mask(mask, p["spins"], p["labels"], p["nside"])
+    n_s, n = lay.n, cfg.n
+    L = p["lmax_signal"]
     beams = OP.beam_by_label(lay, cfg.beam, L)
     cls_s = OP.smoothed_cls(lay, cfg.cls, beams, L)
-    prm = OP.parameter_weights(lay, beams, cfg.params["lmin"], cfg.params["lmax"], L)
-    N = cfg.noise
+    prm = OP.parameter_weights(lay, beams, p["lmin"], p["lmax"], L)
+    # the workload's white noise levels (T first, then Q/U rows)
+    nd = np.concatenate([np.full(f.rows, cfg.noise_diag[0] if f.spin == 0 else cfg.noise_diag[-1])
+                         for f in lay.fields])
+    N = np.diag(nd)
+    x = np.random.default_rng(0).standard_normal((n_s, 1))
     t0 = time.perf_counter()
     S = K.signal_matrix(lay, cls_s, L)
     t_s = time.perf_counter() - t0
@@ -78,34 +130,50 @@ def cpu_baseline(cfg, maps, k_params: int):
     Cinv = LA.matrix_inverse_symm(N + S)
     t_inv = time.perf_counter() - t0
     picks = np.linspace(0, len(prm) - 1, k_params).astype(int)
-    A = []
-    t0 = time.perf_counter()
+    A, t_dc, t_prod = [], 0.0, 0.0
     for i in picks:
         fi, fj, mode, _, w = prm[i]
+        t0 = time.perf_counter()
         dC = K.binned_derivative(lay, fi, fj, mode, w)
+        t1 = time.perf_counter()
         A.append(Cinv @ dC)
-    t_par = (time.perf_counter() - t0) / len(picks)
+        t2 = time.perf_counter()
+        t_dc += t1 - t0
+        t_prod += t2 - t1
+    t_dc /= len(picks)
+    t_prod /= len(picks)
     t0 = time.perf_counter()
     K.matrix_trace(A[0], A[-1])
     t_tr = time.perf_counter() - t0
     t0 = time.perf_counter()
     G = Cinv @ (N @ Cinv)
-    y = Cinv @ maps
+    y = Cinv @ x
     t_g = time.perf_counter() - t0
-    del G, y
+    t0 = time.perf_counter()
+    for i in picks:     # QML per parameter: dC_b y and Tr(G dC_b) (dC_b rebuilt, cache_derivatives=False)
+        fi, fj, mode, _, w = prm[i]
+        dC = K.binned_derivative(lay, fi, fj, mode, w)
+        0.5 * float(y[:, 0] @ (dC @ y[:, 0]))
+        K.matrix_trace(G, dC)
+    t_qp = (time.perf_counter() - t0) / len(picks)
+    del G, y, A
     npar = len(prm)
-    total = t_s + t_inv + npar * t_par + npar * (npar + 1) / 2 * t_tr + t_g + npar * (t_par + t_tr)
+    r2, r3 = (n / n_s) ** 2, (n / n_s) ** 3
+    fisher = t_s * r2 + t_inv * r3 + npar * (t_dc * r2 + t_prod * r3) + npar * (npar + 1) / 2 * t_tr * r2
+    qml = t_g * r3 + npar * t_qp * r2
     blas = ",".join(sorted({d.get("internal_api", "?") for d in threadpool_info()})) or "unknown"
     return {
-        "value": round(total, 2), "unit": "s (extrapolated)", "cores": threads, "kind": "port",
-        "sample": (f"oracle S ({t_s:.2f}s) + C^-1 ({t_inv:.2f}s) + {len(picks)} of {npar} parameters "
-                   f"(dC_b and C^-1 dC_b {t_par:.2f}s each, one trace {t_tr:.3f}s) + QML G and y "
-                   f"({t_g:.2f}s); Fisher extrapolated x{npar}, traces x{npar * (npar + 1) // 2}, "
-                   f"QML x{npar}; BLAS {blas}"),
+        "value": round(fisher + qml, 1), "unit": "s per Fisher+QML solve (extrapolated)", "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle on every {stride}-th active pixel (n_s = {n_s} of {n}): S {t_s:.2f}s (x{r2:.0f}), "
+                   f"C^-1 {t_inv:.2f}s (x{r3:.0f}), {len(picks)} of {npar} parameters (dC_b {t_dc:.2f}s x{r2:.0f}, "
+                   f"C^-1 dC_b {t_prod:.2f}s x{r3:.0f}), one trace {t_tr:.4f}s (x{r2:.0f}, "
+                   f"x{npar * (npar + 1) // 2} pairs), QML G and y {t_g:.2f}s (x{r3:.0f}), per-parameter QML "
+                   f"{t_qp:.3f}s (x{r2:.0f}, x{npar}); BLAS {blas}"),
     }
 
 
-def picslike_leg(ctx, log):
+def picslike_leg(ctx):
     """Config-3 batched exact likelihood: evals/s of the persistent batched
     kernel (mixed and FP64, eager and graph replay) against the plain
     per-point GPU path (the drop-in PICSLike point by point, FP64)."""
@@ -163,7 +231,7 @@ def picslike_leg(ctx, log):
     return out, (patches, tun)
 
 
-def surrogate_leg(ctx, patches, tun, log, n_train=20000, n_cand=65536):
+def surrogate_leg(ctx, patches, tun, plain_rate, n_train=20000, n_cand=65536):
     """Config 4: 4-member BF16 MLP ensemble trained on exact joint -2 ln L of
     the config-3 patches; 65,536 candidates screened (spread > 0.5 or within
     25 of the best prediction -> exact)."""
@@ -205,13 +273,14 @@ def surrogate_leg(ctx, patches, tun, log, n_train=20000, n_cand=65536):
     exact_chk = like(cand[chk], graph=True) if chk.size else np.zeros(0)
     ex = rep.exact
     like.close()
-    jobs_rate = len(patches) * len(ex) / max(t_screen - t_inf, 1e-9)
     return {"train_points": n_train, "train_data_s": round(t_data, 2), "train_s": round(t_train, 2),
             "candidates": n_cand, "inference_s": round(t_inf, 4), "screen_s": round(t_screen, 2),
             "screened_points_per_s": round(n_cand / t_screen, 1),
             "exhaustive_batched_points_per_s": round(n_train / t_data, 1),
+            "plain_per_point_points_per_s": round(plain_rate / len(patches), 2),
+            "speedup_screened_vs_plain_per_point": round(n_cand / t_screen / (plain_rate / len(patches)), 1),
             "exact_fraction": round(len(ex) / n_cand, 4), "fallback_by_spread": int(len(rep.fallback)),
-            "rejected": int(rej.size), "patch_jobs_per_s_in_screen": round(jobs_rate, 1),
+            "rejected": int(rej.size),
             "surrogate_error_on_rejected_sample": {
                 "n": int(chk.size),
                 "max_abs": float(np.max(np.abs(mu[chk] - exact_chk))) if chk.size else None,
@@ -219,24 +288,18 @@ def surrogate_leg(ctx, patches, tun, log, n_train=20000, n_cand=65536):
                 "min_exact_minus_best_kept": float(np.min(exact_chk) - np.min(rep.value[ex])) if chk.size else None}}
 
 
-def rel(a, b) -> float:
-    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4],
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4],
                     help="0-2: Fisher+QML workloads; 3: batched likelihood; 4: surrogate-screened likelihood")
+    ap.add_argument("--no-pixel", action="store_true", help="skip the pixel-route anchor solves")
     ap.add_argument("--no-picslike", action="store_true", help="skip the batched-likelihood leg")
-    ap.add_argument("--precisions", default="fp64,mixed", help="comma list of fp64 / mixed")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--no-harmonic", action="store_true", help="skip the harmonic-mode (basis='harmonic') leg")
-    ap.add_argument("--cpu-params", type=int, default=2)
-    ap.add_argument("--mixed-gemm", default="bf16x3", choices=["bf16x3", "fp32"],
-                    help="whitening arithmetic of the mixed precision")
+    ap.add_argument("--cpu-stride", type=int, default=0, help="CPU sample: every k-th active pixel (0: auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -245,17 +308,13 @@ def main():
 
     import torch
 
-    from cosmoforge_amd import Fisher, Spectra, engine, pipeline, workloads
+    from cosmoforge_amd import Fisher, Spectra, engine, workloads
 
     ctx = engine.Context(0)
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
     if args.config in (3, 4):
-        pl, (patches, tun) = picslike_leg(ctx, log)
-        out = {"metric": "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error",
-               "n_gpus": 1, "steps": 3, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+        pl, (patches, tun) = picslike_leg(ctx)
+        out = {"metric": METRIC, "n_gpus": 1, "steps": 3, "warmup": 1, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "data": "synthetic (seeded discs, maps and parameter draws)"}
         if args.config == 3:
             out.update({"value": pl["mixed_graph_evals_per_s"], "unit": "likelihood evaluations/s",
@@ -263,138 +322,127 @@ def main():
                         "dtype": "f32 MFMA (FP64 diagonal blocks, panels, ln|C|, refinement)",
                         "config": {"workload": pl["workload"], "jobs": pl["jobs"]}, "picslike": pl})
         else:
-            sl = surrogate_leg(ctx, patches, tun, log)
+            sl = surrogate_leg(ctx, patches, tun, pl["plain_per_point_evals_per_s"])
             out.update({"value": sl["screened_points_per_s"], "unit": "parameter points/s (joint over 64 patches)",
                         "ms_per_step": round(1e3 * sl["screen_s"], 2), "dtype": "bf16 MFMA surrogate + f32 MFMA exact",
                         "config": {"workload": "nside16_QU_64discs_surrogate", "candidates": sl["candidates"]},
-                        "surrogate": sl, "picslike": pl,
-                        "speedup_screened_vs_plain_per_point": round(
-                            sl["screened_points_per_s"] / (pl["plain_per_point_evals_per_s"] / 64.0), 1)})
+                        "surrogate": sl, "picslike": pl})
         print(json.dumps(out))
         return
 
     cfg = workloads.config(args.config)
     maps = make_map(ctx, cfg, workloads.SEED0 + args.config)
-    print(f"[bench] config {args.config} ({cfg.name}): n={cfg.n}", file=sys.stderr, flush=True)
-
+    log(f"[bench] config {args.config} ({cfg.name}): n={cfg.n}")
     # Inputs resident in HBM before the timed region: the (n, n) noise
-    # covariance is uploaded once (the PCIe-inclusive figure adds ~n^2 8 B
-    # per solve, ~35 ms at n = 5952).
+    # covariance is uploaded once (a host array would add n^2 8 B of PCIe
+    # per solve).
     noise_dev = torch.as_tensor(cfg.noise).to(ctx.device)
 
-    def step(prec, timer=None, lt=None, basis=False):
-        f = Fisher(cfg.params, basis=basis, mask=cfg.mask, noise_cov1=noise_dev, cls_data=cfg.cls,
-                   beam=cfg.beam, context=ctx, timer=timer, launch_timer=lt, precision=prec,
-                   mixed_gemm=args.mixed_gemm)
+    def solve(prec, basis="harmonic", solver="trsm"):
+        f = Fisher(cfg.params, basis=basis, mask=cfg.mask, noise_cov1=noise_dev, cls_data=cfg.cls, beam=cfg.beam,
+                   context=ctx, precision=prec, harmonic_solver=solver)
         f.run()
         s = Spectra(fisher=f, maps1=maps)
         s.run()
-        out = (f.get_fisher_matrix(), s.get_power_spectra(), s.qml_noise_bias, s.qml_results,
-               f.get_error_bars())
+        out = (f.get_fisher_matrix(), s.get_power_spectra(), s.qml_noise_bias, s.qml_results, f.get_error_bars())
         if basis == "harmonic":
             out = out + (f.derivs.nt, f.derivs.flops)
         f.release()
         del f, s
         return out
 
-    runs = {}
-    for prec in args.precisions.split(","):
+    def timed(prec, record=False):
         for w in range(args.warmup):
-            step(prec)
-            print(f"[bench] {prec} warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+            solve(prec)
+            log(f"[bench] {prec} warmup {w + 1}/{args.warmup} done")
+        rec = engine.S8Recorder() if record else None
         torch.cuda.synchronize()
-        timer = pipeline.StageTimer()
-        lt = pipeline.LaunchTimer()
+        engine.S8_RECORDER = rec
         t0 = time.perf_counter()
         for k in range(args.steps):
-            res = step(prec, timer, lt)
-            print(f"[bench] {prec} step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+            res = solve(prec)
+            log(f"[bench] {prec} step {k + 1}/{args.steps} done")
         torch.cuda.synchronize()
-        per_step = (time.perf_counter() - t0) / args.steps
-        runs[prec] = dict(s=per_step, res=res, timer=timer, lt=lt)
+        dt = (time.perf_counter() - t0) / args.steps
+        engine.S8_RECORDER = None
+        return dt, res, rec
 
-    main_prec = "mixed" if "mixed" in runs else "fp64"
-    r = runs[main_prec]
-    timer, lt = r["timer"], r["lt"]
-    white_ms = lt.total_ms() / args.steps
-    launches = len(lt.launches) / args.steps
-    achieved = timer.flops / (white_ms * 1e-3) / 1e12
-    b3 = main_prec == "mixed" and args.mixed_gemm == "bf16x3"
-    dt = "bf16x3" if b3 else ("f32" if main_prec == "mixed" else "f64")
-    peak = {"bf16x3": BF16_MFMA_PEAK_TFLOPS, "f32": FP32_MFMA_PEAK_TFLOPS, "f64": FP64_MFMA_PEAK_TFLOPS}[dt]
-    kernel = {"bf16x3": "gemm_b3_kernel (three-piece BF16 operands, six BF16 MFMA products per FP32-accurate "
-                        "product; whitened derivatives W dC_b W^T, batched over bins)",
-              "f32": "gemm_kernel<float> (whitened derivatives W dC_b W^T, batched over bins)",
-              "f64": "gemm_kernel<double> (whitened derivatives W dC_b W^T, batched over bins)"}[dt]
-    F = r["res"][0]
+    t_mixed, res_m, rec = timed("mixed", record=True)
+    t_fp64, res_f, _ = timed("fp64")
+    n_modes, flops = res_m[5], res_m[6]
+    fl, ms, launches, prod = rec.total()
+    achieved = fl / (ms * 1e-3) / 1e12
     out = {
-        "metric": "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error",
-        "value": round(r["s"], 4),
+        "metric": METRIC,
+        "value": round(t_mixed, 4),
         "unit": "s per Fisher+QML solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["s"] * 1e3, 2),
+        "ms_per_step": round(t_mixed * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("bf16x3 MFMA at f32 accuracy" if b3 else "f32 MFMA") +
-                 " (FP64 recurrences, factorisation, Gram accumulation, QML)" if main_prec == "mixed" else "f64",
+        "dtype": "int8 MFMA on sliced FP64 operands (exact int32 sums) for C^-1 V, V^T C^-1 V, C^-1 x; FP64 "
+                 "Cholesky, diagonal solves, recurrences, y = C^-1 x",
         "data": "synthetic (seeded map of S+N; reference-recipe spectra, mask, beam, noise)",
-        "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(F.shape[0]),
-                   "lmax_signal": cfg.params["lmax_signal"], "nsims": 1, "precision": main_prec,
-                   "mixed_gemm": args.mixed_gemm if main_prec == "mixed" else None,
-                   "row_bands": len(timer.bands)},
-        "stages_ms": {"deriv_stacks": round(timer.elapsed_ms("deriv0", "white0") / args.steps, 2),
-                      "whitened_derivatives": round(timer.elapsed_ms("white0", "white1") / args.steps, 2),
-                      "fisher_gram": round(timer.elapsed_ms("white1", "gram1") / args.steps, 2)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4),
-                     "traffic": WHITENING_TRAFFIC_BYTES_PER_LAUNCH.get((args.config, dt)),
-                     "kernel": kernel,
-                     "launches_per_step": launches, "flops_per_launch": timer.flops / max(launches, 1),
-                     "ms_per_launch": round(white_ms / max(launches, 1), 3)},
+        "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(res_m[0].shape[0]),
+                   "lmax_signal": cfg.params["lmax_signal"], "nsims": 1, "n_modes": int(n_modes),
+                   "route": "harmonic-mode contraction (basis='harmonic'), blocked triangular solves",
+                   "precision": "mixed"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_MFMA_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / INT8_MFMA_PEAK_TOPS, 4),
+                     "traffic": S8_TRAFFIC_BYTES_PER_LAUNCH.get(args.config),
+                     "kernel": "gemm_s8_kernel (FP64-accurate products on v_mfma_i32_16x16x64_i8: sliced operands, "
+                               "s(s+1)/2 int8 products per multiply-add, exact int32 sums)",
+                     "launches_per_step": launches / args.steps, "flops_per_launch": fl / max(launches, 1),
+                     "ms_per_launch": round(ms / max(launches, 1), 3),
+                     "int8_products_per_flop": round(prod, 2),
+                     "mfma_issue_frac": round(prod * achieved / INT8_MFMA_PEAK_TOPS, 4),
+                     "vs_fp64_mfma_peak": round(achieved / FP64_MFMA_PEAK_TFLOPS, 3)},
+        "mixed_s": round(t_mixed, 4),
+        "fp64_s": round(t_fp64, 4),
+        "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "mixed_error_vs_fp64": errors(res_m, res_f),
+        "harmonic_gemm_flops_per_solve": flops,
     }
-    if b3:
-        # achieved counts the FP32-equivalent (algorithmic) flops; the matrix
-        # cores execute six BF16 products for each of them.
-        out["roofline"]["mfma_products_per_flop"] = B3_PRODUCTS
-        out["roofline"]["mfma_issue_frac"] = round(B3_PRODUCTS * achieved / peak, 4)
-    for prec, rr in runs.items():
-        out[f"{prec}_s"] = round(rr["s"], 4)
-    if "fp64" in runs and "mixed" in runs:
-        (F, C, nb, q, sig), (Fm, Cm, nbm, qm, _) = runs["fp64"]["res"], runs["mixed"]["res"]
-        out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["s"] / runs["mixed"]["s"], 3)
-        out["mixed_error_vs_fp64"] = {
-            "F_maxrel": rel(Fm, F), "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.abs(np.diag(F)))),
-            "Cl_maxrel": rel(Cm, C), "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
-            "noise_bias_maxrel": rel(nbm, nb), "qml_raw_maxrel": rel(qm, q), "cond_F": float(np.linalg.cond(F))}
-    if not args.no_harmonic and args.config < 3:
-        # the same estimator with its traces contracted over harmonic modes
-        # (FP64): M = (W V)^T (W V) and mode reductions instead of the
-        # per-parameter whitening products
-        step("fp64", basis="harmonic")
+    log("[bench] harmonic legs done")
+    # hand the cached device memory back before each leg (the library's own
+    # allocations, e.g. the batched likelihood's workspace, do not see it)
+    torch.cuda.empty_cache()
+    if not args.no_pixel:
+        # the reference's formulation (pixel-space whitened derivatives), once
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(args.steps):
-            hres = step("fp64", basis="harmonic")
+        res_p = solve("fp64", basis=False)
         torch.cuda.synchronize()
-        hs = (time.perf_counter() - t0) / args.steps
-        print("[bench] harmonic steps done", file=sys.stderr, flush=True)
-        Fh, Ch, nbh, qh, _, nmodes, hflops = hres
-        hleg = {"s": round(hs, 4), "modes": int(nmodes), "gemm_flops_per_solve": hflops,
-                "speedup_vs_pixel": {p: round(rr["s"] / hs, 2) for p, rr in runs.items()}}
-        if "fp64" in runs:
-            F, C, nb, q, sig = runs["fp64"]["res"]
-            hleg["error_vs_pixel_fp64"] = {
-                "F_maxrel": rel(Fh, F),
-                "F_diag_rel_max": float(np.max(np.abs(np.diag(Fh) - np.diag(F)) / np.abs(np.diag(F)))),
-                "Cl_over_sigma_max": float(np.max(np.abs(Ch - C) / sig)), "noise_bias_maxrel": rel(nbh, nb)}
-        out["harmonic_fp64"] = hleg
+        t_p = time.perf_counter() - t0
+        log(f"[bench] pixel-route FP64 solve done ({t_p:.1f} s)")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res_pm = solve("mixed", basis=False)
+        torch.cuda.synchronize()
+        t_pm = time.perf_counter() - t0
+        log(f"[bench] pixel-route mixed solve done ({t_pm:.1f} s)")
+        out["pixel_route"] = {
+            "fp64_s": round(t_p, 3), "mixed_s": round(t_pm, 3), "speedup_mixed_vs_fp64": round(t_p / t_pm, 3),
+            "mixed_error_vs_pixel_fp64": errors(res_pm, res_p),
+            "harmonic_fp64_error_vs_pixel_fp64": errors(res_f, res_p),
+            "harmonic_mixed_error_vs_pixel_fp64": errors(res_m, res_p),
+            "note": "one solve each, outside the timed loops; the pixel route carries the reference's acos rotation "
+                    "angles, so the routes agree to that level"}
+        del res_p, res_pm
+        torch.cuda.empty_cache()
     if not args.no_picslike:
-        out["picslike"] = picslike_leg(ctx, log)[0]
-    if not args.no_cpu and args.config < 2:
-        out["cpu_baseline"] = cpu_baseline(cfg, maps, args.cpu_params)
+        pl, (patches, tun) = picslike_leg(ctx)
+        out["picslike"] = pl
+        if not args.no_surrogate:
+            out["surrogate"] = surrogate_leg(ctx, patches, tun, pl["plain_per_point_evals_per_s"])
+            log("[bench] surrogate leg done")
+    if not args.no_cpu:
+        stride = args.cpu_stride or {0: 1, 1: 1, 2: 4}[args.config]
+        out["cpu_baseline"] = cpu_baseline(cfg, stride)
+        log("[bench] cpu baseline done")
     print(json.dumps(out))
 
 

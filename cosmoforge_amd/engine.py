@@ -706,6 +706,28 @@ def slice8(ctx: Context, s: int, layout: int, rows: int, k: int, src: torch.Tens
     return out
 
 
+class S8Recorder:
+    """HIP events (torch's, on the context's stream) around every
+    ``gemm_s8`` launch while installed as ``engine.S8_RECORDER``, with the
+    algorithmic flops 2 m n k (m n k for a symmetric result, whose upper
+    tiles are not computed) and the slice count of each launch."""
+
+    def __init__(self):
+        self.launches = []
+
+    def total(self):
+        """(sum of algorithmic flops, sum of launch times in ms, launches,
+        int8 products issued per algorithmic multiply-add, averaged)."""
+        torch.cuda.synchronize()
+        fl = sum(f for _, _, f, _ in self.launches)
+        ms = sum(a.elapsed_time(b) for a, b, _, _ in self.launches)
+        prod = sum(f * s * (s + 1) / 2 for _, _, f, s in self.launches) / max(fl, 1.0)
+        return fl, ms, len(self.launches), prod
+
+
+S8_RECORDER: S8Recorder | None = None
+
+
 def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row0: int, B: Slices, b_row0: int,
             beta: float, C: torch.Tensor, offc: int, ldc: int, flags: int = 0):
     """C[m x n] (column-major FP64, at offc) = alpha A B^T + beta C over rows
@@ -714,9 +736,18 @@ def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row
         return
     if A.s != B.s:
         raise ValueError("operands sliced with different slice counts")
+    rec = S8_RECORDER
+    if rec is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
     ctx.check(ctx.lib.cfmi_gemm_s8(ctx.h, A.s, m, n, k, alpha, P(A.data, a_row0 * A.ldk), A.ldk, A.ps,
                                    P(A.mu, a_row0), P(B.data, b_row0 * B.ldk), B.ldk, B.ps, P(B.mu, b_row0), beta,
                                    P(C, offc), ldc, flags), "gemm_s8")
+    if rec is not None:
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        flops = (1.0 if flags & 1 else 2.0) * m * n * k
+        rec.launches.append((e0, e1, flops, A.s))
 
 
 def transpose(ctx: Context, A: torch.Tensor, m: int, n: int, lda: int) -> torch.Tensor:
