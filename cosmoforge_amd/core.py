@@ -90,7 +90,40 @@ def read_params(params) -> SimpleNamespace:
     if p.lmax is not None and p.lmax < p.lmin:
         raise ValueError(f"lmax={p.lmax} < lmin={p.lmin}")
     p.do_cross = bool(p.do_cross)
+    p.smoothing_type = normalize_smoothing_type(p.smoothing_type)
     return p
+
+
+SMOOTHING_TYPES = ("none", "gaussian", "cosine_legacy", "cosine_npipe", "file")
+# legacy integer codes of ``smoothing_type`` (``settings.py:353-380``)
+SMOOTHING_CODES = {0: "none", 1: "gaussian", 2: "cosine_legacy", 3: "file"}
+
+
+def normalize_smoothing_type(value) -> str:
+    """A canonical ``smoothing_type``: the names above in any case, the
+    deprecated alias "cosine" (= "cosine_legacy") and the deprecated integer
+    codes 0-3, each alias with a DeprecationWarning (``settings.py:353-380``)."""
+    import warnings
+
+    if isinstance(value, bool):
+        raise ValueError(f"Unknown smoothing_type {value!r}")
+    if isinstance(value, (int, float, np.integer, np.floating)):
+        code = int(value)
+        if code != value or code not in SMOOTHING_CODES:
+            raise ValueError(f"Unknown smoothing_type={value!r}; integer codes are "
+                             + ", ".join(f"{k}={v}" for k, v in SMOOTHING_CODES.items()))
+        warnings.warn(f"integer smoothing_type={code} is deprecated; use '{SMOOTHING_CODES[code]}'",
+                      DeprecationWarning, stacklevel=3)
+        return SMOOTHING_CODES[code]
+    if isinstance(value, str):
+        key = value.strip().lower()
+        if key == "cosine":
+            warnings.warn("smoothing_type='cosine' is deprecated; use 'cosine_legacy' or 'cosine_npipe'",
+                          DeprecationWarning, stacklevel=3)
+            return "cosine_legacy"
+        if key in SMOOTHING_TYPES:
+            return key
+    raise ValueError(f"Unknown smoothing_type {value!r}; expected one of {', '.join(SMOOTHING_TYPES)}")
 
 
 def resolve_basis(basis):

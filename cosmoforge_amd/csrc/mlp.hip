@@ -241,19 +241,28 @@ __global__ void loss_kernel(int B, const float* y, const float* T, const int* id
 // delta3 = dy W4 SiLU'(z3) (BF16) for batch rows of chunk blockIdx.x; the
 // chunk's partial sums of gW4[j] = sum_b dy_b h3[b][j] and gb4 = sum_b dy_b
 // go to part[e][chunk][0..256] (reduced in a fixed order afterwards).
-constexpr int RCH = 64;  // batch rows per reduction chunk
+// Chunks are short (RCH rows, B / RCH blocks per member) so that the batch
+// spreads over the CUs; each chunk's dy (and, in colsum_kernel, its input
+// rows) is staged in LDS first, so the row loop issues only independent,
+// coalesced loads.
+constexpr int RCH = 8;  // batch rows per reduction chunk
 
 __global__ void l4_bwd_kernel(int B, const float* dy, const bf16* h3, const float* z3, int64_t sAct, const bf16* wb,
                               bf16* d3, float* part, int nchunk)
 {
+    __shared__ float g_s[RCH];
     const int e = blockIdx.y, c = blockIdx.x;
     const int j = threadIdx.x;  // column (256 threads)
+    const int b0 = c * RCH, nb = min(RCH, B - b0);
+    if (j < nb) g_s[j] = dy[(int64_t)e * B + b0 + j];
+    __syncthreads();
     const float w4 = bf2f(wb[e * NPAR + OFF_W4 + j]);
     float gw = 0.0f, gb = 0.0f;
-    const int b1 = min(B, (c + 1) * RCH);
-    for (int b = c * RCH; b < b1; ++b) {
-        const float g = dy[(int64_t)e * B + b];
-        const int64_t o = e * sAct + (int64_t)b * H + j;
+#pragma unroll
+    for (int r = 0; r < RCH; ++r) {
+        if (r >= nb) break;
+        const float g = g_s[r];
+        const int64_t o = e * sAct + (int64_t)(b0 + r) * H + j;
         gw += g * bf2f(h3[o]);
         gb += g;
         d3[o] = f2bf(g * w4 * dsilu(z3[o]));
@@ -268,17 +277,25 @@ __global__ void l4_bwd_kernel(int B, const float* dy, const bf16* h3, const floa
 __global__ void colsum_kernel(int B, const bf16* d, int64_t sAct, const float* X, const int* idx, float* part,
                               int nchunk)
 {
+    __shared__ float x_s[RCH][NIN];
     const int e = blockIdx.y, c = blockIdx.x;
     const int j = threadIdx.x;
+    const int b0 = c * RCH, nb = min(RCH, B - b0);
+    if (X && j < nb * NIN) {
+        const int r = j / NIN, k = j % NIN;
+        const int row = idx ? idx[(int64_t)e * B + b0 + r] : b0 + r;
+        x_s[r][k] = X[(int64_t)row * NIN + k];
+    }
+    __syncthreads();
     float sb = 0.0f, sw[NIN] = {0.0f, 0.0f, 0.0f};
-    const int b1 = min(B, (c + 1) * RCH);
-    for (int b = c * RCH; b < b1; ++b) {
-        const float v = bf2f(d[e * sAct + (int64_t)b * H + j]);
+#pragma unroll
+    for (int r = 0; r < RCH; ++r) {
+        if (r >= nb) break;
+        const float v = bf2f(d[e * sAct + (int64_t)(b0 + r) * H + j]);
         sb += v;
         if (X) {
-            const int row = idx ? idx[(int64_t)e * B + b] : b;
 #pragma unroll
-            for (int k = 0; k < NIN; ++k) sw[k] += v * X[(int64_t)row * NIN + k];
+            for (int k = 0; k < NIN; ++k) sw[k] += v * x_s[r][k];
         }
     }
     float* pp = part + ((int64_t)e * nchunk + c) * (4 * H);

@@ -173,10 +173,16 @@ def signal_matrix(ctx: Context, layout: Layout, vecs: list, cls: dict, lmax: int
                     raise NotImplementedError("Cross-correlation for spin-2 fields not implemented yet.")
                 ctx.check(lib.cfmi_signal_22(h, P(cls[lab(0)]), P(cls[lab(1)]), P(cls[lab(2)]), lmax,
                                              P(vecs[i]), fi.npix, P(S, ri + ri * n), n), "signal_22")
+            elif fi.spin == 2:
+                # A spin-2 field listed before a spin-0 field: the reference
+                # writes this block above the diagonal and its final
+                # lower-to-upper copy then overwrites it with the (unwritten)
+                # block below, so its S carries no T x P correlation
+                # (signal_kernels.py:489-499 with linalg.py:101-120).  Kept
+                # for parity: both blocks are zero.
+                ctx.check(lib.cfmi_fill(h, F64, fj.rows, fi.rows, P(S, rj + ri * n), n, 0.0), "fill")
+                ctx.check(lib.cfmi_fill(h, F64, fi.rows, fj.rows, P(S, ri + rj * n), n, 0.0), "fill")
             else:
-                if fi.spin == 2:
-                    raise NotImplementedError(
-                        "a spin-2 field listed before a spin-0 field is not supported; list spin-0 first")
                 ctx.check(lib.cfmi_signal_02(h, P(cls[lab(0)]), P(cls[lab(1)]), lmax, P(vecs[i]), fi.npix,
                                              P(vecs[j]), fj.npix, P(S, rj + ri * n), n, P(S, ri + rj * n), n),
                           "signal_02")
@@ -214,10 +220,11 @@ def block_geometry(layout: Layout) -> list:
         fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
         if fi.spin == 2 and fj.spin == 2 and sp.fi != sp.fj:
             raise NotImplementedError("spin-2 x spin-2 cross-field derivative")
-        if fi.spin == 2 and fj.spin == 0:
-            raise NotImplementedError("spin-2 field before spin-0 field")
         if sp.fi == sp.fj:
             out.append(Block(fi.offset, fi.rows, fi.offset, fi.rows))
+        elif fi.spin == 2 and fj.spin == 0:
+            # P rows x T columns, as for a spin-0 field listed first
+            out.append(Block(fi.offset, fi.rows, fj.offset, fj.rows))
         else:
             out.append(Block(fj.offset, fj.rows, fi.offset, fi.rows))
     return out
@@ -229,10 +236,10 @@ def deriv_state(ctx: Context, layout: Layout, group_spectrum: int) -> torch.Tens
     fi, fj = layout.fields[sp.fi], layout.fields[sp.fj]
     if sp.fi == sp.fj:
         k = ctx.lib.cfmi_deriv_state_doubles(fi.npix, fi.npix, 1)
-    elif fi.spin == 0 and fj.spin == 0:
+    elif fi.spin == 0:
         k = ctx.lib.cfmi_deriv_state_doubles(fj.npix, fi.npix, 0)
-    else:
-        k = ctx.lib.cfmi_deriv_state_doubles(fj.npix, fi.npix, 0)
+    else:   # a spin-2 field listed before the spin-0 one: T first, as the 0 x 2 pass runs
+        k = ctx.lib.cfmi_deriv_state_doubles(fi.npix, fj.npix, 0)
     return torch.empty(int(k), dtype=torch.float64, device=ctx.device)
 
 
@@ -306,7 +313,7 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
                 blk.mirror = mr
             blk.stack = st
             done.add(k)
-        elif fi.spin == 2:
+        elif fi.spin == 2 and fj.spin == 2:
             # EE, BB, EB of this field: one pass, only the requested planes.
             members = [m for m, s2 in enumerate(specs) if s2.fi == sp.fi and s2.fj == sp.fj]
             sts = [None, None, None]
@@ -323,21 +330,24 @@ def deriv_stacks(ctx: Context, layout: Layout, vecs: list, weights: dict, lmax: 
                     out[m].stack = sts[specs[m].mode]
                     done.add(m)
         else:
+            # TE, TB: the spin-0 field (listed first or second) and the spin-2 one
+            it, ip = (sp.fi, sp.fj) if fi.spin == 0 else (sp.fj, sp.fi)
+            ft, fp = layout.fields[it], layout.fields[ip]
             members = [m for m, s2 in enumerate(specs) if s2.fi == sp.fi and s2.fj == sp.fj]
             sts = [None, None]
             mrs = [None, None]
             for m in members:
                 if m in want:
-                    sts[specs[m].mode] = alloc((nb, fi.rows, fj.rows))
+                    sts[specs[m].mode] = alloc((nb, ft.rows, fp.rows))
                     if b3 and mirrors:
-                        mrs[specs[m].mode] = alloc((nb, fj.rows, fi.rows))
+                        mrs[specs[m].mode] = alloc((nb, fp.rows, ft.rows))
             labs = [layout.spectra_map[(sp.fi, sp.fj, md)] for md in range(2)]
             has_m = mrs[0] is not None or mrs[1] is not None
             ctx.check(lib.cfmi_deriv_stack_02(h, dcode, P(weights[labs[0]]), P(weights[labs[1]]), lmax, P(lo),
-                                              P(hi), nb, P(vecs[sp.fi]), fi.npix, P(vecs[sp.fj]), fj.npix,
-                                              P(sts[0]), P(sts[1]), fj.rows, fi.rows * fj.rows, P(mrs[0]),
-                                              P(mrs[1]), fi.rows if has_m else 0,
-                                              fi.rows * fj.rows if has_m else 0, P(state), lresume),
+                                              P(hi), nb, P(vecs[it]), ft.npix, P(vecs[ip]), fp.npix,
+                                              P(sts[0]), P(sts[1]), fp.rows, ft.rows * fp.rows, P(mrs[0]),
+                                              P(mrs[1]), ft.rows if has_m else 0,
+                                              ft.rows * fp.rows if has_m else 0, P(state), lresume),
                       "deriv_stack_02")
             for m in members:
                 if m in want:

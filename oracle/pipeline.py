@@ -2,8 +2,9 @@
 
 Restates the reference's traditional pixel-space path:
 
-* beams and smoothing -- ``src/cosmoforge.cosmocore/cosmocore/beam.py:128-414``
-  and ``spectra_io.py:446-507`` (C_l <- C_l * B1(l) B2(l));
+* smoothing -- ``src/cosmoforge.cosmocore/cosmocore/beam.py:223-414`` and
+  ``spectra_io.py:446-507`` (C_l <- C_l * B1(l) B2(l)) of a given (3, L+1)
+  beam array (every oracle call passes the beam explicitly);
 * Fisher -- ``src/cosmoforge.qube/qube/fisher.py:311-344`` (C = N + S, C^-1),
   ``:506-531`` (A_i = C^-1 dC_i) and ``:557-603`` (F_ij = 1/2 Tr(A_i A_j)),
   with the beam-smoothing rows of ``:790-811``;
@@ -27,27 +28,6 @@ import numpy as np
 
 from . import kernels as K
 from .linalg import matrix_inverse_symm, matrix_slogdet_symm
-
-
-def gauss_beam(fwhm_rad: float, lmax: int) -> np.ndarray:
-    """(3, lmax+1) T, E, B Gaussian windows with healpy's polarisation factor
-    exp(2 sigma^2) on E and B (``beam.py:180-190``)."""
-    sigma = fwhm_rad / np.sqrt(8.0 * np.log(2.0))
-    ell = np.arange(lmax + 1)
-    g = np.exp(-0.5 * ell * (ell + 1) * sigma**2)
-    pol = np.exp(2.0 * sigma**2)
-    return np.vstack([g, g * pol, g * pol])
-
-
-def coswinbeam(nside: int, ell1=None, ell2=None) -> np.ndarray:
-    """Raised-cosine window of ``beam.py:27-74``."""
-    ell1 = nside if ell1 is None else ell1
-    ell2 = 3 * nside if ell2 is None else ell2
-    b = np.zeros(4 * nside + 1)
-    b[: ell1 + 1] = 1.0
-    ell = np.arange(ell1 + 1, ell2 + 1)
-    b[ell1 + 1: ell2 + 1] = 0.5 * (1.0 + np.cos((ell - ell1) * np.pi / (ell2 - ell1)))
-    return b
 
 
 def beam_by_label(layout: K.Layout, beam3: np.ndarray, lmax: int) -> dict:
@@ -144,22 +124,6 @@ def fisher(layout: K.Layout, cls_s: dict, lmax_signal: int, N: np.ndarray, param
             F[i, j] = v
             F[j, i] = v
     return F, Cinv, S
-
-
-def fisher_rows(layout: K.Layout, Cinv: np.ndarray, params, rows) -> np.ndarray:
-    """Selected full rows of F via E_i = C^-1 dC_i C^-1, F_ij = 1/2 <E_i, dC_j>
-    (the same trace, reassociated; used for bounded samples at large n)."""
-    E = []
-    for i in rows:
-        fi, fj, mode, _, w = params[i]
-        dC = K.binned_derivative(layout, fi, fj, mode, w)
-        E.append(Cinv @ dC @ Cinv)
-    out = np.zeros((len(rows), len(params)))
-    for j, (fi, fj, mode, _, w) in enumerate(params):
-        dC = K.binned_derivative(layout, fi, fj, mode, w)
-        for r, Ei in enumerate(E):
-            out[r, j] = 0.5 * float(np.sum(Ei * dC))
-    return out
 
 
 def qml(layout: K.Layout, Cinv: np.ndarray, N: np.ndarray, maps: np.ndarray, F: np.ndarray,

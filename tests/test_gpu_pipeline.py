@@ -131,6 +131,48 @@ def test_qml_tqu_with_eb_tb_vs_oracle():
     assert _rel(sp.get_noise_bias(), nbho) < 1e-9
 
 
+def test_spin2_field_listed_first_vs_oracle():
+    """spins=[2, 0] (Q/U rows before T): S, Fisher rows, QML estimates and
+    noise bias against the oracle's restatement of the reference's assembly
+    (``signal_kernels.py:479-499``), whose T x P block of S ends at zero; the
+    TE/TB derivatives are those of the [0, 2] order with the blocks moved;
+    the harmonic route gives the same F on the same covariance."""
+    from cosmoforge_amd import Fisher, Spectra
+    from oracle import pipeline as OP
+
+    base = cases.case_fisher_nside4("TQU")
+    npix = base.mask.shape[0]
+    case = cases.Case("tqu_p_first", base.nside, [2, 0], ["E", "B", "T"], base.mask[:, [1, 2, 0]], dict(base.cls),
+                      base.beam, base.lmax_signal, lmin=base.lmin, lmax=base.lmax, nest=base.nest)
+    # noise in the Q, U, T row order
+    nt = np.diag(base.noise)[: npix]
+    nq = np.diag(base.noise)[npix:]
+    case.noise = np.diag(np.r_[nq, nt])
+    lay = R.layout(case)
+    cls_s, beams = R.smoothed(case, lay)
+    prm = R.params(case, lay, beams)
+    Fo, Cinv, So = OP.fisher(lay, cls_s, case.lmax_signal, case.noise, prm)
+    maps = np.linalg.cholesky(So + case.noise) @ np.random.default_rng(4).standard_normal((lay.n, 2))
+    q, nb, cho, _ = OP.qml(lay, Cinv, case.noise, maps, Fo, prm)
+    f = Fisher(_params(case, nsims=2), basis=False, mask=case.mask, noise_cov1=case.noise, cls_data=case.cls,
+               beam=case.beam)
+    f.run()
+    assert f.spectra_list == [lab for *_, lab in lay.spectra_list()]
+    S = f.signal_matrix
+    assert np.max(np.abs(S - So)) <= 1e-12 * np.max(np.abs(So))
+    assert not S[2 * npix:, : 2 * npix].any()
+    assert _rel(f.get_fisher_matrix(), Fo) < 1e-10
+    sp = Spectra(fisher=f, maps1=maps)
+    sp.run()
+    assert _rel(sp.qml_results, q) < 1e-10
+    assert _rel(sp.qml_noise_bias, nb) < 1e-10
+    assert _rel(sp.get_power_spectra(), cho) < 1e-9
+    fh = Fisher(_params(case, nsims=2), basis="harmonic", mask=case.mask, noise_cov1=case.noise,
+                cls_data=case.cls, beam=case.beam, context=f.ctx)
+    fh.run()
+    assert _rel(fh.get_fisher_matrix(), Fo) < 1e-7       # the pixel route's acos angles (polarisation)
+
+
 def test_v110_pins_gpu():
     from cosmoforge_amd import PICSLike
 
@@ -143,6 +185,45 @@ def test_v110_pins_gpu():
     assert chi2[0] == pytest.approx(case.expect["quadform"], rel=1e-10)
     S = pl.cov.S.cpu().numpy().T
     C = S + case.noise
+    assert np.trace(C) == pytest.approx(case.expect["cov_trace"], rel=1e-10)
+    assert np.linalg.norm(C) == pytest.approx(case.expect["cov_fro"], rel=1e-10)
+
+
+def test_default_cosine_beam_reproduces_signal_golden():
+    """smoothing_type='cosine_legacy' and no beam= (the reference default,
+    ``settings.py:219``; ``beam.py:27,192-194``): S equals ref_TQU_signal.dat,
+    which the reference's test generated with coswinbeam(4, 4, 12); the
+    legacy integer code 2 selects the same beam."""
+    import warnings
+
+    from cosmoforge_amd import engine, pipeline
+    from cosmoforge_amd.core import Inputs
+
+    case = cases.case_signal_tqu()
+    ctx = engine.Context(0)
+    for st in ("cosine_legacy", 2):
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", DeprecationWarning)
+            inp = Inputs(_params(case, smoothing_type=st), case.mask, np.eye(576), case.cls, None)
+        S = engine.to_numpy_cm(pipeline.Covariance(ctx, inp).signal())
+        np.testing.assert_allclose(S, case.expect["S"], rtol=1e-12, atol=1e-20)
+
+
+def test_v110_pins_with_default_gaussian_beam():
+    """The v1.1.0 pins (``tests/test_core_pipeline.py:276-308``) reached
+    through smoothing_type='gaussian', fwhmarcmin=60 and no beam=: pins the
+    T column of the Gaussian window (the field is T only; the E/B factor
+    exp(2 sigma^2) has no reference fixture)."""
+    from cosmoforge_amd import PICSLike
+
+    case = cases.case_v110()
+    pl = PICSLike(_params(case, nsims=1, smoothing_type="gaussian", fwhmarcmin=60.0), basis=False, mask=case.mask,
+                  noise_cov1=case.noise, cls_data=case.cls, maps1=case.maps)
+    chi2, lnl = pl._compute_likelihood_point(None)
+    logdet = -2.0 * lnl[0] - chi2[0]
+    assert logdet == pytest.approx(case.expect["logdet"], rel=1e-10)
+    assert chi2[0] == pytest.approx(case.expect["quadform"], rel=1e-10)
+    C = pl.cov.S.cpu().numpy().T + case.noise
     assert np.trace(C) == pytest.approx(case.expect["cov_trace"], rel=1e-10)
     assert np.linalg.norm(C) == pytest.approx(case.expect["cov_fro"], rel=1e-10)
 

@@ -354,3 +354,58 @@ def test_grouped_tile_order_reaches_every_tile_once(m, n, batch, gm, gn):
             assert len(seen) == tm * tn * batch
             if tri & 1 and m == n and m >= 3968:     # the trimmed whitening products
                 assert max(load) / min(load) < 1.06
+
+
+def test_smoothing_type_normalisation():
+    """Names in any case, the deprecated 'cosine' alias and the legacy
+    integer codes 0-3 (``settings.py:353-380``); anything else raises."""
+    import warnings
+
+    from cosmoforge_amd.core import read_params
+
+    cases = {"Gaussian": "gaussian", "NONE": "none", " cosine_npipe ": "cosine_npipe", "file": "file"}
+    for v, want in cases.items():
+        assert read_params({"smoothing_type": v}).smoothing_type == want
+    for v, want in {"cosine": "cosine_legacy", 0: "none", 1: "gaussian", 2: "cosine_legacy", 3: "file"}.items():
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            assert read_params({"smoothing_type": v}).smoothing_type == want
+        assert any(issubclass(x.category, DeprecationWarning) for x in w)
+    for bad in (4, -1, 1.5, "cos", "gauss", True):
+        with pytest.raises(ValueError):
+            read_params({"smoothing_type": bad})
+
+
+def test_default_beams_follow_smoothing_type():
+    """Without beam=: 'none' is unity, 'cosine_legacy' / 'cosine_npipe' the
+    raised cosine from nside / 1 to 3 nside in all three rows (the window of
+    the reference's signal fixture, coswinbeam(4, 4, 12)), 'gaussian' the
+    healpy convention exp(-l(l+1) s^2 / 2) with exp(2 s^2) on E and B, and
+    an integer code selects the same beam as its name."""
+    import math
+    import warnings
+
+    from cosmoforge_amd.core import read_params, resolve_beam
+
+    L = 16
+    one = resolve_beam(read_params({"nside": 4, "smoothing_type": "none"}), None, L)
+    assert np.array_equal(one, np.ones((3, L + 1)))
+    cl = resolve_beam(read_params({"nside": 4, "smoothing_type": "cosine_legacy"}), None, L)
+    assert np.array_equal(cl[0], cl[1]) and np.array_equal(cl[0], cl[2])
+    assert np.all(cl[0, :5] == 1.0) and np.all(cl[0, 12:] == 0.0)
+    np.testing.assert_allclose(cl[0, 8], 0.5, atol=1e-15)                 # half-way between 4 and 12
+    assert np.all(np.diff(cl[0, 4:13]) < 0)
+    cn = resolve_beam(read_params({"nside": 4, "smoothing_type": "cosine_npipe"}), None, L)
+    assert cn[0, 1] == 1.0 and cn[0, 2] < 1.0 and cn[0, 12] == 0.0
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert np.array_equal(resolve_beam(read_params({"nside": 4, "smoothing_type": 2}), None, L), cl)
+    g = resolve_beam(read_params({"nside": 4, "smoothing_type": "gaussian", "fwhmarcmin": 60.0}), None, L)
+    sigma = math.radians(1.0) / math.sqrt(8.0 * math.log(2.0))
+    ell = np.arange(L + 1)
+    np.testing.assert_allclose(np.log(g[0]), -0.5 * ell * (ell + 1) * sigma**2, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(g[1] / g[0], math.exp(2 * sigma**2), rtol=1e-14)
+    np.testing.assert_array_equal(g[1], g[2])
+    # an injected beam wins over smoothing_type
+    inj = np.full((4, L + 3), 0.5)
+    assert np.array_equal(resolve_beam(read_params({"smoothing_type": "gaussian"}), inj, L), inj[:3, : L + 1])
