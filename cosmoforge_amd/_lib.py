@@ -87,6 +87,7 @@ SIGNATURES = {
     "cfmi_mlp_forward": (ci, [vp, ci, ci, vp, vp, vp, vp, vp, vp]),
     "cfmi_mlp_loss": (ci, [vp, ci, ci, vp, vp, vp, vp, vp]),
     "cfmi_mlp_backward": (ci, [vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci]),
+    "cfmi_mlp_backward_noalloc": (ci, [vp, ci, ci, vp, vp, vp, vp, vp, vp, ci]),
     "cfmi_mlp_adam": (ci, [vp, i64, vp, vp, vp, vp, cd, cd, cd, cd, ci, vp]),
     "cfmi_mlp_to_bf16": (ci, [vp, i64, vp, vp]),
     "cfmi_mlp_ensemble_stats": (ci, [vp, ci, ci, vp, vp, vp]),

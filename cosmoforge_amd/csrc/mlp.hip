@@ -20,6 +20,8 @@
 //   132865 values.  W[out][in] row-major.
 #include "cfmi_internal.h"
 
+#include <mutex>
+
 #include <cmath>
 #include <cstring>
 
@@ -422,55 +424,39 @@ Acts acts_of(void* ws, int E, int B)
     return a;
 }
 
-}  // namespace
+// Scratch of cfmi_mlp_backward_noalloc (32 MiB; each GPU the library runs
+// on holds its own copy until the process ends).
+constexpr int64_t MLP_SCRATCH_FLOATS = (int64_t)8 << 20;
+static __device__ __align__(256) float scratch[MLP_SCRATCH_FLOATS];
 
-extern "C" {
+// Device address of `scratch` on the context's device: obtained once per
+// device, with that device current, by hipGetSymbolAddress (the array's name
+// in host code is not a device address).
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
 
-int64_t cfmi_mlp_param_count(void) { return NPAR; }
-
-int64_t cfmi_mlp_workspace_bytes(int E, int B)
+int scratch_address(cfmi_ctx* c, float** out)
 {
-    const int64_t s = (int64_t)B * H * E;
-    return 3 * s * 4 + 6 * s * 2;
-}
-
-int cfmi_mlp_forward(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const float* w32, const void* wb16,
-                     float* y, void* ws)
-{
-    if (!c) return CFMI_EINVAL;
-    if (E < 1 || E > 64 || B < 1) return cfmi_fail(c, CFMI_EINVAL, "mlp: bad ensemble %d / batch %d", E, B);
-    const bf16* wb = (const bf16*)wb16;
-    Acts a = acts_of(ws, E, B);
-    hipLaunchKernelGGL(l1_fwd_kernel, dim3(g1((int64_t)B * H).x, (unsigned)E), dim3(256), 0, c->stream, B, X, idx,
-                       wb, w32, a.z[0], a.h[0], a.s);
-    CFMI_LAUNCH_CHECK(c);
-    for (int l = 1; l < 3; ++l) {
-        MG g = {};
-        g.M = B; g.N = H; g.K = H; g.kchunk = H; g.nsplit = 1;
-        g.A = a.h[l - 1]; g.sA = a.s;
-        g.B = wb + (l == 1 ? OFF_W2 : OFF_W3); g.sB = NPAR;
-        g.C = a.z[l]; g.sC = a.s;
-        g.Hout = a.h[l]; g.sH = a.s;
-        g.bias = w32 + (l == 1 ? OFF_B2 : OFF_B3); g.sbias = NPAR;
-        int rc = launch_gemm<0, 1, EPI_FWD>(c, g, E);
-        if (rc) return rc;
+    if (c->device < 0 || c->device >= MAX_DEVICES) return cfmi_fail(c, CFMI_EINVAL, "device %d out of range", c->device);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[c->device]) {
+        int cur = -1;
+        CFMI_HIP(c, hipGetDevice(&cur));
+        if (cur != c->device) CFMI_HIP(c, hipSetDevice(c->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != c->device) CFMI_HIP(c, hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return cfmi_fail(c, CFMI_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        scratch_ptr[c->device] = p;
     }
-    hipLaunchKernelGGL(l4_fwd_kernel, dim3((unsigned)((B + 3) / 4), (unsigned)E), dim3(256), 0, c->stream, B, a.h[2],
-                       a.s, wb, w32, y);
-    CFMI_LAUNCH_CHECK(c);
+    *out = static_cast<float*>(scratch_ptr[c->device]);
     return CFMI_OK;
 }
 
-int cfmi_mlp_loss(cfmi_ctx* c, int E, int B, const float* y, const float* T, const int* idx, float* dy, float* loss)
-{
-    if (!c) return CFMI_EINVAL;
-    hipLaunchKernelGGL(loss_kernel, dim3((unsigned)E), dim3(256), 0, c->stream, B, y, T, idx, dy, loss);
-    CFMI_LAUNCH_CHECK(c);
-    return CFMI_OK;
-}
-
-int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const void* wb16, const float* dy,
-                      void* ws, float* grad, float* part, int chunk)
+static int mlp_backward_impl(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const void* wb16,
+                             const float* dy, void* ws, float* grad, float* part, int chunk)
 {
     if (!c) return CFMI_EINVAL;
     if (chunk < 32 || chunk % 32) return cfmi_fail(c, CFMI_EINVAL, "mlp: gradient chunk must be a multiple of 32");
@@ -524,6 +510,84 @@ int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx,
                        1);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t cfmi_mlp_param_count(void) { return NPAR; }
+
+int64_t cfmi_mlp_workspace_bytes(int E, int B)
+{
+    const int64_t s = (int64_t)B * H * E;
+    return 3 * s * 4 + 6 * s * 2;
+}
+
+int cfmi_mlp_forward(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const float* w32, const void* wb16,
+                     float* y, void* ws)
+{
+    if (!c) return CFMI_EINVAL;
+    if (E < 1 || E > 64 || B < 1) return cfmi_fail(c, CFMI_EINVAL, "mlp: bad ensemble %d / batch %d", E, B);
+    const bf16* wb = (const bf16*)wb16;
+    Acts a = acts_of(ws, E, B);
+    hipLaunchKernelGGL(l1_fwd_kernel, dim3(g1((int64_t)B * H).x, (unsigned)E), dim3(256), 0, c->stream, B, X, idx,
+                       wb, w32, a.z[0], a.h[0], a.s);
+    CFMI_LAUNCH_CHECK(c);
+    for (int l = 1; l < 3; ++l) {
+        MG g = {};
+        g.M = B; g.N = H; g.K = H; g.kchunk = H; g.nsplit = 1;
+        g.A = a.h[l - 1]; g.sA = a.s;
+        g.B = wb + (l == 1 ? OFF_W2 : OFF_W3); g.sB = NPAR;
+        g.C = a.z[l]; g.sC = a.s;
+        g.Hout = a.h[l]; g.sH = a.s;
+        g.bias = w32 + (l == 1 ? OFF_B2 : OFF_B3); g.sbias = NPAR;
+        int rc = launch_gemm<0, 1, EPI_FWD>(c, g, E);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(l4_fwd_kernel, dim3((unsigned)((B + 3) / 4), (unsigned)E), dim3(256), 0, c->stream, B, a.h[2],
+                       a.s, wb, w32, y);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_mlp_loss(cfmi_ctx* c, int E, int B, const float* y, const float* T, const int* idx, float* dy, float* loss)
+{
+    if (!c) return CFMI_EINVAL;
+    hipLaunchKernelGGL(loss_kernel, dim3((unsigned)E), dim3(256), 0, c->stream, B, y, T, idx, dy, loss);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_mlp_backward(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const void* wb16, const float* dy,
+                      void* ws, float* grad, float* part, int chunk)
+{
+    return mlp_backward_impl(c, E, B, X, idx, wb16, dy, ws, grad, part, chunk);
+}
+
+// The backward without a workspace argument: its reduction partials live in
+// the file-scope array `scratch` (MLP_SCRATCH_FLOATS floats, E * ceil(B /
+// chunk) * 256 * 256 of them used).  No allocation happens here; host code
+// reaches the array only through the per-device address that
+// scratch_address() obtains once with hipGetSymbolAddress.  Every partial is
+// written before it is read, so the array needs no clearing; calls that
+// overlap on one GPU share it and race.
+int cfmi_mlp_backward_noalloc(cfmi_ctx* c, int E, int B, const float* X, const int* idx, const void* wb16,
+                              const float* dy, void* ws, float* grad, int chunk)
+{
+    if (!c) return CFMI_EINVAL;
+    if (E <= 0 || B <= 0) return cfmi_fail(c, CFMI_EINVAL, "mlp_backward_noalloc: E and B must be positive");
+    if (chunk < 32 || chunk % 32) return cfmi_fail(c, CFMI_EINVAL, "mlp: gradient chunk must be a multiple of 32");
+    const int64_t need = (int64_t)E * ((B + chunk - 1) / chunk) * H * H;
+    if (need > MLP_SCRATCH_FLOATS)
+        return cfmi_fail(c, CFMI_EINVAL,
+                         "mlp_backward_noalloc: E * ceil(B / chunk) * 256 * 256 = %lld floats exceed the %lld of the "
+                         "static scratch (E * ceil(B / chunk) <= %lld); use cfmi_mlp_backward with a workspace",
+                         (long long)need, (long long)MLP_SCRATCH_FLOATS, (long long)(MLP_SCRATCH_FLOATS / (H * H)));
+    float* part = nullptr;
+    int rc = scratch_address(c, &part);
+    if (rc) return rc;
+    return mlp_backward_impl(c, E, B, X, idx, wb16, dy, ws, grad, part, chunk);
 }
 
 int cfmi_mlp_adam(cfmi_ctx* c, int64_t total, float* w, const float* g, float* m, float* v, double lr, double beta1,

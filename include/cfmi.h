@@ -398,6 +398,15 @@ int cfmi_mlp_loss(cfmi_ctx *ctx, int E, int B, const float *y, const float *T, c
                   float *loss);
 int cfmi_mlp_backward(cfmi_ctx *ctx, int E, int B, const float *X, const int *idx, const void *wb16,
                       const float *dy, void *ws, float *grad, float *part, int chunk);
+/* cfmi_mlp_backward without the workspace argument: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, training loops that forbid hipMalloc).  The reduction
+ * partials live in a static 32 MiB device array of the library (one per
+ * GPU, kept until the process ends), so E * ceil(B / chunk) <= 128; larger
+ * sizes return CFMI_EINVAL.  Allocates nothing.  Calls that overlap on one
+ * GPU share the array and race.  Same results as cfmi_mlp_backward.        */
+int cfmi_mlp_backward_noalloc(cfmi_ctx *ctx, int E, int B, const float *X, const int *idx, const void *wb16,
+                              const float *dy, void *ws, float *grad, int chunk);
 /* Adam (bias-corrected, step >= 1) on total = E * P values; refreshes wb16. */
 int cfmi_mlp_adam(cfmi_ctx *ctx, int64_t total, float *w, const float *g, float *m, float *v, double lr,
                   double beta1, double beta2, double eps, int step, void *wb16);
