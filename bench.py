@@ -67,28 +67,6 @@ def make_map(ctx, cfg, seed):
     return engine.to_numpy_cm(x) / f.d.cpu().numpy()[:, None]
 
 
-def errors(res, ref) -> dict:
-    """Per-quantity differences of one solve against another.
-
-    F: |dF_ij| / sqrt(F_ii F_jj) (the diagonal: plain relative error).
-    C_l: |dC| / max(|C|, sigma), sigma = sqrt((F^-1)_ll): relative where the
-    estimate exceeds its error bar, in units of the error bar where it does
-    not (EB, TB and noise-dominated bins scatter around zero).  Noise bias:
-    |d nb_p| / sqrt(F_pp), the shift in units of the standard deviation of
-    q_p.  q: |dq_p| / sqrt(F_pp)."""
-    F, C, nb, q, sig = ref[:5]
-    Fm, Cm, nbm, qm = res[:4]
-    d = np.sqrt(np.diag(F))
-    E = np.abs(Fm - F) / np.outer(d, d)
-    C, Cm = np.ravel(C), np.ravel(Cm)
-    return {"F_entry_max": float(E.max()),
-            "F_diag_rel_max": float(np.max(np.abs(np.diag(Fm) - np.diag(F)) / np.diag(F))),
-            "Cl_rel_max": float(np.max(np.abs(Cm - C) / np.maximum(np.abs(C), sig))),
-            "Cl_over_sigma_max": float(np.max(np.abs(Cm - C) / sig)),
-            "noise_bias_over_sigma_q_max": float(np.max(np.abs(nbm - nb) / d)),
-            "q_over_sigma_q_max": float(np.max(np.abs(qm - q) / d[None, :]))}
-
-
 def cpu_baseline(cfg, stride: int, k_params: int = 2):
     """The oracle in the reference formulation (FP64, LAPACK + OpenMP C
     recurrences) on the host cores, on a bounded sample of the workload:
@@ -309,6 +287,7 @@ def main():
     import torch
 
     from cosmoforge_amd import Fisher, Spectra, engine, workloads
+    from cosmoforge_amd.accuracy import errors
 
     ctx = engine.Context(0)
 

@@ -62,7 +62,8 @@ SIGNATURES = {
     "cfmi_unslice8": (ci, [vp, ci, i64, i64, vp, i64, i64, vp, vp, i64]),
     "cfmi_transpose": (ci, [vp, i64, i64, vp, i64, vp, i64]),
     "cfmi_s8_kmax": (i64, [ci]),
-    "cfmi_gemm_s8": (ci, [vp, ci, i64, i64, i64, cd, vp, i64, i64, vp, vp, i64, i64, vp, cd, vp, i64, ci]),
+    "cfmi_gemm_s8": (ci, [vp, ci, i64, i64, i64, cd, vp, i64, i64, vp, i64, vp, i64, i64, vp, i64, cd, vp, i64, ci,
+                          i64, i64, i64, cd]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),

@@ -199,7 +199,11 @@ struct S8Args {
     double* C;
     int64_t ldc;
     int lower;                 // compute the tiles on or below the diagonal, store m >= n only
-    int tri;                   // group_tile: no trims
+    int tri;                   // bit 0: A[m][k] = 0 for k > m + offA; bit 2: B[n][k] = 0 for k > n + offB
+    int64_t offA, offB;
+    int packed;                // 1: C is the packed lower row band (row0, col0, pbase, pdiag)
+    int64_t row0, col0, pbase;
+    double pdiag;
     int64_t tm, tn;
     int64_t gm, gn, gtm, gtmp, gtn, ngroups;
 };
@@ -267,6 +271,17 @@ __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (
     }
 }
 
+// K range of tile (m0, n0): the triangular trims shorten it to the last
+// k that can be nonzero, rounded up to a whole slab (the operands hold zeros
+// there, and up to their padded row pitch).
+__device__ __forceinline__ int64_t tile_kend(const S8Args& g, int64_t m0, int64_t n0)
+{
+    int64_t k = g.K;
+    if (g.tri & 1) k = min(k, (m0 + BT + g.offA + SK - 1) / SK * SK);
+    if (g.tri & 4) k = min(k, (n0 + BT + g.offB + SK - 1) / SK * SK);
+    return k;
+}
+
 // DB = 1: two LDS stages, one barrier per slab: the slab prefetched into
 // registers during the MFMAs of slab k is stored into the other stage at the
 // start of step k + 1, while the stage of slab k + 1 (stored during step k)
@@ -281,6 +296,8 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
     if (!group_tile(g, blockIdx.x, z, ti, tj)) return;
     const int64_t m0 = ti * BT, n0 = tj * BT;
     if (g.lower && n0 > m0 + BT - 1) return;
+    if (g.packed && g.col0 + n0 > g.row0 + m0 + BT - 1) return;   // band tile above the diagonal
+    const int64_t K = tile_kend(g, m0, n0);
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 64;
     const int lm = lane & 15, lk = lane >> 4;
 
@@ -293,17 +310,19 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
             for (int j = 0; j < 4; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
 
     Stage8<S> st;
-    st.load(g, m0, n0, 0, tid);
-    if constexpr (DB) {
+    if (K > 0) st.load(g, m0, n0, 0, tid);
+    if (K <= 0) {
+        // the trims leave no k: the product is zero
+    } else if constexpr (DB) {
         st.store(As[0], Bs[0], tid);
-        if (SK < g.K) st.load(g, m0, n0, SK, tid);
+        if (SK < K) st.load(g, m0, n0, SK, tid);
         __syncthreads();
         for (int64_t k0 = 0;; k0 += SK) {
             const int cur = (int)((k0 / SK) & 1);
-            const bool more = k0 + SK < g.K;
+            const bool more = k0 + SK < K;
             if (more) {
                 st.store(As[cur ^ 1], Bs[cur ^ 1], tid);
-                if (k0 + 2 * SK < g.K) st.load(g, m0, n0, k0 + 2 * SK, tid);
+                if (k0 + 2 * SK < K) st.load(g, m0, n0, k0 + 2 * SK, tid);
             }
             slab_mfma<S>(As[cur], Bs[cur], acc, wm, wn, lm, lk);
             if (!more) break;
@@ -311,7 +330,7 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
         }
     } else {
         int64_t k0 = 0;
-        for (; k0 + SK < g.K; k0 += SK) {
+        for (; k0 + SK < K; k0 += SK) {
             __syncthreads();
             st.store(As[0], Bs[0], tid);
             __syncthreads();
@@ -350,7 +369,15 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 #pragma unroll
                 for (int d = S - 2; d >= 0; --d) v = (double)acc[d][i][j][r] + v * kInv254;
                 v = (v * kInv127sq) * (sa[i][r] * sb[j]);
-                double* p = g.C + gm + gn * g.ldc;
+                double* p;
+                if (g.packed) {
+                    const int64_t ii = g.row0 + gm, cc = g.col0 + gn;
+                    if (cc > ii) continue;
+                    if (cc == ii) v = v * g.pdiag;
+                    p = g.C + (ii * (ii + 1) / 2 + cc - g.pbase);
+                } else {
+                    p = g.C + gm + gn * g.ldc;
+                }
                 *p = g.beta == 0.0 ? v : v + g.beta * *p;
             }
 }
@@ -441,7 +468,6 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
 {
     g.tm = (g.M + BT - 1) / BT;
     g.tn = (g.N + BT - 1) / BT;
-    g.tri = 0;
     g.gm = 2;
     g.gn = 8;
     g.gtm = (g.tm + g.gm - 1) / g.gm;
@@ -523,8 +549,9 @@ int cfmi_unslice8(cfmi_ctx* c, int s, int64_t rows, int64_t k, const void* src, 
 }
 
 int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alpha, const void* A, int64_t lda,
-                 int64_t psA, const double* muA, const void* B, int64_t ldb, int64_t psB, const double* muB,
-                 double beta, double* C, int64_t ldc, int flags)
+                 int64_t psA, const double* muA, int64_t off_a, const void* B, int64_t ldb, int64_t psB,
+                 const double* muB, int64_t off_b, double beta, double* C, int64_t ldc, int flags, int64_t row0,
+                 int64_t col0, int64_t pbase, double pdiag)
 {
     if (!c) return CFMI_EINVAL;
     if (m < 0 || n < 0 || k < 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: negative size");
@@ -537,9 +564,11 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
     if (psA < m * lda || psB < n * ldb || psA % 16 || psB % 16)
         return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: slice planes overlap or are misaligned");
     if (!aligned16(A) || !aligned16(B)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: operands not 16-byte aligned");
-    if (ldc < m) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: ldc < m");
-    const bool lower = flags & 1;
-    if (lower && m != n) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
+    const bool lower = flags & 1, packed = flags & 16;
+    if (!packed && ldc < m) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: ldc < m");
+    if (lower && (m != n || packed)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
+    if (packed && (row0 < 0 || col0 < 0 || pbase < 0 || pbase > row0 * (row0 + 1) / 2))
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: packed row band starts after row %lld", (long long)row0);
     if (k == 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: k must be positive");
     // K chunks within the exact int32 range, added in FP64 (beta = 1 after the first)
     const int64_t kmax = cfmi_s8_kmax(s);
@@ -552,6 +581,10 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
         g.A = (const char*)A + k0; g.lda = lda; g.psA = psA; g.muA = muA;
         g.B = (const char*)B + k0; g.ldb = ldb; g.psB = psB; g.muB = muB;
         g.C = C; g.ldc = ldc; g.lower = lower ? 1 : 0;
+        // triangular K-trims, relative to this chunk's first k
+        g.tri = ((flags & 4) ? 1 : 0) | ((flags & 8) ? 4 : 0);
+        g.offA = off_a - k0; g.offB = off_b - k0;
+        g.packed = packed ? 1 : 0; g.row0 = row0; g.col0 = col0; g.pbase = pbase; g.pdiag = pdiag;
         // two LDS stages for s <= 4 (s = 5 would spill registers); flag 2: one stage
         const bool db = s <= 4 && !(flags & 2);
         int rc = s == 3 ? gemm_s8_launch<3>(c, g, db) : s == 4 ? gemm_s8_launch<4>(c, g, db)

@@ -739,9 +739,13 @@ S8_RECORDER: S8Recorder | None = None
 
 
 def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row0: int, B: Slices, b_row0: int,
-            beta: float, C: torch.Tensor, offc: int, ldc: int, flags: int = 0):
+            beta: float, C: torch.Tensor, offc: int, ldc: int, flags: int = 0, *, tri_a: int | None = None,
+            tri_b: int | None = None, packed=None):
     """C[m x n] (column-major FP64, at offc) = alpha A B^T + beta C over rows
-    a_row0.. of A and b_row0.. of B (``cfmi_gemm_s8``)."""
+    a_row0.. of A and b_row0.. of B (``cfmi_gemm_s8``).  ``tri_a`` / ``tri_b``:
+    A[m][k] = 0 for k > m + tri_a (B likewise), which shortens K per tile;
+    ``packed`` = (row0, col0, pbase, pdiag): C (at offc) is the packed lower
+    row band (ldc unused)."""
     if m <= 0 or n <= 0:
         return
     if A.s != B.s:
@@ -750,9 +754,19 @@ def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row
     if rec is not None:
         e0 = torch.cuda.Event(enable_timing=True)
         e0.record()
+    if tri_a is not None:
+        flags |= 4
+    if tri_b is not None:
+        flags |= 8
+    row0 = col0 = pbase = 0
+    pdiag = 1.0
+    if packed is not None:
+        flags |= 16
+        row0, col0, pbase, pdiag = packed
     ctx.check(ctx.lib.cfmi_gemm_s8(ctx.h, A.s, m, n, k, alpha, P(A.data, a_row0 * A.ldk), A.ldk, A.ps,
-                                   P(A.mu, a_row0), P(B.data, b_row0 * B.ldk), B.ldk, B.ps, P(B.mu, b_row0), beta,
-                                   P(C, offc), ldc, flags), "gemm_s8")
+                                   P(A.mu, a_row0), tri_a or 0, P(B.data, b_row0 * B.ldk), B.ldk, B.ps,
+                                   P(B.mu, b_row0), tri_b or 0, beta, P(C, offc), ldc, flags, row0, col0, pbase,
+                                   pdiag), "gemm_s8")
     if rec is not None:
         e1 = torch.cuda.Event(enable_timing=True)
         e1.record()
@@ -837,3 +851,115 @@ def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs
 def transpose_tri(ctx: Context, L: torch.Tensor, n: int) -> torch.Tensor:
     """L^T of a column-major n x n matrix (its rows become contiguous)."""
     return transpose(ctx, L, n, n, n)
+
+
+class S8Whitener:
+    """The whitener W (lower triangular, column-major n x n FP64) as int8
+    slices per field column block: for a block of columns [r0, r0 + m),
+    rows r0..n-1 of W[:, block] (the rows above r0 are zero there), each row
+    scaled by its largest entry within the block."""
+
+    def __init__(self, ctx: Context, W: torch.Tensor, n: int, geometry, s: int = 5):
+        self.s, self.n = int(s), int(n)
+        self.blocks = {}
+        for g in geometry:
+            for r0, m in ((g.ra, g.ma), (g.rb, g.mb)):
+                if (r0, m) not in self.blocks:
+                    self.blocks[(r0, m)] = slice8(ctx, self.s, 1, n - r0, m, W, r0 + r0 * n, n)
+
+    def __getitem__(self, key) -> Slices:
+        return self.blocks[key]
+
+    @property
+    def nbytes(self) -> int:
+        return sum(v.data.numel() + 8 * v.mu.numel() for v in self.blocks.values())
+
+
+class S8Work:
+    """Reused buffers of ``whiten_band_s8``: one bin's FP64 intermediate
+    product and the slices of it and of the derivative block."""
+
+    def __init__(self, ctx: Context, s: int):
+        self.ctx, self.s = ctx, s
+        self.z = None
+        self.sl = {}
+
+    def fp64(self, count: int) -> torch.Tensor:
+        if self.z is None or self.z.numel() < count:
+            self.z = None
+            self.z = torch.empty(count, dtype=torch.float64, device=self.ctx.device)
+        return self.z
+
+    def slices(self, name: str, rows: int, k: int) -> Slices:
+        b = self.sl.get(name)
+        if b is None or b.data.shape[1] < rows or b.ldk < k:
+            self.sl[name] = None
+            b = self.sl[name] = Slices(self.ctx, self.s, rows, k)
+        return b
+
+    @staticmethod
+    def bytes_for(s: int, rows: int, width: int) -> int:
+        """Device bytes one bin needs for ``rows`` intermediate rows of a
+        block ``width`` wide (FP64 product, its slices, the block's slices)."""
+        ld = (width + 63) // 64 * 64
+        return rows * width * 8 + s * rows * ld + 2 * s * width * ld + 16 * (rows + width)
+
+
+def whiten_band_s8(ctx: Context, Ws: S8Whitener, n: int, blk: Block, X: torch.Tensor, r0: int, r1: int,
+                   work: S8Work | None = None, counter: list | None = None) -> S8Work:
+    """``whiten_band`` with every product on the int8 matrix cores at FP64
+    accuracy: X[b] (FP64, packed rows r0..r1-1) += the lower triangle of
+    W dC_b W^T, diagonal scaled by 1/sqrt(2), bin by bin.
+
+    Per bin, Z^T = D^T W[i, a]^T is formed in FP64 from the sliced columns
+    of D and the sliced rows of W (their zeros right of the diagonal trimmed
+    per tile), re-sliced by rows of Z, and multiplied by W[c, b]^T into the
+    packed band; the transposed term of an off-diagonal block reuses Z for a
+    full band, or forms (W[j, b] D^T)^T from the sliced rows of D for a
+    partial one."""
+    s = Ws.s
+    if work is None:
+        work = S8Work(ctx, s)
+    D = blk.stack
+    nbb = D.shape[0]
+    ra, ma, rb, mb = blk.ra, blk.ma, blk.rb, blk.mb
+    Wa, Wb = Ws[(ra, ma)], Ws[(rb, mb)]
+    sX = X.shape[1]
+    pbase = r0 * (r0 + 1) // 2
+    full = r0 == 0 and r1 == n
+    i0, j0 = max(r0, ra), max(r0, rb)
+    rr, rj = r1 - i0, r1 - j0
+
+    def gemm(*a, **k):
+        if counter is not None:
+            counter.append(1)
+        gemm_s8(*a, **k)
+
+    for b in range(nbb):
+        off = b * ma * mb
+        if rr > 0:
+            # term 1: X[i, c] += sum_k (W[i, a] D)[k] W[c, rb + k],  i >= ra, c >= rb
+            Dc = slice8(ctx, s, 0, mb, ma, D, off, ma, out=work.slices("dcol", mb, ma))
+            ZT = work.fp64(rr * mb)
+            gemm(ctx, mb, rr, ma, 1.0, Dc, 0, Wa, i0 - ra, 0.0, ZT, 0, mb, tri_b=i0 - ra)
+            Zs = slice8(ctx, s, 0, rr, mb, ZT, 0, mb, out=work.slices("z", rr, mb))
+            if r1 > rb:
+                gemm(ctx, rr, r1 - rb, mb, 1.0, Zs, 0, Wb, 0, 1.0, X, b * sX, 0, tri_b=0,
+                     packed=(i0, rb, pbase, SQRT_HALF))
+        if blk.sym:
+            continue
+        # term 2, the transpose: X[i, c] += sum_k W[i, rb + k] (W[c, a] D)[k],  i >= rb, c >= ra
+        if full and rr > 0:
+            gemm(ctx, n - rb, n - ra, mb, 1.0, Wb, 0, Zs, 0, 1.0, X, b * sX, 0, tri_a=0,
+                 packed=(rb, ra, 0, SQRT_HALF))
+            continue
+        if rj <= 0 or r1 <= ra:
+            continue
+        # partial band: Z2^T = D (W[j, b])^T, then X[j, c] += sum_q Z2[j, q] W[c, ra + q]
+        Dr = slice8(ctx, s, 1, ma, mb, D, off, ma, out=work.slices("drow", ma, mb))
+        Z2T = work.fp64(rj * ma)
+        gemm(ctx, ma, rj, mb, 1.0, Dr, 0, Wb, j0 - rb, 0.0, Z2T, 0, ma, tri_b=j0 - rb)
+        Z2s = slice8(ctx, s, 0, rj, ma, Z2T, 0, ma, out=work.slices("z2", rj, ma))
+        gemm(ctx, rj, r1 - ra, ma, 1.0, Z2s, 0, Wa, 0, 1.0, X, b * sX, 0, tri_b=0,
+             packed=(j0, ra, pbase, SQRT_HALF))
+    return work

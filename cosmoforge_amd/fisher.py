@@ -5,10 +5,11 @@ Constructor kwargs and getters follow ``src/cosmoforge.qube/qube/fisher.py``
 F_ij = 1/2 Tr[C^-1 dC_i C^-1 dC_j] over parameters ordered spectrum-major
 (``spectra_list`` order, then bins), beam-smoothed as the reference's.
 All device work runs in libcfmi on one GPU.  ``precision="mixed"`` (no
-reference counterpart) runs the whitening products at FP32 accuracy on the
-matrix cores (``mixed_gemm="bf16x3"``: three-piece BF16 operands, the
-default; ``"fp32"``: FP32 MFMA), with FP64 recurrences, factorisation,
-FP64-accumulated Gram matrix and FP64 F^-1.
+reference counterpart) runs the large products on the int8 matrix cores at
+FP64 accuracy (operands as int8 slices, exact int32 sums): the whitening
+products of the pixel route (``mixed_gemm="s8"``, the default; "bf16x3" and
+"fp32" keep the FP32-accurate variants) and the solves and contractions of
+the harmonic route; recurrences, factorisations and F^-1 stay FP64.
 """
 
 from __future__ import annotations
@@ -85,7 +86,7 @@ class Fisher:
         self.harmonic_solver = kwargs.get("harmonic_solver", "whitener")
         if self.harmonic_solver not in ("whitener", "trsm"):
             raise ValueError(f"harmonic_solver must be 'whitener' or 'trsm'; got {self.harmonic_solver!r}")
-        self.mixed_gemm = kwargs.get("mixed_gemm", "bf16x3")
+        self.mixed_gemm = kwargs.get("mixed_gemm", "s8")
         if self.mixed_gemm not in pipeline.MIXED_GEMMS:
             raise ValueError(f"mixed_gemm must be one of {pipeline.MIXED_GEMMS}; got {self.mixed_gemm!r}")
 

@@ -249,7 +249,22 @@ def whitener_planes(cov: Covariance) -> torch.Tensor:
     return cov._wp
 
 
-MIXED_GEMMS = ("bf16x3", "fp32")
+MIXED_GEMMS = ("s8", "bf16x3", "fp32")
+
+# int8 slices per operand of the sliced whitening products (each entry held
+# to 2^-39 of its row's largest entry; 15 int8 products per multiply-add)
+S8_WHITEN_SLICES = 5
+
+
+def s8_whitener(cov: Covariance, geometry) -> engine.S8Whitener:
+    """W as int8 slices per field column block, made once per factor."""
+    key = tuple((g.ra, g.ma, g.rb, g.mb) for g in geometry)
+    cached = getattr(cov, "_ws8", None)
+    if cached is None or cached[0] is not cov.factor or cached[1] != key:
+        cov._ws8 = None
+        cov._ws8 = (cov.factor, key, engine.S8Whitener(cov.ctx, cov.factor.W, cov.n, geometry,
+                                                         S8_WHITEN_SLICES))
+    return cov._ws8[2]
 
 
 def _stack_bytes(dtype) -> int:
@@ -454,28 +469,40 @@ def plan_memory(n: int, npar: int, es: int, geometry, nbins: int, stack_bytes: i
 def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | None = None,
            precision: str = "fp64", keep_stacks: bool = True, mem_fraction: float = 0.8,
            launch_timer: LaunchTimer | None = None, mem_budget: int | None = None, progress=None,
-           mixed_gemm: str = "bf16x3"):
+           mixed_gemm: str = "s8"):
     """(F, derivative source): F is (n_p, n_p) on the host.
 
     F_ij = 1/2 <W dC_i W^T, W dC_j W^T>, accumulated over row bands of the
     packed lower triangles (one band when everything fits on the device).
-    ``precision="mixed"``: the whitening products at FP32 accuracy, the
-    whitened derivatives stored in FP32, their Gram matrix from FP32
-    products accumulated in FP64 across 256-term chunks; recurrences,
-    factorisation and F^-1 stay FP64.  ``mixed_gemm`` picks the whitening
-    arithmetic: "bf16x3" (default: W and D as three BF16 pieces, six BF16
-    MFMA products per tile, ``cfmi_gemm_b3``) or "fp32" (W and D rounded to
-    FP32, FP32 MFMA)."""
+    ``precision="mixed"`` runs the whitening products on the low-precision
+    matrix cores; ``mixed_gemm`` picks how:
+
+    * "s8" (default): FP64-accurate products from int8 slices of W, D and
+      the intermediate W D (``cfmi_gemm_s8``, exact int32 sums); the
+      whitened derivatives, their Gram matrix and everything else stay FP64;
+    * "bf16x3": FP32-accurate products from three-piece BF16 operands
+      (``cfmi_gemm_b3``), FP32 whitened derivatives, their Gram matrix from
+      FP32 products accumulated in FP64 across 256-term chunks;
+    * "fp32": W and D rounded to FP32, FP32 MFMA, FP32 storage as above.
+
+    The FP32 variants lose accuracy in proportion to C's conditioning (the
+    error of W D W^T is cancelled in F only to that level); the sliced
+    products do not."""
     ctx, n = cov.ctx, cov.n
-    T = _dtype(precision)
     if mixed_gemm not in MIXED_GEMMS:
         raise ValueError(f"mixed_gemm must be one of {MIXED_GEMMS}; got {mixed_gemm!r}")
+    s8 = precision == "mixed" and mixed_gemm == "s8"
+    T = torch.float64 if s8 else _dtype(precision)
     b3 = precision == "mixed" and mixed_gemm == "bf16x3"
     es = 4 if T == torch.float32 else 8
-    W = whitener_planes(cov) if b3 else whitener(cov, precision)
     sdt = engine.BF16X3 if b3 else T
-    es_z = _stack_bytes(sdt)
     src = make_source(cov, bins, convention, sdt)
+    if s8:
+        W = s8_whitener(cov, src.geometry)
+        es_z = 8 + 3 * W.s
+    else:
+        W = whitener_planes(cov) if b3 else whitener(cov, precision)
+        es_z = _stack_bytes(sdt)
     nb = len(bins)
     npar = len(src.geometry) * nb
     # Whitening products (K <= a few 10^4, terms of both signs) accumulate in
@@ -485,8 +512,9 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
     # across 256-term chunks.
     flags = 0
     free = int(mem_budget) if mem_budget is not None else int(_free_bytes(ctx) * mem_fraction)
-    bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes, free,
-                                                        es_z)
+    # the sliced whitening takes one bin at a time
+    bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, 1 if s8 else nb,
+                                                        src.total_bytes, free, es_z)
     if b3 and len(bands) > 1 and not all(g.sym for g in src.geometry):
         src.set_mirrors(True)             # partial bands multiply the a != b blocks by W D^T
         bands, resident, chunk_budget, zbytes = plan_memory(n, npar, es, src.geometry, nb, src.total_bytes,
@@ -512,7 +540,9 @@ def fisher(cov: Covariance, bins, convention: str = "Cl", timer: StageTimer | No
             if launch_timer is not None:
                 launch_timer.start()
             cnt = launch_timer.launches if launch_timer is not None else None
-            if b3:
+            if s8:
+                zbuf = engine.whiten_band_s8(ctx, W, n, blk, Xs, r0, r1, zbuf, counter=cnt)
+            elif b3:
                 zbuf = engine.whiten_band_b3(ctx, W, n, blk, Xs, r0, r1, zbuf, zbytes // es_z, counter=cnt)
             else:
                 zbuf = engine.whiten_band(ctx, W, n, blk, Xs, r0, r1, zbuf, flags, zbytes // es_z, counter=cnt)

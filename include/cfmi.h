@@ -244,13 +244,19 @@ int64_t cfmi_s8_kmax(int s);
  * p + q < s on v_mfma_i32_16x16x64_i8 (exact int32 sums), levels combined
  * in FP64; k beyond cfmi_s8_kmax is split and the parts added in FP64.
  * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored;
- * bit 1: one LDS stage instead of two (s <= 4; s = 5 always uses one).
- * Serves the C^-1 V, V^T C^-1 V and C^-1 dC products that replace
+ * bit 1: one LDS stage instead of two (s <= 4; s = 5 always uses one);
+ * bit 2: A[m][k] = 0 for k > m + off_a, bit 3: B[n][k] = 0 for k > n + off_b
+ * (the operands hold those zeros; the bits only shorten K per tile);
+ * bit 4: C is the packed lower row band of the whitening (element (i, j <= i)
+ * at i (i + 1) / 2 + j - pbase for i = row0 + m, j = col0 + n, the diagonal
+ * scaled by pdiag; ldc unused).
+ * Serves the C^-1 V, V^T C^-1 V and W dC_b W^T products that replace
  * matrix_mult at qube/qube/fisher.py:506-531 and spectra.py:1144-1161 in
  * the mixed precision (no reference counterpart of the arithmetic).       */
 int cfmi_gemm_s8(cfmi_ctx *ctx, int s, int64_t m, int64_t n, int64_t k, double alpha, const void *A, int64_t lda,
-                 int64_t psA, const double *muA, const void *B, int64_t ldb, int64_t psB, const double *muB,
-                 double beta, double *C, int64_t ldc, int flags);
+                 int64_t psA, const double *muA, int64_t off_a, const void *B, int64_t ldb, int64_t psB,
+                 const double *muB, int64_t off_b, double beta, double *C, int64_t ldc, int flags, int64_t row0,
+                 int64_t col0, int64_t pbase, double pdiag);
 
 /* d[i] = 1/sqrt(|A_ii|) (1 where A_ii == 0), then A <- D A D (FP64). */
 int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *d);

@@ -48,9 +48,12 @@ def kmax(s: int) -> int:
     return (0x7FFFFFFF // (16129 * s)) // 64 * 64
 
 
-def gemm(A: np.ndarray, B: np.ndarray, s: int, alpha: float = 1.0, beta: float = 0.0, C=None) -> np.ndarray:
+def gemm(A: np.ndarray, B: np.ndarray, s: int, alpha: float = 1.0, beta: float = 0.0, C=None,
+         scale=None) -> np.ndarray:
     """alpha A B^T (+ beta C) with A (m, k) and B (n, k) sliced by rows; k
-    beyond ``kmax(s)`` in chunks added in FP64, as the device splits it."""
+    beyond ``kmax(s)`` in chunks added in FP64, as the device splits it.
+    ``scale``: elementwise factors applied to each chunk's product before the
+    addition (the packed band's diagonal factor)."""
     muA, sa = slices(A, s)
     muB, sb = slices(B, s)
     k = A.shape[1]
@@ -61,15 +64,19 @@ def gemm(A: np.ndarray, B: np.ndarray, s: int, alpha: float = 1.0, beta: float =
         k1 = min(k, k0 + step)
         acc = []
         for d in range(s):
-            t = np.zeros((A.shape[0], B.shape[0]), dtype=np.int64)
+            # integer sums below 2^31 are exact in float64 products (BLAS)
+            t = np.zeros((A.shape[0], B.shape[0]))
             for p in range(d + 1):
-                t += sa[p][:, k0:k1] @ sb[d - p][:, k0:k1].T
+                t += sa[p][:, k0:k1].astype(np.float64) @ sb[d - p][:, k0:k1].T.astype(np.float64)
+            assert np.max(np.abs(t), initial=0.0) < 2.0**31
             acc.append(t)
         v = acc[s - 1].astype(np.float64)
         for d in range(s - 2, -1, -1):
             v = acc[d].astype(np.float64) + v * INV254
         sc = (alpha * muA)[:, None] * muB[None, :]
         part = (v * INV127SQ) * sc
+        if scale is not None:
+            part = part * scale
         prev = out if out is not None else C
         out = part if b == 0.0 else part + b * prev
         b = 1.0

@@ -127,16 +127,55 @@ def test_gemm_s8_k_split(ctx):
     assert np.all(np.abs(got - A @ B.T) <= E.bound(A, B, s))
 
 
+@pytest.mark.parametrize("k_extra", [0, 1])
+def test_gemm_s8_trims_and_packed_band(ctx, k_extra):
+    """Triangular K-trims on A and B (they skip only zeros, so the result is
+    the full product bit for bit) into a packed lower row band with a scaled
+    diagonal, accumulated onto its contents; once more with K beyond the
+    exact int32 range of one pass (the trims are then relative to each
+    pass)."""
+    import torch
+    from cosmoforge_amd import engine
+
+    s = 4
+    rng = np.random.default_rng(31 + k_extra)
+    m, n = 300, 200
+    k = (E.kmax(s) + 700) if k_extra else 900
+    offa, offb = (E.kmax(s) - 150, E.kmax(s) + 90) if k_extra else (37, 290)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    A[np.arange(k)[None, :] > np.arange(m)[:, None] + offa] = 0.0
+    B[np.arange(k)[None, :] > np.arange(n)[:, None] + offb] = 0.0
+    Sa, Sb = _dev_slices(ctx, A, s, 0), _dev_slices(ctx, B, s, 0)
+    row0, col0 = 250, 180                     # band rows 250..549, columns 180..379
+    pbase = row0 * (row0 + 1) // 2
+    size = (row0 + m) * (row0 + m + 1) // 2 - pbase
+    X0 = rng.standard_normal(size)
+    X = torch.tensor(X0, device=ctx.device)
+    engine.gemm_s8(ctx, m, n, k, 1.0, Sa, 0, Sb, 0, 1.0, X, 0, 0, tri_a=offa, tri_b=offb,
+                   packed=(row0, col0, pbase, 0.7))
+    ii = row0 + np.arange(m)[:, None]
+    cc = col0 + np.arange(n)[None, :]
+    ok = cc <= ii
+    idx = np.where(ok, ii * (ii + 1) // 2 + cc - pbase, 0)
+    C0 = np.where(ok, X0[idx], 0.0)
+    full = E.gemm(A, B, s, beta=1.0, C=C0, scale=np.where(cc == ii, 0.7, 1.0))
+    want = X0.copy()
+    want[idx[ok]] = full[ok]
+    np.testing.assert_array_equal(X.cpu().numpy(), want)
+
+
 def test_gemm_s8_rejects_bad_arguments(ctx):
     from cosmoforge_amd import engine
 
     S = engine.Slices(ctx, 4, 10, 64)
     C = ctx.empty(10, 10)
-    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 6, 10, 10, 64, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu),
-                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0.0, engine.P(C), 10, 0)
+    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 6, 10, 10, 64, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0,
+                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0, 0.0, engine.P(C), 10, 0, 0, 0, 0, 1.0)
     assert bad < 0 and b"slices" in ctx.lib.cfmi_last_error(ctx.h)
-    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 4, 10, 10, 200, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu),
-                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0.0, engine.P(C), 10, 0)
+    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 4, 10, 10, 200, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0,
+                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0, 0.0, engine.P(C), 10, 0, 0, 0, 0,
+                               1.0)
     assert bad < 0   # row pitch 64 below k = 200
 
 
