@@ -66,6 +66,7 @@ SIGNATURES = {
                           i64, i64, i64, cd]),
     "cfmi_jacobi_scale": (ci, [vp, i64, vp, i64, vp]),
     "cfmi_potrf": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
+    "cfmi_potrf_s8": (ci, [vp, ci, i64, vp, i64, vp, i64, _c.POINTER(ci)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),
     "cfmi_logdet_chol": (ci, [vp, i64, vp, i64, vp, _c.POINTER(cd)]),
     "cfmi_scale": (ci, [vp, ci, i64, i64, vp, i64, vp]),

@@ -9,6 +9,7 @@
 #include "cfmi_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -659,7 +660,7 @@ int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, in
 }
 
 template <typename T>
-int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu, int* info)
+int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu, int* info, int slices = 0)
 {
     T* A = (T*)Av;
     T* U = (T*)Uv;
@@ -712,8 +713,15 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
         const int64_t m2 = n - K0 - ob2;
         if (m2 > 0) {
             T* P2 = A + (K0 + ob2) + K0 * lda;
-            int rc = gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A + (K0 + ob2) + (K0 + ob2) * lda,
-                               lda, 16);
+            T* A22 = A + (K0 + ob2) + (K0 + ob2) * lda;
+            int rc;
+            if constexpr (std::is_same<T, double>::value) {
+                // FP64-accurate on the int8 matrix cores: P2's rows as int8 slices
+                rc = slices ? cfmi_s8_syrk_lower(c, slices, m2, ob2, -1.0, P2, lda, A22, lda)
+                            : gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A22, lda, 16);
+            } else {
+                rc = gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A22, lda, 16);
+            }
             if (rc) return rc;
         }
     }
@@ -892,6 +900,18 @@ int cfmi_potrf(cfmi_ctx* c, int dtype, int64_t n, void* A, int64_t lda, void* U,
     if (dtype == CFMI_F64) return potrf_t<double>(c, n, A, lda, U, ldu, info);
     if (dtype == CFMI_F32) return potrf_t<float>(c, n, A, lda, U, ldu, info);
     return cfmi_fail(c, CFMI_EINVAL, "potrf: unsupported dtype %d", dtype);
+}
+
+int cfmi_potrf_s8(cfmi_ctx* c, int s, int64_t n, double* A, int64_t lda, double* U, int64_t ldu, int* info)
+{
+    if (!c) return CFMI_EINVAL;
+    if (s < 3 || s > 5) return cfmi_fail(c, CFMI_EINVAL, "potrf_s8: %d slices (3 to 5 are built)", s);
+    if (n < 0 || lda < n) return cfmi_fail(c, CFMI_EINVAL, "potrf_s8: bad size n=%lld lda=%lld",
+                                           (long long)n, (long long)lda);
+    if (U && ldu < n) return cfmi_fail(c, CFMI_EINVAL, "potrf_s8: ldu < n");
+    if (info) *info = 0;
+    if (n == 0) return CFMI_OK;
+    return potrf_t<double>(c, n, A, lda, U, ldu, info, s);
 }
 
 int cfmi_trtri(cfmi_ctx* c, int dtype, int64_t n, const void* L, int64_t ldl, void* U, int64_t ldu)

@@ -22,7 +22,7 @@
 // Tiles 128 x 128, 8 waves of 32 x 64 (2 x 4 MFMA tiles of 16 x 16, S
 // int32 accumulators each), one 64-deep K-slab of every slice in LDS per
 // step (64-byte rows, 16-byte chunks XOR-swizzled by row), the next slab
-// prefetched into registers during the MFMAs.  Operand lane layout
+// copied into the other LDS stage during the MFMAs.  Operand lane layout
 // (measured on gfx950): lane l reads A[m = l % 16][k = 16 (l / 16) + j] and
 // B[k = 16 (l / 16) + j][n = l % 16], j = 0..15 (16 bytes); output row
 // 4 (l / 16) + r, column l % 16.
@@ -246,6 +246,48 @@ struct Stage8 {
 };
 #undef CFMI_S8_EACH
 
+// Per-lane source of the asynchronous slab copies: wave w moves rows
+// 16 w .. 16 w + 15 of every slice plane of both operands, one 1 KB
+// global_load_lds_dwordx4 per plane and operand (lane l lands at byte 16 l
+// of the wave's LDS destination: row 16 w + l / 4, chunk l % 4, so it reads
+// the global chunk that the XOR swizzle stores there).  Rows past the
+// matrix edge repeat the last row; the epilogue drops them.
+struct DmaSrc {
+    const char* a;             // tile bases (wave-uniform)
+    const char* b;
+    unsigned oa, ob;           // this lane's byte offsets from them
+
+    __device__ __forceinline__ DmaSrc(const S8Args& g, int64_t m0, int64_t n0, int wave, int lane)
+    {
+        const int r = 16 * wave + (lane >> 2);
+        const int cg = ((lane & 3) ^ swz(r)) * 16;
+        const int64_t ra = m0 + r < g.M ? r : g.M - 1 - m0;
+        const int64_t rb = n0 + r < g.N ? r : g.N - 1 - n0;
+        a = g.A + m0 * g.lda;
+        b = g.B + n0 * g.ldb;
+        oa = (unsigned)(ra * g.lda + cg);
+        ob = (unsigned)(rb * g.ldb + cg);
+    }
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int S>
+__device__ __forceinline__ void dma_slab(const S8Args& g, const DmaSrc& src, int64_t k0, char (*As)[BT][SK],
+                                         char (*Bs)[BT][SK], int wave)
+{
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        const char* pa = src.a + (p * g.psA + k0);
+        const char* pb = src.b + (p * g.psB + k0);
+        __builtin_amdgcn_global_load_lds(pa + src.oa, (lds_void*)&As[p][16 * wave][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(pb + src.ob, (lds_void*)&Bs[p][16 * wave][0], 16, 0, 0);
+    }
+}
+
+// The copies this wave issued have landed in LDS.
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <int S>
 __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (*Bs)[BT][SK], i4 (&acc)[S][2][4],
                                           int wm, int wn, int lm, int lk)
@@ -261,10 +303,13 @@ __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (
         i4 b[S];
 #pragma unroll
         for (int p = 0; p < S; ++p) b[p] = *reinterpret_cast<const i4*>(&Bs[p][wn + 16 * j + lm][kk]);
+        // p outer: consecutive products go to different accumulators, so a
+        // product waits on the one before it into the same accumulator only
+        // after 2 (S - p) others have issued
 #pragma unroll
-        for (int d = 0; d < S; ++d)
+        for (int p = 0; p < S; ++p)
 #pragma unroll
-            for (int p = 0; p <= d; ++p)
+            for (int d = p; d < S; ++d)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
                     acc[d][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][p], b[d - p], acc[d][i][j], 0, 0, 0);
@@ -282,15 +327,18 @@ __device__ __forceinline__ int64_t tile_kend(const S8Args& g, int64_t m0, int64_
     return k;
 }
 
-// DB = 1: two LDS stages, one barrier per slab: the slab prefetched into
-// registers during the MFMAs of slab k is stored into the other stage at the
-// start of step k + 1, while the stage of slab k + 1 (stored during step k)
-// is multiplied.  DB = 0: one stage, stored between two barriers.
+// DB = 1: two LDS stages filled by asynchronous global -> LDS copies
+// (global_load_lds_dwordx4, no registers in between), one barrier per
+// slab: the copy of slab k + 1 into one stage is in flight while slab k is
+// multiplied out of the other.  DB = 0: one stage, staged through registers
+// and stored between two barriers.
 template <int S, int DB>
 __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 {
-    __shared__ __attribute__((aligned(16))) char As[DB + 1][S][BT][SK];
-    __shared__ __attribute__((aligned(16))) char Bs[DB + 1][S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char As[S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char Bs[S][BT][SK];
+    __shared__ __attribute__((aligned(16))) char As1[DB ? S : 1][DB ? BT : 1][SK];
+    __shared__ __attribute__((aligned(16))) char Bs1[DB ? S : 1][DB ? BT : 1][SK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t z, ti, tj;
     if (!group_tile(g, blockIdx.x, z, ti, tj)) return;
@@ -309,38 +357,41 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
 
-    Stage8<S> st;
-    if (K > 0) st.load(g, m0, n0, 0, tid);
     if (K <= 0) {
         // the trims leave no k: the product is zero
     } else if constexpr (DB) {
-        st.store(As[0], Bs[0], tid);
-        if (SK < K) st.load(g, m0, n0, SK, tid);
+        const int uw = __builtin_amdgcn_readfirstlane(wave);   // wave-uniform: LDS targets in scalars
+        const DmaSrc src(g, m0, n0, uw, lane);
+        dma_slab<S>(g, src, 0, As, Bs, uw);
+        dma_wait();
         __syncthreads();
         for (int64_t k0 = 0;; k0 += SK) {
-            const int cur = (int)((k0 / SK) & 1);
+            const bool odd = (k0 / SK) & 1;
             const bool more = k0 + SK < K;
             if (more) {
-                st.store(As[cur ^ 1], Bs[cur ^ 1], tid);
-                if (k0 + 2 * SK < K) st.load(g, m0, n0, k0 + 2 * SK, tid);
+                if (odd) dma_slab<S>(g, src, k0 + SK, As, Bs, uw);
+                else dma_slab<S>(g, src, k0 + SK, As1, Bs1, uw);
             }
-            slab_mfma<S>(As[cur], Bs[cur], acc, wm, wn, lm, lk);
+            slab_mfma<S>(odd ? As1 : As, odd ? Bs1 : Bs, acc, wm, wn, lm, lk);
             if (!more) break;
+            dma_wait();
             __syncthreads();
         }
     } else {
+        Stage8<S> st;
+        st.load(g, m0, n0, 0, tid);
         int64_t k0 = 0;
         for (; k0 + SK < K; k0 += SK) {
             __syncthreads();
-            st.store(As[0], Bs[0], tid);
+            st.store(As, Bs, tid);
             __syncthreads();
             st.load(g, m0, n0, k0 + SK, tid);
-            slab_mfma<S>(As[0], Bs[0], acc, wm, wn, lm, lk);
+            slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
         }
         __syncthreads();
-        st.store(As[0], Bs[0], tid);
+        st.store(As, Bs, tid);
         __syncthreads();
-        slab_mfma<S>(As[0], Bs[0], acc, wm, wn, lm, lk);
+        slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
     }
 
     double sa[2][4], sb[4];
@@ -476,13 +527,9 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
     g.ngroups = g.gtmp * g.gtn;
     const int64_t nwg = (g.ngroups + 7) / 8 * 8 * g.gm * g.gn;
     if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: too many tiles");
-    if constexpr (S <= 4) {
-        if (db) {
-            hipLaunchKernelGGL((gemm_s8_kernel<S, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
-            CFMI_LAUNCH_CHECK(c);
-            return CFMI_OK;
-        }
-    }
+    if (db)
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    else
         hipLaunchKernelGGL((gemm_s8_kernel<S, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
@@ -564,7 +611,7 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
     if (psA < m * lda || psB < n * ldb || psA % 16 || psB % 16)
         return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: slice planes overlap or are misaligned");
     if (!aligned16(A) || !aligned16(B)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: operands not 16-byte aligned");
-    const bool lower = flags & 1, packed = flags & 16;
+    const bool lower = flags & (1 | 32), mirror = flags & 1, packed = flags & 16;
     if (!packed && ldc < m) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: ldc < m");
     if (lower && (m != n || packed)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
     if (packed && (row0 < 0 || col0 < 0 || pbase < 0 || pbase > row0 * (row0 + 1) / 2))
@@ -585,14 +632,14 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
         g.tri = ((flags & 4) ? 1 : 0) | ((flags & 8) ? 4 : 0);
         g.offA = off_a - k0; g.offB = off_b - k0;
         g.packed = packed ? 1 : 0; g.row0 = row0; g.col0 = col0; g.pbase = pbase; g.pdiag = pdiag;
-        // two LDS stages for s <= 4 (s = 5 would spill registers); flag 2: one stage
-        const bool db = s <= 4 && !(flags & 2);
+        // two LDS stages filled asynchronously; flag 2: one stage staged through registers
+        const bool db = !(flags & 2);
         int rc = s == 3 ? gemm_s8_launch<3>(c, g, db) : s == 4 ? gemm_s8_launch<4>(c, g, db)
-                                                              : gemm_s8_launch<5>(c, g, false);
+                                                              : gemm_s8_launch<5>(c, g, db);
         if (rc) return rc;
         b = 1.0;
     }
-    if (lower) {
+    if (mirror) {
         const unsigned t = (unsigned)((n + 63) / 64);
         hipLaunchKernelGGL(mirror_lower_s8_kernel, dim3(t, t), dim3(256), 0, c->stream, n, C, ldc);
         CFMI_LAUNCH_CHECK(c);
@@ -601,3 +648,16 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
 }
 
 }  // extern "C"
+
+int cfmi_s8_syrk_lower(cfmi_ctx* c, int s, int64_t n, int64_t k, double alpha, const double* P, int64_t ldp,
+                       double* C, int64_t ldc)
+{
+    if (n <= 0 || k <= 0) return CFMI_OK;
+    const int64_t ldk = round64(k), ps = n * ldk;
+    char* ws = (char*)cfmi_workspace(c, (size_t)(s * ps + 8 * n));
+    if (!ws) return cfmi_fail(c, CFMI_ENOMEM, "s8 syrk: workspace of %lld bytes", (long long)(s * ps + 8 * n));
+    double* mu = reinterpret_cast<double*>(ws + s * ps);    // s ps is a multiple of 64
+    int rc = cfmi_slice8(c, s, 1, n, k, P, ldp, ws, ldk, ps, mu);
+    if (rc) return rc;
+    return cfmi_gemm_s8(c, s, n, n, k, alpha, ws, ldk, ps, mu, 0, ws, ldk, ps, mu, 0, 1.0, C, ldc, 32, 0, 0, 0, 1.0);
+}

@@ -369,19 +369,26 @@ class Factor:
     logdet: float        # ln|C|
     U: torch.Tensor | None = None   # inverses of L's diagonal nb-blocks (when W is not formed)
     nb: int = 0
+    slices: int = 0                 # > 0: trailing updates on int8 slices (L L^T = D C D to ~2^-(8 s))
 
 
-def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, diag_blocks: int = 0) -> Factor:
+def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, diag_blocks: int = 0,
+           slices: int = 0) -> Factor:
     """Jacobi-scaled Cholesky of C (overwritten by L) and the whitener.
 
     ``diag_blocks`` = nb > 0 (instead of the whitener): ``Factor.U`` holds
     the inverses of L's nb x nb diagonal blocks at their places (the rest of
-    U is unused), the diagonal solves of the blocked ``trsm``."""
+    U is unused), the diagonal solves of the blocked ``trsm``.  ``slices``
+    = s > 0: the trailing updates of the factorisation on the int8 matrix
+    cores, operands as s int8 slices (``cfmi_potrf_s8``)."""
     d = torch.empty(n, dtype=torch.float64, device=ctx.device)
     ctx.check(ctx.lib.cfmi_jacobi_scale(ctx.h, n, P(C), n, P(d)), "jacobi_scale")
     U = ctx.empty(n, n) if (need_whitener or diag_blocks) else None
     info = ctypes.c_int(0)
-    rc = ctx.lib.cfmi_potrf(ctx.h, F64, n, P(C), n, P(U), n, ctypes.byref(info))
+    if slices:
+        rc = ctx.lib.cfmi_potrf_s8(ctx.h, slices, n, P(C), n, P(U), n, ctypes.byref(info))
+    else:
+        rc = ctx.lib.cfmi_potrf(ctx.h, F64, n, P(C), n, P(U), n, ctypes.byref(info))
     ctx.check(rc, "potrf")
     if info.value:
         raise LinAlgError(info.value)
@@ -390,7 +397,7 @@ def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, di
     if need_whitener:
         ctx.check(ctx.lib.cfmi_trtri(ctx.h, F64, n, P(C), n, P(U), n), "trtri")
         ctx.check(ctx.lib.cfmi_scale(ctx.h, 0, n, n, P(U), n, P(d)), "scale")
-        return Factor(C, U, d, float(ld.value))
+        return Factor(C, U, d, float(ld.value), slices=slices)
     if diag_blocks:
         if diag_blocks % 64:
             raise ValueError("diagonal blocks must be a multiple of 64 rows")
@@ -399,9 +406,9 @@ def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, di
             o = j0 + j0 * n
             ctx.check(ctx.lib.cfmi_trtri(ctx.h, F64, w, P(C, o), n, P(U, o), n), "trtri")
         f = Factor(C, None, d, float(ld.value))
-        f.U, f.nb = U, diag_blocks
+        f.U, f.nb, f.slices = U, diag_blocks, slices
         return f
-    return Factor(C, None, d, float(ld.value))
+    return Factor(C, None, d, float(ld.value), slices=slices)
 
 
 def gemm(ctx: Context, ta: bool, tb: bool, m: int, n: int, k: int, alpha: float, A, offa: int, lda: int,
@@ -782,7 +789,7 @@ def transpose(ctx: Context, A: torch.Tensor, m: int, n: int, lda: int) -> torch.
 
 
 def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs: int, transpose: bool = False,
-         slices: int = 0, outer: int = 2048):
+         slices: int = 0, outer: int = 2048, diag_slices: int = 0):
     """X = L^-1 B (or L^-T B with ``transpose``) for the factor's L, n x nrhs
     column-major with leading dimension n; B is consumed (it holds the
     running right-hand side).
@@ -795,7 +802,9 @@ def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs
     side is read and written n / outer times instead of n / nb.  Updates run
     in FP64, or on the int8 matrix cores with ``slices`` = s: the rows of L's
     panel (rows of L^T, kept once per factor) and the solved block's columns
-    are sliced per update, each row scaled over the panel's width."""
+    are sliced per update, each row scaled over the panel's width.  The
+    diagonal solves likewise with ``diag_slices`` (the block inverses sliced
+    once per factor; worth it for large nb, where K = nb is long)."""
     nb = f.nb
     if not nb or f.U is None:
         raise ValueError("trsm needs the factor's diagonal-block inverses (factor(..., diag_blocks=nb))")
@@ -826,6 +835,28 @@ def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs
         slice8(ctx, slices, 0, nrhs, w, X, k0, n, Bs)
         gemm_s8(ctx, m, nrhs, w, -1.0, As, 0, Bs, 0, 1.0, B, r0, n)
 
+    def diag_solve(j0: int, j1: int):
+        """X[j0:j1] = U_jj B[j0:j1] (forward) or U_jj^T B[j0:j1] (transpose)."""
+        nonlocal Ds
+        w = j1 - j0
+        o = j0 + j0 * n
+        if not diag_slices:
+            gemm(ctx, transpose, False, w, nrhs, w, 1.0, U, o, n, B, j0, n, 0.0, X, j0, n, 2 if transpose else 1)
+            return
+        # rows of U_jj (forward) or of U_jj^T = columns of U_jj (transpose)
+        if getattr(f, "u_slices", None) is None:
+            f.u_slices = {}
+        key = (diag_slices, transpose, j0)
+        if key not in f.u_slices:
+            f.u_slices[key] = (slice8(ctx, diag_slices, 0, w, w, U, o, n) if transpose
+                               else slice8(ctx, diag_slices, 1, w, w, U, o, n))
+        if Ds is None:
+            Ds = Slices(ctx, diag_slices, nrhs, nb)
+        slice8(ctx, diag_slices, 0, nrhs, w, B, j0, n, Ds)
+        gemm_s8(ctx, w, nrhs, w, 1.0, f.u_slices[key], 0, Ds, 0, 0.0, X, j0, n, tri_a=None if transpose else 0)
+
+    Ds = None
+
     outers = [(o0, min(n, o0 + outer)) for o0 in range(0, n, outer)]
     if transpose:
         outers = outers[::-1]
@@ -834,9 +865,7 @@ def trsm(ctx: Context, f: Factor, n: int, B: torch.Tensor, X: torch.Tensor, nrhs
         if transpose:
             inner = inner[::-1]
         for j0, j1 in inner:
-            w = j1 - j0
-            o = j0 + j0 * n
-            gemm(ctx, transpose, False, w, nrhs, w, 1.0, U, o, n, B, j0, n, 0.0, X, j0, n, 2 if transpose else 1)
+            diag_solve(j0, j1)
             if not transpose:
                 update(j1, o1, j0, j1)
             else:

@@ -131,7 +131,9 @@ class Fisher:
         if self.basis == "harmonic" and (self.precision == "mixed" or self.harmonic_solver == "trsm"):
             from .harmonic import HarmonicContraction
 
-            self.cov.build(diag_blocks=HarmonicContraction.TRSM_BLOCK)
+            mixed = self.precision == "mixed"
+            self.cov.build(diag_blocks=HarmonicContraction.MIXED_TRSM_BLOCK if mixed else HarmonicContraction.TRSM_BLOCK,
+                           slices=HarmonicContraction.FACTOR_SLICES if mixed else 0)
         else:
             self.cov.build()
         if not in_out._falsy(self.params.outinvcovmatfile1):

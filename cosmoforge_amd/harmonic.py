@@ -46,7 +46,13 @@ class HarmonicContraction:
 
     # int8 slices of the mixed products: Z = L^-1 D V, M = Z^T Z, R = D L^-T Z
     MIXED_SLICES = (4, 5, 4)
-    TRSM_BLOCK = 512
+    TRSM_BLOCK = 512          # diagonal blocks of the FP64 solves
+    # mixed: 2048-row diagonal blocks, their solves (products with the
+    # block inverses, K = 2048) on 5 int8 slices; the factorisation's
+    # trailing updates on 5 slices (Covariance.build(slices=...))
+    MIXED_TRSM_BLOCK = 2048
+    DIAG_SLICES = 5
+    FACTOR_SLICES = 5
 
     def __init__(self, cov: pipeline.Covariance, bins, convention: str = "Cl",
                  cov2: pipeline.Covariance | None = None, precision: str = "fp64", slices=None):
@@ -145,7 +151,7 @@ class HarmonicContraction:
             if self._blocked(cov):
                 B = self.V.clone()
                 ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(B), n, P(cov.factor.d)), "scale")
-                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self._s(0))
+                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self._s(0), diag_slices=self._ds())
                 del B
             else:
                 engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
@@ -171,7 +177,7 @@ class HarmonicContraction:
         R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
         if self._blocked(cov):
             del self._Z[id(cov)]
-            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self._s(2))
+            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self._s(2), diag_slices=self._ds())
             del Z
             ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(R), n, P(cov.factor.d)), "scale")
             return R
@@ -188,17 +194,37 @@ class HarmonicContraction:
         """Slices of product i (0 when the product runs in FP64)."""
         return self.slices[i] if self.precision == "mixed" else 0
 
-    def _solve(self, cov, maps: np.ndarray) -> torch.Tensor:
-        """y = C^-1 x = D L^-T L^-1 D x in FP64 (column-major n x nsims)."""
+    def _ds(self) -> int:
+        """Slices of the blocked solves' diagonal products (long K only)."""
+        nb = getattr(self.cov.factor, "nb", 0)
+        return self.DIAG_SLICES if self.precision == "mixed" and nb >= self.MIXED_TRSM_BLOCK else 0
+
+    @staticmethod
+    def _apply_inv(cov, X: torch.Tensor, ns: int) -> torch.Tensor:
+        """D L^-T L^-1 D X in FP64 (X is consumed)."""
         ctx, n = cov.ctx, cov.n
-        ns = maps.shape[1]
-        X = ctx.upload_cm(maps)
         ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, ns, P(X), n, P(cov.factor.d)), "scale")
         T = ctx.empty(n, ns)
         engine.trsm(ctx, cov.factor, n, X, T, ns)
         Y = ctx.empty(n, ns)
         engine.trsm(ctx, cov.factor, n, T, Y, ns, transpose=True)
         ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, ns, P(Y), n, P(cov.factor.d)), "scale")
+        return Y
+
+    def _solve(self, cov, maps: np.ndarray) -> torch.Tensor:
+        """y = C^-1 x (column-major n x nsims), FP64 solves with the factor.
+        A factor with sliced trailing updates reproduces C only to ~2^-(8 s)
+        relative, so y then takes one step of FP64 refinement against
+        C = S + N: y += C~^-1 (x - S y - N y)."""
+        ctx, n = cov.ctx, cov.n
+        ns = maps.shape[1]
+        Y = self._apply_inv(cov, ctx.upload_cm(maps), ns)
+        if cov.factor.slices:
+            Rr = ctx.upload_cm(maps)
+            engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.S, 0, n, Y, 0, n, 1.0, Rr, 0, n)
+            engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.N, 0, n, Y, 0, n, 1.0, Rr, 0, n)
+            D = self._apply_inv(cov, Rr, ns)
+            ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, ns, 1.0, P(D), n, 1.0, P(Y), n), "axpby")
         return Y
 
     def _ut(self, R: torch.Tensor, maps: np.ndarray) -> torch.Tensor:

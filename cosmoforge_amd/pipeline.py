@@ -87,17 +87,18 @@ class Covariance:
         self.S = engine.signal_matrix(ctx, self.inp.layout, self.vecs, dev, self.inp.lmax_signal, out=self.S)
         return self.S
 
-    def build(self, cls=None, need_whitener: bool = True, diag_blocks: int = 0) -> engine.Factor:
+    def build(self, cls=None, need_whitener: bool = True, diag_blocks: int = 0, slices: int = 0) -> engine.Factor:
         """S, then C = S + N factorised in a separate buffer (S is kept).
         ``diag_blocks``: keep the inverses of L's diagonal blocks (for
-        ``engine.trsm``) instead of forming the whitener."""
+        ``engine.trsm``) instead of forming the whitener; ``slices``: the
+        factorisation's trailing updates on int8 slices (``engine.factor``)."""
         ctx, n = self.ctx, self.n
         S = self.signal(cls)
         C = ctx.empty(n, n)
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(S), n, 0.0, P(C), n), "axpby")
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
         self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener and not diag_blocks,
-                                    diag_blocks=diag_blocks)
+                                    diag_blocks=diag_blocks, slices=slices)
         self._w32 = None
         self._wp = None
         return self.factor

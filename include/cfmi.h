@@ -244,7 +244,10 @@ int64_t cfmi_s8_kmax(int s);
  * p + q < s on v_mfma_i32_16x16x64_i8 (exact int32 sums), levels combined
  * in FP64; k beyond cfmi_s8_kmax is split and the parts added in FP64.
  * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored;
- * bit 1: one LDS stage instead of two (s <= 4; s = 5 always uses one);
+ * bit 5: the lower triangle only (m == n; the strict upper part of C is
+ * left as it was);
+ * bit 1: one LDS stage staged through registers instead of two filled by
+ * asynchronous global -> LDS copies;
  * bit 2: A[m][k] = 0 for k > m + off_a, bit 3: B[n][k] = 0 for k > n + off_b
  * (the operands hold those zeros; the bits only shorten K per tile);
  * bit 4: C is the packed lower row band of the whitening (element (i, j <= i)
@@ -266,6 +269,14 @@ int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *
  * be completed by cfmi_trtri.  *info (host) receives the LAPACK info.      */
 int cfmi_potrf(cfmi_ctx *ctx, int dtype, int64_t n, void *A, int64_t lda, void *U, int64_t ldu,
                int *info);
+
+/* cfmi_potrf (FP64) with the rank-2048 trailing updates, nearly all of
+ * the n^3/3 flops, on the int8 matrix cores: each panel's rows as s int8
+ * slices (3 <= s <= 5; each entry to 2^-(8 s) of its row's largest),
+ * exact int32 sums combined in FP64.  No reference counterpart: the
+ * mixed-precision factorisation of the covariance for the harmonic route. */
+int cfmi_potrf_s8(cfmi_ctx *ctx, int s, int64_t n, double *A, int64_t lda, double *U, int64_t ldu,
+                  int *info);
 
 /* Complete U = L^-1 from cfmi_potrf's L and diagonal blocks of U.        */
 int cfmi_trtri(cfmi_ctx *ctx, int dtype, int64_t n, const void *L, int64_t ldl, void *U,

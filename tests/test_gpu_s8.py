@@ -179,8 +179,11 @@ def test_gemm_s8_rejects_bad_arguments(ctx):
     assert bad < 0   # row pitch 64 below k = 200
 
 
-@pytest.mark.parametrize("slices,outer", [(0, 2048), (4, 2048), (5, 2048), (0, 384), (4, 384)])
-def test_blocked_trsm(ctx, slices, outer):
+@pytest.mark.parametrize("slices,outer,diag", [(0, 2048, 0), (4, 2048, 0), (5, 2048, 0), (0, 384, 0), (4, 384, 0),
+                                               (4, 2048, 5), (0, 384, 5)])
+def test_blocked_trsm(ctx, slices, outer, diag):
+    """Blocked L^-1 B and L^-T B against the host solves; updates and
+    diagonal products in FP64 or on int8 slices."""
     import torch
     from scipy.linalg import solve_triangular
 
@@ -195,12 +198,52 @@ def test_blocked_trsm(ctx, slices, outer):
     Lh = np.tril(engine.to_numpy_cm(f.L))
     Bh = rng.standard_normal((n, nrhs))
     X = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
-    engine.trsm(ctx, f, n, ctx.upload_cm(Bh), X, nrhs, slices=slices, outer=outer)
+    engine.trsm(ctx, f, n, ctx.upload_cm(Bh), X, nrhs, slices=slices, outer=outer, diag_slices=diag)
     want = solve_triangular(Lh, Bh, lower=True)
     err = np.max(np.abs(engine.to_numpy_cm(X) - want)) / np.max(np.abs(want))
     Y = torch.zeros((nrhs, n), dtype=torch.float64, device=ctx.device)
-    engine.trsm(ctx, f, n, X.clone(), Y, nrhs, transpose=True, slices=slices, outer=outer)
+    engine.trsm(ctx, f, n, X.clone(), Y, nrhs, transpose=True, slices=slices, outer=outer, diag_slices=diag)
     want2 = solve_triangular(Lh.T, want, lower=False)
     err2 = np.max(np.abs(engine.to_numpy_cm(Y) - want2)) / np.max(np.abs(want2))
-    tol = {0: 1e-12, 4: 1e-7, 5: 1e-9}[slices]
+    tol = {0: 1e-12, 4: 1e-7, 5: 1e-9}[slices] if not diag else {0: 1e-9, 4: 1e-7}[slices]
     assert err < tol and err2 < tol, (err, err2)
+
+
+@pytest.mark.parametrize("s", [4, 5])
+def test_potrf_s8_matches_fp64_factor(ctx, s):
+    """The Cholesky factor with its rank-2048 trailing updates on int8
+    slices equals the FP64 factor to the slicing level (n = 5000: two
+    sliced updates), keeps the strict upper triangle zero, and fills the
+    diagonal-block inverses as cfmi_potrf does."""
+    import ctypes
+
+    import torch
+    from cosmoforge_amd import engine
+
+    n = 5000
+    rng = np.random.default_rng(40 + s)
+    X = rng.standard_normal((n, n // 2))
+    A = X @ X.T / n + np.eye(n)                     # SPD, cond ~ 10
+    d = 1.0 / np.sqrt(np.diag(A))
+    A = d[:, None] * A * d[None, :]
+    out = {}
+    for mode in ("fp64", "s8"):
+        C = ctx.upload_cm(A)
+        U = ctx.empty(n, n)
+        info = ctypes.c_int(0)
+        if mode == "fp64":
+            rc = ctx.lib.cfmi_potrf(ctx.h, engine.F64, n, engine.P(C), n, engine.P(U), n, ctypes.byref(info))
+        else:
+            rc = ctx.lib.cfmi_potrf_s8(ctx.h, s, n, engine.P(C), n, engine.P(U), n, ctypes.byref(info))
+        assert rc == 0 and info.value == 0
+        out[mode] = (engine.to_numpy_cm(C), engine.to_numpy_cm(U))
+        del C, U
+        torch.cuda.empty_cache()
+    (L, U), (Ls, Us) = out["fp64"], out["s8"]
+    assert not np.triu(Ls, 1).any()
+    err = np.max(np.abs(Ls - L))
+    print(f"s={s}: max |L_s8 - L_fp64| = {err:.2e}")
+    assert err < (1e-6 if s == 4 else 1e-8)
+    np.testing.assert_allclose(Us[:64, :64], np.linalg.inv(Ls[:64, :64]), atol=1e-12)
+    bad = ctx.lib.cfmi_potrf_s8(ctx.h, 6, 10, engine.P(ctx.empty(10, 10)), 10, None, 10, None)
+    assert bad < 0 and b"slices" in ctx.lib.cfmi_last_error(ctx.h)

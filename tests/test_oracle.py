@@ -266,3 +266,38 @@ def test_filtered_oracle_pins():
     Fd, _, _ = P.fisher_filtered(lay, cls_s, c.lmax_signal, c.noise, prm, Wd, None, S=S)
     assert np.linalg.eigvalsh(F - Fd).min() > -1e-9 * np.abs(F).max()
     assert np.abs(F - Fd).max() > 1e-6 * np.abs(F).max()
+
+
+def test_config1_fixture_against_the_direct_trace():
+    """tests/golden/config1_nside16_TQU.npz holds F as 1/2 <B_p, B_q> with
+    B_p = R^T dC_p R (C^-1 = R R^T), formed block by block: three entries
+    (an auto, an off-diagonal pair across spectra, a TB x TE pair) against
+    the oracle's own 1/2 Tr[C^-1 dC_p C^-1 dC_q], and q_p, nb_p of one
+    parameter against the oracle's formulas on the stored map."""
+    import os
+
+    from cosmoforge_amd import workloads
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config1_nside16_TQU.npz"))
+    cfg = workloads.config(1)
+    p = cfg.params
+    lay = K.layout_from_mask(np.array(cfg.mask), p["spins"], p["labels"], p["nside"])
+    L = p["lmax_signal"]
+    beams = P.beam_by_label(lay, cfg.beam, L)
+    cls_s = P.smoothed_cls(lay, cfg.cls, beams, L)
+    prm = P.parameter_weights(lay, beams, p["lmin"], p["lmax"], L)
+    assert int(g["n"]) == lay.n and np.array_equal(g["weights"], np.stack([w for *_, w in prm]))
+    N = np.diag(cfg.noise_diag)
+    Cinv = LA.matrix_inverse_symm(K.signal_matrix(lay, cls_s, L) + N)
+    A = {}
+    for k in (3, 120, 250, 210):
+        fi, fj, mode, _, w = prm[k]
+        dC = K.binned_derivative(lay, fi, fj, mode, w)
+        A[k] = Cinv @ dC
+        if k == 120:
+            y = Cinv @ g["map"]
+            nb = 0.5 * K.matrix_trace(Cinv @ N @ Cinv, dC)
+            assert nb == pytest.approx(g["nb"][k], rel=1e-10)
+            assert 0.5 * y @ (dC @ y) - nb == pytest.approx(g["q"][k], rel=1e-10, abs=1e-10 * abs(nb))
+    for i, j in ((3, 3), (3, 120), (250, 210)):
+        assert 0.5 * K.matrix_trace(A[i], A[j]) == pytest.approx(g["F"][i, j], rel=1e-10)
