@@ -7,10 +7,14 @@ One step = one complete Fisher + QML solve with the inputs resident in HBM:
 pointings, S, C = S + N and its Cholesky factor, the harmonic-mode
 contraction of the traces (F from M = V^T C^-1 V, q from y = C^-1 x, the
 noise bias from C^-1 V), and the F^-1 normalisation.  The mixed precision
-runs the three large products on the int8 matrix cores with sliced operands
-(exact integer sums; the Cholesky, diagonal solves and y in FP64); the FP64
-precision runs the same algorithm with FP64 MFMA products.  Both are timed
-over K steps after W warm-ups; ``value`` is the mixed seconds per solve.
+runs the large products on the int8 matrix cores with sliced operands
+(exact integer sums): the Cholesky's trailing updates, the blocked solves
+for C^-1 V (updates and 2048-row diagonal products) and M; the Cholesky
+panels, y = C^-1 x (refined once against S + N) and the recurrences stay
+FP64.  The FP64 precision runs the same algorithm with FP64 MFMA products
+(blocked solves with 512-row diagonal blocks, its fastest measured setting).
+Both are timed over K steps after W warm-ups; ``value`` is the mixed
+seconds per solve.
 
 Outside the timed loops: the pixel-space route of the reference's
 formulation (per-parameter whitened derivatives, FP64) once, as the accuracy
@@ -334,22 +338,21 @@ def main():
         for w in range(args.warmup):
             solve(prec)
             log(f"[bench] {prec} warmup {w + 1}/{args.warmup} done")
-        rec = engine.S8Recorder() if record else None
         torch.cuda.synchronize()
-        engine.S8_RECORDER = rec
+        rec = engine.S8Recorder(ctx) if record else None
         t0 = time.perf_counter()
         for k in range(args.steps):
             res = solve(prec)
             log(f"[bench] {prec} step {k + 1}/{args.steps} done")
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
-        engine.S8_RECORDER = None
         return dt, res, rec
 
     t_mixed, res_m, rec = timed("mixed", record=True)
     t_fp64, res_f, _ = timed("fp64")
     n_modes, flops = res_m[5], res_m[6]
     fl, ms, launches, prod = rec.total()
+    rec.close()
     achieved = fl / (ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -362,8 +365,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 MFMA on sliced FP64 operands (exact int32 sums) for C^-1 V, V^T C^-1 V, C^-1 x; FP64 "
-                 "Cholesky, diagonal solves, recurrences, y = C^-1 x",
+        "dtype": "int8 MFMA on sliced FP64 operands (exact int32 sums) for the Cholesky trailing updates, C^-1 V "
+                 "and V^T C^-1 V; FP64 Cholesky panels, recurrences, y = C^-1 x",
         "data": "synthetic (seeded map of S+N; reference-recipe spectra, mask, beam, noise)",
         "config": {"workload": cfg.name, "n_rows": cfg.n, "n_params": int(res_m[0].shape[0]),
                    "lmax_signal": cfg.params["lmax_signal"], "nsims": 1, "n_modes": int(n_modes),

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/cfmi.h"
 
@@ -23,6 +24,17 @@ struct cfmi_ctx {
     // Growable workspace for split-K partials and trtri temporaries.
     void* d_work = nullptr;
     size_t work_bytes = 0;
+    // Sliced-GEMM launch record (cfmi_s8_stats_*): HIP events around every
+    // gemm_s8 kernel launch on the stream while enabled, with each launch's
+    // algorithmic flops and slice count.
+    bool s8_stats = false;
+    struct S8Launch {
+        hipEvent_t start, stop;
+        double flops;
+        int s;
+    };
+    std::vector<S8Launch> s8_launches;
+    std::vector<hipEvent_t> s8_event_pool;
 };
 
 #define CFMI_RED_SLOTS 65536

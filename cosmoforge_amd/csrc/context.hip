@@ -73,6 +73,11 @@ int cfmi_destroy(cfmi_ctx* c)
 {
     if (!c) return CFMI_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& l : c->s8_launches) {
+        hipEventDestroy(l.start);
+        hipEventDestroy(l.stop);
+    }
+    for (auto e : c->s8_event_pool) hipEventDestroy(e);
     if (c->d_work) hipFree(c->d_work);
     if (c->d_info) hipFree(c->d_info);
     if (c->d_red) hipFree(c->d_red);

@@ -514,6 +514,18 @@ int slice_t(cfmi_ctx* c, int layout, int64_t rows, int64_t k, const double* src,
     return CFMI_OK;
 }
 
+hipEvent_t s8_event(cfmi_ctx* c)
+{
+    if (!c->s8_event_pool.empty()) {
+        hipEvent_t e = c->s8_event_pool.back();
+        c->s8_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
 template <int S>
 int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
 {
@@ -527,11 +539,23 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
     g.ngroups = g.gtmp * g.gtn;
     const int64_t nwg = (g.ngroups + 7) / 8 * 8 * g.gm * g.gn;
     if (nwg > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: too many tiles");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->s8_stats) {
+        e0 = s8_event(c);
+        e1 = s8_event(c);
+        if (!e0 || !e1) return cfmi_fail(c, CFMI_EHIP, "gemm_s8: hipEventCreate failed");
+        CFMI_HIP(c, hipEventRecord(e0, c->stream));
+    }
     if (db)
         hipLaunchKernelGGL((gemm_s8_kernel<S, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     else
         hipLaunchKernelGGL((gemm_s8_kernel<S, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     CFMI_LAUNCH_CHECK(c);
+    if (c->s8_stats) {
+        CFMI_HIP(c, hipEventRecord(e1, c->stream));
+        const double f = 2.0 * (double)g.M * (double)g.N * (double)g.K * (g.lower ? 0.5 : 1.0);
+        c->s8_launches.push_back({e0, e1, f, S});
+    }
     return CFMI_OK;
 }
 
@@ -592,6 +616,38 @@ int cfmi_unslice8(cfmi_ctx* c, int s, int64_t rows, int64_t k, const void* src, 
     hipLaunchKernelGGL(unslice_kernel, grid_of(rows * k), dim3(256), 0, c->stream, s, rows, k,
                        (const signed char*)src, ldk, ps, mu, dst, ldd);
     CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_s8_stats_enable(cfmi_ctx* c, int on)
+{
+    if (!c) return CFMI_EINVAL;
+    CFMI_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& l : c->s8_launches) {
+        c->s8_event_pool.push_back(l.start);
+        c->s8_event_pool.push_back(l.stop);
+    }
+    c->s8_launches.clear();
+    c->s8_stats = on != 0;
+    return CFMI_OK;
+}
+
+int cfmi_s8_stats_read(cfmi_ctx* c, int64_t* launches, double* ms, double* flops, double* products)
+{
+    if (!c) return CFMI_EINVAL;
+    CFMI_HIP(c, hipStreamSynchronize(c->stream));
+    double t = 0.0, f = 0.0, p = 0.0;
+    for (auto& l : c->s8_launches) {
+        float dt = 0.0f;
+        CFMI_HIP(c, hipEventElapsedTime(&dt, l.start, l.stop));
+        t += dt;
+        f += l.flops;
+        p += l.flops * l.s * (l.s + 1) / 2;
+    }
+    if (launches) *launches = (int64_t)c->s8_launches.size();
+    if (ms) *ms = t;
+    if (flops) *flops = f;
+    if (products) *products = p;
     return CFMI_OK;
 }
 

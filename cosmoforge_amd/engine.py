@@ -724,25 +724,26 @@ def slice8(ctx: Context, s: int, layout: int, rows: int, k: int, src: torch.Tens
 
 
 class S8Recorder:
-    """HIP events (torch's, on the context's stream) around every
-    ``gemm_s8`` launch while installed as ``engine.S8_RECORDER``, with the
-    algorithmic flops 2 m n k (m n k for a symmetric result, whose upper
-    tiles are not computed) and the slice count of each launch."""
+    """The context's sliced-GEMM launch record while open
+    (``cfmi_s8_stats_*``): HIP events on the context's stream around every
+    gemm_s8 kernel launch, those issued from here and those the library
+    issues itself (the sliced Cholesky updates), with each launch's
+    algorithmic flops 2 m n k (m n k for a lower or symmetric result)."""
 
-    def __init__(self):
-        self.launches = []
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        ctx.check(ctx.lib.cfmi_s8_stats_enable(ctx.h, 1), "s8_stats_enable")
 
     def total(self):
         """(sum of algorithmic flops, sum of launch times in ms, launches,
         int8 products issued per algorithmic multiply-add, averaged)."""
-        torch.cuda.synchronize()
-        fl = sum(f for _, _, f, _ in self.launches)
-        ms = sum(a.elapsed_time(b) for a, b, _, _ in self.launches)
-        prod = sum(f * s * (s + 1) / 2 for _, _, f, s in self.launches) / max(fl, 1.0)
-        return fl, ms, len(self.launches), prod
+        n, ms, fl, prod = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        self.ctx.check(self.ctx.lib.cfmi_s8_stats_read(self.ctx.h, ctypes.byref(n), ctypes.byref(ms),
+                                                       ctypes.byref(fl), ctypes.byref(prod)), "s8_stats_read")
+        return fl.value, ms.value, int(n.value), prod.value / max(fl.value, 1.0)
 
-
-S8_RECORDER: S8Recorder | None = None
+    def close(self):
+        self.ctx.check(self.ctx.lib.cfmi_s8_stats_enable(self.ctx.h, 0), "s8_stats_enable")
 
 
 def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row0: int, B: Slices, b_row0: int,
@@ -757,10 +758,6 @@ def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row
         return
     if A.s != B.s:
         raise ValueError("operands sliced with different slice counts")
-    rec = S8_RECORDER
-    if rec is not None:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e0.record()
     if tri_a is not None:
         flags |= 4
     if tri_b is not None:
@@ -774,11 +771,6 @@ def gemm_s8(ctx: Context, m: int, n: int, k: int, alpha: float, A: Slices, a_row
                                    P(A.mu, a_row0), tri_a or 0, P(B.data, b_row0 * B.ldk), B.ldk, B.ps,
                                    P(B.mu, b_row0), tri_b or 0, beta, P(C, offc), ldc, flags, row0, col0, pbase,
                                    pdiag), "gemm_s8")
-    if rec is not None:
-        e1 = torch.cuda.Event(enable_timing=True)
-        e1.record()
-        flops = (1.0 if flags & 1 else 2.0) * m * n * k
-        rec.launches.append((e0, e1, flops, A.s))
 
 
 def transpose(ctx: Context, A: torch.Tensor, m: int, n: int, lda: int) -> torch.Tensor:

@@ -270,6 +270,16 @@ int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *
 int cfmi_potrf(cfmi_ctx *ctx, int dtype, int64_t n, void *A, int64_t lda, void *U, int64_t ldu,
                int *info);
 
+/* Launch record of the sliced GEMM on this context (no reference
+ * counterpart; measurement): while enabled, HIP events on the context's
+ * stream bracket every gemm_s8 kernel launch, including those the library
+ * issues itself (cfmi_potrf_s8).  enable(on) synchronizes and clears the
+ * record; read synchronizes and returns the launch count, the summed
+ * launch time (ms), the algorithmic flops (2 m n k, m n k for a lower /
+ * symmetric result) and the int8 products (flops s (s + 1) / 2).          */
+int cfmi_s8_stats_enable(cfmi_ctx *ctx, int on);
+int cfmi_s8_stats_read(cfmi_ctx *ctx, int64_t *launches, double *ms, double *flops, double *products);
+
 /* cfmi_potrf (FP64) with the rank-2048 trailing updates, nearly all of
  * the n^3/3 flops, on the int8 matrix cores: each panel's rows as s int8
  * slices (3 <= s <= 5; each entry to 2^-(8 s) of its row's largest),
