@@ -251,12 +251,16 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
         y1[q] = 1.0 - xv * xv;
     }
 
+    // The recurrence and weight tables, read through the constant address
+    // space: with l uniform these become scalar loads (one per step for the
+    // whole wave) instead of a vector load per lane.
+    typedef const __attribute__((address_space(4))) double ctab;
     const int L1 = p.L + 1;
-    const double* cA = p.tab;
-    const double* cB = p.tab + L1;
-    const double* w0 = p.tab + 2 * L1;
-    const double* w1 = p.tab + 3 * L1;
-    const double* w2 = p.tab + 4 * L1;
+    ctab* cA = (ctab*)p.tab;
+    ctab* cB = (ctab*)p.tab + L1;
+    ctab* w0 = (ctab*)p.tab + 2 * L1;
+    ctab* w1 = (ctab*)p.tab + 3 * L1;
+    ctab* w2 = (ctab*)p.tab + 4 * L1;
 
     // Accumulators: 00 -> acc0; 22 -> A1,A2 (w0), B1,B2 (w1), E (w2); 02 -> tq (w0), tu (w1)
     double acc[5][PPT];
@@ -274,9 +278,13 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs p)
     // Recurrence state slot of this thread's pairs (tile-major).
     double* st = p.state ? p.state + ((int64_t)(blockIdx.x + (int64_t)blockIdx.y * gridDim.x) * 256 + t) * (2 * PPT)
                          : nullptr;
+    // l, and with it the table reads and the bin tests below, is the same
+    // for every lane: decided from kernel arguments only, so that it lives
+    // in a scalar register (a per-lane l turns every table read into a
+    // vector load and every branch into an exec-masked one)
     int lstart = (KIND == K00S || KIND == K00G) ? 0 : 2;
-    if (st && p.lresume > lstart) {
-        lstart = p.lresume;
+    if (p.state && p.lresume > lstart) {
+        lstart = __builtin_amdgcn_readfirstlane(p.lresume);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             pm1[q] = st[2 * q];
