@@ -44,11 +44,11 @@ sys.path.insert(0, ROOT)
 METRIC = "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error"
 INT8_MFMA_PEAK_TOPS = 5000.0       # MI355X dense int8 MFMA
 FP64_MFMA_PEAK_TFLOPS = 78.6
-# HBM bytes per gemm_s8 launch of the config-2 mixed solve, from rocprofv3
-# --pmc FETCH_SIZE and --pmc WRITE_SIZE over one solve (each scaled by what
-# the counter reads for a plain 4 GiB copy), divided by the launches; not
-# measured live.  Keyed by config.
-S8_TRAFFIC_BYTES_PER_LAUNCH = {}
+# HBM bytes per gemm_s8 launch of the config-2 mixed solve (mean over its 58
+# launches), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one
+# solve, each scaled by what the counter reads for a plain 1 GiB device copy
+# (FETCH x 2.0, WRITE x 1.0); not measured live.  Keyed by config.
+S8_TRAFFIC_BYTES_PER_LAUNCH = {2: 11.74e9}
 
 
 def log(msg):
@@ -270,6 +270,56 @@ def surrogate_leg(ctx, patches, tun, plain_rate, n_train=20000, n_cand=65536):
                 "min_exact_minus_best_kept": float(np.min(exact_chk) - np.min(rep.value[ex])) if chk.size else None}}
 
 
+def stages_leg(ctx, cfg, reps):
+    """S (pair recurrences), the FP64 Cholesky and the sliced one (5 int8
+    slices in the trailing updates) of the workload's C, ``reps`` times
+    each, with the FP64 VALU flops of S and n^3/3 per factorisation."""
+    import ctypes
+
+    import torch
+
+    from cosmoforge_amd import engine, pipeline
+    from cosmoforge_amd.core import Inputs
+
+    inp = Inputs(cfg.params, cfg.mask, cfg.noise, cfg.cls, cfg.beam)
+    cov = pipeline.Covariance(ctx, inp)
+    n = inp.n
+    out = {"workload": cfg.name, "n": n, "reps": reps}
+    ts = []
+    for _ in range(reps):
+        cov.S = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S = cov.signal()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    out["signal_ms"] = round(1e3 * min(ts), 2)
+    for name, s in (("potrf_fp64", 0), ("potrf_s8", 5)):
+        ts = []
+        for _ in range(reps):
+            C = ctx.empty(n, n)
+            ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, engine.P(S), n, 0.0, engine.P(C), n), "axpby")
+            ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, engine.P(cov.N), n, 1.0, engine.P(C), n), "axpby")
+            d = torch.empty(n, dtype=torch.float64, device=ctx.device)
+            ctx.check(ctx.lib.cfmi_jacobi_scale(ctx.h, n, engine.P(C), n, engine.P(d)), "jacobi_scale")
+            info = ctypes.c_int(0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if s:
+                rc = ctx.lib.cfmi_potrf_s8(ctx.h, s, n, engine.P(C), n, None, n, ctypes.byref(info))
+            else:
+                rc = ctx.lib.cfmi_potrf(ctx.h, engine.F64, n, engine.P(C), n, None, n, ctypes.byref(info))
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            ctx.check(rc, name)
+            del C, d
+            torch.cuda.empty_cache()
+        t = min(ts)
+        out[name + "_ms"] = round(1e3 * t, 2)
+        out[name + "_tflops_n3_over_3"] = round(n ** 3 / 3 / t / 1e12, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +332,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-stride", type=int, default=0, help="CPU sample: every k-th active pixel (0: auto)")
+    ap.add_argument("--stages", action="store_true",
+                    help="time the covariance stages alone (S, the FP64 and the sliced Cholesky) and exit; "
+                         "the command the kernel profiles of those stages run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -294,6 +347,10 @@ def main():
     from cosmoforge_amd.accuracy import errors
 
     ctx = engine.Context(0)
+
+    if args.stages:
+        print(json.dumps(stages_leg(ctx, workloads.config(args.config), args.steps)))
+        return
 
     if args.config in (3, 4):
         pl, (patches, tun) = picslike_leg(ctx)
