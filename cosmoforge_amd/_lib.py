@@ -81,6 +81,8 @@ SIGNATURES = {
     "cfmi_harmonic_fisher": (ci, [vp, vp, vp, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci, vp]),
     "cfmi_harmonic_pairsum": (ci, [vp, i64, vp, i64, i64, vp, i64, i64, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci,
                                    cd, vp, i64]),
+    "cfmi_harmonic_coldots": (ci, [vp, i64, vp, i64, vp, i64, vp, ci, i64, ci, ci, vp, vp, ci, vp, vp, vp, ci, cd,
+                                   vp]),
     "cfmi_batch_create": (ci, [vp, ci, ci, vp, ci, vp, vp, vp, vp, ci, vp, ci, ci, ci, _c.POINTER(vp)]),
     "cfmi_batch_run": (ci, [vp, vp, vp, vp, vp]),
     "cfmi_batch_capture": (ci, [vp, vp, vp, vp, vp]),

@@ -181,6 +181,94 @@ __device__ double block_sum(double v, double* red)
     return s;
 }
 
+// Column dot products of every mode for every pair of mode groups:
+// dots[(k G + a) G + b] = sum_i wrow_i X[i, goff[a] + k] Y[i, goff[b] + k]
+// (wrow = 1 when null), a, b < G <= 3.  One block per mode k reads each of
+// its G (or 2 G) columns once, however many group pairs use them; the sums
+// are reduced in a fixed order.
+__global__ __launch_bounds__(RT) void harm_coldots_kernel(int64_t ni, const double* __restrict__ X, int64_t ldx,
+                                                          const double* __restrict__ Y, int64_t ldy,
+                                                          const double* __restrict__ wrow,
+                                                          const int64_t* __restrict__ goff, int G,
+                                                          double* __restrict__ dots)
+{
+    __shared__ double red[9][RT / 64];
+    const int64_t k = blockIdx.x;
+    const bool same = X == Y && ldx == ldy;
+    const double* xc[3];
+    const double* yc[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const int64_t col = g < G ? goff[g] + k : goff[0] + k;
+        xc[g] = X + col * ldx;
+        yc[g] = Y + col * ldy;
+    }
+    double acc[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc[a][b] = 0.0;
+    for (int64_t i = threadIdx.x; i < ni; i += RT) {
+        const double wi = wrow ? wrow[i] : 1.0;
+        double x[3], y[3];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            if (g < G) {
+                const double xv = xc[g][i];
+                x[g] = xv * wi;
+                y[g] = same ? xv : yc[g][i];
+            } else {
+                x[g] = 0.0;
+                y[g] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a][b] += x[a] * y[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            double v = acc[a][b];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0) red[3 * a + b][threadIdx.x >> 6] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int a = threadIdx.x / 3, b = threadIdx.x % 3;
+        if (a < G && b < G) {
+            double t = 0.0;
+            for (int q = 0; q < RT / 64; ++q) t += red[threadIdx.x][q];
+            dots[(k * G + a) * G + b] = t;
+        }
+    }
+}
+
+// out[p] = alpha sum_{k in bin(p)} w(s, l(k)) sum_pairs dots[k, u-group, v-group], one thread per parameter.
+__global__ void harm_binsum_kernel(const double* __restrict__ dots, int G, HarmTables t, int npar, double alpha,
+                                   double* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npar) return;
+    const int s = p / t.nbins, b = p % t.nbins;
+    const int64_t lmin2 = (int64_t)t.lmin * t.lmin;
+    const int64_t ka = (int64_t)t.lo[b] * t.lo[b] - lmin2, K = (int64_t)(t.hi[b] + 1) * (t.hi[b] + 1) - lmin2 - ka;
+    const int* Pp = t.pairs + 4 * s;
+    const int np = Pp[2] < 0 ? 1 : 2;
+    double acc = 0.0;
+    for (int64_t k = ka; k < ka + K; ++k) {
+        const double a = t.w[s * t.lw + ell_of(k, t.lmin)];
+        if (a == 0.0) continue;
+        double term = 0.0;
+        for (int e = 0; e < np; ++e) term += dots[(k * G + Pp[2 * e]) * G + Pp[2 * e + 1]];
+        acc += a * term;
+    }
+    out[p] = alpha * acc;
+}
+
 // One block per lower-triangle entry (p1 >= p2) of F.
 // M2: V^T C2^-1 V of a cross analysis (M itself otherwise):
 // Tr[C1^-1 u' v'^T C2^-1 u v^T] = M1[v, u'] M2[u, v'].
@@ -340,6 +428,32 @@ int cfmi_harmonic_pairsum(cfmi_ctx* c, int64_t ni, const double* X, int64_t ldx,
     HarmTables t{bin_lo, bin_hi, pairs, goff, w, lw, nbins, lmin};
     hipLaunchKernelGGL(harm_pairsum_kernel, dim3((unsigned)npar, (unsigned)ncols), dim3(RT), 0, c->stream, ni, X,
                        ldx, sxj, Y, ldy, syj, t, alpha, out, ldo);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
+int cfmi_harmonic_coldots(cfmi_ctx* c, int64_t ni, const double* X, int64_t ldx, const double* Y, int64_t ldy,
+                          const double* wrow, int ngroups, int64_t nmodes, int lmin, int nbins, const int* bin_lo,
+                          const int* bin_hi, int nspec, const int* pairs, const int64_t* goff, const double* w,
+                          int lw, double alpha, double* out)
+{
+    if (!c) return CFMI_EINVAL;
+    int rc = check_tables(c, lmin, nbins, bin_lo, bin_hi, nspec, pairs, goff, w, lw);
+    if (rc) return rc;
+    if (ni <= 0 || nmodes <= 0 || ngroups < 1 || ngroups > 3)
+        return cfmi_fail(c, CFMI_EINVAL, "harmonic_coldots: ni, nmodes > 0 and 1 <= ngroups <= 3 required");
+    if (nmodes > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "harmonic_coldots: too many modes");
+    if (!X || !Y || !out) return cfmi_fail(c, CFMI_EINVAL, "harmonic_coldots: null pointer");
+    if (ldx < ni || ldy < ni) return cfmi_fail(c, CFMI_EINVAL, "harmonic_coldots: leading dimension < ni");
+    double* dots = (double*)cfmi_workspace(c, (size_t)nmodes * ngroups * ngroups * sizeof(double));
+    if (!dots) return cfmi_fail(c, CFMI_ENOMEM, "harmonic_coldots: workspace");
+    hipLaunchKernelGGL(harm_coldots_kernel, dim3((unsigned)nmodes), dim3(RT), 0, c->stream, ni, X, ldx, Y, ldy, wrow,
+                       goff, ngroups, dots);
+    CFMI_LAUNCH_CHECK(c);
+    const int npar = nspec * nbins;
+    HarmTables t{bin_lo, bin_hi, pairs, goff, w, lw, nbins, lmin};
+    hipLaunchKernelGGL(harm_binsum_kernel, dim3((unsigned)((npar + 255) / 256)), dim3(256), 0, c->stream, dots,
+                       ngroups, t, npar, alpha, out);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }

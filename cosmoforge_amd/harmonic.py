@@ -288,18 +288,20 @@ class HarmonicContraction:
             Ut = self._ut(R, maps)
         ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, 1, P(Ut), ns, 1, P(Ut), ns, 1, ns, *self._tables(), 0.5,
                                                 P(q), self.npar), "harmonic_pairsum")
-        NR = torch.empty_like(R)
+        # nb_p = 1/2 sum w sum_pairs R_u^T N R_v: a diagonal N weights the
+        # rows of R (no N R formed), a dense one is applied by a product
+        nb = torch.empty(self.npar, dtype=torch.float64, device=ctx.device)
         if self.cov.noise_is_diagonal():
             d = torch.empty(n, dtype=torch.float64, device=ctx.device)
             ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, 1, n, P(self.cov.N), n + 1, P(d), 1), "copy")
-            ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, n, nt, P(R), n, P(NR), n), "copy")
-            ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(NR), n, P(d)), "scale")
+            X, wrow = R, d
         else:
-            engine.gemm(ctx, False, False, n, nt, n, 1.0, self.cov.N, 0, n, R, 0, n, 0.0, NR, 0, n)
-        nb = torch.empty(self.npar, dtype=torch.float64, device=ctx.device)
-        ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, n, P(NR), n, 0, P(R), n, 0, 1, *self._tables(), 0.5,
-                                                P(nb), self.npar), "harmonic_pairsum")
-        del R, NR
+            X = torch.empty_like(R)
+            engine.gemm(ctx, False, False, n, nt, n, 1.0, self.cov.N, 0, n, R, 0, n, 0.0, X, 0, n)
+            wrow = None
+        ctx.check(ctx.lib.cfmi_harmonic_coldots(ctx.h, n, P(X), n, P(R), n, P(wrow), len(self.group), self.ng,
+                                                *self._tables(), 0.5, P(nb)), "harmonic_coldots")
+        del R, X
         qh = q.cpu().numpy()              # (ns, npar): row s holds simulation s
         nbh = nb.cpu().numpy()
         if remove_nb:

@@ -405,6 +405,18 @@ int cfmi_harmonic_pairsum(cfmi_ctx *ctx, int64_t ni, const double *X, int64_t ld
                           const int64_t *goff, const double *w, int lw, double alpha, double *out,
                           int64_t ldo);
 
+/* The noise-bias form of cfmi_harmonic_pairsum over whole columns (ni = n),
+ * with optional row weights: out[p] = alpha sum_{l in bin(p)} w[s][l] sum_m
+ * sum_{(u,v)} sum_i wrow_i X[i + (goff[u] + k) ldx] Y[i + (goff[v] + k) ldy]
+ * (wrow NULL: 1).  Each mode's columns of every group (ngroups <= 3, nmodes
+ * modes each) are read once for all group pairs; X == Y with wrow = diag N
+ * gives 1/2 Tr[C^-1 N C^-1 dC_p] from R = C^-1 V without forming N R.
+ * Replaces spectra.py:1124-1186 (the noise bias, harmonic form).         */
+int cfmi_harmonic_coldots(cfmi_ctx *ctx, int64_t ni, const double *X, int64_t ldx, const double *Y,
+                          int64_t ldy, const double *wrow, int ngroups, int64_t nmodes, int lmin, int nbins,
+                          const int *bin_lo, const int *bin_hi, int nspec, const int *pairs,
+                          const int64_t *goff, const double *w, int lw, double alpha, double *out);
+
 /* ---- A21: ensemble MLP surrogate of -2 ln L ---------------------------- */
 /* E networks 3 -> 256 -> 256 -> 256 -> 1 (SiLU).  Parameters per member
  * (cfmi_mlp_param_count() values, layout W1[256][3], b1, W2[256][256], b2,

@@ -492,3 +492,50 @@ def test_harmonic_modes_vs_oracle(ctx, lmin, lmax):
     assert ctx.lib.cfmi_harmonic_modes(ctx.h, 2, 4, engine.P(vec), 1, 8, engine.P(V), n, 3 * ng, 0, 0, 0, 0) < 0
     assert ctx.lib.cfmi_harmonic_modes(ctx.h, 0, 4, engine.P(vec), lmin, lmax, engine.P(V), n, ng - 1, 0, 0, 0,
                                        0) < 0
+
+
+def test_harmonic_coldots_equals_pairsum(ctx):
+    """The column-dot noise-bias reduction against the per-parameter pair
+    sums over an explicit N R: diagonal row weights and a dense X."""
+    import torch
+
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(12)
+    n, lmin, lmax, G = 700, 2, 9, 3
+    ng = (lmax + 1) ** 2 - lmin ** 2
+    nt = G * ng
+    bins = [(2, 3), (4, 6), (7, 9)]
+    pairs = np.array([[0, 0, -1, -1], [1, 1, -1, -1], [2, 2, -1, -1], [1, 2, 2, 1], [0, 1, 1, 0], [0, 2, 2, 0]],
+                     dtype=np.int32)
+    lw = lmax + 1
+    w = rng.uniform(0.5, 2.0, (len(pairs), lw))
+    tabs = (lmin, len(bins), ctx.upload(np.array([a for a, _ in bins], dtype=np.int32), dtype=torch.int32),
+            ctx.upload(np.array([b for _, b in bins], dtype=np.int32), dtype=torch.int32), len(pairs),
+            ctx.upload(pairs, dtype=torch.int32), ctx.upload(np.arange(G, dtype=np.int64) * ng, dtype=torch.int64),
+            ctx.upload(w), lw)
+    targs = (tabs[0], tabs[1], engine.P(tabs[2]), engine.P(tabs[3]), tabs[4], engine.P(tabs[5]), engine.P(tabs[6]),
+             engine.P(tabs[7]), tabs[8])
+    npar = len(pairs) * len(bins)
+    R = rng.standard_normal((n, nt))
+    d = rng.uniform(0.5, 2.0, n)
+    dR, dd = ctx.upload_cm(R), ctx.upload(d)
+    want = torch.empty(npar, dtype=torch.float64, device=ctx.device)
+    NR = ctx.upload_cm(d[:, None] * R)
+    ctx.check(ctx.lib.cfmi_harmonic_pairsum(ctx.h, n, engine.P(NR), n, 0, engine.P(dR), n, 0, 1, *targs, 0.5,
+                                            engine.P(want), npar), "pairsum")
+    got = torch.empty(npar, dtype=torch.float64, device=ctx.device)
+    ctx.check(ctx.lib.cfmi_harmonic_coldots(ctx.h, n, engine.P(dR), n, engine.P(dR), n, engine.P(dd), G, ng, *targs,
+                                            0.5, engine.P(got)), "coldots")
+    np.testing.assert_allclose(got.cpu().numpy(), want.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    got2 = torch.empty(npar, dtype=torch.float64, device=ctx.device)
+    ctx.check(ctx.lib.cfmi_harmonic_coldots(ctx.h, n, engine.P(NR), n, engine.P(dR), n, None, G, ng, *targs, 0.5,
+                                            engine.P(got2)), "coldots")
+    np.testing.assert_allclose(got2.cpu().numpy(), want.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    # host check of one parameter: TE bin 1 (pairs (0, 1) and (1, 0))
+    p = 4 * len(bins) + 1
+    ks = range(4 * 4 - 4, 7 * 7 - 4)
+    ell = lambda k: int(np.floor(np.sqrt(k + lmin * lmin) + 1e-9))  # noqa: E731
+    ref = 0.5 * sum(w[4, ell(k)] * (np.sum(d * R[:, k] * R[:, ng + k]) + np.sum(d * R[:, ng + k] * R[:, k]))
+                    for k in ks)
+    assert got.cpu().numpy()[p] == pytest.approx(ref, rel=1e-12)
