@@ -51,6 +51,8 @@ class HarmonicContraction:
     # block inverses, K = 2048) on 5 int8 slices; the factorisation's
     # trailing updates on 5 slices (Covariance.build(slices=...))
     MIXED_TRSM_BLOCK = 2048
+    TRSM_OUTER = 2048         # rows per outer block of the blocked solves (engine.trsm), FP64
+    MIXED_TRSM_OUTER = 4096   # mixed: each right-hand-side pass covers more of L (measured 3% faster)
     DIAG_SLICES = 5
     FACTOR_SLICES = 5
 
@@ -151,7 +153,8 @@ class HarmonicContraction:
             if self._blocked(cov):
                 B = self.V.clone()
                 ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(B), n, P(cov.factor.d)), "scale")
-                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self._s(0), diag_slices=self._ds())
+                engine.trsm(ctx, cov.factor, n, B, Z, nt, slices=self._s(0), outer=self._outer(),
+                            diag_slices=self._ds())
                 del B
             else:
                 engine.gemm(ctx, False, False, n, nt, n, 1.0, cov.factor.W, 0, n, self.V, 0, n, 0.0, Z, 0, n, 1)
@@ -177,7 +180,8 @@ class HarmonicContraction:
         R = torch.empty((nt, n), dtype=torch.float64, device=ctx.device)
         if self._blocked(cov):
             del self._Z[id(cov)]
-            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self._s(2), diag_slices=self._ds())
+            engine.trsm(ctx, cov.factor, n, Z, R, nt, transpose=True, slices=self._s(2), outer=self._outer(),
+                        diag_slices=self._ds())
             del Z
             ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, nt, P(R), n, P(cov.factor.d)), "scale")
             return R
@@ -193,6 +197,9 @@ class HarmonicContraction:
     def _s(self, i: int) -> int:
         """Slices of product i (0 when the product runs in FP64)."""
         return self.slices[i] if self.precision == "mixed" else 0
+
+    def _outer(self) -> int:
+        return self.MIXED_TRSM_OUTER if self.precision == "mixed" else self.TRSM_OUTER
 
     def _ds(self) -> int:
         """Slices of the blocked solves' diagonal products (long K only)."""
