@@ -20,8 +20,13 @@
 //   * C is Jacobi-scaled (D C D, unit diagonal) as the reference scales it;
 //   * solves are blocked: a wave solves each diagonal block with lane
 //     shuffles while the workgroup applies the off-diagonal blocks;
-//   * refinement in the same workgroup: r = D x - (D C D) y from the FP64
-//     components, one more pair of solves, chi2 = (D x)^T y in FP64.
+//   * refinement against FP64 residuals from the components, without
+//     re-reading them per job: the first kernel stores each job's y as a
+//     column of its patch's Y_p, one FP64 GEMM per patch forms
+//     T_p = [M_1 .. M_c]^T Y_p (every component read once for the patch's
+//     points instead of once per point), and a second persistent kernel
+//     takes r = D (x - sum_k c_k T_p[k]) per job, one more pair of solves
+//     and chi2 = (D x)^T y in FP64.
 #include "cfmi_internal.h"
 
 #include <algorithm>
@@ -56,6 +61,13 @@ struct BatchArgs {
     double* chi2;
     int* info;
     unsigned* queue;            // [0]: job counter
+    // refinement (iters > 0)
+    int last;                   // batch_refine_kernel: the last step (writes chi2)
+    double* ys;                 // scaled y per job (job_vec offsets)
+    double* Y;                  // per patch n x J_p, column r = the unscaled y of the patch's r-th job
+    const int64_t* job_ycol;    // element offset of the job's column of Y
+    const double* T;            // per patch (ncomp n) x J_p: T[(k n + i) + r (ncomp n)] = (M_k Y_p)[i, r]
+    const int64_t* job_tcol;    // element offset of the job's column of T
 };
 
 template <typename T> struct Mf;
@@ -443,6 +455,10 @@ __global__ __launch_bounds__(BT, 2) void batch_factor_kernel(BatchArgs a)
         __syncthreads();
         const int bad = factor<TS, TC, NB>(n, A, sd, si, &s_bad, &s_ld);
         if (bad) {
+            if (a.iters > 0) {
+                double* yc = a.Y + a.job_ycol[job];
+                for (int i = tid; i < n; i += BT) yc[i] = 0.0;
+            }
             if (tid == 0) {
                 a.info[job] = bad;
                 a.logdet[job] = __builtin_nan("");
@@ -455,38 +471,20 @@ __global__ __launch_bounds__(BT, 2) void batch_factor_kernel(BatchArgs a)
         for (int i = tid; i < n; i += BT) sv[i] = d[i] * x[i];
         __syncthreads();
         solve_inplace<TS, NB>(n, A, sv, sd, red);
-        // refinement: y += (DCD)^-1 (b - DCD y), the residual from the FP64 components
-        for (int it = 0; it < a.iters; ++it) {
-            for (int i = tid; i < n; i += BT) sy[i] = sv[i] * d[i];  // unscaled y
-            __syncthreads();
-            double acc[RPT];
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) acc[r] = 0.0;
-            for (int j = 0; j < n; ++j) {
-                const double yj = sy[j];
-#pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int i = tid + r * BT;
-                    if (i < n) {
-                        double mm = 0.0;
-#pragma unroll
-                        for (int k = 0; k < MAXC; ++k)
-                            if (k < a.ncomp) mm += c[k] * M[k * nn + i + j * n];
-                        acc[r] += mm * yj;
-                    }
-                }
+        if (a.iters > 0) {
+            // refinement follows (per-patch residual products, batch_refine_kernel)
+            double* ys = a.ys + a.job_vec[job];
+            double* yc = a.Y + a.job_ycol[job];
+            for (int i = tid; i < n; i += BT) {
+                ys[i] = sv[i];
+                yc[i] = d[i] * sv[i];
+            }
+            if (tid == 0) {
+                a.info[job] = 0;
+                a.logdet[job] = s_ld - 2.0 * sum_logd;
             }
             __syncthreads();
-            // sy <- residual (scaled), sv keeps y
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int i = tid + r * BT;
-                if (i < n) sy[i] = d[i] * (x[i] - acc[r]);
-            }
-            __syncthreads();
-            solve_inplace<TS, NB>(n, A, sy, sd, red);
-            for (int i = tid; i < n; i += BT) sv[i] += sy[i];
-            __syncthreads();
+            continue;
         }
         double part = 0.0;
         for (int i = tid; i < n; i += BT) part += d[i] * x[i] * sv[i];
@@ -496,6 +494,58 @@ __global__ __launch_bounds__(BT, 2) void batch_factor_kernel(BatchArgs a)
             a.logdet[job] = s_ld - 2.0 * sum_logd;
             a.chi2[job] = chi;
         }
+        __syncthreads();
+    }
+}
+
+// One refinement step per job: r = D (x - sum_k c_k T[k]) with T = M Y from
+// the patch product, y += (D C D)^-1 r; the last step writes chi2.
+template <typename TS, int NB>
+__global__ __launch_bounds__(BT, 2) void batch_refine_kernel(BatchArgs a)
+{
+    __shared__ double sd[NB][NB + 1];
+    __shared__ double sy[VMAX];
+    __shared__ double red[(BT / 64) * NB];
+    __shared__ int s_job;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int q = next_job(&a.queue[0], &s_job);
+        if (q >= a.njobs) break;
+        const int job = a.order[q];
+        if (a.info[job] != 0) continue;              // not positive definite: reported by the factor kernel
+        const int p = a.job_patch[job];
+        const int n = (int)a.patch_n[p];
+        const double* x = a.xs + a.patch_x[p];
+        const TS* A = (const TS*)a.work + a.job_work[job];
+        const double* d = a.dsc + a.job_vec[job];
+        double* ys = a.ys + a.job_vec[job];
+        const double* t = a.T + a.job_tcol[job];
+        double c[MAXC];
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) c[k] = k < a.ncomp ? a.coef[(int64_t)job * a.ncomp + k] : 0.0;
+        for (int i = tid; i < n; i += BT) {
+            double cy = 0.0;
+#pragma unroll
+            for (int k = 0; k < MAXC; ++k)
+                if (k < a.ncomp) cy += c[k] * t[(int64_t)k * n + i];
+            sy[i] = d[i] * (x[i] - cy);
+        }
+        __syncthreads();
+        solve_inplace<TS, NB>(n, A, sy, sd, red);
+        if (!a.last) {
+            double* yc = a.Y + a.job_ycol[job];
+            for (int i = tid; i < n; i += BT) {
+                const double v = ys[i] + sy[i];
+                ys[i] = v;
+                yc[i] = d[i] * v;
+            }
+            __syncthreads();
+            continue;
+        }
+        double part = 0.0;
+        for (int i = tid; i < n; i += BT) part += d[i] * x[i] * (ys[i] + sy[i]);
+        const double chi = block_sum(part, red);
+        if (tid == 0) a.chi2[job] = chi;
         __syncthreads();
     }
 }
@@ -524,6 +574,13 @@ struct cfmi_batch {
     int64_t* d_patch = nullptr;  // patch_n | patch_comp | patch_x
     int64_t* d_job = nullptr;    // job_work | job_vec
     unsigned* d_queue = nullptr;
+    // refinement: per-job y, the patches' Y_p and T_p = [M_1 .. M_c]^T Y_p
+    double* d_ys = nullptr;
+    double* d_Y = nullptr;
+    double* d_T = nullptr;
+    int64_t* d_jobext = nullptr;  // job_ycol | job_tcol
+    std::vector<int64_t> pn, pcomp, pY, pT, pJ;   // per patch (host)
+    const double* comps = nullptr;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -545,6 +602,34 @@ void launch_nb(cfmi_batch* b, hipStream_t s)
     else launch_factor<TS, TC, 32>(b, s);
 }
 
+template <typename TS>
+void launch_refine(cfmi_batch* b, hipStream_t s, int last)
+{
+    BatchArgs a = b->args;
+    a.last = last;
+    dim3 grid((unsigned)(b->ncu * b->wg_per_cu));
+    if (b->nb == 16) hipLaunchKernelGGL((batch_refine_kernel<TS, 16>), grid, dim3(BT), 0, s, a);
+    else hipLaunchKernelGGL((batch_refine_kernel<TS, 32>), grid, dim3(BT), 0, s, a);
+}
+
+// T_p = [M_1 .. M_c]^T Y_p for every patch (FP64 GEMMs on stream s, no
+// split-K workspace so that the sequence can be captured).
+int patch_products(cfmi_batch* b, hipStream_t s)
+{
+    cfmi_ctx* c = b->ctx;
+    hipStream_t keep = c->stream;
+    c->stream = s;
+    int rc = CFMI_OK;
+    for (size_t p = 0; p < b->pn.size() && rc == CFMI_OK; ++p) {
+        const int64_t n = b->pn[p], J = b->pJ[p];
+        if (J == 0) continue;
+        rc = cfmi_gemm_impl(c, CFMI_F64, 1, 0, b->ncomp * n, J, n, 1.0, b->comps + b->pcomp[p], n,
+                            b->d_Y + b->pY[p], n, 0.0, b->d_T + b->pT[p], b->ncomp * n, CFMI_TRI_NOSPLIT);
+    }
+    c->stream = keep;
+    return rc;
+}
+
 int enqueue(cfmi_batch* b, hipStream_t s)
 {
     cfmi_ctx* c = b->ctx;
@@ -554,6 +639,16 @@ int enqueue(cfmi_batch* b, hipStream_t s)
     else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
     else launch_nb<double, double>(b, s);
     CFMI_LAUNCH_CHECK(c);
+    for (int it = 0; it < b->iters; ++it) {
+        int rc = patch_products(b, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
+        CFMI_LAUNCH_CHECK(c);
+        const int last = it == b->iters - 1;
+        if (b->mode == CFMI_BATCH_MIXED) launch_refine<float>(b, s, last);
+        else launch_refine<double>(b, s, last);
+        CFMI_LAUNCH_CHECK(c);
+    }
     return CFMI_OK;
 }
 
@@ -569,6 +664,10 @@ void release(cfmi_batch* b)
     (void)hipFree(b->d_patch);
     (void)hipFree(b->d_job);
     (void)hipFree(b->d_queue);
+    (void)hipFree(b->d_ys);
+    (void)hipFree(b->d_Y);
+    (void)hipFree(b->d_T);
+    (void)hipFree(b->d_jobext);
 }
 
 }  // namespace
@@ -631,6 +730,31 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
         jb[j] = jw[j];
         jb[njobs + j] = jv[j];
     }
+    // refinement layout: patch p's jobs are the columns of Y_p (n_p x J_p)
+    // and T_p (ncomp n_p x J_p), in job order
+    b->pn.assign(n, n + npatch);
+    b->pcomp.assign(comp_off, comp_off + npatch);
+    b->pY.assign(npatch, 0);
+    b->pT.assign(npatch, 0);
+    b->pJ.assign(npatch, 0);
+    for (int j = 0; j < njobs; ++j) b->pJ[job_patch[j]] += 1;
+    int64_t ytot = 0;
+    for (int p = 0; p < npatch; ++p) {
+        b->pY[p] = ytot;
+        b->pT[p] = ytot * ncomp;
+        ytot += n[p] * b->pJ[p];
+    }
+    std::vector<int64_t> je(2 * (size_t)njobs);
+    {
+        std::vector<int64_t> rank(npatch, 0);
+        for (int j = 0; j < njobs; ++j) {
+            const int p = job_patch[j];
+            je[j] = b->pY[p] + rank[p] * n[p];
+            je[njobs + j] = b->pT[p] + rank[p] * ncomp * n[p];
+            rank[p] += 1;
+        }
+    }
+    b->comps = comps;
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     chk(hipMalloc(&b->d_work, (size_t)w * esz));
@@ -640,6 +764,14 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
     chk(hipMalloc((void**)&b->d_patch, pt.size() * sizeof(int64_t)));
     chk(hipMalloc((void**)&b->d_job, jb.size() * sizeof(int64_t)));
     chk(hipMalloc((void**)&b->d_queue, 2 * sizeof(unsigned)));
+    if (refine_iters > 0) {
+        chk(hipMalloc((void**)&b->d_ys, (size_t)v * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_Y, (size_t)ytot * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_T, (size_t)ytot * ncomp * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_jobext, je.size() * sizeof(int64_t)));
+        if (e == hipSuccess)
+            chk(hipMemcpy(b->d_jobext, je.data(), je.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
     if (e == hipSuccess) {
         chk(hipMemcpy(b->d_order, order.data(), (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
         chk(hipMemcpy(b->d_job_patch, job_patch, (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
@@ -667,6 +799,11 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
     a.job_vec = b->d_job + njobs;
     a.dsc = b->d_dsc;
     a.queue = b->d_queue;
+    a.ys = b->d_ys;
+    a.Y = b->d_Y;
+    a.T = b->d_T;
+    a.job_ycol = b->d_jobext;
+    a.job_tcol = b->d_jobext ? b->d_jobext + njobs : nullptr;
     *out = b;
     return CFMI_OK;
 }

@@ -68,6 +68,8 @@ int cfmi_s8_syrk_lower(cfmi_ctx* c, int s, int64_t n, int64_t k, double alpha, c
 // tri: bit 0 = op(A) lower (A[m,k]=0 for k>m), bit 1 = op(A) upper,
 //      bit 2 = op(B) upper (B[k,n]=0 for k>n), bit 3 = op(B) lower,
 //      bit 4 = write only the lower triangle of C (m >= n).
+//      bit 20 (CFMI_TRI_NOSPLIT) = never split K (no workspace; for graph capture).
+constexpr int CFMI_TRI_NOSPLIT = 1 << 20;
 int cfmi_gemm_impl(cfmi_ctx* c, int dtype, int ta, int tb, int64_t M, int64_t N, int64_t K,
                    double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
                    double beta, void* C, int64_t ldc, int tri);

@@ -425,7 +425,7 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     int64_t nz = 1;
     int64_t tiles = tm * tn * batch;
     const int64_t active = (tri & 64) && tm == tn ? tm * (tm + 1) / 2 * batch : tiles;
-    if (K > 4 * BK && tiles < 512 && batch == 1) {
+    if (!(tri & CFMI_TRI_NOSPLIT) && K > 4 * BK && tiles < 512 && batch == 1) {
         nz = std::min<int64_t>((2048 + active - 1) / active, (K + 255) / 256);
         nz = std::max<int64_t>(nz, 1);
         // Bound the partial buffer to 1 GiB.
@@ -447,6 +447,7 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
     g.M = M; g.N = N; g.K = K > 0 ? K : 0;
     g.alpha = alpha; g.beta = beta;
     g.A = (const T*)A; g.lda = lda; g.B = (const T*)B; g.ldb = ldb; g.C = (TO*)C; g.ldc = ldc;
+    tri &= ~CFMI_TRI_NOSPLIT;
     g.tri = tri; g.offA = sh.offA; g.offB = sh.offB;
     g.kchunk = kchunk; g.part = part; g.tm = tm; g.tn = tn; g.o = sh.om;
     g.sA = sh.sA; g.sB = sh.sB; g.sC = sh.sC;

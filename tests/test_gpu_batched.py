@@ -91,11 +91,33 @@ def test_batched_mixed_within_1e3(setup):
             assert err < 1e-3
 
 
+@pytest.mark.parametrize("iters", [0, 2])
+def test_batched_refinement_steps(setup, iters):
+    """No refinement and two refinement steps (per-patch residual products
+    between the factor and refine kernels): FP64 still at 1e-10 of the
+    oracle, mixed within 1e-3 of FP64."""
+    from cosmoforge_amd import batched
+
+    ctx, masks, basis, patches, job_patch, coef, params = setup
+    ref = _oracle(masks, basis, patches, job_patch, params)
+    b = batched.BatchedLikelihood(ctx, patches, job_patch, "fp64", refine_iters=iters)
+    chi2, logdet, info = b.run(coef)
+    assert not np.any(info.cpu().numpy())
+    np.testing.assert_allclose(chi2.cpu().numpy(), ref[:, 0], rtol=1e-10)
+    np.testing.assert_allclose(logdet.cpu().numpy(), ref[:, 1], rtol=1e-10)
+    b.close()
+    exact = ref[:, 0] + ref[:, 1]
+    m = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=iters).minus2lnl(coef)
+    err = np.max(np.abs(m - exact))
+    print(f"mixed, {iters} refinement steps: max |d(-2lnL)| = {err:.3e}")
+    assert err < (1e-3 if iters else 1.0)
+
+
 def test_graph_replay_matches_eager(setup):
     from cosmoforge_amd import batched
 
     ctx, masks, basis, patches, job_patch, coef, params = setup
-    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed")
+    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=2)
     eager1 = b.minus2lnl(coef)
     coef2 = coef.copy()
     coef2[:, 3] *= 0.5
