@@ -70,6 +70,7 @@ SIGNATURES = {
     "cfmi_s8_stats_enable": (ci, [vp, ci]),
     "cfmi_s8_stats_read": (ci, [vp, _c.POINTER(i64), _c.POINTER(cd), _c.POINTER(cd), _c.POINTER(cd)]),
     "cfmi_trtri": (ci, [vp, ci, i64, vp, i64, vp, i64]),
+    "cfmi_trtri_blocks": (ci, [vp, ci, i64, i64, vp, i64, vp, i64]),
     "cfmi_logdet_chol": (ci, [vp, i64, vp, i64, vp, _c.POINTER(cd)]),
     "cfmi_scale": (ci, [vp, ci, i64, i64, vp, i64, vp]),
     "cfmi_axpby": (ci, [vp, i64, i64, cd, vp, i64, cd, vp, i64]),

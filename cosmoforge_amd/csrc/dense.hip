@@ -523,16 +523,33 @@ constexpr int64_t OB = 512;   // Cholesky panel width (a multiple of NB)
 constexpr int64_t OB2 = 2048; // outer panel width (a multiple of OB)
 
 template <typename T>
+__device__ __forceinline__ T lane_value(T v, int lane)
+{
+    if constexpr (sizeof(T) == 8) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+        return __hiloint2double(hi, lo);
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
                                                         T* linv, T* U, int64_t ldu)
 {
-    // One wave: lane r holds row r of the block in registers.  Step q takes
-    // the pivot from lane q, scales column q, publishes it through LDS and
-    // every lane updates its row (right-looking, no block-wide sweeps).  The
-    // inverse is then solved column by column, lane c owning column c, with
-    // L read from LDS as broadcasts.
-    __shared__ T col[NB];
-    __shared__ T Ls[NB][NB + 1];
+    // One wave.  Factor: lane r holds row r of the block in registers; step
+    // q reads the pivot from lane q (a scalar), forms 1 / L_qq once, scales
+    // column q by it, publishes the column through LDS and every lane
+    // updates its row (right-looking; a single wave needs no barrier between
+    // its LDS store and load, which the LDS unit keeps in order).
+    // Inverse: lane c owns column c of L^-1 and solves L u = e_c right-
+    // looking, u_k = s_k / L_kk then s_i -= L_ik u_k for i > k, so the
+    // dependent chain is one multiply and one FMA per k while the rest of
+    // the updates are independent; L's columns are LDS broadcasts.  The
+    // inverse leaves through LDS so that the global stores are by rows.
+    __shared__ T lt[NB][NB + 1];      // lt[k][i] = L[i][k] (columns of L contiguous), then U^T
+    __shared__ T rdiag[NB];           // 1 / L_kk
     if (*info != 0) return;
     const int r = threadIdx.x;
     T row[NB];
@@ -542,19 +559,25 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         if (q < kb && !bad) {
-            const T d = __shfl(row[q], q, 64);
+            const T d = lane_value(row[q], q);
             if (!(d > T(0))) {
                 bad = q + 1;
             } else {
                 const T piv = sqrt(d);
-                const T l = r > q ? row[q] / piv : (r == q ? piv : T(0));
+                const T rp = T(1) / piv;
+                const T l = r > q ? row[q] * rp : (r == q ? piv : T(0));
                 row[q] = l;
-                col[r] = l;
-                __syncthreads();
+                lt[q][r] = l;
+                if (r == 0) rdiag[q] = rp;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-                for (int p = q + 1; p < NB; ++p) row[p] -= l * col[p];
-                __syncthreads();
+                for (int p = q + 1; p < NB; ++p) row[p] -= l * lt[q][p];
             }
+        } else if (q >= kb) {
+            lt[q][r] = T(0);
+            if (r == 0) rdiag[q] = T(0);
         }
     }
     if (bad) {
@@ -562,26 +585,31 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
         return;
     }
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        const T v = c <= r ? row[c] : T(0);
-        Ls[r][c] = v;
-        if (r < kb && c < kb) A[r + c * lda] = v;
-    }
+    for (int c = 0; c < NB; ++c)
+        if (r < kb && c < kb) A[r + c * lda] = c <= r ? row[c] : T(0);
     __syncthreads();
-    // U = L^-1: lane c solves L u = e_c (u[k] = 0 for k < c).
+    // U = L^-1: lane c solves L u = e_c (u_k = 0 for k < c falls out).
     const int c = r;
     T u[NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        T sum = (i == c) ? T(1) : T(0);
+    for (int i = 0; i < NB; ++i) u[i] = (i == c) ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < i; ++k) sum -= Ls[i][k] * u[k];
-        u[i] = (i < c || i >= kb) ? T(0) : sum / Ls[i][i];
+    for (int k = 0; k < NB; ++k) {
+        u[k] *= rdiag[k];
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i) u[i] -= lt[k][i] * u[k];
     }
+    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        linv[i + c * NB] = u[i];
-        if (U && i < kb && c < kb) U[i + c * ldu] = u[i];
+    for (int i = 0; i < NB; ++i) lt[c][i] = u[i];   // lt[c][i] = U[i][c]
+    __syncthreads();
+    // lane i stores row i of U: U[i][c] = lt[c][i], consecutive lanes at consecutive addresses
+    const int i = r;
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc) {
+        const T v = lt[cc][i];
+        linv[i + cc * NB] = v;
+        if (U && i < kb && cc < kb) U[i + cc * ldu] = v;
     }
 }
 
@@ -735,27 +763,45 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
     return h > 0 ? h : CFMI_OK;
 }
 
-// U = L^-1 by recursive 2 x 2 blocking on 64-aligned splits; the diagonal
-// 64 x 64 blocks of U come from potrf.  U21 = -U22 (L21 U11).
+// Inverse of L's diagonal blocks [j nb, (j + 1) nb) (nb = 64 * 2^k; nb >= n
+// inverts the whole of L), bottom up.  The 64 x 64 diagonal blocks of U come
+// from potrf.  Level b joins each pair of adjacent inverted b-blocks at
+// lo = t 2b, [[U11, 0], [U21, U22]] with U21 = -U22 (L21 U11): the pairs
+// of a level are 2b (1 + ld) apart on the diagonal, so every full pair of
+// the level is one batched product (two launches per level instead of two
+// per pair), and a trailing pair whose second block is short takes one
+// more product.  A pair never straddles an nb boundary (2b divides nb).
 template <typename T>
-int trtri_rec(cfmi_ctx* c, int64_t lo, int64_t hi, const T* L, int64_t ldl, T* U, int64_t ldu, T* tmp)
+int trtri_levels(cfmi_ctx* c, int64_t n, int64_t nb, const T* L, int64_t ldl, T* U, int64_t ldu, T* tmp)
 {
-    int64_t len = hi - lo;
-    if (len <= NB) return CFMI_OK;
-    int64_t half = ((len / 2 + NB - 1) / NB) * NB;
-    int64_t mid = lo + half;
-    int rc = trtri_rec<T>(c, lo, mid, L, ldl, U, ldu, tmp);
-    if (rc) return rc;
-    rc = trtri_rec<T>(c, mid, hi, L, ldl, U, ldu, tmp);
-    if (rc) return rc;
-    int64_t m2 = hi - mid, m1 = mid - lo;
-    // tmp (m2 x m1) = L21 * U11   (U11 lower: op(B)[k,n] = 0 for k < n)
-    rc = gemm_t<T>(c, 0, 0, m2, m1, m1, 1.0, L + mid + lo * ldl, ldl, U + lo + lo * ldu, ldu, 0.0, tmp,
-                   m2, 8);
-    if (rc) return rc;
-    // U21 = -U22 * tmp   (U22 lower: op(A)[m,k] = 0 for k > m)
-    return gemm_t<T>(c, 0, 0, m2, m1, m2, -1.0, U + mid + mid * ldu, ldu, tmp, m2, 0.0,
-                     U + mid + lo * ldu, ldu, 1);
+    for (int64_t b = NB; b < nb && b < n; b *= 2) {
+        const int64_t full = n / (2 * b);
+        const int64_t sL = 2 * b * (1 + ldl), sU = 2 * b * (1 + ldu);
+        for (int64_t t0 = 0; t0 < full; t0 += 65535) {
+            const int64_t cnt = std::min<int64_t>(65535, full - t0);
+            const T* L0 = L + t0 * sL;
+            T* U0 = U + t0 * sU;
+            // tmp_t (b x b) = L21 U11   (U11 lower: op(B)[k,n] = 0 for k < n)
+            int rc = gemm_t<T>(c, 0, 0, b, b, b, 1.0, L0 + b, ldl, U0, ldu, 0.0, tmp, b, 8, OutMap(), cnt, sL, sU,
+                               b * b);
+            if (rc) return rc;
+            // U21 = -U22 tmp   (U22 lower: op(A)[m,k] = 0 for k > m)
+            rc = gemm_t<T>(c, 0, 0, b, b, b, -1.0, U0 + b + b * ldu, ldu, tmp, b, 0.0, U0 + b, ldu, 1, OutMap(),
+                           cnt, sU, b * b, sU);
+            if (rc) return rc;
+        }
+        const int64_t lo = full * 2 * b, m2 = n - lo - b;
+        if (m2 > 0) {
+            const int64_t mid = lo + b;
+            int rc = gemm_t<T>(c, 0, 0, m2, b, b, 1.0, L + mid + lo * ldl, ldl, U + lo + lo * ldu, ldu, 0.0, tmp,
+                               m2, 8);
+            if (rc) return rc;
+            rc = gemm_t<T>(c, 0, 0, m2, b, m2, -1.0, U + mid + mid * ldu, ldu, tmp, m2, 0.0, U + mid + lo * ldu, ldu,
+                           1);
+            if (rc) return rc;
+        }
+    }
+    return CFMI_OK;
 }
 
 __global__ void diag_scale_kernel(int64_t n, const double* A, int64_t lda, double* d)
@@ -915,26 +961,53 @@ int cfmi_potrf_s8(cfmi_ctx* c, int s, int64_t n, double* A, int64_t lda, double*
     return potrf_t<double>(c, n, A, lda, U, ldu, info, s);
 }
 
+// Device scratch of trtri_levels: the largest level's b x b products.
+static int64_t trtri_tmp_elems(int64_t n, int64_t nb)
+{
+    int64_t big = 0;
+    for (int64_t b = NB; b < nb && b < n; b *= 2) {
+        const int64_t full = std::min<int64_t>(n / (2 * b), 65535);
+        big = std::max<int64_t>(big, std::max<int64_t>(full * b * b, (n - n / (2 * b) * 2 * b - b) * b));
+    }
+    return big;
+}
+
+static int trtri_run(cfmi_ctx* c, int dtype, int64_t n, int64_t nb, const void* L, int64_t ldl, void* U, int64_t ldu)
+{
+    const int64_t elems = trtri_tmp_elems(n, nb);
+    if (elems <= 0) return CFMI_OK;
+    const size_t esz = dtype == CFMI_F64 ? 8 : 4;
+    // Kept apart from the shared workspace, which gemm's split-K may grow.
+    void* tmp2 = nullptr;
+    CFMI_HIP(c, hipMallocAsync(&tmp2, (size_t)elems * esz, c->stream));
+    int rc;
+    if (dtype == CFMI_F64)
+        rc = trtri_levels<double>(c, n, nb, (const double*)L, ldl, (double*)U, ldu, (double*)tmp2);
+    else
+        rc = trtri_levels<float>(c, n, nb, (const float*)L, ldl, (float*)U, ldu, (float*)tmp2);
+    hipFreeAsync(tmp2, c->stream);
+    return rc;
+}
+
 int cfmi_trtri(cfmi_ctx* c, int dtype, int64_t n, const void* L, int64_t ldl, void* U, int64_t ldu)
 {
     if (!c) return CFMI_EINVAL;
-    if (n <= NB) return CFMI_OK;
-    size_t esz = dtype == CFMI_F64 ? 8 : 4;
-    // The largest temporary is at the top split: (hi - mid) x (mid - lo).
-    int64_t half = ((n / 2 + NB - 1) / NB) * NB;
-    int64_t big = std::max<int64_t>(half, n - half);
-    // Kept apart from the shared workspace, which gemm's split-K may grow.
-    void* tmp2 = nullptr;
-    CFMI_HIP(c, hipMallocAsync(&tmp2, (size_t)big * big * esz, c->stream));
-    int rc;
-    if (dtype == CFMI_F64)
-        rc = trtri_rec<double>(c, 0, n, (const double*)L, ldl, (double*)U, ldu, (double*)tmp2);
-    else if (dtype == CFMI_F32)
-        rc = trtri_rec<float>(c, 0, n, (const float*)L, ldl, (float*)U, ldu, (float*)tmp2);
-    else
-        rc = cfmi_fail(c, CFMI_EINVAL, "trtri: unsupported dtype %d", dtype);
-    hipFreeAsync(tmp2, c->stream);
-    return rc;
+    if (dtype != CFMI_F64 && dtype != CFMI_F32) return cfmi_fail(c, CFMI_EINVAL, "trtri: unsupported dtype %d", dtype);
+    if (n < 0 || ldl < n || ldu < n) return cfmi_fail(c, CFMI_EINVAL, "trtri: bad size or leading dimension");
+    return trtri_run(c, dtype, n, n, L, ldl, U, ldu);
+}
+
+int cfmi_trtri_blocks(cfmi_ctx* c, int dtype, int64_t n, int64_t nb, const void* L, int64_t ldl, void* U,
+                      int64_t ldu)
+{
+    if (!c) return CFMI_EINVAL;
+    if (dtype != CFMI_F64 && dtype != CFMI_F32)
+        return cfmi_fail(c, CFMI_EINVAL, "trtri_blocks: unsupported dtype %d", dtype);
+    if (n < 0 || ldl < n || ldu < n) return cfmi_fail(c, CFMI_EINVAL, "trtri_blocks: bad size or leading dimension");
+    if (nb < NB || (nb / NB) & (nb / NB - 1) || nb % NB)
+        return cfmi_fail(c, CFMI_EINVAL, "trtri_blocks: block size %lld is not 64 times a power of two",
+                         (long long)nb);
+    return trtri_run(c, dtype, n, nb, L, ldl, U, ldu);
 }
 
 int cfmi_jacobi_scale(cfmi_ctx* c, int64_t n, double* A, int64_t lda, double* d)

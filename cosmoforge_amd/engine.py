@@ -399,12 +399,9 @@ def factor(ctx: Context, C: torch.Tensor, n: int, need_whitener: bool = True, di
         ctx.check(ctx.lib.cfmi_scale(ctx.h, 0, n, n, P(U), n, P(d)), "scale")
         return Factor(C, U, d, float(ld.value), slices=slices)
     if diag_blocks:
-        if diag_blocks % 64:
-            raise ValueError("diagonal blocks must be a multiple of 64 rows")
-        for j0 in range(0, n, diag_blocks):
-            w = min(diag_blocks, n - j0)
-            o = j0 + j0 * n
-            ctx.check(ctx.lib.cfmi_trtri(ctx.h, F64, w, P(C, o), n, P(U, o), n), "trtri")
+        if diag_blocks % 64 or (diag_blocks // 64) & (diag_blocks // 64 - 1):
+            raise ValueError("diagonal blocks must be 64 times a power of two rows")
+        ctx.check(ctx.lib.cfmi_trtri_blocks(ctx.h, F64, n, diag_blocks, P(C), n, P(U), n), "trtri_blocks")
         f = Factor(C, None, d, float(ld.value))
         f.U, f.nb, f.slices = U, diag_blocks, slices
         return f

@@ -29,6 +29,7 @@
  *   cfmi_potrf          lapack.dpotrf in matrix_inverse_symm / cholesky_factor
  *                                                           basics/linalg.py:197-214,362-406
  *   cfmi_trtri          lapack.dpotri (via L^-1)            basics/linalg.py:212-215
+ *   cfmi_trtri_blocks   lapack.dpotri on the diagonal blocks basics/linalg.py:212-215
  *   cfmi_jacobi_scale   D M D preconditioning               basics/linalg.py:197-207
  *   cfmi_logdet_chol    matrix_slogdet_symm                 basics/linalg.py:430-474
  *   cfmi_gemm           matrix_mult (np.matmul)             basics/linalg.py:18-32
@@ -291,6 +292,14 @@ int cfmi_potrf_s8(cfmi_ctx *ctx, int s, int64_t n, double *A, int64_t lda, doubl
 /* Complete U = L^-1 from cfmi_potrf's L and diagonal blocks of U.        */
 int cfmi_trtri(cfmi_ctx *ctx, int dtype, int64_t n, const void *L, int64_t ldl, void *U,
                int64_t ldu);
+
+/* The inverses of L's diagonal nb x nb blocks (nb = 64 * 2^k; the last
+ * block may be short) into the same places of U, from cfmi_potrf's L and
+ * 64 x 64 blocks of U: the blocked triangular solves' diagonal products
+ * in the harmonic route, the dpotri step (basics/linalg.py:212-215)
+ * restricted to the blocks that the solves need.                           */
+int cfmi_trtri_blocks(cfmi_ctx *ctx, int dtype, int64_t n, int64_t nb, const void *L, int64_t ldl, void *U,
+                      int64_t ldu);
 
 /* log det of D^-1 L L^T D^-1 = 2 sum log L_ii - 2 sum log d_i (d may be
  * NULL); the result is written to *out (host).                             */

@@ -168,7 +168,29 @@ def test_deriv_stacks_vs_oracle(ctx):
             assert np.max(np.abs(full - ref)) / scale < 1e-12, (s, b)
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 1000])
+@pytest.mark.parametrize("n,nb", [(1000, 256), (2500, 512), (4096, 2048), (300, 64), (130, 1024)])
+def test_diagonal_block_inverses(ctx, n, nb):
+    """cfmi_trtri_blocks: every nb x nb diagonal block of U (the last one
+    short) is the inverse of L's block, L from the same factorisation."""
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(n + nb)
+    A = rng.standard_normal((n, n))
+    C = A @ A.T / n + np.diag(rng.uniform(0.5, 2.0, n))
+    dC = _cm(ctx, C)
+    f = engine.factor(ctx, dC, n, need_whitener=False, diag_blocks=nb)
+    L, U = engine.to_numpy_cm(f.L), engine.to_numpy_cm(f.U)
+    for j0 in range(0, n, nb):
+        j1 = min(n, j0 + nb)
+        Lb, Ub = L[j0:j1, j0:j1], U[j0:j1, j0:j1]
+        assert np.array_equal(np.triu(Ub, 1), np.zeros_like(Ub))
+        ref = np.linalg.inv(Lb)
+        assert np.max(np.abs(Ub - ref)) / np.max(np.abs(ref)) < 1e-12, (j0, j1)
+    with pytest.raises(ValueError):
+        engine.factor(ctx, _cm(ctx, C), n, need_whitener=False, diag_blocks=192)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 1000, 2500])
 def test_factor_and_inverse(ctx, n):
     from cosmoforge_amd import engine
     from oracle import linalg as LA
