@@ -28,7 +28,7 @@ import torch
 from . import engine
 from .engine import P
 
-NB_CHOICES = (16, 32)
+NB_CHOICES = (16, 32, 64)   # 16, 32: one persistent workgroup per matrix; 64: step-synchronous launches
 WG_CHOICES = (1, 2, 4)
 
 

@@ -28,6 +28,7 @@
 //     takes r = D (x - sum_k c_k T_p[k]) per job, one more pair of solves
 //     and chi2 = (D x)^T y in FP64.
 #include "cfmi_internal.h"
+#include "chol64.h"
 
 #include <algorithm>
 #include <numeric>
@@ -105,69 +106,77 @@ __device__ __forceinline__ void tri_tile(int k, int& ti, int& tj)
     tj = k - r * (r + 1) / 2;
 }
 
-// A[r0:, r0:] -= P P^T on the lower triangle, P = A[r0:, kc:kc+kb].
-// TS: storage type, TC: MFMA type.  Each wave owns 32 x 32 output tiles; all
-// NB/4 K-steps of its operands are loaded before the MFMA chain so that the
-// loads of a tile are in flight together.  The product accumulates from zero
-// and is added to the stored tile once.
+// One wave: the 32 x 32 tile A[i0:i0+32, j0:j0+32] -= P_i P_j^T on and
+// below the diagonal, P = A[:, kc:kc+kb] (kb <= NB).  TS: storage type, TC:
+// MFMA type.  All NB/4 K-steps of the operands are loaded before the MFMA
+// chain so that the tile's loads are in flight together; the product
+// accumulates from zero and is added to the stored tile once.
 template <typename TS, typename TC, int NB>
-__device__ void trailing_update(int n, TS* A, int r0, int kc, int kb)
+__device__ __forceinline__ void update_tile32(int n, TS* A, int i0, int j0, int kc, int kb)
 {
     typedef typename Mf<TC>::v4 v4;
     constexpr bool kF64 = sizeof(TC) == 8;
     constexpr int KS = 8;                       // K-steps (of 4) loaded together
-    const int m = n - r0;
-    if (m <= 0) return;
-    const int t = (m + TT - 1) / TT;
-    const int ntiles = t * (t + 1) / 2;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     const int lm = lane & 15, lk = lane >> 4;
-    for (int tile = wave; tile < ntiles; tile += BT / 64) {
-        int ti, tj;
-        tri_tile(tile, ti, tj);
-        const int i0 = r0 + ti * TT, j0 = r0 + tj * TT;
-        v4 acc[2][2];
+    v4 acc[2][2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
-        for (int k4 = 0; k4 < NB; k4 += 4 * KS) {
-            TC av[KS][2], bv[KS][2];
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4{0, 0, 0, 0};
+    for (int k4 = 0; k4 < NB; k4 += 4 * KS) {
+        TC av[KS][2], bv[KS][2];
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const int kk = k4 + 4 * s + lk;
-                const bool kin = kk < kb;
-                const TS* col = A + (kc + (kin ? kk : 0)) * n;
+        for (int s = 0; s < KS; ++s) {
+            const int kk = k4 + 4 * s + lk;
+            const bool kin = kk < kb;
+            const TS* col = A + (int64_t)(kc + (kin ? kk : 0)) * n;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int ri = i0 + 16 * h + lm, rj = j0 + 16 * h + lm;
-                    av[s][h] = (kin && ri < n) ? -(TC)col[ri] : TC(0);
-                    bv[s][h] = (kin && rj < n) ? (TC)col[rj] : TC(0);
-                }
+            for (int h = 0; h < 2; ++h) {
+                const int ri = i0 + 16 * h + lm, rj = j0 + 16 * h + lm;
+                av[s][h] = (kin && ri < n) ? -(TC)col[ri] : TC(0);
+                bv[s][h] = (kin && rj < n) ? (TC)col[rj] : TC(0);
             }
+        }
         // D = P_j P_i^T (the transposed tile): a lane's outputs are rows
         // i0 + 16 a + lm (contiguous in the column-major store) of columns
         // j0 + 16 b + {4 lk + r | lk + 4 r}.
 #pragma unroll
-            for (int s = 0; s < KS; ++s)
+        for (int s = 0; s < KS; ++s)
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) acc[a][b] = Mf<TC>::run(bv[s][b], av[s][a], acc[a][b]);
-        }
+                for (int b = 0; b < 2; ++b) acc[a][b] = Mf<TC>::run(bv[s][b], av[s][a], acc[a][b]);
+    }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int row = i0 + 16 * a + lm;
-                    int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
-                    if (row < n && col < n && row >= col) {
-                        TS* p = A + row + col * n;
-                        *p = (TS)((TS)*p + (TS)acc[a][b][r]);
-                    }
+            for (int r = 0; r < 4; ++r) {
+                int row = i0 + 16 * a + lm;
+                int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
+                if (row < n && col < n && row >= col) {
+                    TS* p = A + row + (int64_t)col * n;
+                    *p = (TS)((TS)*p + (TS)acc[a][b][r]);
                 }
+            }
+}
+
+// A[r0:, r0:] -= P P^T on the lower triangle, P = A[r0:, kc:kc+kb]; each
+// wave owns 32 x 32 output tiles.
+template <typename TS, typename TC, int NB>
+__device__ void trailing_update(int n, TS* A, int r0, int kc, int kb)
+{
+    const int m = n - r0;
+    if (m <= 0) return;
+    const int t = (m + TT - 1) / TT;
+    const int ntiles = t * (t + 1) / 2;
+    const int wave = threadIdx.x >> 6;
+    for (int tile = wave; tile < ntiles; tile += BT / 64) {
+        int ti, tj;
+        tri_tile(tile, ti, tj);
+        update_tile32<TS, TC, NB>(n, A, r0 + ti * TT, r0 + tj * TT, kc, kb);
     }
 }
 
@@ -550,6 +559,329 @@ __global__ __launch_bounds__(BT, 2) void batch_refine_kernel(BatchArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Step-synchronous engine (nb = 64): instead of one workgroup walking one
+// matrix, every job's step k is one launch per stage over all jobs still
+// active at column k0 = 64 k (jobs are ordered largest n first, so they are
+// a prefix of the order):
+//   sw_diag_kernel    one wave per job: the 64 x 64 diagonal block factored
+//                     and inverted in FP64 registers (chol64.h), ln|C|
+//                     accumulated in FP64;
+//   sw_panel_kernel   one workgroup per (job, 64-row tile) of the panel
+//                     below: P <- P L_kk^-T on FP64 MFMA;
+//   sw_update_kernel  one workgroup per (job, 64 x 64 lower tile) of the
+//                     trailing matrix: A22 -= P_i P_j^T on MFMA.
+// Each launch exposes thousands of independent tiles, which is what hides
+// the memory latency that a single workgroup per matrix cannot.  The
+// assembly reads each component tile once per 32 jobs of its patch.  The
+// per-step grids are fixed by the batch's sizes (plans built once), so the
+// whole sequence is captured as one graph.
+// ---------------------------------------------------------------------------
+constexpr int SNB = 64;     // panel width and tile edge of the step-synchronous engine
+constexpr int SJC = 32;     // jobs per assembly workgroup
+
+struct StepArgs {
+    const int* order;
+    const int* job_patch;
+    const int64_t* patch_n;
+    const int64_t* patch_comp;
+    const double* comps;
+    const double* coef;
+    int ncomp;
+    void* work;
+    const int64_t* job_work;
+    const int64_t* job_vec;
+    double* dsc;
+    double* linv;               // per job a 64 x 64 FP64 block inverse
+    double* ldacc;              // per job 2 sum log L_ii (FP64)
+    int* info;
+    int njobs;
+    int k0;                     // the step's first column
+    // segments of equal n in the order (this step's): first position,
+    // per-job tile counts, tile offsets (ascending)
+    int nseg;
+    const int64_t* seg_q0;
+    const int64_t* seg_ppj;
+    const int64_t* seg_upj;
+    const int64_t* seg_poff;
+    const int64_t* seg_uoff;
+    // assembly: per patch its lower 64-tile count, chunk count and WG offset, and its jobs
+    int npatch;
+    const int64_t* asm_tiles;
+    const int64_t* asm_chunks;
+    const int64_t* asm_off;
+    const int* pjobs;
+    const int64_t* pjoff;
+};
+
+// Last index s < cnt with off[s] <= x (off ascending, off[0] = 0).
+__device__ __forceinline__ int find_last_le(const int64_t* off, int cnt, int64_t x)
+{
+    int lo = 0, hi = cnt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Jacobi scales d_i = 1 / sqrt(C_ii) from the FP64 diagonals; clears the
+// job's ln|C| accumulator and info.
+__global__ __launch_bounds__(BT) void sw_scale_kernel(StepArgs a)
+{
+    const int job = blockIdx.x;
+    const int p = a.job_patch[job];
+    const int n = (int)a.patch_n[p];
+    const double* M = a.comps + a.patch_comp[p];
+    double c[MAXC];
+#pragma unroll
+    for (int k = 0; k < MAXC; ++k) c[k] = k < a.ncomp ? a.coef[(int64_t)job * a.ncomp + k] : 0.0;
+    double* d = a.dsc + a.job_vec[job];
+    const int64_t nn = (int64_t)n * n;
+    for (int i = threadIdx.x; i < n; i += BT) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k)
+            if (k < a.ncomp) v += c[k] * M[k * nn + i + (int64_t)i * n];
+        d[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+    }
+    if (threadIdx.x == 0) {
+        a.ldacc[job] = 0.0;
+        a.info[job] = 0;
+    }
+}
+
+// Lower 64 x 64 tile (ti, tj) of D C D for up to SJC jobs of one patch: the
+// component tiles are read once into registers (thread: row tid % 64,
+// columns tid / 64 + 4 u), then each job's combination is stored.
+template <typename TS>
+__global__ __launch_bounds__(BT) void sw_assemble_kernel(StepArgs a)
+{
+    const int p = find_last_le(a.asm_off, a.npatch, (int64_t)blockIdx.x);
+    const int64_t local = (int64_t)blockIdx.x - a.asm_off[p];
+    const int64_t nch = a.asm_chunks[p];
+    const int tile = (int)(local / nch), chunk = (int)(local - tile * nch);
+    int ti, tj;
+    tri_tile(tile, ti, tj);
+    const int n = (int)a.patch_n[p];
+    const int64_t nn = (int64_t)n * n;
+    const double* M = a.comps + a.patch_comp[p];
+    const int i = ti * SNB + (threadIdx.x & 63);
+    const int jb = tj * SNB + (threadIdx.x >> 6);
+    double m[16][MAXC];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = jb + 4 * u;
+        const bool in = i < n && j < n && i >= j;
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) m[u][k] = (in && k < a.ncomp) ? M[k * nn + i + (int64_t)j * n] : 0.0;
+    }
+    const int64_t j0 = a.pjoff[p] + (int64_t)chunk * SJC;
+    const int64_t j1 = min(a.pjoff[p + 1], j0 + SJC);
+    for (int64_t q = j0; q < j1; ++q) {
+        const int job = a.pjobs[q];
+        double c[MAXC];
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k) c[k] = k < a.ncomp ? a.coef[(int64_t)job * a.ncomp + k] : 0.0;
+        const double* d = a.dsc + a.job_vec[job];
+        TS* A = (TS*)a.work + a.job_work[job];
+        const double di = i < n ? d[i] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int j = jb + 4 * u;
+            if (i < n && j < n && i >= j) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < MAXC; ++k) v += c[k] * m[u][k];
+                A[i + (int64_t)j * n] = (TS)(v * di * d[j]);
+            }
+        }
+    }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
+{
+    __shared__ double lt[SNB][SNB + 1];
+    __shared__ double rdiag[SNB];
+    const int job = a.order[blockIdx.x];
+    if (a.info[job] != 0) return;
+    const int n = (int)a.patch_n[a.job_patch[job]];
+    const int k0 = a.k0;
+    const int kb = min(SNB, n - k0);
+    TS* A = (TS*)a.work + a.job_work[job] + k0 + (int64_t)k0 * n;
+    const int r = threadIdx.x;
+    double row[SNB];
+#pragma unroll
+    for (int c = 0; c < SNB; ++c) row[c] = (r < kb && c <= r) ? (double)A[r + (int64_t)c * n] : 0.0;
+    const int bad = wave_chol64<double>(row, kb, lt, rdiag);
+    if (bad) {
+        if (r == 0) a.info[job] = k0 + bad;
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < SNB; ++c)
+        if (r < kb && c <= r) A[r + (int64_t)c * n] = (TS)row[c];
+    double ld = r < kb ? 2.0 * log(lt[r][r]) : 0.0;    // lt[q][i] = L[i][q]
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ld += __shfl_xor(ld, off, 64);
+    if (r == 0) a.ldacc[job] += ld;
+    wave_inv64_to_lds<double>(lt, rdiag);
+    double* li = a.linv + (int64_t)job * SNB * SNB;
+#pragma unroll
+    for (int cc = 0; cc < SNB; ++cc) li[r + cc * SNB] = lt[cc][r];
+}
+
+// The (job, tile) of a flat step grid: seg_* tables, per-job counts.
+__device__ __forceinline__ bool step_item(const StepArgs& a, const int64_t* off, const int64_t* per, int& job,
+                                          int64_t& item)
+{
+    const int s = find_last_le(off, a.nseg, (int64_t)blockIdx.x);
+    const int64_t local = (int64_t)blockIdx.x - off[s];
+    const int64_t pj = per[s];
+    if (pj <= 0) return false;
+    const int64_t q = a.seg_q0[s] + local / pj;
+    item = local - (local / pj) * pj;
+    job = a.order[q];
+    return a.info[job] == 0;
+}
+
+// P (64 rows of the panel below the diagonal block) <- P L_kk^-T, FP64 MFMA;
+// wave w owns rows 32 w .. 32 w + 31 and reads them completely before
+// overwriting them.
+template <typename TS>
+__global__ __launch_bounds__(128) void sw_panel_kernel(StepArgs a)
+{
+    typedef Mf<double>::v4 v4;
+    __shared__ double si[SNB][SNB + 1];
+    int job;
+    int64_t tile;
+    if (!step_item(a, a.seg_poff, a.seg_ppj, job, tile)) return;
+    const int n = (int)a.patch_n[a.job_patch[job]];
+    const int k0 = a.k0;
+    const int kb = min(SNB, n - k0);
+    const double* li = a.linv + (int64_t)job * SNB * SNB;
+    for (int idx = threadIdx.x; idx < SNB * SNB; idx += 128) si[idx % SNB][idx / SNB] = li[idx];
+    __syncthreads();
+    TS* A = (TS*)a.work + a.job_work[job];
+    constexpr int KS = SNB / 4, NT = SNB / 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int i0 = k0 + kb + (int)tile * SNB + 32 * wave;
+    double av[KS][2];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int kk = 4 * s + lk;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ri = i0 + 16 * h + lm;
+            av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (int64_t)(k0 + kk) * n] : 0.0;
+        }
+    }
+    double outv[NT][2][4];
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+        v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+        const int cc = 16 * b + lm;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int kk = 4 * s + lk;
+            const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;   // L^-T[kk][cc] = L^-1[cc][kk]
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[h] = Mf<double>::run(av[s][h], bv, acc[h]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) outv[b][h][r] = acc[h][r];
+    }
+#pragma unroll
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = i0 + 16 * h + lk + 4 * r;
+                const int col = 16 * b + lm;
+                if (row < n && col < kb) A[row + (int64_t)(k0 + col) * n] = (TS)outv[b][h][r];
+            }
+}
+
+// A22 tile (ti, tj) of 64 x 64 -= P_i P_j^T; wave w takes the 32 x 32
+// quadrant (w / 2, w % 2), the upper quadrant of a diagonal tile is skipped.
+template <typename TS, typename TC>
+__global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
+{
+    int job;
+    int64_t tile;
+    if (!step_item(a, a.seg_uoff, a.seg_upj, job, tile)) return;
+    int ti, tj;
+    tri_tile((int)tile, ti, tj);
+    const int wave = threadIdx.x >> 6;
+    const int wa = wave >> 1, wb = wave & 1;
+    if (ti == tj && wb > wa) return;
+    const int n = (int)a.patch_n[a.job_patch[job]];
+    const int k0 = a.k0;
+    const int kb = min(SNB, n - k0);
+    const int r0 = k0 + kb;
+    TS* A = (TS*)a.work + a.job_work[job];
+    update_tile32<TS, TC, SNB>(n, A, r0 + ti * SNB + 32 * wa, r0 + tj * SNB + 32 * wb, k0, kb);
+}
+
+// After the factorisation: ln|C|, y = (D C D)^-1 D x and chi2 (or, with
+// refinement steps, y stored for the patch products), as the tail of
+// batch_factor_kernel.
+template <typename TS>
+__global__ __launch_bounds__(BT, 2) void sw_solve_kernel(BatchArgs a, const double* ldacc)
+{
+    __shared__ double sd[32][33];
+    __shared__ double sv[VMAX];
+    __shared__ double red[(BT / 64) * 32];
+    const int tid = threadIdx.x;
+    const int job = a.order[blockIdx.x];
+    const int p = a.job_patch[job];
+    const int n = (int)a.patch_n[p];
+    if (a.info[job] != 0) {
+        if (a.iters > 0) {
+            double* yc = a.Y + a.job_ycol[job];
+            for (int i = tid; i < n; i += BT) yc[i] = 0.0;
+        }
+        if (tid == 0) {
+            a.logdet[job] = __builtin_nan("");
+            a.chi2[job] = __builtin_nan("");
+        }
+        return;
+    }
+    const double* x = a.xs + a.patch_x[p];
+    const TS* A = (const TS*)a.work + a.job_work[job];
+    const double* d = a.dsc + a.job_vec[job];
+    double ldd = 0.0;
+    for (int i = tid; i < n; i += BT) {
+        ldd += log(d[i]);
+        sv[i] = d[i] * x[i];
+    }
+    const double sum_logd = block_sum(ldd, red);
+    solve_inplace<TS, 32>(n, A, sv, sd, red);
+    if (a.iters > 0) {
+        double* ys = a.ys + a.job_vec[job];
+        double* yc = a.Y + a.job_ycol[job];
+        for (int i = tid; i < n; i += BT) {
+            ys[i] = sv[i];
+            yc[i] = d[i] * sv[i];
+        }
+        if (tid == 0) a.logdet[job] = ldacc[job] - 2.0 * sum_logd;
+        return;
+    }
+    double part = 0.0;
+    for (int i = tid; i < n; i += BT) part += d[i] * x[i] * sv[i];
+    const double chi = block_sum(part, red);
+    if (tid == 0) {
+        a.logdet[job] = ldacc[job] - 2.0 * sum_logd;
+        a.chi2[job] = chi;
+    }
+}
+
 __global__ void reset_queue_kernel(unsigned* q)
 {
     if (threadIdx.x < 2) q[threadIdx.x] = 0u;
@@ -581,6 +913,20 @@ struct cfmi_batch {
     int64_t* d_jobext = nullptr;  // job_ycol | job_tcol
     std::vector<int64_t> pn, pcomp, pY, pT, pJ;   // per patch (host)
     const double* comps = nullptr;
+    // step-synchronous engine (nb = 64): per-job block inverses and ln|C|
+    // accumulators, the per-step segment tables and the assembly tables
+    double* d_linv = nullptr;
+    double* d_ldacc = nullptr;
+    int64_t* d_plan = nullptr;
+    int* d_pjobs = nullptr;
+    struct Step {
+        int64_t jobs, panel, update;   // grid of each stage
+        int nseg;
+        int64_t off;                   // of the step's 5 nseg-long arrays in d_plan
+    };
+    std::vector<Step> steps;
+    int64_t asm_wgs = 0, asm_off = 0, pjoff_off = 0;   // assembly grid; table offsets in d_plan
+    StepArgs sargs = {};
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -630,15 +976,61 @@ int patch_products(cfmi_batch* b, hipStream_t s)
     return rc;
 }
 
+// The step-synchronous factorisation and the solves (nb = 64).
+template <typename TS, typename TC>
+int enqueue_steps(cfmi_batch* b, hipStream_t s)
+{
+    cfmi_ctx* c = b->ctx;
+    StepArgs sa = b->sargs;
+    sa.coef = b->args.coef;
+    sa.info = b->args.info;
+    hipLaunchKernelGGL(sw_scale_kernel, dim3((unsigned)b->njobs), dim3(BT), 0, s, sa);
+    CFMI_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL((sw_assemble_kernel<TS>), dim3((unsigned)b->asm_wgs), dim3(BT), 0, s, sa);
+    CFMI_LAUNCH_CHECK(c);
+    for (size_t k = 0; k < b->steps.size(); ++k) {
+        const cfmi_batch::Step& st = b->steps[k];
+        sa.k0 = (int)(k * SNB);
+        sa.nseg = st.nseg;
+        const int64_t* t = b->d_plan + st.off;
+        sa.seg_q0 = t;
+        sa.seg_ppj = t + st.nseg;
+        sa.seg_upj = t + 2 * st.nseg;
+        sa.seg_poff = t + 3 * st.nseg;
+        sa.seg_uoff = t + 4 * st.nseg;
+        hipLaunchKernelGGL((sw_diag_kernel<TS>), dim3((unsigned)st.jobs), dim3(64), 0, s, sa);
+        CFMI_LAUNCH_CHECK(c);
+        if (st.panel > 0) {
+            hipLaunchKernelGGL((sw_panel_kernel<TS>), dim3((unsigned)st.panel), dim3(128), 0, s, sa);
+            CFMI_LAUNCH_CHECK(c);
+        }
+        if (st.update > 0) {
+            hipLaunchKernelGGL((sw_update_kernel<TS, TC>), dim3((unsigned)st.update), dim3(BT), 0, s, sa);
+            CFMI_LAUNCH_CHECK(c);
+        }
+    }
+    hipLaunchKernelGGL((sw_solve_kernel<TS>), dim3((unsigned)b->njobs), dim3(BT), 0, s, b->args,
+                       (const double*)b->d_ldacc);
+    CFMI_LAUNCH_CHECK(c);
+    return CFMI_OK;
+}
+
 int enqueue(cfmi_batch* b, hipStream_t s)
 {
     cfmi_ctx* c = b->ctx;
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
-    CFMI_LAUNCH_CHECK(c);
-    if (b->mode == CFMI_BATCH_MIXED) launch_nb<float, float>(b, s);
-    else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
-    else launch_nb<double, double>(b, s);
-    CFMI_LAUNCH_CHECK(c);
+    if (b->nb == SNB) {
+        int rc = b->mode == CFMI_BATCH_MIXED ? enqueue_steps<float, float>(b, s)
+               : b->mode == CFMI_BATCH_MIXED_F64_STORAGE ? enqueue_steps<double, float>(b, s)
+                                                          : enqueue_steps<double, double>(b, s);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
+        CFMI_LAUNCH_CHECK(c);
+        if (b->mode == CFMI_BATCH_MIXED) launch_nb<float, float>(b, s);
+        else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
+        else launch_nb<double, double>(b, s);
+        CFMI_LAUNCH_CHECK(c);
+    }
     for (int it = 0; it < b->iters; ++it) {
         int rc = patch_products(b, s);
         if (rc) return rc;
@@ -668,6 +1060,70 @@ void release(cfmi_batch* b)
     (void)hipFree(b->d_Y);
     (void)hipFree(b->d_T);
     (void)hipFree(b->d_jobext);
+    (void)hipFree(b->d_linv);
+    (void)hipFree(b->d_ldacc);
+    (void)hipFree(b->d_plan);
+    (void)hipFree(b->d_pjobs);
+}
+
+// Plans of the step-synchronous engine: per step the active jobs (a prefix
+// of the order, largest n first) grouped by equal n, with their panel and
+// trailing tile counts; per patch the assembly tiles and job chunks.
+void plan_steps(cfmi_batch* b, int npatch, const int64_t* n, const std::vector<int>& order, const int* job_patch,
+                std::vector<int64_t>& tab, std::vector<int>& pjobs)
+{
+    const int njobs = (int)order.size();
+    struct Seg { int64_t q0, cnt, n; };
+    std::vector<Seg> segs;
+    for (int q = 0; q < njobs; ++q) {
+        const int64_t m = n[job_patch[order[q]]];
+        if (segs.empty() || segs.back().n != m) segs.push_back({q, 0, m});
+        segs.back().cnt += 1;
+    }
+    const int64_t nmax = segs.empty() ? 0 : segs.front().n;
+    b->steps.clear();
+    for (int64_t k0 = 0; k0 < nmax; k0 += SNB) {
+        cfmi_batch::Step st{0, 0, 0, 0, (int64_t)tab.size()};
+        std::vector<int64_t> q0, ppj, upj, poff, uoff;
+        for (const Seg& g : segs) {
+            if (g.n <= k0) break;
+            const int64_t kb = std::min<int64_t>(SNB, g.n - k0);
+            const int64_t t = (g.n - k0 - kb + SNB - 1) / SNB;
+            q0.push_back(g.q0);
+            ppj.push_back(t);
+            upj.push_back(t * (t + 1) / 2);
+            poff.push_back(st.panel);
+            uoff.push_back(st.update);
+            st.jobs += g.cnt;
+            st.panel += g.cnt * t;
+            st.update += g.cnt * t * (t + 1) / 2;
+        }
+        st.nseg = (int)q0.size();
+        for (auto* v : {&q0, &ppj, &upj, &poff, &uoff}) tab.insert(tab.end(), v->begin(), v->end());
+        b->steps.push_back(st);
+    }
+    // assembly: per patch its lower tiles x job chunks, jobs listed by patch
+    std::vector<int64_t> tiles(npatch), chunks(npatch), off(npatch), pjoff(npatch + 1, 0);
+    std::vector<int64_t> cnt(npatch, 0);
+    for (int j = 0; j < njobs; ++j) cnt[job_patch[j]] += 1;
+    for (int p = 0; p < npatch; ++p) pjoff[p + 1] = pjoff[p] + cnt[p];
+    pjobs.assign(njobs, 0);
+    {
+        std::vector<int64_t> fill(pjoff.begin(), pjoff.end() - 1);
+        for (int j = 0; j < njobs; ++j) pjobs[fill[job_patch[j]]++] = j;
+    }
+    b->asm_wgs = 0;
+    for (int p = 0; p < npatch; ++p) {
+        const int64_t t = (n[p] + SNB - 1) / SNB;
+        tiles[p] = t * (t + 1) / 2;
+        chunks[p] = (cnt[p] + SJC - 1) / SJC;
+        off[p] = b->asm_wgs;
+        b->asm_wgs += tiles[p] * chunks[p];
+    }
+    b->asm_off = (int64_t)tab.size();
+    for (auto* v : {&tiles, &chunks, &off}) tab.insert(tab.end(), v->begin(), v->end());
+    b->pjoff_off = (int64_t)tab.size();
+    tab.insert(tab.end(), pjoff.begin(), pjoff.end());
 }
 
 }  // namespace
@@ -684,7 +1140,8 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
         return cfmi_fail(c, CFMI_EINVAL, "batch: unknown mode %d", mode);
     if (npatch <= 0 || njobs <= 0) return cfmi_fail(c, CFMI_EINVAL, "batch: empty batch");
     if (ncomp < 1 || ncomp > MAXC) return cfmi_fail(c, CFMI_EINVAL, "batch: 1..%d component matrices", MAXC);
-    if (nb != 16 && nb != 32) return cfmi_fail(c, CFMI_EINVAL, "batch: panel width must be 16 or 32");
+    if (nb != 16 && nb != 32 && nb != SNB)
+        return cfmi_fail(c, CFMI_EINVAL, "batch: panel width must be 16, 32 or 64");
     if (wg_per_cu < 1 || wg_per_cu > 8) return cfmi_fail(c, CFMI_EINVAL, "batch: 1..8 workgroups per CU");
     if (refine_iters < 0 || refine_iters > 8) return cfmi_fail(c, CFMI_EINVAL, "batch: 0..8 refinement iterations");
     for (int p = 0; p < npatch; ++p)
@@ -772,6 +1229,19 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
         if (e == hipSuccess)
             chk(hipMemcpy(b->d_jobext, je.data(), je.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     }
+    std::vector<int64_t> tab;
+    std::vector<int> pjobs;
+    if (nb == SNB) {
+        plan_steps(b, npatch, n, order, job_patch, tab, pjobs);
+        chk(hipMalloc((void**)&b->d_linv, (size_t)njobs * SNB * SNB * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_ldacc, (size_t)njobs * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_plan, tab.size() * sizeof(int64_t)));
+        chk(hipMalloc((void**)&b->d_pjobs, pjobs.size() * sizeof(int)));
+        if (e == hipSuccess) {
+            chk(hipMemcpy(b->d_plan, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+            chk(hipMemcpy(b->d_pjobs, pjobs.data(), pjobs.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     if (e == hipSuccess) {
         chk(hipMemcpy(b->d_order, order.data(), (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
         chk(hipMemcpy(b->d_job_patch, job_patch, (size_t)njobs * sizeof(int), hipMemcpyHostToDevice));
@@ -804,6 +1274,28 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
     a.T = b->d_T;
     a.job_ycol = b->d_jobext;
     a.job_tcol = b->d_jobext ? b->d_jobext + njobs : nullptr;
+    if (nb == SNB) {
+        StepArgs& t = b->sargs;
+        t.order = a.order;
+        t.job_patch = a.job_patch;
+        t.patch_n = a.patch_n;
+        t.patch_comp = a.patch_comp;
+        t.comps = comps;
+        t.ncomp = ncomp;
+        t.work = a.work;
+        t.job_work = a.job_work;
+        t.job_vec = a.job_vec;
+        t.dsc = a.dsc;
+        t.linv = b->d_linv;
+        t.ldacc = b->d_ldacc;
+        t.njobs = njobs;
+        t.npatch = npatch;
+        t.asm_tiles = b->d_plan + b->asm_off;
+        t.asm_chunks = b->d_plan + b->asm_off + npatch;
+        t.asm_off = b->d_plan + b->asm_off + 2 * npatch;
+        t.pjobs = b->d_pjobs;
+        t.pjoff = b->d_plan + b->pjoff_off;
+    }
     *out = b;
     return CFMI_OK;
 }

@@ -7,6 +7,7 @@
 // consumed from LDS.  Split-K writes per-slab partials that a second
 // kernel reduces in a fixed order, so results do not depend on scheduling.
 #include "cfmi_internal.h"
+#include "chol64.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -523,63 +524,19 @@ constexpr int64_t OB = 512;   // Cholesky panel width (a multiple of NB)
 constexpr int64_t OB2 = 2048; // outer panel width (a multiple of OB)
 
 template <typename T>
-__device__ __forceinline__ T lane_value(T v, int lane)
-{
-    if constexpr (sizeof(T) == 8) {
-        const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-        const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-        return __hiloint2double(hi, lo);
-    } else {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-    }
-}
-
-template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
                                                         T* linv, T* U, int64_t ldu)
 {
-    // One wave.  Factor: lane r holds row r of the block in registers; step
-    // q reads the pivot from lane q (a scalar), forms 1 / L_qq once, scales
-    // column q by it, publishes the column through LDS and every lane
-    // updates its row (right-looking; a single wave needs no barrier between
-    // its LDS store and load, which the LDS unit keeps in order).
-    // Inverse: lane c owns column c of L^-1 and solves L u = e_c right-
-    // looking, u_k = s_k / L_kk then s_i -= L_ik u_k for i > k, so the
-    // dependent chain is one multiply and one FMA per k while the rest of
-    // the updates are independent; L's columns are LDS broadcasts.  The
-    // inverse leaves through LDS so that the global stores are by rows.
-    __shared__ T lt[NB][NB + 1];      // lt[k][i] = L[i][k] (columns of L contiguous), then U^T
-    __shared__ T rdiag[NB];           // 1 / L_kk
+    // One wave: factor and inverse of the kb x kb diagonal block (chol64.h);
+    // the inverse leaves through LDS so that the global stores are by rows.
+    __shared__ T lt[NB][NB + 1];
+    __shared__ T rdiag[NB];
     if (*info != 0) return;
     const int r = threadIdx.x;
     T row[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) row[c] = (r < kb && c <= r) ? A[r + c * lda] : T(0);
-    int bad = 0;
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-        if (q < kb && !bad) {
-            const T d = lane_value(row[q], q);
-            if (!(d > T(0))) {
-                bad = q + 1;
-            } else {
-                const T piv = sqrt(d);
-                const T rp = T(1) / piv;
-                const T l = r > q ? row[q] * rp : (r == q ? piv : T(0));
-                row[q] = l;
-                lt[q][r] = l;
-                if (r == 0) rdiag[q] = rp;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-                for (int p = q + 1; p < NB; ++p) row[p] -= l * lt[q][p];
-            }
-        } else if (q >= kb) {
-            lt[q][r] = T(0);
-            if (r == 0) rdiag[q] = T(0);
-        }
-    }
+    const int bad = wave_chol64<T>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) atomicCAS(info, 0, (int)(koff + bad));
         return;
@@ -587,22 +544,7 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         if (r < kb && c < kb) A[r + c * lda] = c <= r ? row[c] : T(0);
-    __syncthreads();
-    // U = L^-1: lane c solves L u = e_c (u_k = 0 for k < c falls out).
-    const int c = r;
-    T u[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) u[i] = (i == c) ? T(1) : T(0);
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        u[k] *= rdiag[k];
-#pragma unroll
-        for (int i = k + 1; i < NB; ++i) u[i] -= lt[k][i] * u[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NB; ++i) lt[c][i] = u[i];   // lt[c][i] = U[i][c]
-    __syncthreads();
+    wave_inv64_to_lds<T>(lt, rdiag);
     // lane i stores row i of U: U[i][c] = lt[c][i], consecutive lanes at consecutive addresses
     const int i = r;
 #pragma unroll

@@ -68,7 +68,7 @@ def test_batched_fp64_matches_oracle(setup):
 
     ctx, masks, basis, patches, job_patch, coef, params = setup
     ref = _oracle(masks, basis, patches, job_patch, params)
-    for nb in (16, 32):
+    for nb in batched.NB_CHOICES:
         b = batched.BatchedLikelihood(ctx, patches, job_patch, "fp64", nb=nb, wg_per_cu=2)
         chi2, logdet, info = b.run(coef)
         chi2, logdet = chi2.cpu().numpy(), logdet.cpu().numpy()
@@ -84,15 +84,16 @@ def test_batched_mixed_within_1e3(setup):
     ctx, masks, basis, patches, job_patch, coef, params = setup
     exact = batched.BatchedLikelihood(ctx, patches, job_patch, "fp64").minus2lnl(coef)
     for mode in ("mixed", "mixed_f64_storage"):
-        for nb in (16, 32):
+        for nb in batched.NB_CHOICES:
             m = batched.BatchedLikelihood(ctx, patches, job_patch, mode, nb=nb).minus2lnl(coef)
             err = np.max(np.abs(m - exact))
             print(f"{mode} nb={nb}: max |d(-2lnL)| = {err:.3e}")
             assert err < 1e-3
 
 
+@pytest.mark.parametrize("nb", [32, 64])
 @pytest.mark.parametrize("iters", [0, 2])
-def test_batched_refinement_steps(setup, iters):
+def test_batched_refinement_steps(setup, iters, nb):
     """No refinement and two refinement steps (per-patch residual products
     between the factor and refine kernels): FP64 still at 1e-10 of the
     oracle, mixed within 1e-3 of FP64."""
@@ -100,24 +101,25 @@ def test_batched_refinement_steps(setup, iters):
 
     ctx, masks, basis, patches, job_patch, coef, params = setup
     ref = _oracle(masks, basis, patches, job_patch, params)
-    b = batched.BatchedLikelihood(ctx, patches, job_patch, "fp64", refine_iters=iters)
+    b = batched.BatchedLikelihood(ctx, patches, job_patch, "fp64", refine_iters=iters, nb=nb)
     chi2, logdet, info = b.run(coef)
     assert not np.any(info.cpu().numpy())
     np.testing.assert_allclose(chi2.cpu().numpy(), ref[:, 0], rtol=1e-10)
     np.testing.assert_allclose(logdet.cpu().numpy(), ref[:, 1], rtol=1e-10)
     b.close()
     exact = ref[:, 0] + ref[:, 1]
-    m = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=iters).minus2lnl(coef)
+    m = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=iters, nb=nb).minus2lnl(coef)
     err = np.max(np.abs(m - exact))
     print(f"mixed, {iters} refinement steps: max |d(-2lnL)| = {err:.3e}")
     assert err < (1e-3 if iters else 1.0)
 
 
-def test_graph_replay_matches_eager(setup):
+@pytest.mark.parametrize("nb", [32, 64])
+def test_graph_replay_matches_eager(setup, nb):
     from cosmoforge_amd import batched
 
     ctx, masks, basis, patches, job_patch, coef, params = setup
-    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=2)
+    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", refine_iters=2, nb=nb)
     eager1 = b.minus2lnl(coef)
     coef2 = coef.copy()
     coef2[:, 3] *= 0.5
@@ -130,20 +132,21 @@ def test_graph_replay_matches_eager(setup):
         assert np.array_equal(g2, eager2), rep
 
 
-def test_non_positive_definite_job_reported(setup):
+@pytest.mark.parametrize("nb", [32, 64])
+def test_non_positive_definite_job_reported(setup, nb):
     import torch
     from cosmoforge_amd import batched
 
     ctx, masks, basis, patches, job_patch, coef, params = setup
     bad = coef.copy()
     bad[2] = [-1.0, 0.0, 0.0, 0.0]          # C = -N
-    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed")
+    b = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", nb=nb)
     chi2, logdet, info = b.run(bad)
     torch.cuda.synchronize()
     info = info.cpu().numpy()
     assert info[2] == 1 and np.isnan(chi2[2].item()) and np.isnan(logdet[2].item())
     assert np.count_nonzero(info) == 1
-    good = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed").minus2lnl(coef)
+    good = batched.BatchedLikelihood(ctx, patches, job_patch, "mixed", nb=nb).minus2lnl(coef)
     mask = np.arange(len(coef)) != 2
     np.testing.assert_array_equal((chi2 + logdet).cpu().numpy()[mask], good[mask])
     with pytest.raises(Exception):
