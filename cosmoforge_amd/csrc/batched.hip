@@ -116,9 +116,21 @@ __device__ __forceinline__ void update_tile32(int n, TS* A, int i0, int j0, int 
 {
     typedef typename Mf<TC>::v4 v4;
     constexpr bool kF64 = sizeof(TC) == 8;
-    constexpr int KS = 8;                       // K-steps (of 4) loaded together
+    constexpr int KS = NB / 4;                  // every K-step (of 4) loaded together
     const int lane = threadIdx.x & 63;
     const int lm = lane & 15, lk = lane >> 4;
+    // the stored tile, fetched with the operands (its latency overlaps theirs)
+    TS cv[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = i0 + 16 * a + lm;
+                const int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
+                cv[a][b][r] = (row < n && col < n && row >= col) ? A[row + (int64_t)col * n] : TS(0);
+            }
     v4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -156,10 +168,7 @@ __device__ __forceinline__ void update_tile32(int n, TS* A, int i0, int j0, int 
             for (int r = 0; r < 4; ++r) {
                 int row = i0 + 16 * a + lm;
                 int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
-                if (row < n && col < n && row >= col) {
-                    TS* p = A + row + (int64_t)col * n;
-                    *p = (TS)((TS)*p + (TS)acc[a][b][r]);
-                }
+                if (row < n && col < n && row >= col) A[row + (int64_t)col * n] = (TS)(cv[a][b][r] + (TS)acc[a][b][r]);
             }
 }
 
@@ -592,9 +601,11 @@ struct StepArgs {
     const int64_t* job_work;
     const int64_t* job_vec;
     double* dsc;
-    double* linv;               // per job a 64 x 64 FP64 block inverse
+    double* linv;               // per job its ceil(n / 64) 64 x 64 FP64 diagonal-block inverses
+    const int64_t* job_inv;     // element offset of the job's first block inverse
     double* ldacc;              // per job 2 sum log L_ii (FP64)
     int* info;
+    double* rv;                 // per job (job_vec offsets) the right-hand side, solved in place
     int njobs;
     int k0;                     // the step's first column
     // segments of equal n in the order (this step's): first position,
@@ -728,7 +739,7 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
     for (int off = 32; off > 0; off >>= 1) ld += __shfl_xor(ld, off, 64);
     if (r == 0) a.ldacc[job] += ld;
     wave_inv64_to_lds<double>(lt, rdiag);
-    double* li = a.linv + (int64_t)job * SNB * SNB;
+    double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
 #pragma unroll
     for (int cc = 0; cc < SNB; ++cc) li[r + cc * SNB] = lt[cc][r];
 }
@@ -747,11 +758,13 @@ __device__ __forceinline__ bool step_item(const StepArgs& a, const int64_t* off,
     return a.info[job] == 0;
 }
 
-// P (64 rows of the panel below the diagonal block) <- P L_kk^-T, FP64 MFMA;
-// wave w owns rows 32 w .. 32 w + 31 and reads them completely before
-// overwriting them.
+// P (SPR rows of the panel below the diagonal block) <- P L_kk^-T, FP64
+// MFMA; each wave takes 32-row tiles and reads a tile completely before
+// overwriting it.  One load of L_kk^-1 into LDS serves SPR rows.
+constexpr int SPR = 256;    // panel rows per workgroup
+
 template <typename TS>
-__global__ __launch_bounds__(128) void sw_panel_kernel(StepArgs a)
+__global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
 {
     typedef Mf<double>::v4 v4;
     __shared__ double si[SNB][SNB + 1];
@@ -761,51 +774,54 @@ __global__ __launch_bounds__(128) void sw_panel_kernel(StepArgs a)
     const int n = (int)a.patch_n[a.job_patch[job]];
     const int k0 = a.k0;
     const int kb = min(SNB, n - k0);
-    const double* li = a.linv + (int64_t)job * SNB * SNB;
-    for (int idx = threadIdx.x; idx < SNB * SNB; idx += 128) si[idx % SNB][idx / SNB] = li[idx];
+    const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
+    for (int idx = threadIdx.x; idx < SNB * SNB; idx += 256) si[idx % SNB][idx / SNB] = li[idx];
     __syncthreads();
     TS* A = (TS*)a.work + a.job_work[job];
     constexpr int KS = SNB / 4, NT = SNB / 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lm = lane & 15, lk = lane >> 4;
-    const int i0 = k0 + kb + (int)tile * SNB + 32 * wave;
-    double av[KS][2];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int kk = 4 * s + lk;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int ri = i0 + 16 * h + lm;
-            av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (int64_t)(k0 + kk) * n] : 0.0;
-        }
-    }
-    double outv[NT][2][4];
-#pragma unroll
-    for (int b = 0; b < NT; ++b) {
-        v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
-        const int cc = 16 * b + lm;
+    for (int half = 0; half < SPR / 128; ++half) {
+        const int i0 = k0 + kb + (int)tile * SPR + 128 * half + 32 * wave;
+        if (i0 >= n) break;
+        double av[KS][2];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int kk = 4 * s + lk;
-            const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;   // L^-T[kk][cc] = L^-1[cc][kk]
 #pragma unroll
-            for (int h = 0; h < 2; ++h) acc[h] = Mf<double>::run(av[s][h], bv, acc[h]);
+            for (int h = 0; h < 2; ++h) {
+                const int ri = i0 + 16 * h + lm;
+                av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (int64_t)(k0 + kk) * n] : 0.0;
+            }
+        }
+        double outv[NT][2][4];
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+            const int cc = 16 * b + lm;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int kk = 4 * s + lk;
+                const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;   // L^-T[kk][cc] = L^-1[cc][kk]
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc[h] = Mf<double>::run(av[s][h], bv, acc[h]);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) outv[b][h][r] = acc[h][r];
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int b = 0; b < NT; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) outv[b][h][r] = acc[h][r];
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = i0 + 16 * h + lk + 4 * r;
+                    const int col = 16 * b + lm;
+                    if (row < n && col < kb) A[row + (int64_t)(k0 + col) * n] = (TS)outv[b][h][r];
+                }
     }
-#pragma unroll
-    for (int b = 0; b < NT; ++b)
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = i0 + 16 * h + lk + 4 * r;
-                const int col = 16 * b + lm;
-                if (row < n && col < kb) A[row + (int64_t)(k0 + col) * n] = (TS)outv[b][h][r];
-            }
 }
 
 // A22 tile (ti, tj) of 64 x 64 -= P_i P_j^T; wave w takes the 32 x 32
@@ -829,57 +845,152 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
     update_tile32<TS, TC, SNB>(n, A, r0 + ti * SNB + 32 * wa, r0 + tj * SNB + 32 * wb, k0, kb);
 }
 
-// After the factorisation: ln|C|, y = (D C D)^-1 D x and chi2 (or, with
-// refinement steps, y stored for the patch products), as the tail of
-// batch_factor_kernel.
-template <typename TS>
-__global__ __launch_bounds__(BT, 2) void sw_solve_kernel(BatchArgs a, const double* ldacc)
+// Solves, step-synchronous and in place on each job's FP64 vector rv: the
+// forward step k takes z_k = L_kk^-1 r_k (the stored FP64 block inverse) and
+// subtracts L[below, k] z_k from the rows below; the backward step k (last
+// block first) forms r_k - L[below, k]^T x_below, one wave per 16 columns
+// with lane-strided rows and a fixed-order wave reduction, then x_k =
+// L_kk^-T of it.  One workgroup per job and step, every active job at once.
+
+// rv = D x (mode 0), or the refinement residual D (x - sum_k c_k (M_k Y)_job)
+// from the patch products (mode 1).
+__global__ __launch_bounds__(BT) void sw_rhs_kernel(BatchArgs a, double* rv, int mode)
 {
-    __shared__ double sd[32][33];
-    __shared__ double sv[VMAX];
-    __shared__ double red[(BT / 64) * 32];
-    const int tid = threadIdx.x;
+    const int job = blockIdx.x;
+    if (a.info[job] != 0) return;
+    const int p = a.job_patch[job];
+    const int n = (int)a.patch_n[p];
+    const double* x = a.xs + a.patch_x[p];
+    const double* d = a.dsc + a.job_vec[job];
+    double* r = rv + a.job_vec[job];
+    if (mode == 0) {
+        for (int i = threadIdx.x; i < n; i += BT) r[i] = d[i] * x[i];
+        return;
+    }
+    const double* t = a.T + a.job_tcol[job];
+    double c[MAXC];
+#pragma unroll
+    for (int k = 0; k < MAXC; ++k) c[k] = k < a.ncomp ? a.coef[(int64_t)job * a.ncomp + k] : 0.0;
+    for (int i = threadIdx.x; i < n; i += BT) {
+        double cy = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXC; ++k)
+            if (k < a.ncomp) cy += c[k] * t[(int64_t)k * n + i];
+        r[i] = d[i] * (x[i] - cy);
+    }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(BT) void sw_fwd_kernel(StepArgs a)
+{
+    __shared__ double rk[SNB];
+    __shared__ double zk[SNB];
     const int job = a.order[blockIdx.x];
+    if (a.info[job] != 0) return;
+    const int n = (int)a.patch_n[a.job_patch[job]];
+    const int k0 = a.k0;
+    const int kb = min(SNB, n - k0);
+    double* r = a.rv + a.job_vec[job];
+    const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
+    const int tid = threadIdx.x;
+    if (tid < SNB) rk[tid] = tid < kb ? r[k0 + tid] : 0.0;
+    __syncthreads();
+    if (tid < SNB) {
+        double z = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < SNB; ++c) z += li[tid + c * SNB] * rk[c];
+        zk[tid] = z;
+        if (tid < kb) r[k0 + tid] = z;
+    }
+    __syncthreads();
+    const TS* L = (const TS*)a.work + a.job_work[job] + (int64_t)k0 * n;
+    for (int i = k0 + kb + tid; i < n; i += BT) {
+        double acc = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < kb; ++c) acc += (double)L[i + (int64_t)c * n] * zk[c];
+        r[i] -= acc;
+    }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(BT) void sw_bwd_kernel(StepArgs a)
+{
+    __shared__ double sl[SNB][SNB + 1];
+    __shared__ double rk[SNB];
+    const int job = a.order[blockIdx.x];
+    if (a.info[job] != 0) return;
+    const int n = (int)a.patch_n[a.job_patch[job]];
+    const int k0 = a.k0;
+    const int kb = min(SNB, n - k0);
+    double* r = a.rv + a.job_vec[job];
+    const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int idx = tid; idx < SNB * SNB; idx += BT) sl[idx % SNB][idx / SNB] = li[idx];
+    // t_c = sum_{i >= k0 + kb} L[i][k0 + c] x_i: wave w takes columns w, w + 4, ...
+    const TS* L = (const TS*)a.work + a.job_work[job] + (int64_t)k0 * n;
+    for (int c = wave; c < kb; c += BT / 64) {
+        double part = 0.0;
+        for (int i = k0 + kb + lane; i < n; i += 64) part += (double)L[i + (int64_t)c * n] * r[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (lane == 0) rk[c] = r[k0 + c] - part;
+    }
+    __syncthreads();
+    if (tid < kb) {
+        // x = L_kk^-T rk: x[t] = sum_c L^-1[c][t] rk[c]
+        double x = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < SNB; ++c)
+            if (c < kb) x += sl[c][tid] * rk[c];
+        r[k0 + tid] = x;
+    }
+}
+
+// After the solves.  mode 0: ln|C| and chi2 = (D x)^T y; 1: ln|C|, y kept
+// for refinement (ys, and the unscaled y as the job's column of Y); 2:
+// ys += y (a refinement step that is not the last); 3: chi2 from ys + y.
+__global__ __launch_bounds__(BT) void sw_finish_kernel(BatchArgs a, const double* rv, const double* ldacc, int mode)
+{
+    __shared__ double red[BT / 64];
+    const int tid = threadIdx.x;
+    const int job = blockIdx.x;
     const int p = a.job_patch[job];
     const int n = (int)a.patch_n[p];
     if (a.info[job] != 0) {
-        if (a.iters > 0) {
+        if (mode != 0 && mode != 3) {
             double* yc = a.Y + a.job_ycol[job];
             for (int i = tid; i < n; i += BT) yc[i] = 0.0;
         }
-        if (tid == 0) {
+        if (tid == 0 && mode <= 1) {
             a.logdet[job] = __builtin_nan("");
             a.chi2[job] = __builtin_nan("");
         }
         return;
     }
     const double* x = a.xs + a.patch_x[p];
-    const TS* A = (const TS*)a.work + a.job_work[job];
     const double* d = a.dsc + a.job_vec[job];
-    double ldd = 0.0;
-    for (int i = tid; i < n; i += BT) {
-        ldd += log(d[i]);
-        sv[i] = d[i] * x[i];
+    const double* y = rv + a.job_vec[job];
+    if (mode <= 1) {
+        double ldd = 0.0;
+        for (int i = tid; i < n; i += BT) ldd += log(d[i]);
+        const double sum_logd = block_sum(ldd, red);
+        if (tid == 0) a.logdet[job] = ldacc[job] - 2.0 * sum_logd;
     }
-    const double sum_logd = block_sum(ldd, red);
-    solve_inplace<TS, 32>(n, A, sv, sd, red);
-    if (a.iters > 0) {
+    if (mode == 1 || mode == 2) {
         double* ys = a.ys + a.job_vec[job];
         double* yc = a.Y + a.job_ycol[job];
         for (int i = tid; i < n; i += BT) {
-            ys[i] = sv[i];
-            yc[i] = d[i] * sv[i];
+            const double v = mode == 1 ? y[i] : ys[i] + y[i];
+            ys[i] = v;
+            yc[i] = d[i] * v;
         }
-        if (tid == 0) a.logdet[job] = ldacc[job] - 2.0 * sum_logd;
         return;
     }
+    const double* ys = a.ys + a.job_vec[job];
     double part = 0.0;
-    for (int i = tid; i < n; i += BT) part += d[i] * x[i] * sv[i];
+    for (int i = tid; i < n; i += BT) part += d[i] * x[i] * (mode == 0 ? y[i] : ys[i] + y[i]);
     const double chi = block_sum(part, red);
-    if (tid == 0) {
-        a.logdet[job] = ldacc[job] - 2.0 * sum_logd;
-        a.chi2[job] = chi;
-    }
+    if (tid == 0) a.chi2[job] = chi;
 }
 
 __global__ void reset_queue_kernel(unsigned* q)
@@ -917,6 +1028,7 @@ struct cfmi_batch {
     // accumulators, the per-step segment tables and the assembly tables
     double* d_linv = nullptr;
     double* d_ldacc = nullptr;
+    double* d_rv = nullptr;
     int64_t* d_plan = nullptr;
     int* d_pjobs = nullptr;
     struct Step {
@@ -926,6 +1038,7 @@ struct cfmi_batch {
     };
     std::vector<Step> steps;
     int64_t asm_wgs = 0, asm_off = 0, pjoff_off = 0;   // assembly grid; table offsets in d_plan
+    int64_t inv_off = 0, inv_total = 0;                 // per-job block-inverse offsets in d_plan; their size
     StepArgs sargs = {};
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1001,7 +1114,7 @@ int enqueue_steps(cfmi_batch* b, hipStream_t s)
         hipLaunchKernelGGL((sw_diag_kernel<TS>), dim3((unsigned)st.jobs), dim3(64), 0, s, sa);
         CFMI_LAUNCH_CHECK(c);
         if (st.panel > 0) {
-            hipLaunchKernelGGL((sw_panel_kernel<TS>), dim3((unsigned)st.panel), dim3(128), 0, s, sa);
+            hipLaunchKernelGGL((sw_panel_kernel<TS>), dim3((unsigned)st.panel), dim3(256), 0, s, sa);
             CFMI_LAUNCH_CHECK(c);
         }
         if (st.update > 0) {
@@ -1009,9 +1122,28 @@ int enqueue_steps(cfmi_batch* b, hipStream_t s)
             CFMI_LAUNCH_CHECK(c);
         }
     }
-    hipLaunchKernelGGL((sw_solve_kernel<TS>), dim3((unsigned)b->njobs), dim3(BT), 0, s, b->args,
-                       (const double*)b->d_ldacc);
-    CFMI_LAUNCH_CHECK(c);
+    for (int it = 0; it <= b->iters; ++it) {
+        if (it > 0) {
+            int rc = patch_products(b, s);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(sw_rhs_kernel, dim3((unsigned)b->njobs), dim3(BT), 0, s, b->args, b->d_rv, it > 0 ? 1 : 0);
+        CFMI_LAUNCH_CHECK(c);
+        for (size_t k = 0; k < b->steps.size(); ++k) {
+            sa.k0 = (int)(k * SNB);
+            hipLaunchKernelGGL((sw_fwd_kernel<TS>), dim3((unsigned)b->steps[k].jobs), dim3(BT), 0, s, sa);
+            CFMI_LAUNCH_CHECK(c);
+        }
+        for (size_t k = b->steps.size(); k-- > 0;) {
+            sa.k0 = (int)(k * SNB);
+            hipLaunchKernelGGL((sw_bwd_kernel<TS>), dim3((unsigned)b->steps[k].jobs), dim3(BT), 0, s, sa);
+            CFMI_LAUNCH_CHECK(c);
+        }
+        const int mode = it == 0 ? (b->iters > 0 ? 1 : 0) : (it < b->iters ? 2 : 3);
+        hipLaunchKernelGGL(sw_finish_kernel, dim3((unsigned)b->njobs), dim3(BT), 0, s, b->args,
+                           (const double*)b->d_rv, (const double*)b->d_ldacc, mode);
+        CFMI_LAUNCH_CHECK(c);
+    }
     return CFMI_OK;
 }
 
@@ -1022,15 +1154,14 @@ int enqueue(cfmi_batch* b, hipStream_t s)
         int rc = b->mode == CFMI_BATCH_MIXED ? enqueue_steps<float, float>(b, s)
                : b->mode == CFMI_BATCH_MIXED_F64_STORAGE ? enqueue_steps<double, float>(b, s)
                                                           : enqueue_steps<double, double>(b, s);
-        if (rc) return rc;
-    } else {
-        hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
-        CFMI_LAUNCH_CHECK(c);
-        if (b->mode == CFMI_BATCH_MIXED) launch_nb<float, float>(b, s);
-        else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
-        else launch_nb<double, double>(b, s);
-        CFMI_LAUNCH_CHECK(c);
+        return rc;
     }
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, b->d_queue);
+    CFMI_LAUNCH_CHECK(c);
+    if (b->mode == CFMI_BATCH_MIXED) launch_nb<float, float>(b, s);
+    else if (b->mode == CFMI_BATCH_MIXED_F64_STORAGE) launch_nb<double, float>(b, s);
+    else launch_nb<double, double>(b, s);
+    CFMI_LAUNCH_CHECK(c);
     for (int it = 0; it < b->iters; ++it) {
         int rc = patch_products(b, s);
         if (rc) return rc;
@@ -1062,6 +1193,7 @@ void release(cfmi_batch* b)
     (void)hipFree(b->d_jobext);
     (void)hipFree(b->d_linv);
     (void)hipFree(b->d_ldacc);
+    (void)hipFree(b->d_rv);
     (void)hipFree(b->d_plan);
     (void)hipFree(b->d_pjobs);
 }
@@ -1088,14 +1220,15 @@ void plan_steps(cfmi_batch* b, int npatch, const int64_t* n, const std::vector<i
         for (const Seg& g : segs) {
             if (g.n <= k0) break;
             const int64_t kb = std::min<int64_t>(SNB, g.n - k0);
-            const int64_t t = (g.n - k0 - kb + SNB - 1) / SNB;
+            const int64_t t = (g.n - k0 - kb + SNB - 1) / SNB;      // trailing 64-tiles per side
+            const int64_t tp = (g.n - k0 - kb + SPR - 1) / SPR;     // panel workgroups
             q0.push_back(g.q0);
-            ppj.push_back(t);
+            ppj.push_back(tp);
             upj.push_back(t * (t + 1) / 2);
             poff.push_back(st.panel);
             uoff.push_back(st.update);
             st.jobs += g.cnt;
-            st.panel += g.cnt * t;
+            st.panel += g.cnt * tp;
             st.update += g.cnt * t * (t + 1) / 2;
         }
         st.nseg = (int)q0.size();
@@ -1124,6 +1257,13 @@ void plan_steps(cfmi_batch* b, int npatch, const int64_t* n, const std::vector<i
     for (auto* v : {&tiles, &chunks, &off}) tab.insert(tab.end(), v->begin(), v->end());
     b->pjoff_off = (int64_t)tab.size();
     tab.insert(tab.end(), pjoff.begin(), pjoff.end());
+    // every job keeps its ceil(n / 64) FP64 diagonal-block inverses (the solves use them)
+    b->inv_off = (int64_t)tab.size();
+    b->inv_total = 0;
+    for (int j = 0; j < njobs; ++j) {
+        tab.push_back(b->inv_total);
+        b->inv_total += (n[job_patch[j]] + SNB - 1) / SNB * SNB * SNB;
+    }
 }
 
 }  // namespace
@@ -1233,8 +1373,9 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
     std::vector<int> pjobs;
     if (nb == SNB) {
         plan_steps(b, npatch, n, order, job_patch, tab, pjobs);
-        chk(hipMalloc((void**)&b->d_linv, (size_t)njobs * SNB * SNB * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_linv, (size_t)b->inv_total * sizeof(double)));
         chk(hipMalloc((void**)&b->d_ldacc, (size_t)njobs * sizeof(double)));
+        chk(hipMalloc((void**)&b->d_rv, (size_t)v * sizeof(double)));
         chk(hipMalloc((void**)&b->d_plan, tab.size() * sizeof(int64_t)));
         chk(hipMalloc((void**)&b->d_pjobs, pjobs.size() * sizeof(int)));
         if (e == hipSuccess) {
@@ -1287,7 +1428,9 @@ int cfmi_batch_create(cfmi_ctx* c, int mode, int npatch, const int64_t* n, int n
         t.job_vec = a.job_vec;
         t.dsc = a.dsc;
         t.linv = b->d_linv;
+        t.job_inv = b->d_plan + b->inv_off;
         t.ldacc = b->d_ldacc;
+        t.rv = b->d_rv;
         t.njobs = njobs;
         t.npatch = npatch;
         t.asm_tiles = b->d_plan + b->asm_off;
