@@ -744,12 +744,22 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
     for (int cc = 0; cc < SNB; ++cc) li[r + cc * SNB] = lt[cc][r];
 }
 
+// Workgroup x of a grid of g runs on XCD x % 8 (round-robin dispatch); it
+// takes item x / 8 of that XCD's contiguous share of the grid, so that the
+// tiles of one job (which share its panel) meet in one XCD's L2.
+__device__ __forceinline__ int64_t xcd_item(int64_t x, int64_t g)
+{
+    const int64_t q = g >> 3, r = g & 7, xcd = x & 7;
+    return xcd * q + (xcd < r ? xcd : r) + (x >> 3);
+}
+
 // The (job, tile) of a flat step grid: seg_* tables, per-job counts.
 __device__ __forceinline__ bool step_item(const StepArgs& a, const int64_t* off, const int64_t* per, int& job,
                                           int64_t& item)
 {
-    const int s = find_last_le(off, a.nseg, (int64_t)blockIdx.x);
-    const int64_t local = (int64_t)blockIdx.x - off[s];
+    const int64_t v = xcd_item(blockIdx.x, gridDim.x);
+    const int s = find_last_le(off, a.nseg, v);
+    const int64_t local = v - off[s];
     const int64_t pj = per[s];
     if (pj <= 0) return false;
     const int64_t q = a.seg_q0[s] + local / pj;
@@ -824,25 +834,93 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
     }
 }
 
-// A22 tile (ti, tj) of 64 x 64 -= P_i P_j^T; wave w takes the 32 x 32
-// quadrant (w / 2, w % 2), the upper quadrant of a diagonal tile is skipped.
+// A22 tile (ti, tj) of 64 x 64 -= P_i P_j^T.  The workgroup stages the two
+// 64 x kb operand blocks of the panel once in LDS (k-major, rows
+// contiguous; a diagonal tile stages one), with coalesced column loads, and
+// fetches its stored tile at the same time; wave w then takes the 32 x 32
+// quadrant (w / 2, w % 2) on MFMA (the upper quadrant of a diagonal tile
+// is skipped).  A k-row of 64 entries padded to 80 keeps the MFMA operand
+// reads (16 rows x 2 k-rows per half-wave) free of LDS bank conflicts for
+// 4- and 8-byte entries.
 template <typename TS, typename TC>
 __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
 {
+    typedef typename Mf<TC>::v4 v4;
+    constexpr bool kF64 = sizeof(TC) == 8;
+    constexpr int LDT = 80;
+    static_assert(LDT >= SNB, "a staged k-row holds the 64 rows of a tile");
+    __shared__ TC sa[SNB][LDT];
+    __shared__ TC sb[SNB][LDT];
     int job;
     int64_t tile;
     if (!step_item(a, a.seg_uoff, a.seg_upj, job, tile)) return;
     int ti, tj;
     tri_tile((int)tile, ti, tj);
-    const int wave = threadIdx.x >> 6;
-    const int wa = wave >> 1, wb = wave & 1;
-    if (ti == tj && wb > wa) return;
     const int n = (int)a.patch_n[a.job_patch[job]];
     const int k0 = a.k0;
     const int kb = min(SNB, n - k0);
     const int r0 = k0 + kb;
+    const int iA = r0 + ti * SNB, iB = r0 + tj * SNB;
+    const bool diag = ti == tj;
     TS* A = (TS*)a.work + a.job_work[job];
-    update_tile32<TS, TC, SNB>(n, A, r0 + ti * SNB + 32 * wa, r0 + tj * SNB + 32 * wb, k0, kb);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wa = wave >> 1, wb = wave & 1;
+    const int lm = lane & 15, lk = lane >> 4;
+    const bool active = !(diag && wb > wa);
+    // the stored quadrant (MFMA output layout)
+    TS cv[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = iA + 32 * wa + 16 * x + lm;
+                const int col = iB + 32 * wb + 16 * y + (kF64 ? lk + 4 * r : 4 * lk + r);
+                cv[x][y][r] = (active && row < n && col < n && row >= col) ? A[row + (int64_t)col * n] : TS(0);
+            }
+    // operand blocks: thread (row = lane, k = wave + 4 p)
+#pragma unroll
+    for (int p = 0; p < SNB / 4; ++p) {
+        const int k = wave + 4 * p;
+        const TS* col = A + (int64_t)(k0 + (k < kb ? k : 0)) * n;
+        const int ra = iA + lane, rb = iB + lane;
+        sa[k][lane] = (k < kb && ra < n) ? (TC)col[ra] : TC(0);
+        if (!diag) sb[k][lane] = (k < kb && rb < n) ? (TC)col[rb] : TC(0);
+    }
+    __syncthreads();
+    if (!active) return;
+    const TC (*pb)[LDT] = diag ? sa : sb;
+    v4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < SNB / 4; ++s) {
+        const int k = 4 * s + lk;
+        TC av[2], bv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            av[h] = sa[k][32 * wa + 16 * h + lm];
+            bv[h] = pb[k][32 * wb + 16 * h + lm];
+        }
+        // D = P_j P_i^T (the transposed tile), as update_tile32
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = Mf<TC>::run(bv[y], av[x], acc[x][y]);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = iA + 32 * wa + 16 * x + lm;
+                const int col = iB + 32 * wb + 16 * y + (kF64 ? lk + 4 * r : 4 * lk + r);
+                if (row < n && col < n && row >= col) A[row + (int64_t)col * n] = (TS)(cv[x][y][r] - (TS)acc[x][y][r]);
+            }
 }
 
 // Solves, step-synchronous and in place on each job's FP64 vector rv: the
@@ -926,14 +1004,29 @@ __global__ __launch_bounds__(BT) void sw_bwd_kernel(StepArgs a)
     const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int idx = tid; idx < SNB * SNB; idx += BT) sl[idx % SNB][idx / SNB] = li[idx];
-    // t_c = sum_{i >= k0 + kb} L[i][k0 + c] x_i: wave w takes columns w, w + 4, ...
+    // t_c = sum_{i >= k0 + kb} L[i][k0 + c] x_i: wave w takes columns
+    // 16 w .. 16 w + 15 together (16 independent sums per lane over the rows
+    // lane, lane + 64, ...), then one fixed-order wave reduction per column
     const TS* L = (const TS*)a.work + a.job_work[job] + (int64_t)k0 * n;
-    for (int c = wave; c < kb; c += BT / 64) {
-        double part = 0.0;
-        for (int i = k0 + kb + lane; i < n; i += 64) part += (double)L[i + (int64_t)c * n] * r[i];
+    constexpr int CW = SNB / (BT / 64);
+    double part[CW];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-        if (lane == 0) rk[c] = r[k0 + c] - part;
+    for (int u = 0; u < CW; ++u) part[u] = 0.0;
+    for (int i = k0 + kb + lane; i < n; i += 64) {
+        const double xi = r[i];
+#pragma unroll
+        for (int u = 0; u < CW; ++u) {
+            const int c = CW * wave + u;
+            if (c < kb) part[u] += (double)L[i + (int64_t)c * n] * xi;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CW; ++u) {
+        double t = part[u];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        const int c = CW * wave + u;
+        if (lane == 0 && c < kb) rk[c] = r[k0 + c] - t;
     }
     __syncthreads();
     if (tid < kb) {
