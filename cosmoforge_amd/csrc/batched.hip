@@ -834,14 +834,20 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
     }
 }
 
-// A22 tile (ti, tj) of 64 x 64 -= P_i P_j^T.  The workgroup stages the two
-// 64 x kb operand blocks of the panel once in LDS (k-major, rows
-// contiguous; a diagonal tile stages one), with coalesced column loads, and
-// fetches its stored tile at the same time; wave w then takes the 32 x 32
-// quadrant (w / 2, w % 2) on MFMA (the upper quadrant of a diagonal tile
-// is skipped).  A k-row of 64 entries padded to 80 keeps the MFMA operand
-// reads (16 rows x 2 k-rows per half-wave) free of LDS bank conflicts for
-// 4- and 8-byte entries.
+// Trailing updates with one step of look-ahead, so that the trailing matrix
+// is read and written once per two panels.  Step k (k0 = 64 k, r0 = k0 +
+// kb the first row below the diagonal block):
+//   k even: the next block column only, A[r0:, r0:r0+64] -= P_k P_k[r0:r0+64]^T
+//           (tiles (ti, 0));
+//   k odd:  the whole trailing matrix from both panels, A[r0:, r0:] -=
+//           [P_{k-1} P_k] [P_{k-1} P_k]^T (K = 64 + kb, columns k0 - 64 ..).
+// The workgroup stages the two 64-row operand blocks in LDS one 64-deep K
+// chunk at a time (k-major, rows contiguous; a diagonal tile stages one),
+// with coalesced column loads, and fetches its stored tile at the start;
+// wave w takes the 32 x 32 quadrant (w / 2, w % 2) on MFMA (the upper
+// quadrant of a diagonal tile is skipped).  A k-row of 64 entries padded to
+// 80 keeps the MFMA operand reads (16 rows x 2 k-rows per half-wave) free of
+// LDS bank conflicts for 4- and 8-byte entries.
 template <typename TS, typename TC>
 __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
 {
@@ -854,12 +860,15 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
     int job;
     int64_t tile;
     if (!step_item(a, a.seg_uoff, a.seg_upj, job, tile)) return;
+    const bool wide = (a.k0 / SNB) & 1;
     int ti, tj;
-    tri_tile((int)tile, ti, tj);
+    if (wide) tri_tile((int)tile, ti, tj);
+    else ti = (int)tile, tj = 0;
     const int n = (int)a.patch_n[a.job_patch[job]];
     const int k0 = a.k0;
     const int kb = min(SNB, n - k0);
     const int r0 = k0 + kb;
+    const int kbeg = wide ? k0 - SNB : k0, kend = k0 + kb;
     const int iA = r0 + ti * SNB, iB = r0 + tj * SNB;
     const bool diag = ti == tj;
     TS* A = (TS*)a.work + a.job_work[job];
@@ -879,38 +888,44 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
                 const int col = iB + 32 * wb + 16 * y + (kF64 ? lk + 4 * r : 4 * lk + r);
                 cv[x][y][r] = (active && row < n && col < n && row >= col) ? A[row + (int64_t)col * n] : TS(0);
             }
-    // operand blocks: thread (row = lane, k = wave + 4 p)
-#pragma unroll
-    for (int p = 0; p < SNB / 4; ++p) {
-        const int k = wave + 4 * p;
-        const TS* col = A + (int64_t)(k0 + (k < kb ? k : 0)) * n;
-        const int ra = iA + lane, rb = iB + lane;
-        sa[k][lane] = (k < kb && ra < n) ? (TC)col[ra] : TC(0);
-        if (!diag) sb[k][lane] = (k < kb && rb < n) ? (TC)col[rb] : TC(0);
-    }
-    __syncthreads();
-    if (!active) return;
     const TC (*pb)[LDT] = diag ? sa : sb;
     v4 acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
+    for (int kc = kbeg; kc < kend; kc += SNB) {
+        const int kw = min(SNB, kend - kc);
+        if (kc != kbeg) __syncthreads();
+        // operand blocks: thread (row = lane, k = wave + 4 p)
 #pragma unroll
-    for (int s = 0; s < SNB / 4; ++s) {
-        const int k = 4 * s + lk;
-        TC av[2], bv[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            av[h] = sa[k][32 * wa + 16 * h + lm];
-            bv[h] = pb[k][32 * wb + 16 * h + lm];
+        for (int p = 0; p < SNB / 4; ++p) {
+            const int k = wave + 4 * p;
+            const TS* col = A + (int64_t)(kc + (k < kw ? k : 0)) * n;
+            const int ra = iA + lane, rb = iB + lane;
+            sa[k][lane] = (k < kw && ra < n) ? (TC)col[ra] : TC(0);
+            if (!diag) sb[k][lane] = (k < kw && rb < n) ? (TC)col[rb] : TC(0);
         }
-        // D = P_j P_i^T (the transposed tile), as update_tile32
+        __syncthreads();
+        if (active) {
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+            for (int s = 0; s < SNB / 4; ++s) {
+                const int k = 4 * s + lk;
+                TC av[2], bv[2];
 #pragma unroll
-            for (int y = 0; y < 2; ++y) acc[x][y] = Mf<TC>::run(bv[y], av[x], acc[x][y]);
+                for (int h = 0; h < 2; ++h) {
+                    av[h] = sa[k][32 * wa + 16 * h + lm];
+                    bv[h] = pb[k][32 * wb + 16 * h + lm];
+                }
+                // D = P_j P_i^T (the transposed tile), as update_tile32
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) acc[x][y] = Mf<TC>::run(bv[y], av[x], acc[x][y]);
+            }
+        }
     }
+    if (!active) return;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1315,14 +1330,16 @@ void plan_steps(cfmi_batch* b, int npatch, const int64_t* n, const std::vector<i
             const int64_t kb = std::min<int64_t>(SNB, g.n - k0);
             const int64_t t = (g.n - k0 - kb + SNB - 1) / SNB;      // trailing 64-tiles per side
             const int64_t tp = (g.n - k0 - kb + SPR - 1) / SPR;     // panel workgroups
+            // look-ahead: even steps update the next block column, odd steps the trailing matrix
+            const int64_t tu = (k0 / SNB) & 1 ? t * (t + 1) / 2 : t;
             q0.push_back(g.q0);
             ppj.push_back(tp);
-            upj.push_back(t * (t + 1) / 2);
+            upj.push_back(tu);
             poff.push_back(st.panel);
             uoff.push_back(st.update);
             st.jobs += g.cnt;
             st.panel += g.cnt * tp;
-            st.update += g.cnt * t * (t + 1) / 2;
+            st.update += g.cnt * tu;
         }
         st.nseg = (int)q0.size();
         for (auto* v : {&q0, &ppj, &upj, &poff, &uoff}) tab.insert(tab.end(), v->begin(), v->end());
