@@ -794,6 +794,7 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
     for (int half = 0; half < SPR / 128; ++half) {
         const int i0 = k0 + kb + (int)tile * SPR + 128 * half + 32 * wave;
         if (i0 >= n) break;
+        // the wave's 32 rows, read completely before any of them is written
         double av[KS][2];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
@@ -804,33 +805,26 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
                 av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (int64_t)(k0 + kk) * n] : 0.0;
             }
         }
-        double outv[NT][2][4];
 #pragma unroll
         for (int b = 0; b < NT; ++b) {
             v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
             const int cc = 16 * b + lm;
+            // L^-T[kk][cc] = L^-1[cc][kk] is zero for kk > cc: K-steps past 16 b + 15 are skipped
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
+            for (int s = 0; s < 4 * (b + 1); ++s) {
                 const int kk = 4 * s + lk;
-                const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;   // L^-T[kk][cc] = L^-1[cc][kk]
+                const double bv = (kk < kb && cc < kb) ? si[cc][kk] : 0.0;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) acc[h] = Mf<double>::run(av[s][h], bv, acc[h]);
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) outv[b][h][r] = acc[h][r];
-        }
-#pragma unroll
-        for (int b = 0; b < NT; ++b)
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = i0 + 16 * h + lk + 4 * r;
-                    const int col = 16 * b + lm;
-                    if (row < n && col < kb) A[row + (int64_t)(k0 + col) * n] = (TS)outv[b][h][r];
+                    if (row < n && cc < kb) A[row + (int64_t)(k0 + cc) * n] = (TS)acc[h][r];
                 }
+        }
     }
 }
 
