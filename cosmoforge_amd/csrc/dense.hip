@@ -408,6 +408,111 @@ struct GemmShape {
     int gm = 0, gn = 0;              // tile group of batched products (group_tile); 0: default
 };
 
+// Matrix-vector products in FP64 (M = 1 or N = 1 of a plain product),
+// bound by reading the matrix once: a GEMM tile would leave 127 of its 128
+// columns idle.
+//   gemv_n: y[i] = alpha sum_k A[i + k lda] x[k incx] + beta y[i incy]
+//           (rows contiguous): a thread per row over one chunk of GV_KC
+//           columns (x chunk in LDS), the chunks' partial sums reduced in a
+//           fixed order;
+//   gemv_t: y[j] = alpha sum_k A[k + j lda] x[k incx] + beta y[j incy]
+//           (columns contiguous): a wave per column, a fixed-order wave
+//           reduction.
+constexpr int GV_KC = 512;
+
+__global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t k, const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ x, int64_t incx,
+                                                     double* __restrict__ part)
+{
+    __shared__ double xs[GV_KC];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k0 = (int64_t)blockIdx.y * GV_KC;
+    const int64_t kc = min((int64_t)GV_KC, k - k0);
+    for (int j = threadIdx.x; j < kc; j += 256) xs[j] = x[(k0 + j) * incx];
+    __syncthreads();
+    if (i >= m) return;
+    const double* a = A + i + k0 * lda;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 4 <= kc; j += 4) {
+        s0 += a[(j + 0) * lda] * xs[j + 0];
+        s1 += a[(j + 1) * lda] * xs[j + 1];
+        s2 += a[(j + 2) * lda] * xs[j + 2];
+        s3 += a[(j + 3) * lda] * xs[j + 3];
+    }
+    for (; j < kc; ++j) s0 += a[j * lda] * xs[j];
+    part[(int64_t)blockIdx.y * m + i] = (s0 + s1) + (s2 + s3);
+}
+
+__global__ void gemv_reduce_kernel(int64_t m, int64_t nz, const double* __restrict__ part, double alpha, double beta,
+                                   double* __restrict__ y, int64_t incy)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int64_t z = 0; z < nz; ++z) s += part[z * m + i];
+        double* p = y + i * incy;
+        *p = alpha * s + (beta == 0.0 ? 0.0 : beta * *p);
+    }
+}
+
+__global__ __launch_bounds__(256) void gemv_t_kernel(int64_t n, int64_t k, const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ x, int64_t incx, double alpha,
+                                                     double beta, double* __restrict__ y, int64_t incy)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    const double* a = A + j * lda;
+    double s0 = 0.0, s1 = 0.0;
+    int64_t q = lane;
+    for (; q + 64 < k; q += 128) {
+        s0 += a[q] * x[q * incx];
+        s1 += a[q + 64] * x[(q + 64) * incx];
+    }
+    if (q < k) s0 += a[q] * x[q * incx];
+    double s = s0 + s1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+        double* p = y + j * incy;
+        *p = alpha * s + (beta == 0.0 ? 0.0 : beta * *p);
+    }
+}
+
+// The GEMV cases of gemm_x (FP64, plain output); returns false when the
+// product is not one.
+bool gemv_dispatch(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                   int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int tri, int& rc)
+{
+    rc = CFMI_OK;
+    if (tri != 0 || K <= 0) return false;
+    if (N == 1 && !ta) {
+        // y (M) = A (M x K, rows contiguous) x, x = B's column (tb = 0) or row (tb = 1)
+        const int64_t nz = (K + GV_KC - 1) / GV_KC;
+        if (nz > 65535) return false;
+        double* part = (double*)cfmi_workspace(c, (size_t)nz * M * sizeof(double));
+        if (!part) {
+            rc = cfmi_fail(c, CFMI_ENOMEM, "gemv: workspace allocation failed");
+            return true;
+        }
+        hipLaunchKernelGGL(gemv_n_kernel, dim3((unsigned)((M + 255) / 256), (unsigned)nz), dim3(256), 0, c->stream, M,
+                           K, A, lda, B, tb ? ldb : (int64_t)1, part);
+        if (hipGetLastError() != hipSuccess) { rc = cfmi_fail(c, CFMI_EHIP, "gemv launch failed"); return true; }
+        hipLaunchKernelGGL(gemv_reduce_kernel, elem_grid(M), dim3(256), 0, c->stream, M, nz, (const double*)part,
+                           alpha, beta, C, (int64_t)1);
+        if (hipGetLastError() != hipSuccess) rc = cfmi_fail(c, CFMI_EHIP, "gemv launch failed");
+        return true;
+    }
+    if (M == 1 && !tb) {
+        // y (N, stride ldc) = B^T x, B K x N (columns contiguous), x = A's row (ta = 0) or column (ta = 1)
+        hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, c->stream, N, K, B, ldb, A,
+                           ta ? (int64_t)1 : lda, alpha, beta, C, ldc);
+        if (hipGetLastError() != hipSuccess) rc = cfmi_fail(c, CFMI_EHIP, "gemv launch failed");
+        return true;
+    }
+    return false;
+}
+
 template <typename T, typename TO, typename TE>
 int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
            int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc, int tri,
@@ -415,6 +520,14 @@ int gemm_x(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
 {
     const int64_t batch = sh.batch;
     if (M <= 0 || N <= 0 || batch <= 0) return CFMI_OK;
+    if constexpr (std::is_same<T, double>::value && std::is_same<TO, double>::value &&
+                  std::is_same<TE, double>::value) {
+        int rc = CFMI_OK;
+        if (batch == 1 && sh.om.mode == 0 &&
+            gemv_dispatch(c, ta, tb, M, N, K, alpha, (const double*)A, lda, (const double*)B, ldb, beta, (double*)C,
+                          ldc, tri, rc))
+            return rc;
+    }
     // FP64 accumulators of FP32 products fit in registers only with 96 x 96 tiles.
     constexpr bool kAcc64 = sizeof(TE) > sizeof(T);
     const bool small = kAcc64 || (M <= 3 * BSMALL && N <= 3 * BSMALL && K > 4096);

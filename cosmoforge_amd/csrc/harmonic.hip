@@ -19,7 +19,9 @@
 // signal_kernels.py (Q + iU = -sum (a_E + i a_B) 2Y).
 #include "cfmi_internal.h"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace {
 
@@ -312,6 +314,103 @@ __global__ __launch_bounds__(RT) void harm_fisher_kernel(const double* __restric
     }
 }
 
+// One block per bin pair (b1 >= b2), all spectrum pairs at once: each mode
+// pair (k1 in b1, k2 in b2) reads the G x G entries M[g k1, g' k2] (and
+// those of M2) once, into thread-private LDS slots, and every (s1, s2) term
+// M[v_e, u_f] M2[u_e, v_f] picks its two entries there by the pair table of
+// the spectra (uniform indices, no bank conflicts).  The G^2 reads per mode
+// pair replace the sum over spectrum pairs of their 2 n1 n2 reads in
+// harm_fisher_kernel (9 against 2 x 81 for T, E, B).  Fixed-order sums.
+constexpr int HFG = 3;    // mode groups (T, E, B)
+constexpr int HFS = 6;    // spectra
+__global__ __launch_bounds__(RT) void harm_fisher_bins_kernel(const double* __restrict__ M,
+                                                              const double* __restrict__ M2, int64_t ldm,
+                                                              HarmTables t, int nspec, int npar,
+                                                              double* __restrict__ F)
+{
+    __shared__ double mv[HFG * HFG][RT];
+    __shared__ double mv2[HFG * HFG][RT];
+    __shared__ int ent[HFS * HFS][4][2];     // per (s1, s2): up to 4 (index into mv, index into mv2)
+    __shared__ int nent[HFS * HFS];
+    __shared__ double red[HFS * HFS][RT / 64];
+    const int64_t idx = blockIdx.x;
+    int b1 = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(b1 + 1) * (b1 + 2) / 2 <= idx) ++b1;
+    while ((int64_t)b1 * (b1 + 1) / 2 > idx) --b1;
+    const int b2 = (int)(idx - (int64_t)b1 * (b1 + 1) / 2);
+    const int tid = threadIdx.x;
+    int G = 1;
+    for (int s = 0; s < nspec; ++s)
+        for (int e = 0; e < 4; ++e) G = max(G, t.pairs[4 * s + e] + 1);   // <= HFG (checked by the host)
+    if (tid < nspec * nspec) {
+        const int s1 = tid / nspec, s2 = tid % nspec;
+        const int* P1 = t.pairs + 4 * s1;
+        const int* P2 = t.pairs + 4 * s2;
+        const int n1 = P1[2] < 0 ? 1 : 2, n2 = P2[2] < 0 ? 1 : 2;
+        int c = 0;
+        for (int e = 0; e < n1; ++e)
+            for (int f = 0; f < n2; ++f) {
+                // M[v_e(k1), u_f(k2)] M2[u_e(k1), v_f(k2)]
+                ent[tid][c][0] = P1[2 * e + 1] * HFG + P2[2 * f];
+                ent[tid][c][1] = P1[2 * e] * HFG + P2[2 * f + 1];
+                ++c;
+            }
+        nent[tid] = c;
+    }
+    __syncthreads();
+    const int64_t lmin2 = (int64_t)t.lmin * t.lmin;
+    const int64_t k1a = (int64_t)t.lo[b1] * t.lo[b1] - lmin2, K1 = (int64_t)(t.hi[b1] + 1) * (t.hi[b1] + 1) - lmin2 - k1a;
+    const int64_t k2a = (int64_t)t.lo[b2] * t.lo[b2] - lmin2, K2 = (int64_t)(t.hi[b2] + 1) * (t.hi[b2] + 1) - lmin2 - k2a;
+    double acc[HFS * HFS];
+#pragma unroll
+    for (int i = 0; i < HFS * HFS; ++i) acc[i] = 0.0;
+    for (int64_t q = tid; q < K1 * K2; q += RT) {
+        const int64_t k1 = k1a + q % K1, k2 = k2a + q / K1;
+        const int l1 = ell_of(k1, t.lmin), l2 = ell_of(k2, t.lmin);
+        for (int g1 = 0; g1 < G; ++g1)
+            for (int g2 = 0; g2 < G; ++g2) {
+                const int64_t o = (t.goff[g1] + k1) + (t.goff[g2] + k2) * ldm;
+                mv[g1 * HFG + g2][tid] = M[o];
+                mv2[g1 * HFG + g2][tid] = M2[o];
+            }
+#pragma unroll
+        for (int s1 = 0; s1 < HFS; ++s1) {
+            if (s1 >= nspec) break;
+            const double w1 = t.w[s1 * t.lw + l1];
+#pragma unroll
+            for (int s2 = 0; s2 < HFS; ++s2) {
+                if (s2 >= nspec) break;
+                const int pq = s1 * nspec + s2;
+                double term = 0.0;
+                for (int c = 0; c < nent[pq]; ++c) term += mv[ent[pq][c][0]][tid] * mv2[ent[pq][c][1]][tid];
+                acc[s1 * HFS + s2] += w1 * t.w[s2 * t.lw + l2] * term;
+            }
+        }
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < HFS * HFS; ++i) {
+        double v = acc[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[i][wave] = v;
+    }
+    __syncthreads();
+    if (tid < HFS * HFS) {
+        const int s1 = tid / HFS, s2 = tid % HFS;
+        if (s1 < nspec && s2 < nspec) {
+            double v = 0.0;
+            for (int w = 0; w < RT / 64; ++w) v += red[tid][w];
+            const int p1 = s1 * t.nbins + b1, p2 = s2 * t.nbins + b2;
+            // a diagonal bin pair gives each entry twice, (s1, s2) and (s2, s1): the lower one writes it
+            if (b1 != b2 || p1 >= p2) {
+                F[p1 + (int64_t)p2 * npar] = 0.5 * v;
+                F[p2 + (int64_t)p1 * npar] = 0.5 * v;
+            }
+        }
+    }
+}
+
 // out[p + j ldo] = alpha sum_{k in bin(p)} w sum_pairs sum_{i < ni} X[i + u_k ldx + j sxj] Y[i + v_k ldy + j syj]
 __global__ __launch_bounds__(RT) void harm_pairsum_kernel(int64_t ni, const double* __restrict__ X, int64_t ldx,
                                                           int64_t sxj, const double* __restrict__ Y, int64_t ldy,
@@ -407,6 +506,20 @@ int cfmi_harmonic_fisher(cfmi_ctx* c, const double* M, const double* M2, int64_t
     const int64_t nent = npar * (npar + 1) / 2;
     if (nent > 0x7fffffffLL) return cfmi_fail(c, CFMI_EINVAL, "harmonic_fisher: too many entries");
     HarmTables t{bin_lo, bin_hi, pairs, goff, w, lw, nbins, lmin};
+    int G = 0;
+    if (nspec <= HFS) {
+        // the mode groups the spectra pair (a small device table, read once per Fisher matrix)
+        std::vector<int> ph(4 * (size_t)nspec);
+        CFMI_HIP(c, hipMemcpyAsync(ph.data(), pairs, ph.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        CFMI_HIP(c, hipStreamSynchronize(c->stream));
+        for (int v : ph) G = std::max(G, v + 1);
+    }
+    if (nspec <= HFS && G <= HFG) {
+        hipLaunchKernelGGL(harm_fisher_bins_kernel, dim3((unsigned)((int64_t)nbins * (nbins + 1) / 2)), dim3(RT), 0,
+                           c->stream, M, M2 ? M2 : M, ldm, t, nspec, (int)npar, F);
+        CFMI_LAUNCH_CHECK(c);
+        return CFMI_OK;
+    }
     hipLaunchKernelGGL(harm_fisher_kernel, dim3((unsigned)nent), dim3(RT), 0, c->stream, M, M2 ? M2 : M, ldm, t,
                        (int)npar, F);
     CFMI_LAUNCH_CHECK(c);

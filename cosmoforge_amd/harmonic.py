@@ -229,7 +229,16 @@ class HarmonicContraction:
         if cov.factor.slices:
             Rr = ctx.upload_cm(maps)
             engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.S, 0, n, Y, 0, n, 1.0, Rr, 0, n)
-            engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.N, 0, n, Y, 0, n, 1.0, Rr, 0, n)
+            if cov.noise_is_diagonal():
+                # N y for a diagonal N: y's rows scaled by N's diagonal
+                NY = Y.clone()
+                d = torch.empty(n, dtype=torch.float64, device=ctx.device)
+                ctx.check(ctx.lib.cfmi_copy(ctx.h, engine.F64, 1, n, P(cov.N), n + 1, P(d), 1), "copy")
+                ctx.check(ctx.lib.cfmi_scale(ctx.h, 1, n, ns, P(NY), n, P(d)), "scale")
+                ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, ns, -1.0, P(NY), n, 1.0, P(Rr), n), "axpby")
+                del NY, d
+            else:
+                engine.gemm(ctx, False, False, n, ns, n, -1.0, cov.N, 0, n, Y, 0, n, 1.0, Rr, 0, n)
             D = self._apply_inv(cov, Rr, ns)
             ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, ns, 1.0, P(D), n, 1.0, P(Y), n), "axpby")
         return Y

@@ -96,7 +96,11 @@ class Covariance:
         S = self.signal(cls)
         C = ctx.empty(n, n)
         ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(S), n, 0.0, P(C), n), "axpby")
-        ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
+        if self.noise_is_diagonal():
+            # only the diagonal of N: an n-vector with stride n + 1 on both sides
+            ctx.check(ctx.lib.cfmi_axpby(ctx.h, 1, n, 1.0, P(self.N), n + 1, 1.0, P(C), n + 1), "axpby")
+        else:
+            ctx.check(ctx.lib.cfmi_axpby(ctx.h, n, n, 1.0, P(self.N), n, 1.0, P(C), n), "axpby")
         self.factor = engine.factor(ctx, C, n, need_whitener=need_whitener and not diag_blocks,
                                     diag_blocks=diag_blocks, slices=slices)
         self._w32 = None
