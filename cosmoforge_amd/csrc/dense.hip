@@ -668,28 +668,73 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
     }
 }
 
-// P (m x kb) <- P * Linv^T, 64 rows per workgroup.
+// P (m x kb) <- P Linv^T, 256 rows per workgroup on MFMA (16 x 16 x 4):
+// wave w takes rows 32 w .. of each 128-row half, reads its 32 rows
+// completely before storing any of them, and skips the K-steps past each
+// 16-column block's diagonal (Linv^T is upper triangular).
+template <typename T> struct PanelMf;
+template <> struct PanelMf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct PanelMf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int64_t m, int kb,
                                                          const T* linv, const int* info)
 {
-    __shared__ T sp[NB][NB + 1];
+    typedef typename PanelMf<T>::v4 v4;
     __shared__ T sl[NB][NB + 1];
     if (*info != 0) return;
     const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int r = idx & (NB - 1), q = idx >> 6;
-        sp[r][q] = (r0 + r < m && q < kb) ? P[(r0 + r) + q * ldp] : T(0);
-        sl[r][q] = linv[r + q * NB];
-    }
+    for (int idx = tid; idx < NB * NB; idx += 256) sl[idx & (NB - 1)][idx >> 6] = linv[idx];   // sl[i][j] = Linv[i][j]
     __syncthreads();
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int r = idx & (NB - 1), q = idx >> 6;
-        if (r0 + r >= m || q >= kb) continue;
-        T sum = 0;
-        for (int j = 0; j <= q; ++j) sum += sp[r][j] * sl[q][j];
-        P[(r0 + r) + q * ldp] = sum;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    constexpr int KS = NB / 4, NT = NB / 16;
+    for (int half = 0; half < 2; ++half) {
+        const int64_t i0 = (int64_t)blockIdx.x * 256 + 128 * half + 32 * wave;
+        if (i0 >= m) break;
+        T av[KS][2];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int kk = 4 * s + lk;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t ri = i0 + 16 * h + lm;
+                av[s][h] = (kk < kb && ri < m) ? P[ri + (int64_t)kk * ldp] : T(0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+            const int cc = 16 * b + lm;
+#pragma unroll
+            for (int s = 0; s < 4 * (b + 1); ++s) {
+                const int kk = 4 * s + lk;
+                const T bv = (kk < kb && cc < kb) ? sl[cc][kk] : T(0);   // Linv^T[kk][cc]
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc[h] = PanelMf<T>::run(av[s][h], bv, acc[h]);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t row = i0 + 16 * h + PanelMf<T>::row(lk, r);
+                    if (row < m && cc < kb) P[row + (int64_t)cc * ldp] = acc[h][r];
+                }
+        }
     }
 }
 
@@ -772,7 +817,7 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
                 CFMI_LAUNCH_CHECK(c);
                 int64_t m = n - k - kb;
                 if (m > 0) {
-                    hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + NB - 1) / NB)), dim3(256), 0,
+                    hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                                        c->stream, A + (k + kb) + k * lda, lda, m, kb, (const T*)linv,
                                        (const int*)c->d_info);
                     CFMI_LAUNCH_CHECK(c);

@@ -84,14 +84,18 @@ __device__ __forceinline__ double absmax(double m, double v)
 }
 
 // Layout 0: row r at src + r lds (k contiguous).  One wave per row: the
-// row maximum by a wave reduction, then 16-value chunks, zero beyond k.
+// row maximum by a wave reduction, then 1024-value chunks, read coalesced
+// (64 consecutive values per load) into the wave's LDS buffer and sliced
+// as 16 consecutive values per lane (a pad entry every 16 keeps both LDS
+// sides free of bank conflicts); zero beyond k.
 template <int S>
 __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k, const double* __restrict__ src,
                                                          int64_t lds, char* __restrict__ dst, int64_t ldk,
                                                          int64_t ps, double* __restrict__ mu)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ double buf[4][1024 + 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 4 + wave;
     if (r >= rows) return;
     const double* row = src + r * lds;
     double m = 0.0;
@@ -101,17 +105,30 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k
     m = finish_mu(m);
     if (lane == 0) mu[r] = m;
     const double inv = 1.0 / m;
-    for (int64_t c = lane; c * 16 < ldk; c += 64) {
-        double x[16];
+    double* b = buf[wave];
+    for (int64_t base = 0; base < ldk; base += 1024) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int64_t kk = c * 16 + j;
-            x[j] = kk < k ? row[kk] : 0.0;
+        for (int t = 0; t < 16; ++t) {
+            const int e = t * 64 + lane;                  // chunk element
+            const int64_t kk = base + e;
+            b[e + (e >> 4)] = kk < k ? row[kk] : 0.0;
         }
-        u4 o[S];
-        slice16<S>(x, inv, o);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int64_t c = base / 16 + lane;              // this lane's 16-value group
+        if (c * 16 < ldk) {
+            double x[16];
 #pragma unroll
-        for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + c * 16) = o[p];
+            for (int j = 0; j < 16; ++j) x[j] = b[lane * 17 + j];
+            u4 o[S];
+            slice16<S>(x, inv, o);
+#pragma unroll
+            for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + c * 16) = o[p];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
