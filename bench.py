@@ -156,8 +156,9 @@ def cpu_baseline(cfg, stride: int, k_params: int = 2):
 
 
 def picslike_leg(ctx):
-    """Config-3 batched exact likelihood: evals/s of the persistent batched
-    kernel (mixed and FP64, eager and graph replay) against the plain
+    """Config-3 batched exact likelihood: evals/s of the batched engine the
+    autotuner picks (step-synchronous launches or one persistent workgroup
+    per matrix; mixed and FP64, eager and graph replay) against the plain
     per-point GPU path (the drop-in PICSLike point by point, FP64)."""
     import torch
 
