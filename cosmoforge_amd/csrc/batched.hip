@@ -129,7 +129,7 @@ __device__ __forceinline__ void update_tile32(int n, TS* A, int i0, int j0, int 
             for (int r = 0; r < 4; ++r) {
                 const int row = i0 + 16 * a + lm;
                 const int col = j0 + 16 * b + (kF64 ? lk + 4 * r : 4 * lk + r);
-                cv[a][b][r] = (row < n && col < n && row >= col) ? A[row + (int64_t)col * n] : TS(0);
+                cv[a][b][r] = load_or_zero((const TS*)A, row + (int64_t)col * n, row < n && col < n && row >= col);
             }
     v4 acc[2][2];
 #pragma unroll
@@ -146,8 +146,8 @@ __device__ __forceinline__ void update_tile32(int n, TS* A, int i0, int j0, int 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int ri = i0 + 16 * h + lm, rj = j0 + 16 * h + lm;
-                av[s][h] = (kin && ri < n) ? -(TC)col[ri] : TC(0);
-                bv[s][h] = (kin && rj < n) ? (TC)col[rj] : TC(0);
+                av[s][h] = -(TC)load_or_zero(col, ri, kin && ri < n);
+                bv[s][h] = (TC)load_or_zero(col, rj, kin && rj < n);
             }
         }
         // D = P_j P_i^T (the transposed tile): a lane's outputs are rows
@@ -198,7 +198,7 @@ __device__ __forceinline__ void load_diag(int n, const T* A, int k0, int kb, dou
         const int idx = threadIdx.x + s * BT;
         if (idx < NB * NB) {
             int r = idx % NB, q = idx / NB;
-            sd[r][q] = (r < kb && q < kb && r >= q) ? (double)A[(k0 + r) + (k0 + q) * n] : 0.0;
+            sd[r][q] = (double)load_or_zero(A, (int64_t)(k0 + r) + (int64_t)(k0 + q) * n, r < kb && q < kb && r >= q);
         }
     }
 }
@@ -229,7 +229,7 @@ __device__ void panel_solve(int n, TS* A, int k0, int kb, const double (*si)[NB 
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int ri = i0 + 16 * h + lm;
-                av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (k0 + kk) * n] : 0.0;
+                av[s][h] = (double)load_or_zero((const TS*)A, ri + (int64_t)(k0 + kk) * n, kk < kb && ri < n);
             }
         }
         for (int b = 0; b < NT; ++b) {      // 16 output columns at a time
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(BT) void sw_assemble_kernel(StepArgs a)
         const int j = jb + 4 * u;
         const bool in = i < n && j < n && i >= j;
 #pragma unroll
-        for (int k = 0; k < MAXC; ++k) m[u][k] = (in && k < a.ncomp) ? M[k * nn + i + (int64_t)j * n] : 0.0;
+        for (int k = 0; k < MAXC; ++k) m[u][k] = load_or_zero(M, k * nn + i + (int64_t)j * n, in && k < a.ncomp);
     }
     const int64_t j0 = a.pjoff[p] + (int64_t)chunk * SJC;
     const int64_t j1 = min(a.pjoff[p + 1], j0 + SJC);
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
     const int r = threadIdx.x;
     double row[SNB];
 #pragma unroll
-    for (int c = 0; c < SNB; ++c) row[c] = (r < kb && c <= r) ? (double)A[r + (int64_t)c * n] : 0.0;
+    for (int c = 0; c < SNB; ++c) row[c] = (double)load_or_zero((const TS*)A, r + (int64_t)c * n, r < kb && c <= r);
     const int bad = wave_chol64<double>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) a.info[job] = k0 + bad;
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int ri = i0 + 16 * h + lm;
-                av[s][h] = (kk < kb && ri < n) ? (double)A[ri + (int64_t)(k0 + kk) * n] : 0.0;
+                av[s][h] = (double)load_or_zero((const TS*)A, ri + (int64_t)(k0 + kk) * n, kk < kb && ri < n);
             }
         }
 #pragma unroll
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
             for (int r = 0; r < 4; ++r) {
                 const int row = iA + 32 * wa + 16 * x + lm;
                 const int col = iB + 32 * wb + 16 * y + (kF64 ? lk + 4 * r : 4 * lk + r);
-                cv[x][y][r] = (active && row < n && col < n && row >= col) ? A[row + (int64_t)col * n] : TS(0);
+                cv[x][y][r] = load_or_zero((const TS*)A, row + (int64_t)col * n, active && row < n && col < n && row >= col);
             }
     const TC (*pb)[LDT] = diag ? sa : sb;
     v4 acc[2][2];
@@ -897,8 +897,8 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
             const int k = wave + 4 * p;
             const TS* col = A + (int64_t)(kc + (k < kw ? k : 0)) * n;
             const int ra = iA + lane, rb = iB + lane;
-            sa[k][lane] = (k < kw && ra < n) ? (TC)col[ra] : TC(0);
-            if (!diag) sb[k][lane] = (k < kw && rb < n) ? (TC)col[rb] : TC(0);
+            sa[k][lane] = (TC)load_or_zero(col, ra, k < kw && ra < n);
+            if (!diag) sb[k][lane] = (TC)load_or_zero(col, rb, k < kw && rb < n);
         }
         __syncthreads();
         if (active) {
@@ -993,8 +993,14 @@ __global__ __launch_bounds__(BT) void sw_fwd_kernel(StepArgs a)
     const TS* L = (const TS*)a.work + a.job_work[job] + (int64_t)k0 * n;
     for (int i = k0 + kb + tid; i < n; i += BT) {
         double acc = 0.0;
-#pragma unroll 16
-        for (int c = 0; c < kb; ++c) acc += (double)L[i + (int64_t)c * n] * zk[c];
+#pragma unroll
+        for (int c0 = 0; c0 < SNB; c0 += 16) {
+            double lv[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lv[c] = (double)load_or_zero(L, i + (int64_t)(c0 + c) * n, c0 + c < kb);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc += lv[c] * zk[c0 + c];
+        }
         r[i] -= acc;
     }
 }
@@ -1026,7 +1032,7 @@ __global__ __launch_bounds__(BT) void sw_bwd_kernel(StepArgs a)
 #pragma unroll
         for (int u = 0; u < CW; ++u) {
             const int c = CW * wave + u;
-            if (c < kb) part[u] += (double)L[i + (int64_t)c * n] * xi;
+            part[u] += (double)load_or_zero(L, i + (int64_t)c * n, c < kb) * xi;
         }
     }
 #pragma unroll

@@ -74,6 +74,17 @@ int cfmi_gemm_impl(cfmi_ctx* c, int dtype, int ta, int tb, int64_t M, int64_t N,
                    double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
                    double beta, void* C, int64_t ldc, int tri);
 
+// p[i] when ok, else zero, from an unconditional load at a clamped address
+// (p[0] must be readable): a conditional load compiles to a branch that
+// waits for its data, so a run of them is serialised, while the
+// unconditional loads of a run are all in flight together.
+template <typename T>
+__device__ __forceinline__ T load_or_zero(const T* p, int64_t i, bool ok)
+{
+    const T v = p[ok ? i : 0];
+    return ok ? v : T(0);
+}
+
 // XCD-grouped tile order of the batched GEMMs (gemm_b3_kernel, and the
 // batched launches of gemm_kernel).  Workgroup x runs on XCD x % 8
 // (round-robin dispatch); its slot x / 8 there picks group 8 (slot / G) +

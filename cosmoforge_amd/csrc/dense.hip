@@ -648,7 +648,7 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
     const int r = threadIdx.x;
     T row[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) row[c] = (r < kb && c <= r) ? A[r + c * lda] : T(0);
+    for (int c = 0; c < NB; ++c) row[c] = load_or_zero((const T*)A, r + c * lda, r < kb && c <= r);
     const int bad = wave_chol64<T>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) atomicCAS(info, 0, (int)(koff + bad));
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int6
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int64_t ri = i0 + 16 * h + lm;
-                av[s][h] = (kk < kb && ri < m) ? P[ri + (int64_t)kk * ldp] : T(0);
+                av[s][h] = load_or_zero((const T*)P, ri + (int64_t)kk * ldp, kk < kb && ri < m);
             }
         }
 #pragma unroll

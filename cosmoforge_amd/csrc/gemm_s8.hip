@@ -99,7 +99,13 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k
     if (r >= rows) return;
     const double* row = src + r * lds;
     double m = 0.0;
-    for (int64_t j = lane; j < k; j += 64) m = absmax(m, row[j]);
+    for (int64_t j0 = 0; j0 < k; j0 += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = load_or_zero(row, j0 + 64 * u + lane, j0 + 64 * u + lane < k);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = absmax(m, v[u]);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = absmax(m, __shfl_xor(m, off, 64));
     m = finish_mu(m);
@@ -107,11 +113,13 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k
     const double inv = 1.0 / m;
     double* b = buf[wave];
     for (int64_t base = 0; base < ldk; base += 1024) {
+        double v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = load_or_zero(row, base + t * 64 + lane, base + t * 64 + lane < k);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const int e = t * 64 + lane;                  // chunk element
-            const int64_t kk = base + e;
-            b[e + (e >> 4)] = kk < k ? row[kk] : 0.0;
+            b[e + (e >> 4)] = v[t];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -146,7 +154,13 @@ __global__ __launch_bounds__(256) void rowmax_cm_kernel(int64_t rows, int64_t k,
     const int64_t k0 = (int64_t)blockIdx.y * kchunk;
     const int64_t k1 = k0 + kchunk < k ? k0 + kchunk : k;
     double m = 0.0;
-    for (int64_t j = k0; j < k1; ++j) m = absmax(m, src[r + j * lds]);
+    for (int64_t j0 = k0; j0 < k1; j0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = load_or_zero(src, r + (j0 + u) * lds, j0 + u < k1);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = absmax(m, v[u]);
+    }
     unsigned long long b;
     memcpy(&b, &m, 8);
     if (b) atomicMax(bits + r, b);
@@ -176,7 +190,7 @@ __global__ __launch_bounds__(256) void slice_cm_kernel(int64_t rows, int64_t k, 
     for (int e = 0; e < 16; ++e) {
         const int idx = tid + 256 * e, rr = idx & 63, kk = idx >> 6;
         const int64_t r = r0 + rr, kg = k0 + kk;
-        tile[kk][rr] = (r < rows && kg < k) ? src[r + kg * lds] : 0.0;
+        tile[kk][rr] = load_or_zero(src, r + kg * lds, r < rows && kg < k);
     }
     __syncthreads();
     const int rr = tid >> 2, c = tid & 3;
@@ -305,7 +319,12 @@ __device__ __forceinline__ void dma_slab(const S8Args& g, const DmaSrc& src, int
 // The copies this wave issued have landed in LDS.
 __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-template <int S>
+// TN = 1: the products are issued with the operands exchanged, so each
+// accumulator holds the transposed 16 x 16 tile (the same exact integers):
+// consecutive lanes then own consecutive rows of C, and a column-major C is
+// read and written in 128-byte runs.  TN = 0 keeps consecutive columns per
+// lane (contiguous in the packed row band).
+template <int S, int TN>
 __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (*Bs)[BT][SK], i4 (&acc)[S][2][4],
                                           int wm, int wn, int lm, int lk)
 {
@@ -329,7 +348,8 @@ __device__ __forceinline__ void slab_mfma(const char (*As)[BT][SK], const char (
             for (int d = p; d < S; ++d)
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
-                    acc[d][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][p], b[d - p], acc[d][i][j], 0, 0, 0);
+                    acc[d][i][j] = TN ? __builtin_amdgcn_mfma_i32_16x16x64_i8(b[d - p], a[i][p], acc[d][i][j], 0, 0, 0)
+                                      : __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][p], b[d - p], acc[d][i][j], 0, 0, 0);
     }
 }
 
@@ -349,7 +369,7 @@ __device__ __forceinline__ int64_t tile_kend(const S8Args& g, int64_t m0, int64_
 // slab: the copy of slab k + 1 into one stage is in flight while slab k is
 // multiplied out of the other.  DB = 0: one stage, staged through registers
 // and stored between two barriers.
-template <int S, int DB>
+template <int S, int DB, int TN>
 __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
 {
     __shared__ __attribute__((aligned(16))) char As[S][BT][SK];
@@ -389,7 +409,7 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
                 if (odd) dma_slab<S>(g, src, k0 + SK, As, Bs, uw);
                 else dma_slab<S>(g, src, k0 + SK, As1, Bs1, uw);
             }
-            slab_mfma<S>(odd ? As1 : As, odd ? Bs1 : Bs, acc, wm, wn, lm, lk);
+            slab_mfma<S, TN>(odd ? As1 : As, odd ? Bs1 : Bs, acc, wm, wn, lm, lk);
             if (!more) break;
             dma_wait();
             __syncthreads();
@@ -403,14 +423,59 @@ __global__ __launch_bounds__(NTH, 1) void gemm_s8_kernel(S8Args g)
             st.store(As, Bs, tid);
             __syncthreads();
             st.load(g, m0, n0, k0 + SK, tid);
-            slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
+            slab_mfma<S, TN>(As, Bs, acc, wm, wn, lm, lk);
         }
         __syncthreads();
         st.store(As, Bs, tid);
         __syncthreads();
-        slab_mfma<S>(As, Bs, acc, wm, wn, lm, lk);
+        slab_mfma<S, TN>(As, Bs, acc, wm, wn, lm, lk);
     }
 
+    if constexpr (TN) {
+        // lane (lm, lk), register r: row m0 + wm + 16 i + lm, column n0 + wn + 16 j + 4 lk + r.
+        // Per column block j the eight stored values of C are fetched together
+        // (unconditional loads at clamped addresses), then combined and stored.
+        double sa[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t gm = m0 + wm + 16 * i + lm;
+            const double mu = g.muA[gm < g.M ? gm : g.M - 1];
+            sa[i] = gm < g.M ? g.alpha * mu : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double sb[4], cold[2][4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gn = n0 + wn + 16 * j + 4 * lk + r;
+                const double mu = g.muB[gn < g.N ? gn : g.N - 1];
+                sb[r] = gn < g.N ? mu : 0.0;
+            }
+            if (g.beta != 0.0) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t gm = m0 + wm + 16 * i + lm, gn = n0 + wn + 16 * j + 4 * lk + r;
+                        const bool ok = gm < g.M && gn < g.N && !(g.lower && gn > gm);
+                        cold[i][r] = g.C[ok ? gm + gn * g.ldc : 0];
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t gm = m0 + wm + 16 * i + lm, gn = n0 + wn + 16 * j + 4 * lk + r;
+                    if (gm >= g.M || gn >= g.N || (g.lower && gn > gm)) continue;
+                    double v = (double)acc[S - 1][i][j][r];
+#pragma unroll
+                    for (int d = S - 2; d >= 0; --d) v = (double)acc[d][i][j][r] + v * kInv254;
+                    v = (v * kInv127sq) * (sa[i] * sb[r]);
+                    g.C[gm + gn * g.ldc] = g.beta == 0.0 ? v : v + g.beta * cold[i][r];
+                }
+        }
+        return;
+    }
     double sa[2][4], sb[4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -461,7 +526,7 @@ __global__ __launch_bounds__(256) void mirror_lower_s8_kernel(int64_t n, double*
     for (int e = 0; e < 16; ++e) {
         const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
         const int64_t r = bi * 64 + rr, c = bj * 64 + cc;
-        t[cc][rr] = (r < n && c < n) ? C[r + c * ldc] : 0.0;
+        t[cc][rr] = load_or_zero((const double*)C, r + c * ldc, r < n && c < n);
     }
     __syncthreads();
 #pragma unroll
@@ -485,7 +550,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(int64_t m, int64_t n, co
     for (int e = 0; e < 16; ++e) {
         const int idx = tid + 256 * e, rr = idx & 63, cc = idx >> 6;
         const int64_t r = bi * 64 + rr, c = bj * 64 + cc;
-        t[cc][rr] = (r < m && c < n) ? src[r + c * lds] : 0.0;
+        t[cc][rr] = load_or_zero(src, r + c * lds, r < m && c < n);
     }
     __syncthreads();
 #pragma unroll
@@ -563,10 +628,15 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
         if (!e0 || !e1) return cfmi_fail(c, CFMI_EHIP, "gemm_s8: hipEventCreate failed");
         CFMI_HIP(c, hipEventRecord(e0, c->stream));
     }
-    if (db)
-        hipLaunchKernelGGL((gemm_s8_kernel<S, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    // the transposed accumulator layout for a column-major C, the plain one for the packed band
+    if (db && !g.packed)
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 1, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    else if (db)
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 1, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+    else if (!g.packed)
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 0, 1>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     else
-        hipLaunchKernelGGL((gemm_s8_kernel<S, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
+        hipLaunchKernelGGL((gemm_s8_kernel<S, 0, 0>), dim3((unsigned)nwg), dim3(NTH), 0, c->stream, g);
     CFMI_LAUNCH_CHECK(c);
     if (c->s8_stats) {
         CFMI_HIP(c, hipEventRecord(e1, c->stream));
