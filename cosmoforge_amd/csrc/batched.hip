@@ -731,9 +731,10 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
         if (r == 0) a.info[job] = k0 + bad;
         return;
     }
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < SNB; ++c)
-        if (r < kb && c <= r) A[r + (int64_t)c * n] = (TS)row[c];
+        if (r < kb && c <= r) A[r + (int64_t)c * n] = (TS)lt[c][r];   // L[r][c]
     double ld = r < kb ? 2.0 * log(lt[r][r]) : 0.0;    // lt[q][i] = L[i][q]
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ld += __shfl_xor(ld, off, 64);

@@ -28,9 +28,11 @@ __device__ __forceinline__ T lane_value(T v, int lane)
 }
 
 // row: lane r's row of the kb x kb block, zero above the diagonal and beyond
-// kb.  On return row holds L's row r, lt[k][i] = L[i][k] (zero for k >= kb),
-// rdiag[k] = 1 / L_kk (zero for k >= kb).  Returns 0, or the 1-based index of
-// the first non-positive pivot (row, lt and rdiag are then incomplete).
+// kb (consumed: entry q is dead after step q, so the registers free up as
+// the steps go).  On return lt[k][i] = L[i][k] (zero for k >= kb and above
+// the diagonal) and rdiag[k] = 1 / L_kk (zero for k >= kb).  Returns 0, or
+// the 1-based index of the first non-positive pivot (lt and rdiag are then
+// incomplete).
 template <typename T>
 __device__ __forceinline__ int wave_chol64(T (&row)[64], int kb, T (*lt)[65], T* rdiag)
 {
@@ -46,7 +48,6 @@ __device__ __forceinline__ int wave_chol64(T (&row)[64], int kb, T (*lt)[65], T*
                 const T piv = sqrt(d);
                 const T rp = T(1) / piv;
                 const T l = r > q ? row[q] * rp : (r == q ? piv : T(0));
-                row[q] = l;
                 lt[q][r] = l;
                 if (r == 0) rdiag[q] = rp;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -65,6 +66,9 @@ __device__ __forceinline__ int wave_chol64(T (&row)[64], int kb, T (*lt)[65], T*
 
 // From wave_chol64's lt and rdiag: U = L^-1, left in LDS as lt[c][i] = U[i][c]
 // (so that lane i can then read row i of U, lt[.][i], for coalesced stores).
+// u_k is final at step k and goes to lt[c][k] at once: step k reads row k
+// of lt beyond column k only, and row c is read at step c only beyond
+// column c, so no later step reads an entry written this way.
 template <typename T>
 __device__ __forceinline__ void wave_inv64_to_lds(T (*lt)[65], const T* rdiag)
 {
@@ -78,9 +82,7 @@ __device__ __forceinline__ void wave_inv64_to_lds(T (*lt)[65], const T* rdiag)
         u[k] *= rdiag[k];
 #pragma unroll
         for (int i = k + 1; i < 64; ++i) u[i] -= lt[k][i] * u[k];
+        lt[c][k] = u[k];
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 64; ++i) lt[c][i] = u[i];
     __syncthreads();
 }

@@ -654,9 +654,10 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
         if (r == 0) atomicCAS(info, 0, (int)(koff + bad));
         return;
     }
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-        if (r < kb && c < kb) A[r + c * lda] = c <= r ? row[c] : T(0);
+        if (r < kb && c < kb) A[r + c * lda] = lt[c][r];   // L[r][c], zero above the diagonal
     wave_inv64_to_lds<T>(lt, rdiag);
     // lane i stores row i of U: U[i][c] = lt[c][i], consecutive lanes at consecutive addresses
     const int i = r;

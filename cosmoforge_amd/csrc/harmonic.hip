@@ -323,6 +323,7 @@ __global__ __launch_bounds__(RT) void harm_fisher_kernel(const double* __restric
 // harm_fisher_kernel (9 against 2 x 81 for T, E, B).  Fixed-order sums.
 constexpr int HFG = 3;    // mode groups (T, E, B)
 constexpr int HFS = 6;    // spectra
+template <int G>
 __global__ __launch_bounds__(RT) void harm_fisher_bins_kernel(const double* __restrict__ M,
                                                               const double* __restrict__ M2, int64_t ldm,
                                                               HarmTables t, int nspec, int npar,
@@ -339,9 +340,6 @@ __global__ __launch_bounds__(RT) void harm_fisher_bins_kernel(const double* __re
     while ((int64_t)b1 * (b1 + 1) / 2 > idx) --b1;
     const int b2 = (int)(idx - (int64_t)b1 * (b1 + 1) / 2);
     const int tid = threadIdx.x;
-    int G = 1;
-    for (int s = 0; s < nspec; ++s)
-        for (int e = 0; e < 4; ++e) G = max(G, t.pairs[4 * s + e] + 1);   // <= HFG (checked by the host)
     if (tid < nspec * nspec) {
         const int s1 = tid / nspec, s2 = tid % nspec;
         const int* P1 = t.pairs + 4 * s1;
@@ -367,11 +365,22 @@ __global__ __launch_bounds__(RT) void harm_fisher_bins_kernel(const double* __re
     for (int64_t q = tid; q < K1 * K2; q += RT) {
         const int64_t k1 = k1a + q % K1, k2 = k2a + q / K1;
         const int l1 = ell_of(k1, t.lmin), l2 = ell_of(k2, t.lmin);
+        // the G^2 entries of M and M2 for this mode pair, all loads in flight together
+        double v1[G][G], v2[G][G];
+#pragma unroll
         for (int g1 = 0; g1 < G; ++g1)
+#pragma unroll
             for (int g2 = 0; g2 < G; ++g2) {
                 const int64_t o = (t.goff[g1] + k1) + (t.goff[g2] + k2) * ldm;
-                mv[g1 * HFG + g2][tid] = M[o];
-                mv2[g1 * HFG + g2][tid] = M2[o];
+                v1[g1][g2] = M[o];
+                v2[g1][g2] = M2[o];
+            }
+#pragma unroll
+        for (int g1 = 0; g1 < G; ++g1)
+#pragma unroll
+            for (int g2 = 0; g2 < G; ++g2) {
+                mv[g1 * HFG + g2][tid] = v1[g1][g2];
+                mv2[g1 * HFG + g2][tid] = v2[g1][g2];
             }
 #pragma unroll
         for (int s1 = 0; s1 < HFS; ++s1) {
@@ -515,8 +524,16 @@ int cfmi_harmonic_fisher(cfmi_ctx* c, const double* M, const double* M2, int64_t
         for (int v : ph) G = std::max(G, v + 1);
     }
     if (nspec <= HFS && G <= HFG) {
-        hipLaunchKernelGGL(harm_fisher_bins_kernel, dim3((unsigned)((int64_t)nbins * (nbins + 1) / 2)), dim3(RT), 0,
-                           c->stream, M, M2 ? M2 : M, ldm, t, nspec, (int)npar, F);
+        const dim3 grid((unsigned)((int64_t)nbins * (nbins + 1) / 2));
+        if (G == 3)
+            hipLaunchKernelGGL(harm_fisher_bins_kernel<3>, grid, dim3(RT), 0, c->stream, M, M2 ? M2 : M, ldm, t, nspec,
+                               (int)npar, F);
+        else if (G == 2)
+            hipLaunchKernelGGL(harm_fisher_bins_kernel<2>, grid, dim3(RT), 0, c->stream, M, M2 ? M2 : M, ldm, t, nspec,
+                               (int)npar, F);
+        else
+            hipLaunchKernelGGL(harm_fisher_bins_kernel<1>, grid, dim3(RT), 0, c->stream, M, M2 ? M2 : M, ldm, t, nspec,
+                               (int)npar, F);
         CFMI_LAUNCH_CHECK(c);
         return CFMI_OK;
     }

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(MG g)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int gk = k0 + kc + u;
-                As[r][kc + u] = (gm < g.M && gk < kend) ? A[(int64_t)gm * g.K + gk] : (bf16)0;
+                As[r][kc + u] = load_or_zero(A, (int64_t)gm * g.K + gk, gm < g.M && gk < kend);
             }
         } else {
             const int kr = tid >> 3, mc = (tid & 7) * 8;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(MG g)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int gm = m0 + mc + u;
-                As[mc + u][kr] = (gm < g.M && gk < kend) ? A[(int64_t)gk * g.M + gm] : (bf16)0;
+                As[mc + u][kr] = load_or_zero(A, (int64_t)gk * g.M + gm, gm < g.M && gk < kend);
             }
         }
         // B tile -> Bs[n][k]
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(MG g)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int gk = k0 + kc + u;
-                Bs[r][kc + u] = (gn < g.N && gk < kend) ? B[(int64_t)gn * g.K + gk] : (bf16)0;
+                Bs[r][kc + u] = load_or_zero(B, (int64_t)gn * g.K + gk, gn < g.N && gk < kend);
             }
         } else {
             const int kr = tid >> 3, nc = (tid & 7) * 8;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(MG g)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int gn = n0 + nc + u;
-                Bs[nc + u][kr] = (gn < g.N && gk < kend) ? B[(int64_t)gk * g.N + gn] : (bf16)0;
+                Bs[nc + u][kr] = load_or_zero(B, (int64_t)gk * g.N + gn, gn < g.N && gk < kend);
             }
         }
         __syncthreads();
