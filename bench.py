@@ -48,7 +48,7 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 # launches), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one
 # solve, each scaled by what the counter reads for a plain 1 GiB device copy
 # (FETCH x 2.0, WRITE x 1.0); not measured live.  Keyed by config.
-S8_TRAFFIC_BYTES_PER_LAUNCH = {2: 11.74e9}
+S8_TRAFFIC_BYTES_PER_LAUNCH = {2: 10.84e9}
 
 
 def log(msg):
