@@ -91,14 +91,16 @@ MODES = {"mixed": 1, "mixed_f64_storage": 2, "fp64": 3}
 
 
 class BatchedLikelihood:
-    """chi2 and ln|C| for jobs (patch j_p, coefficients c_j) in one launch.
+    """chi2 and ln|C| for jobs (patch j_p, coefficients c_j) in one captured
+    sequence: ``nb`` = 64 runs the step-synchronous launches (every job's
+    step k at once), 16 or 32 one persistent workgroup per matrix.
     ``precision``: "mixed" (factor stored in FP32, FP32 MFMA trailing
     updates; FP64 diagonal blocks, panels, ln|C| and refinement),
     "mixed_f64_storage" (factor stored in FP64, trailing products in FP32
     MFMA) or "fp64"."""
 
     def __init__(self, ctx: engine.Context, patches: list, job_patch: np.ndarray, precision: str = "mixed",
-                 nb: int = 32, wg_per_cu: int = 2, refine_iters: int = 1):
+                 nb: int = 64, wg_per_cu: int = 2, refine_iters: int = 1):
         if precision not in MODES:
             raise ValueError(f"precision must be one of {list(MODES)}; got {precision!r}")
         self.ctx = ctx
@@ -275,7 +277,7 @@ class JointLikelihood:
     rows; evaluated with ``BatchedLikelihood`` in chunks of at most
     ``max_jobs`` (patch, point) jobs, one plan per chunk size."""
 
-    def __init__(self, ctx: engine.Context, patches: list, precision: str = "mixed", nb: int = 32,
+    def __init__(self, ctx: engine.Context, patches: list, precision: str = "mixed", nb: int = 64,
                  wg_per_cu: int = 4, max_jobs: int = 16384):
         self.ctx, self.patches, self.precision = ctx, patches, precision
         self.nb, self.wg_per_cu = nb, wg_per_cu
