@@ -725,7 +725,15 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
     const int r = threadIdx.x;
     double row[SNB];
 #pragma unroll
-    for (int c = 0; c < SNB; ++c) row[c] = (double)load_or_zero((const TS*)A, r + (int64_t)c * n, r < kb && c <= r);
+    for (int c = 0; c < SNB; ++c) row[c] = (double)A[min(r, kb - 1) + (int64_t)min(c, kb - 1) * n];
+    // Consume the row before masking it: a load whose address or result
+    // select waits on a per-column condition is issued one at a time, and
+    // the compiler would sink each load into the factor step that first
+    // reads it.
+#pragma unroll
+    for (int c = 0; c < SNB; ++c) asm volatile("" ::"v"(row[c]));
+#pragma unroll
+    for (int c = 0; c < SNB; ++c) row[c] = (r < kb && c <= r) ? row[c] : 0.0;
     const int bad = wave_chol64<double>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) a.info[job] = k0 + bad;
@@ -786,7 +794,7 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
     const int k0 = a.k0;
     const int kb = min(SNB, n - k0);
     const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
-    for (int idx = threadIdx.x; idx < SNB * SNB; idx += 256) si[idx % SNB][idx / SNB] = li[idx];
+    block64_to_lds<256>(si, li, (int)threadIdx.x);
     __syncthreads();
     TS* A = (TS*)a.work + a.job_work[job];
     constexpr int KS = SNB / 4, NT = SNB / 16;
@@ -889,19 +897,32 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
-    for (int kc = kbeg; kc < kend; kc += SNB) {
+    // operand blocks, thread (row = lane, k = wave + 4 p): all 32 loads of a
+    // chunk issued together (a diagonal tile loads its block twice rather
+    // than branch), the next chunk's during this chunk's products
+    TS va[SNB / 4], vb[SNB / 4];
+    auto fetch = [&](int kc) {
         const int kw = min(SNB, kend - kc);
-        if (kc != kbeg) __syncthreads();
-        // operand blocks: thread (row = lane, k = wave + 4 p)
 #pragma unroll
         for (int p = 0; p < SNB / 4; ++p) {
             const int k = wave + 4 * p;
             const TS* col = A + (int64_t)(kc + (k < kw ? k : 0)) * n;
             const int ra = iA + lane, rb = iB + lane;
-            sa[k][lane] = (TC)load_or_zero(col, ra, k < kw && ra < n);
-            if (!diag) sb[k][lane] = (TC)load_or_zero(col, rb, k < kw && rb < n);
+            va[p] = load_or_zero(col, ra, k < kw && ra < n);
+            vb[p] = load_or_zero(col, rb, k < kw && rb < n);
+        }
+    };
+    fetch(kbeg);
+    for (int kc = kbeg; kc < kend; kc += SNB) {
+        if (kc != kbeg) __syncthreads();
+#pragma unroll
+        for (int p = 0; p < SNB / 4; ++p) {
+            const int k = wave + 4 * p;
+            sa[k][lane] = (TC)va[p];
+            sb[k][lane] = (TC)vb[p];
         }
         __syncthreads();
+        if (kc + SNB < kend) fetch(kc + SNB);
         if (active) {
 #pragma unroll
             for (int s = 0; s < SNB / 4; ++s) {
@@ -987,7 +1008,7 @@ __global__ __launch_bounds__(BT) void sw_fwd_kernel(StepArgs a)
         double z = 0.0;
 #pragma unroll 16
         for (int c = 0; c < SNB; ++c) z += li[tid + c * SNB] * rk[c];
-        zk[tid] = z;
+        zk[tid] = tid < kb ? z : 0.0;   // zero past kb: the clamped column loads below add nothing
         if (tid < kb) r[k0 + tid] = z;
     }
     __syncthreads();
@@ -996,11 +1017,15 @@ __global__ __launch_bounds__(BT) void sw_fwd_kernel(StepArgs a)
         double acc = 0.0;
 #pragma unroll
         for (int c0 = 0; c0 < SNB; c0 += 16) {
-            double lv[16];
+            TS lv[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) lv[c] = (double)load_or_zero(L, i + (int64_t)(c0 + c) * n, c0 + c < kb);
+            for (int c = 0; c < 16; ++c) lv[c] = L[i + (int64_t)min(c0 + c, kb - 1) * n];
+            // all 16 in flight before the first use (the chain of sums
+            // would otherwise take them one at a time)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) acc += lv[c] * zk[c0 + c];
+            for (int c = 0; c < 16; ++c) asm volatile("" ::"v"(lv[c]));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc += (double)lv[c] * zk[c0 + c];
         }
         r[i] -= acc;
     }
@@ -1019,7 +1044,7 @@ __global__ __launch_bounds__(BT) void sw_bwd_kernel(StepArgs a)
     double* r = a.rv + a.job_vec[job];
     const double* li = a.linv + a.job_inv[job] + (int64_t)(k0 / SNB) * SNB * SNB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int idx = tid; idx < SNB * SNB; idx += BT) sl[idx % SNB][idx / SNB] = li[idx];
+    block64_to_lds<BT>(sl, li, tid);
     // t_c = sum_{i >= k0 + kb} L[i][k0 + c] x_i: wave w takes columns
     // 16 w .. 16 w + 15 together (16 independent sums per lane over the rows
     // lane, lane + 64, ...), then one fixed-order wave reduction per column
@@ -1033,7 +1058,7 @@ __global__ __launch_bounds__(BT) void sw_bwd_kernel(StepArgs a)
 #pragma unroll
         for (int u = 0; u < CW; ++u) {
             const int c = CW * wave + u;
-            part[u] += (double)load_or_zero(L, i + (int64_t)c * n, c < kb) * xi;
+            part[u] += (double)L[i + (int64_t)min(c, kb - 1) * n] * xi;   // columns past kb are never read back
         }
     }
 #pragma unroll

@@ -27,6 +27,25 @@ __device__ __forceinline__ T lane_value(T v, int lane)
     }
 }
 
+// A column-major 64 x 64 block (src, 4096 contiguous elements) into LDS as
+// dst[i][j] = block[i][j], by NT threads: each thread issues its 4096 / NT
+// loads together before its LDS stores (a load-store loop waits for every
+// load in turn).
+template <int NT, typename T, int LD>
+__device__ __forceinline__ void block64_to_lds(T (*dst)[LD], const T* __restrict__ src, int tid)
+{
+    constexpr int PER = 4096 / NT;
+    static_assert(PER * NT == 4096 && LD >= 64, "block64_to_lds: thread count must divide 4096");
+    T v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) v[e] = src[tid + e * NT];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = tid + e * NT;
+        dst[idx & 63][idx >> 6] = v[e];
+    }
+}
+
 // row: lane r's row of the kb x kb block, zero above the diagonal and beyond
 // kb (consumed: entry q is dead after step q, so the registers free up as
 // the steps go).  On return lt[k][i] = L[i][k] (zero for k >= kb and above

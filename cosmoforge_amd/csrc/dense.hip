@@ -91,20 +91,29 @@ __device__ __forceinline__ int64_t packed_index(int64_t i, int64_t j, int64_t n)
     return j * (2 * n - j + 1) / 2 + (i - j);
 }
 
+// Offset of output element (gm, gn) in g.C and the scale its value takes.
+template <typename T, typename TO, typename TE>
+__device__ __forceinline__ int64_t out_offset(const GemmArgs<T, TO, TE>& g, int64_t gm, int64_t gn, TE& scale)
+{
+    const int64_t i = g.o.row0 + gm, j = g.o.col0 + gn;
+    scale = TE(1);
+    if (g.o.mode == 1) {
+        scale = (TE)(i == j ? g.o.pdiag : g.o.poff);
+        return packed_index(i, j, g.o.pn);
+    }
+    if (g.o.mode == 2) {
+        scale = (TE)(i == j ? g.o.pdiag : g.o.poff);
+        return i * (i + 1) / 2 + j - g.o.pbase;
+    }
+    return gm + gn * g.ldc;
+}
+
 template <typename T, typename TO, typename TE>
 __device__ __forceinline__ void store_out(const GemmArgs<T, TO, TE>& g, int64_t gm, int64_t gn, TE v)
 {
-    const int64_t i = g.o.row0 + gm, j = g.o.col0 + gn;
-    TO* p;
-    if (g.o.mode == 1) {
-        v *= (TE)(i == j ? g.o.pdiag : g.o.poff);
-        p = g.C + packed_index(i, j, g.o.pn);
-    } else if (g.o.mode == 2) {
-        v *= (TE)(i == j ? g.o.pdiag : g.o.poff);
-        p = g.C + (i * (i + 1) / 2 + j - g.o.pbase);
-    } else {
-        p = g.C + gm + gn * g.ldc;
-    }
+    TE scale;
+    TO* p = g.C + out_offset(g, gm, gn, scale);
+    v *= scale;
     *p = g.beta == 0.0 ? (TO)v : (TO)(v + (TE)g.beta * (TE)*p);
 }
 
@@ -353,25 +362,54 @@ __global__ __launch_bounds__(NT, CFMI_GEMM_WAVES) void gemm_kernel(GemmArgs<T, T
 
     // Epilogue: lane holds column lm of each 16 x 16 tile; its four values
     // are rows lk + 4r (f64 16x16x4) or 4 lk + r (f32 16x16x4), as measured
-    // on gfx950 with an explicit layout probe.
+    // on gfx950 with an explicit layout probe.  One 16-row group of tiles at
+    // a time: the C values that beta needs are fetched together (from a
+    // valid address for elements that are not stored), then the stores.
     constexpr bool kF64 = sizeof(T) == 8;
+    const bool read_c = g.part == nullptr && g.beta != 0.0;
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+        TE old[FN][4];
+        if (read_c) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t gm = m0 + wm + i * 16 + (kF64 ? lk + 4 * r : lk * 4 + r);
+                    const int64_t gn = n0 + wn + j * 16 + lm;
+                    const bool ok = gm < M && gn < N && !(lower_only && g.o.col0 + gn > g.o.row0 + gm);
+                    TE sc;
+                    old[j][r] = (TE)g.C[ok ? out_offset(g, gm, gn, sc) : 0];
+                }
+            // Consume the values here, so that the compiler cannot sink each
+            // load into its element's store branch (one wait per element).
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(old[j][r]));
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int64_t gm = m0 + wm + i * 16 + (kF64 ? lk + 4 * r : lk * 4 + r);
-                int64_t gn = n0 + wn + j * 16 + lm;
+                const int64_t gm = m0 + wm + i * 16 + (kF64 ? lk + 4 * r : lk * 4 + r);
+                const int64_t gn = n0 + wn + j * 16 + lm;
                 if (gm >= M || gn >= N) continue;
                 if (lower_only && g.o.col0 + gn > g.o.row0 + gm) continue;
                 TE v;
                 if (ACC64) v = acc64[ACC64 ? i : 0][ACC64 ? j : 0][r];
                 else v = (TE)acc[i][j][r];
                 v *= (TE)g.alpha;
-                if (g.part) g.part[(int64_t)z * M * N + gm + gn * M] = v;
-                else store_out(g, gm, gn, v);
+                if (g.part) {
+                    g.part[(int64_t)z * M * N + gm + gn * M] = v;
+                } else {
+                    TE sc;
+                    const int64_t off = out_offset(g, gm, gn, sc);
+                    v *= sc;
+                    g.C[off] = read_c ? (TO)(v + (TE)g.beta * old[j][r]) : (TO)v;
+                }
             }
+    }
 }
 
 template <typename T, typename TO, typename TE>
@@ -699,7 +737,7 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int6
     __shared__ T sl[NB][NB + 1];
     if (*info != 0) return;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < NB * NB; idx += 256) sl[idx & (NB - 1)][idx >> 6] = linv[idx];   // sl[i][j] = Linv[i][j]
+    block64_to_lds<256>(sl, linv, tid);   // sl[i][j] = Linv[i][j]
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     const int lm = lane & 15, lk = lane >> 4;
