@@ -723,17 +723,18 @@ __global__ __launch_bounds__(64) void sw_diag_kernel(StepArgs a)
     const int kb = min(SNB, n - k0);
     TS* A = (TS*)a.work + a.job_work[job] + k0 + (int64_t)k0 * n;
     const int r = threadIdx.x;
+    TS raw[SNB];
+#pragma unroll
+    for (int c = 0; c < SNB; ++c) raw[c] = A[min(r, kb - 1) + (int64_t)min(c, kb - 1) * n];
+    // Consume the stored values before converting and masking them: a load
+    // whose address or result select waits on a per-column condition is
+    // issued one at a time, and the compiler would otherwise sink each load
+    // into the factor step that first reads it.
+#pragma unroll
+    for (int c = 0; c < SNB; ++c) asm volatile("" ::"v"(raw[c]));
     double row[SNB];
 #pragma unroll
-    for (int c = 0; c < SNB; ++c) row[c] = (double)A[min(r, kb - 1) + (int64_t)min(c, kb - 1) * n];
-    // Consume the row before masking it: a load whose address or result
-    // select waits on a per-column condition is issued one at a time, and
-    // the compiler would sink each load into the factor step that first
-    // reads it.
-#pragma unroll
-    for (int c = 0; c < SNB; ++c) asm volatile("" ::"v"(row[c]));
-#pragma unroll
-    for (int c = 0; c < SNB; ++c) row[c] = (r < kb && c <= r) ? row[c] : 0.0;
+    for (int c = 0; c < SNB; ++c) row[c] = (r < kb && c <= r) ? (double)raw[c] : 0.0;
     const int bad = wave_chol64<double>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) a.info[job] = k0 + bad;
@@ -899,7 +900,8 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
         for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
     // operand blocks, thread (row = lane, k = wave + 4 p): all 32 loads of a
     // chunk issued together (a diagonal tile loads its block twice rather
-    // than branch), the next chunk's during this chunk's products
+    // than branch); loading the next chunk during this chunk's products
+    // measured no faster
     TS va[SNB / 4], vb[SNB / 4];
     auto fetch = [&](int kc) {
         const int kw = min(SNB, kend - kc);
@@ -912,8 +914,8 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
             vb[p] = load_or_zero(col, rb, k < kw && rb < n);
         }
     };
-    fetch(kbeg);
     for (int kc = kbeg; kc < kend; kc += SNB) {
+        fetch(kc);
         if (kc != kbeg) __syncthreads();
 #pragma unroll
         for (int p = 0; p < SNB / 4; ++p) {
@@ -922,7 +924,6 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
             sb[k][lane] = (TC)vb[p];
         }
         __syncthreads();
-        if (kc + SNB < kend) fetch(kc + SNB);
         if (active) {
 #pragma unroll
             for (int s = 0; s < SNB / 4; ++s) {

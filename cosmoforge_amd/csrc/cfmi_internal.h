@@ -42,11 +42,12 @@ struct cfmi_ctx {
 int cfmi_fail(cfmi_ctx* c, int code, const char* fmt, ...);
 void* cfmi_workspace(cfmi_ctx* c, size_t bytes);
 
-// C (lower triangle only, FP64, column-major) += alpha P P^T, P n x k
-// column-major: P's rows as s int8 slices (in the context workspace) on
-// the int8 matrix cores, exact integer sums (gemm_s8.hip).
-int cfmi_s8_syrk_lower(cfmi_ctx* c, int s, int64_t n, int64_t k, double alpha, const double* P, int64_t ldp,
-                       double* C, int64_t ldc);
+// C (m x w, FP64, column-major; only the entries on or below the diagonal)
+// += alpha P P[0:w]^T, P m x k column-major (w <= m): P's rows as s int8
+// slices (in the context workspace) on the int8 matrix cores, exact integer
+// sums (gemm_s8.hip).  w = m is the lower triangle of P P^T.
+int cfmi_s8_update_lower(cfmi_ctx* c, int s, int64_t m, int64_t w, int64_t k, double alpha, const double* P,
+                         int64_t ldp, double* C, int64_t ldc);
 
 #define CFMI_HIP(ctx, expr)                                                              \
     do {                                                                                 \

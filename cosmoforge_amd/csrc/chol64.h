@@ -50,44 +50,64 @@ __device__ __forceinline__ void block64_to_lds(T (*dst)[LD], const T* __restrict
 // kb (consumed: entry q is dead after step q, so the registers free up as
 // the steps go).  On return lt[k][i] = L[i][k] (zero for k >= kb and above
 // the diagonal) and rdiag[k] = 1 / L_kk (zero for k >= kb).  Returns 0, or
-// the 1-based index of the first non-positive pivot (lt and rdiag are then
-// incomplete).
+// the 1-based index of the first non-positive pivot (lt and rdiag then hold
+// no factor).
+//
+// The steps have no branches (a branch per step costs register copies at
+// every join, and keeps a step's updates from overlapping the next pivot's
+// square root and reciprocal): a non-positive pivot is recorded and replaced
+// by 1.  Past kb the rows are zero, so those steps publish the identity's
+// columns, cleared at the end, and record their zero pivots past kb, which
+// are not failures.  (A test of q against kb inside the steps would be
+// computed for all 64 steps up front and kept in scalar registers.)
 template <typename T>
 __device__ __forceinline__ int wave_chol64(T (&row)[64], int kb, T (*lt)[65], T* rdiag)
 {
     const int r = threadIdx.x & 63;
     int bad = 0;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int q = 0; q < 64; ++q) {
-        if (q < kb && !bad) {
-            const T d = lane_value(row[q], q);
-            if (!(d > T(0))) {
-                bad = q + 1;
-            } else {
-                const T piv = sqrt(d);
-                const T rp = T(1) / piv;
-                const T l = r > q ? row[q] * rp : (r == q ? piv : T(0));
-                lt[q][r] = l;
-                if (r == 0) rdiag[q] = rp;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const T d = lane_value(row[q], q);
+        const bool pos = d > T(0);
+        bad = (bad == 0 && !pos) ? q + 1 : bad;
+        const T piv = sqrt(pos ? d : T(1));
+        const T rp = T(1) / piv;
+        const T l = r > q ? row[q] * rp : (r == q ? piv : T(0));
+        lt[q][r] = l;
+        rdiag[q] = rp;   // the same value from every lane
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the column in groups of 16 reads, each group consumed (waited for)
+        // as a whole before its updates: left alone, the compiler waits for
+        // every pair of reads in turn
+#pragma clang loop unroll(full)
+        for (int g = 0; g < 4; ++g) {
+            if (16 * g + 15 <= q) continue;
+            T v[16];
 #pragma unroll
-                for (int p = q + 1; p < 64; ++p) row[p] -= l * lt[q][p];
-            }
-        } else if (q >= kb) {
-            lt[q][r] = T(0);
-            if (r == 0) rdiag[q] = T(0);
+            for (int j = 0; j < 16; ++j) v[j] = lt[q][16 * g + j];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) asm volatile("" ::"v"(v[j]));
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (16 * g + j > q) row[16 * g + j] -= l * v[j];
         }
     }
-    return bad;
+    if (r >= kb) {
+        lt[r][r] = T(0);
+        rdiag[r] = T(0);
+    }
+    return bad > kb ? 0 : bad;
 }
 
 // From wave_chol64's lt and rdiag: U = L^-1, left in LDS as lt[c][i] = U[i][c]
 // (so that lane i can then read row i of U, lt[.][i], for coalesced stores).
-// u_k is final at step k and goes to lt[c][k] at once: step k reads row k
-// of lt beyond column k only, and row c is read at step c only beyond
-// column c, so no later step reads an entry written this way.
+// u_k is final at step k and goes to lt[c][k] at once: the values that
+// step k uses from row k of lt lie beyond column k, and those that step c
+// uses from row c beyond column c, so no step uses an entry written this
+// way (lane k's own write to lt[k][k] is read but not used).  (Kept in
+// registers until the end instead, the 64 columns measured slower.)
 template <typename T>
 __device__ __forceinline__ void wave_inv64_to_lds(T (*lt)[65], const T* rdiag)
 {
@@ -96,12 +116,23 @@ __device__ __forceinline__ void wave_inv64_to_lds(T (*lt)[65], const T* rdiag)
     T u[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) u[i] = (i == c) ? T(1) : T(0);
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 0; k < 64; ++k) {
         u[k] *= rdiag[k];
-#pragma unroll
-        for (int i = k + 1; i < 64; ++i) u[i] -= lt[k][i] * u[k];
         lt[c][k] = u[k];
+        // L's column k in groups of 16 reads, each consumed as a whole
+#pragma clang loop unroll(full)
+        for (int g = 0; g < 4; ++g) {
+            if (16 * g + 15 <= k) continue;
+            T v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = lt[k][16 * g + j];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) asm volatile("" ::"v"(v[j]));
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (16 * g + j > k) u[16 * g + j] -= v[j] * u[k];
+        }
     }
     __syncthreads();
 }

@@ -205,6 +205,9 @@ struct Slab {
         }
     }
 
+    // Edge slabs: every load from a clamped (valid) index, issued together,
+    // then the out-of-range entries zeroed (a load under its own condition
+    // is issued and waited for one at a time).  kend > k0, M, N >= 1.
     __device__ __forceinline__ void load_guarded(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                  int64_t ldb, int64_t m0, int64_t n0, int64_t k0, int64_t M,
                                                  int64_t N, int64_t kend, int tid)
@@ -215,18 +218,36 @@ struct Slab {
             amap(idx, m, k);
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                int64_t gm = m0 + m + (TA == 0 ? e : 0), gk = k0 + k + (TA == 0 ? 0 : e);
-                T v = 0;
-                if (gm < M && gk < kend) v = TA == 0 ? A[gm + gk * lda] : A[gk + gm * lda];
-                ra[2 * r + e] = v;
+                const int64_t gm = min(m0 + m + (TA == 0 ? e : 0), M - 1);
+                const int64_t gk = min(k0 + k + (TA == 0 ? 0 : e), kend - 1);
+                ra[2 * r + e] = TA == 0 ? A[gm + gk * lda] : A[gk + gm * lda];
             }
             bmap(idx, k, n);
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                int64_t gk = k0 + k + (TB == 0 ? e : 0), gn = n0 + n + (TB == 0 ? 0 : e);
-                T v = 0;
-                if (gn < N && gk < kend) v = TB == 0 ? B[gk + gn * ldb] : B[gn + gk * ldb];
-                rb[2 * r + e] = v;
+                const int64_t gk = min(k0 + k + (TB == 0 ? e : 0), kend - 1);
+                const int64_t gn = min(n0 + n + (TB == 0 ? 0 : e), N - 1);
+                rb[2 * r + e] = TB == 0 ? B[gk + gn * ldb] : B[gn + gk * ldb];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2 * PA; ++r) asm volatile("" ::"v"(ra[r]));
+#pragma unroll
+        for (int r = 0; r < 2 * PB; ++r) asm volatile("" ::"v"(rb[r]));
+#pragma unroll
+        for (int r = 0; r < PA; ++r) {
+            int idx = tid + NT * r, m, k, n;
+            amap(idx, m, k);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int64_t gm = m0 + m + (TA == 0 ? e : 0), gk = k0 + k + (TA == 0 ? 0 : e);
+                ra[2 * r + e] = (gm < M && gk < kend) ? ra[2 * r + e] : T(0);
+            }
+            bmap(idx, k, n);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int64_t gk = k0 + k + (TB == 0 ? e : 0), gn = n0 + n + (TB == 0 ? 0 : e);
+                rb[2 * r + e] = (gn < N && gk < kend) ? rb[2 * r + e] : T(0);
             }
         }
     }
@@ -671,8 +692,8 @@ int gemm_t(cfmi_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double 
 // that inverse (TRSM), and the trailing lower triangle is updated by GEMM.
 // ---------------------------------------------------------------------------
 constexpr int NB = 64;
-constexpr int64_t OB = 512;   // Cholesky panel width (a multiple of NB)
-constexpr int64_t OB2 = 2048; // outer panel width (a multiple of OB)
+constexpr int64_t OB2 = 2048; // outer panel width (a multiple of NB)
+constexpr int64_t S8_PANEL_MIN_K = 256;   // sliced in-panel updates from this K on
 
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
@@ -686,7 +707,12 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
     const int r = threadIdx.x;
     T row[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) row[c] = load_or_zero((const T*)A, r + c * lda, r < kb && c <= r);
+    for (int c = 0; c < NB; ++c) row[c] = A[min(r, kb - 1) + (int64_t)min(c, kb - 1) * lda];
+    // all 64 loads in flight before the first use, then the mask
+#pragma unroll
+    for (int c = 0; c < NB; ++c) asm volatile("" ::"v"(row[c]));
+#pragma unroll
+    for (int c = 0; c < NB; ++c) row[c] = (r < kb && c <= r) ? row[c] : T(0);
     const int bad = wave_chol64<T>(row, kb, lt, rdiag);
     if (bad) {
         if (r == 0) atomicCAS(info, 0, (int)(koff + bad));
@@ -707,8 +733,8 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
     }
 }
 
-// P (m x kb) <- P Linv^T, 256 rows per workgroup on MFMA (16 x 16 x 4):
-// wave w takes rows 32 w .. of each 128-row half, reads its 32 rows
+// P (m x kb) <- P Linv^T, TRSM_ROWS rows per workgroup on MFMA (16 x 16 x 4):
+// wave w takes rows 32 w .. of each 128-row group, reads its 32 rows
 // completely before storing any of them, and skips the K-steps past each
 // 16-column block's diagonal (Linv^T is upper triangular).
 template <typename T> struct PanelMf;
@@ -729,6 +755,8 @@ template <> struct PanelMf<float> {
     static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
+constexpr int TRSM_ROWS = 128;   // panel rows per workgroup (more workgroups: the panel is latency-bound)
+
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int64_t m, int kb,
                                                          const T* linv, const int* info)
@@ -742,8 +770,8 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int6
     const int wave = tid >> 6, lane = tid & 63;
     const int lm = lane & 15, lk = lane >> 4;
     constexpr int KS = NB / 4, NT = NB / 16;
-    for (int half = 0; half < 2; ++half) {
-        const int64_t i0 = (int64_t)blockIdx.x * 256 + 128 * half + 32 * wave;
+    for (int half = 0; half < TRSM_ROWS / 128; ++half) {
+        const int64_t i0 = (int64_t)blockIdx.x * TRSM_ROWS + 128 * half + 32 * wave;
         if (i0 >= m) break;
         T av[KS][2];
 #pragma unroll
@@ -827,6 +855,52 @@ int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, in
     return CFMI_OK;
 }
 
+// Columns [k0, k0 + w) of the n x n lower triangle, all rows from k0 down,
+// already updated by every column left of k0: factored recursively.  The
+// left half (a multiple of 64 columns, at least the right half) is factored,
+// the right half's columns are updated by one product with K = the left
+// half's width, then the right half is factored; 64-column leaves are the
+// one-wave diagonal factor and the triangular solve of all rows below.  A
+// 2048-column panel thus takes updates of K = 1024, 512, ..., 64 whose
+// outputs are only as wide as K: far less traffic than a rank-64 update of
+// all the panel's remaining columns after every 64-column step.
+template <typename T>
+int potrf_panel(cfmi_ctx* c, int64_t n, T* A, int64_t lda, T* U, int64_t ldu, T* linv, int64_t k0, int64_t w,
+                int slices)
+{
+    if (w <= NB) {
+        const int kb = (int)w;
+        hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k0 + k0 * lda, lda, kb,
+                           c->d_info, k0, linv, U ? U + k0 + k0 * ldu : (T*)nullptr, ldu);
+        CFMI_LAUNCH_CHECK(c);
+        const int64_t m = n - k0 - kb;
+        if (m > 0) {
+            hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + TRSM_ROWS - 1) / TRSM_ROWS)), dim3(256),
+                               0, c->stream, A + (k0 + kb) + k0 * lda, lda, m, kb, (const T*)linv,
+                               (const int*)c->d_info);
+            CFMI_LAUNCH_CHECK(c);
+        }
+        return CFMI_OK;
+    }
+    const int64_t blocks = (w + NB - 1) / NB;
+    const int64_t h = (blocks + 1) / 2 * NB;
+    int rc = potrf_panel<T>(c, n, A, lda, U, ldu, linv, k0, h, slices);
+    if (rc) return rc;
+    const int64_t m1 = n - (k0 + h);
+    T* P = A + (k0 + h) + k0 * lda;
+    T* C = A + (k0 + h) + (k0 + h) * lda;
+    if constexpr (std::is_same<T, double>::value) {
+        // sliced (FP64-accurate on the int8 matrix cores) once K is deep
+        // enough for the products to outweigh the slicing
+        if (slices && h >= S8_PANEL_MIN_K) rc = cfmi_s8_update_lower(c, slices, m1, w - h, h, -1.0, P, lda, C, lda);
+        else rc = gemm_t<T>(c, 0, 1, m1, w - h, h, -1.0, P, lda, P, lda, 1.0, C, lda, 16);
+    } else {
+        rc = gemm_t<T>(c, 0, 1, m1, w - h, h, -1.0, P, lda, P, lda, 1.0, C, lda, 16);
+    }
+    if (rc) return rc;
+    return potrf_panel<T>(c, n, A, lda, U, ldu, linv, k0 + h, w - h, slices);
+}
+
 template <typename T>
 int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu, int* info, int slices = 0)
 {
@@ -838,54 +912,21 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
         int rc = fill_t<T>(c, n, n, U, ldu, 0.0, 0);
         if (rc) return rc;
     }
-    // Three levels of column panels.  Inside an OB-column panel, 64-column
-    // steps (one-wave diagonal factor, row-block triangular solve, update of
-    // the panel's remaining columns); after each OB panel, a rank-OB update
-    // of the columns left in its OB2-column panel; after each OB2 panel one
-    // rank-OB2 update of the trailing matrix, which carries almost all of
-    // the n^3/3 flops at a K that keeps the matrix cores busy (a rank-64 or
-    // rank-512 update of the whole trailing matrix is bound by re-reading C).
+    // OB2-column panels, each factored recursively (potrf_panel); after each
+    // panel one rank-OB2 update of the trailing matrix, which carries almost
+    // all of the n^3/3 flops at a K that keeps the matrix cores busy (a
+    // narrower update of the whole trailing matrix is bound by re-reading C).
     for (int64_t K0 = 0; K0 < n; K0 += OB2) {
         const int64_t ob2 = std::min<int64_t>(OB2, n - K0);
-        for (int64_t k0 = K0; k0 < K0 + ob2; k0 += OB) {
-            const int64_t ob = std::min<int64_t>(OB, K0 + ob2 - k0);
-            for (int64_t k = k0; k < k0 + ob; k += NB) {
-                int kb = (int)std::min<int64_t>(NB, k0 + ob - k);
-                hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k + k * lda,
-                                   lda, kb, c->d_info, k, linv, U ? U + k + k * ldu : (T*)nullptr, ldu);
-                CFMI_LAUNCH_CHECK(c);
-                int64_t m = n - k - kb;
-                if (m > 0) {
-                    hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
-                                       c->stream, A + (k + kb) + k * lda, lda, m, kb, (const T*)linv,
-                                       (const int*)c->d_info);
-                    CFMI_LAUNCH_CHECK(c);
-                    const int64_t w = k0 + ob - (k + kb);   // panel columns still to update
-                    if (w > 0) {
-                        T* P = A + (k + kb) + k * lda;
-                        int rc = gemm_t<T>(c, 0, 1, m, w, kb, -1.0, P, lda, P, lda, 1.0,
-                                           A + (k + kb) + (k + kb) * lda, lda, 16);
-                        if (rc) return rc;
-                    }
-                }
-            }
-            const int64_t w2 = K0 + ob2 - (k0 + ob);       // OB2-panel columns still to update
-            const int64_t m1 = n - k0 - ob;
-            if (w2 > 0 && m1 > 0) {
-                T* P1 = A + (k0 + ob) + k0 * lda;
-                int rc = gemm_t<T>(c, 0, 1, m1, w2, ob, -1.0, P1, lda, P1, lda, 1.0,
-                                   A + (k0 + ob) + (k0 + ob) * lda, lda, 16);
-                if (rc) return rc;
-            }
-        }
+        int rc = potrf_panel<T>(c, n, A, lda, U, ldu, linv, K0, ob2, slices);
+        if (rc) return rc;
         const int64_t m2 = n - K0 - ob2;
         if (m2 > 0) {
             T* P2 = A + (K0 + ob2) + K0 * lda;
             T* A22 = A + (K0 + ob2) + (K0 + ob2) * lda;
-            int rc;
             if constexpr (std::is_same<T, double>::value) {
                 // FP64-accurate on the int8 matrix cores: P2's rows as int8 slices
-                rc = slices ? cfmi_s8_syrk_lower(c, slices, m2, ob2, -1.0, P2, lda, A22, lda)
+                rc = slices ? cfmi_s8_update_lower(c, slices, m2, m2, ob2, -1.0, P2, lda, A22, lda)
                             : gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A22, lda, 16);
             } else {
                 rc = gemm_t<T>(c, 0, 1, m2, m2, ob2, -1.0, P2, lda, P2, lda, 1.0, A22, lda, 16);

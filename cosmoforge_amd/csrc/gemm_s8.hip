@@ -640,7 +640,8 @@ int gemm_s8_launch(cfmi_ctx* c, S8Args g, bool db)
     CFMI_LAUNCH_CHECK(c);
     if (c->s8_stats) {
         CFMI_HIP(c, hipEventRecord(e1, c->stream));
-        const double f = 2.0 * (double)g.M * (double)g.N * (double)g.K * (g.lower ? 0.5 : 1.0);
+        const double entries = (double)g.M * (double)g.N - (g.lower ? 0.5 * (double)g.N * (double)(g.N - 1) : 0.0);
+        const double f = 2.0 * (double)g.K * entries;
         c->s8_launches.push_back({e0, e1, f, S});
     }
     return CFMI_OK;
@@ -756,7 +757,9 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
     if (!aligned16(A) || !aligned16(B)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: operands not 16-byte aligned");
     const bool lower = flags & (1 | 32), mirror = flags & 1, packed = flags & 16;
     if (!packed && ldc < m) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: ldc < m");
-    if (lower && (m != n || packed)) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
+    if (mirror && m != n) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a symmetric result needs m == n");
+    if (lower && (n > m || packed))
+        return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: a lower result needs n <= m and a dense C");
     if (packed && (row0 < 0 || col0 < 0 || pbase < 0 || pbase > row0 * (row0 + 1) / 2))
         return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: packed row band starts after row %lld", (long long)row0);
     if (k == 0) return cfmi_fail(c, CFMI_EINVAL, "gemm_s8: k must be positive");
@@ -792,15 +795,17 @@ int cfmi_gemm_s8(cfmi_ctx* c, int s, int64_t m, int64_t n, int64_t k, double alp
 
 }  // extern "C"
 
-int cfmi_s8_syrk_lower(cfmi_ctx* c, int s, int64_t n, int64_t k, double alpha, const double* P, int64_t ldp,
-                       double* C, int64_t ldc)
+int cfmi_s8_update_lower(cfmi_ctx* c, int s, int64_t m, int64_t w, int64_t k, double alpha, const double* P,
+                         int64_t ldp, double* C, int64_t ldc)
 {
-    if (n <= 0 || k <= 0) return CFMI_OK;
-    const int64_t ldk = round64(k), ps = n * ldk;
-    char* ws = (char*)cfmi_workspace(c, (size_t)(s * ps + 8 * n));
-    if (!ws) return cfmi_fail(c, CFMI_ENOMEM, "s8 syrk: workspace of %lld bytes", (long long)(s * ps + 8 * n));
+    if (m <= 0 || w <= 0 || k <= 0) return CFMI_OK;
+    if (w > m) return cfmi_fail(c, CFMI_EINVAL, "s8 update: %lld columns > %lld rows", (long long)w, (long long)m);
+    const int64_t ldk = round64(k), ps = m * ldk;
+    char* ws = (char*)cfmi_workspace(c, (size_t)(s * ps + 8 * m));
+    if (!ws) return cfmi_fail(c, CFMI_ENOMEM, "s8 update: workspace of %lld bytes", (long long)(s * ps + 8 * m));
     double* mu = reinterpret_cast<double*>(ws + s * ps);    // s ps is a multiple of 64
-    int rc = cfmi_slice8(c, s, 1, n, k, P, ldp, ws, ldk, ps, mu);
+    int rc = cfmi_slice8(c, s, 1, m, k, P, ldp, ws, ldk, ps, mu);
     if (rc) return rc;
-    return cfmi_gemm_s8(c, s, n, n, k, alpha, ws, ldk, ps, mu, 0, ws, ldk, ps, mu, 0, 1.0, C, ldc, 32, 0, 0, 0, 1.0);
+    // B is the first w rows of the same slices (the same row scales)
+    return cfmi_gemm_s8(c, s, m, w, k, alpha, ws, ldk, ps, mu, 0, ws, ldk, ps, mu, 0, 1.0, C, ldc, 32, 0, 0, 0, 1.0);
 }

@@ -725,7 +725,8 @@ class S8Recorder:
     (``cfmi_s8_stats_*``): HIP events on the context's stream around every
     gemm_s8 kernel launch, those issued from here and those the library
     issues itself (the sliced Cholesky updates), with each launch's
-    algorithmic flops 2 m n k (m n k for a lower or symmetric result)."""
+    algorithmic flops (2 k per entry computed: 2 m n k, about m n k for a
+    lower or symmetric result)."""
 
     def __init__(self, ctx: Context):
         self.ctx = ctx

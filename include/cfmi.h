@@ -245,8 +245,9 @@ int64_t cfmi_s8_kmax(int s);
  * p + q < s on v_mfma_i32_16x16x64_i8 (exact int32 sums), levels combined
  * in FP64; k beyond cfmi_s8_kmax is split and the parts added in FP64.
  * flags bit 0: symmetric result (m == n), lower tiles computed and mirrored;
- * bit 5: the lower triangle only (m == n; the strict upper part of C is
- * left as it was);
+ * bit 5: only the entries on or below the diagonal (n <= m: the lower
+ * triangle, or a lower trapezoid; the strict upper part of C is left as
+ * it was);
  * bit 1: one LDS stage staged through registers instead of two filled by
  * asynchronous global -> LDS copies;
  * bit 2: A[m][k] = 0 for k > m + off_a, bit 3: B[n][k] = 0 for k > n + off_b
@@ -276,8 +277,9 @@ int cfmi_potrf(cfmi_ctx *ctx, int dtype, int64_t n, void *A, int64_t lda, void *
  * stream bracket every gemm_s8 kernel launch, including those the library
  * issues itself (cfmi_potrf_s8).  enable(on) synchronizes and clears the
  * record; read synchronizes and returns the launch count, the summed
- * launch time (ms), the algorithmic flops (2 m n k, m n k for a lower /
- * symmetric result) and the int8 products (flops s (s + 1) / 2).          */
+ * launch time (ms), the algorithmic flops (2 k per entry computed: 2 m n k,
+ * k (m n - n (n - 1) / 2) for a lower or symmetric result, i.e. about m n k
+ * for m = n) and the int8 products (flops s (s + 1) / 2).                 */
 int cfmi_s8_stats_enable(cfmi_ctx *ctx, int on);
 int cfmi_s8_stats_read(cfmi_ctx *ctx, int64_t *launches, double *ms, double *flops, double *products);
 

@@ -110,6 +110,31 @@ def test_gemm_s8_symmetric_and_offsets(ctx):
     np.testing.assert_array_equal(engine.to_numpy_cm(C), full[40:140, 7:57])
 
 
+def test_gemm_s8_lower_trapezoid(ctx):
+    """Flag bit 5 with n < m (the sliced Cholesky's in-panel updates): the
+    entries on or below the diagonal equal the full product's, bit for bit,
+    the strict upper part keeps C's values; n > m is rejected."""
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(6)
+    P = rng.standard_normal((300, 320))
+    S = _dev_slices(ctx, P, 5, 0)
+    C0 = rng.standard_normal((300, 140))
+    C = ctx.upload_cm(C0)
+    engine.gemm_s8(ctx, 300, 140, 320, -1.0, S, 0, S, 0, 1.0, C, 0, 300, flags=32)
+    got = engine.to_numpy_cm(C)
+    want = C0 - E.gemm(P, P[:140], 5)
+    low = np.tril(np.ones((300, 140), dtype=bool))
+    np.testing.assert_allclose(got[low], want[low], rtol=0, atol=1e-13 * np.max(np.abs(want)))
+    np.testing.assert_array_equal(got[~low], C0[~low])
+    W = torch.zeros((300, 140), dtype=torch.float64, device=ctx.device)
+    bad = ctx.lib.cfmi_gemm_s8(ctx.h, 5, 140, 300, 320, 1.0, engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0,
+                               engine.P(S.data), S.ldk, S.ps, engine.P(S.mu), 0, 0.0, engine.P(W), 140, 32, 0, 0, 0,
+                               1.0)
+    assert bad < 0 and b"n <= m" in ctx.lib.cfmi_last_error(ctx.h)
+
+
 def test_gemm_s8_k_split(ctx):
     """k beyond the exact int32 range of one pass: chunks added in FP64."""
     from cosmoforge_amd import engine
@@ -211,10 +236,11 @@ def test_blocked_trsm(ctx, slices, outer, diag):
 
 @pytest.mark.parametrize("s", [4, 5])
 def test_potrf_s8_matches_fp64_factor(ctx, s):
-    """The Cholesky factor with its rank-2048 trailing updates on int8
-    slices equals the FP64 factor to the slicing level (n = 5000: two
-    sliced updates), keeps the strict upper triangle zero, and fills the
-    diagonal-block inverses as cfmi_potrf does."""
+    """The Cholesky factor with its rank-2048 trailing updates and its
+    in-panel updates of K >= 256 on int8 slices equals the FP64 factor to
+    the slicing level (n = 5000: two sliced trailing updates), keeps the
+    strict upper triangle zero, and fills the diagonal-block inverses as
+    cfmi_potrf does."""
     import ctypes
 
     import torch
