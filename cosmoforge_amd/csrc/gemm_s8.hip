@@ -87,8 +87,37 @@ __device__ __forceinline__ double absmax(double m, double v)
 // row maximum by a wave reduction, then 1024-value chunks, read coalesced
 // (64 consecutive values per load) into the wave's LDS buffer and sliced
 // as 16 consecutive values per lane (a pad entry every 16 keeps both LDS
-// sides free of bank conflicts); zero beyond k.
+// sides free of bank conflicts); zero beyond k.  NCH > 0: the row (at most
+// NCH chunks) is read once and held in registers for both passes; NCH = 0:
+// read twice.
 template <int S>
+__device__ __forceinline__ void slice_chunk(double* b, const double (&v)[16], int lane, int64_t base, int64_t r,
+                                            int64_t ldk, int64_t ps, double inv, char* __restrict__ dst)
+{
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int e = t * 64 + lane;                  // chunk element
+        b[e + (e >> 4)] = v[t];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t c = base / 16 + lane;              // this lane's 16-value group
+    if (c * 16 < ldk) {
+        double x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = b[lane * 17 + j];
+        u4 o[S];
+        slice16<S>(x, inv, o);
+#pragma unroll
+        for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + c * 16) = o[p];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int S, int NCH>
 __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k, const double* __restrict__ src,
                                                          int64_t lds, char* __restrict__ dst, int64_t ldk,
                                                          int64_t ps, double* __restrict__ mu)
@@ -98,45 +127,48 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(int64_t rows, int64_t k
     const int64_t r = (int64_t)blockIdx.x * 4 + wave;
     if (r >= rows) return;
     const double* row = src + r * lds;
-    double m = 0.0;
-    for (int64_t j0 = 0; j0 < k; j0 += 8 * 64) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = load_or_zero(row, j0 + 64 * u + lane, j0 + 64 * u + lane < k);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) m = absmax(m, v[u]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = absmax(m, __shfl_xor(m, off, 64));
-    m = finish_mu(m);
-    if (lane == 0) mu[r] = m;
-    const double inv = 1.0 / m;
     double* b = buf[wave];
-    for (int64_t base = 0; base < ldk; base += 1024) {
-        double v[16];
+    double m = 0.0;
+    if constexpr (NCH > 0) {
+        double v[NCH][16];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) v[t] = load_or_zero(row, base + t * 64 + lane, base + t * 64 + lane < k);
+        for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const int e = t * 64 + lane;                  // chunk element
-            b[e + (e >> 4)] = v[t];
+            for (int t = 0; t < 16; ++t) {
+                const int64_t e = ch * 1024 + t * 64 + lane;
+                v[ch][t] = load_or_zero(row, e, e < k);
+            }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+            for (int t = 0; t < 16; ++t) m = absmax(m, v[ch][t]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = absmax(m, __shfl_xor(m, off, 64));
+        m = finish_mu(m);
+        if (lane == 0) mu[r] = m;
+        const double inv = 1.0 / m;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+            if (ch * 1024 < ldk) slice_chunk<S>(b, v[ch], lane, ch * 1024, r, ldk, ps, inv, dst);
+    } else {
+        for (int64_t j0 = 0; j0 < k; j0 += 8 * 64) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = load_or_zero(row, j0 + 64 * u + lane, j0 + 64 * u + lane < k);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) m = absmax(m, v[u]);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int64_t c = base / 16 + lane;              // this lane's 16-value group
-        if (c * 16 < ldk) {
-            double x[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) x[j] = b[lane * 17 + j];
-            u4 o[S];
-            slice16<S>(x, inv, o);
+        for (int off = 32; off > 0; off >>= 1) m = absmax(m, __shfl_xor(m, off, 64));
+        m = finish_mu(m);
+        if (lane == 0) mu[r] = m;
+        const double inv = 1.0 / m;
+        for (int64_t base = 0; base < ldk; base += 1024) {
+            double v[16];
 #pragma unroll
-            for (int p = 0; p < S; ++p) *reinterpret_cast<u4*>(dst + p * ps + r * ldk + c * 16) = o[p];
+            for (int t = 0; t < 16; ++t) v[t] = load_or_zero(row, base + t * 64 + lane, base + t * 64 + lane < k);
+            slice_chunk<S>(b, v, lane, base, r, ldk, ps, inv, dst);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -574,8 +606,17 @@ int slice_t(cfmi_ctx* c, int layout, int64_t rows, int64_t k, const double* src,
             int64_t ldk, int64_t ps, double* mu)
 {
     if (layout == 0) {
-        hipLaunchKernelGGL((slice_rows_kernel<S>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, rows,
-                           k, src, lds, dst, ldk, ps, mu);
+        // rows of up to 2048 values are read once (held in registers)
+        const dim3 grid((unsigned)((rows + 3) / 4));
+        if (ldk <= 1024)
+            hipLaunchKernelGGL((slice_rows_kernel<S, 1>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
+                               ps, mu);
+        else if (ldk <= 2048)
+            hipLaunchKernelGGL((slice_rows_kernel<S, 2>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
+                               ps, mu);
+        else
+            hipLaunchKernelGGL((slice_rows_kernel<S, 0>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
+                               ps, mu);
         CFMI_LAUNCH_CHECK(c);
         return CFMI_OK;
     }

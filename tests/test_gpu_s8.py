@@ -85,6 +85,26 @@ def test_gemm_s8_bit_exact_and_bound(ctx, s, shape):
     assert np.all(np.abs(got - exact) <= 0.75 * E.bound(A, B, s) + 1e-15 * np.abs(C0))
 
 
+@pytest.mark.parametrize("s", [3, 5])
+def test_gemm_s8_single_stage(ctx, s):
+    """Flag bit 1 (one LDS stage staged through registers) gives the same
+    result as the two asynchronous stages, bit for bit."""
+    from cosmoforge_amd import engine
+
+    m, n, k = 257, 190, 700
+    rng = np.random.default_rng(70 + s)
+    A, B = _mat(rng, m, k), _mat(rng, n, k)
+    C0 = rng.standard_normal((m, n))
+    Sa, Sb = _dev_slices(ctx, A, s, 1), _dev_slices(ctx, B, s, 0)
+    got = {}
+    for flags in (0, 2):
+        C = ctx.upload_cm(C0)
+        engine.gemm_s8(ctx, m, n, k, 1.25, Sa, 0, Sb, 0, -0.5, C, 0, m, flags=flags)
+        got[flags] = engine.to_numpy_cm(C)
+    np.testing.assert_array_equal(got[2], got[0])
+    np.testing.assert_array_equal(got[0], E.gemm(A, B, s, alpha=1.25, beta=-0.5, C=C0))
+
+
 def test_gemm_s8_symmetric_and_offsets(ctx):
     """Lower tiles + mirror equal the full product; row offsets into the
     slice buffers address the right rows."""
