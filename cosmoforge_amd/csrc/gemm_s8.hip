@@ -606,13 +606,16 @@ int slice_t(cfmi_ctx* c, int layout, int64_t rows, int64_t k, const double* src,
             int64_t ldk, int64_t ps, double* mu)
 {
     if (layout == 0) {
-        // rows of up to 2048 values are read once (held in registers)
+        // rows of up to 4096 values are read once (held in registers)
         const dim3 grid((unsigned)((rows + 3) / 4));
         if (ldk <= 1024)
             hipLaunchKernelGGL((slice_rows_kernel<S, 1>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
                                ps, mu);
         else if (ldk <= 2048)
             hipLaunchKernelGGL((slice_rows_kernel<S, 2>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
+                               ps, mu);
+        else if (ldk <= 4096)
+            hipLaunchKernelGGL((slice_rows_kernel<S, 4>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
                                ps, mu);
         else
             hipLaunchKernelGGL((slice_rows_kernel<S, 0>), grid, dim3(256), 0, c->stream, rows, k, src, lds, dst, ldk,
