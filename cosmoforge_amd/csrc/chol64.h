@@ -136,3 +136,107 @@ __device__ __forceinline__ void wave_inv64_to_lds(T (*lt)[65], const T* rdiag)
     }
     __syncthreads();
 }
+
+// U = L^-1 for one wave on the FP64 matrix cores, from wave_chol64's lt and
+// rdiag, into ut as ut[c][i] = U[i][c] (the layout of wave_inv64_to_lds):
+// the four 16 x 16 diagonal blocks by substitution (lane c takes column c %
+// 16 of block c / 16, all four blocks at once), then U_BA = -U_BB L_BA U_AA
+// for the two pairs of 16-blocks and once for the two 32-blocks, each
+// product on v_mfma_f64_16x16x4 (lane (m, k) holds A[m][k] and B[k][m], and
+// C[k + 4 r][m] after) with the operands read from LDS.  tb: 32 x 33
+// staging for the intermediate products.  Entries past kb come out zero.
+typedef double chol_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void wave_inv64_mfma(double (*lt)[65], const double* rdiag, double (*ut)[65],
+                                                double (*tb)[33])
+{
+    __syncthreads();
+    const int c = threadIdx.x & 63;
+    const int lm = c & 15, lk = c >> 4;
+    // zero U, then the 16 x 16 diagonal blocks by substitution
+#pragma unroll
+    for (int i = 0; i < 64; ++i) ut[c][i] = 0.0;
+    {
+        const int b = c >> 4, j = c & 15, o = 16 * b;
+        double u[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) u[i] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            u[k] *= rdiag[o + k];
+#pragma unroll
+            for (int i = k + 1; i < 16; ++i) u[i] -= lt[o + k][o + i] * u[k];
+        }
+        lds_fence();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ut[o + j][o + i] = u[i];
+    }
+    lds_fence();
+    // pairs of 16-blocks (A, B) = (0, 1) and (2, 3): T = L_BA U_AA, U_BA = -U_BB T
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr) {
+        const int a = 32 * pr, bb = a + 16;
+        chol_d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = 4 * s + lk;
+            t = __builtin_amdgcn_mfma_f64_16x16x4f64(lt[a + k][bb + lm], ut[a + lm][a + k], t, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tb[lk + 4 * r][lm] = t[r];
+        lds_fence();
+        chol_d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = 4 * s + lk;
+            y = __builtin_amdgcn_mfma_f64_16x16x4f64(ut[bb + k][bb + lm], tb[k][lm], y, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ut[a + lm][bb + lk + 4 * r] = -y[r];
+        lds_fence();
+    }
+    // the 32-blocks: T = L21 U11 (32 x 32), U21 = -U22 T; U11 and U22 are
+    // lower triangular, so their zero 16 x 16 corners skip half the K-steps
+    chol_d4 t[2][2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            t[ti][tj] = chol_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 4 * tj; s < 8; ++s) {
+                const int k = 4 * s + lk;
+                t[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lt[k][32 + 16 * ti + lm], ut[16 * tj + lm][k],
+                                                                 t[ti][tj], 0, 0, 0);
+            }
+        }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tb[16 * ti + lk + 4 * r][16 * tj + lm] = t[ti][tj][r];
+    lds_fence();
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            chol_d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4 * (ti + 1); ++s) {
+                const int k = 4 * s + lk;
+                y = __builtin_amdgcn_mfma_f64_16x16x4f64(ut[32 + k][32 + 16 * ti + lm], tb[k][16 * tj + lm], y, 0, 0,
+                                                         0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ut[16 * tj + lm][32 + 16 * ti + lk + 4 * r] = -y[r];
+        }
+    __syncthreads();
+}

@@ -695,17 +695,17 @@ constexpr int NB = 64;
 constexpr int64_t OB2 = 2048; // outer panel width (a multiple of NB)
 constexpr int64_t S8_PANEL_MIN_K = 256;   // sliced in-panel updates from this K on
 
-// One wave: the Cholesky factor of the kb x kb diagonal block (chol64.h),
-// stored in place, and 1 / L_jj for the panel solve (rdiag).  The inverse
-// of the block is not on this path: the panel solve substitutes with L
-// itself, and the diagonal-block inverses come from one batched launch
-// after the factorisation (inv_blocks_kernel).
+// One wave: factor and inverse of the kb x kb diagonal block (chol64.h);
+// the inverse (FP64: blocked, on the matrix cores) leaves through LDS so
+// that the global stores are by rows.
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int kb, int* info, int64_t koff,
-                                                        T* rdiag_out)
+                                                        T* linv, T* U, int64_t ldu)
 {
     __shared__ T lt[NB][NB + 1];
     __shared__ T rdiag[NB];
+    __shared__ T ut[std::is_same<T, double>::value ? NB : 1][NB + 1];
+    __shared__ T tb[std::is_same<T, double>::value ? 32 : 1][33];
     if (*info != 0) return;
     const int r = threadIdx.x;
     T row[NB];
@@ -722,104 +722,98 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* A, int64_t lda, int k
         return;
     }
     __syncthreads();
-    rdiag_out[r] = rdiag[r];
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         if (r < kb && c < kb) A[r + c * lda] = lt[c][r];   // L[r][c], zero above the diagonal
+    T (*uo)[NB + 1] = lt;
+    if constexpr (std::is_same<T, double>::value) {
+        wave_inv64_mfma(lt, rdiag, ut, tb);
+        uo = ut;
+    } else {
+        wave_inv64_to_lds<T>(lt, rdiag);
+    }
+    // lane i stores row i of U: U[i][c] = uo[c][i], consecutive lanes at consecutive addresses
+    const int i = r;
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc) {
+        const T v = uo[cc][i];
+        linv[i + cc * NB] = v;
+        if (U && i < kb && cc < kb) U[i + cc * ldu] = v;
+    }
 }
 
-constexpr int TRSM_ROWS = 128;   // panel rows (threads) per workgroup of the panel solve
+// P (m x kb) <- P Linv^T, TRSM_ROWS rows per workgroup on MFMA (16 x 16 x 4):
+// wave w takes rows 32 w .. of each 128-row group, reads its 32 rows
+// completely before storing any of them, and skips the K-steps past each
+// 16-column block's diagonal (Linv^T is upper triangular).
+template <typename T> struct PanelMf;
+template <> struct PanelMf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct PanelMf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
 
-// P (m x kb) <- P L^-T for the factored kb x kb diagonal block L: thread t
-// solves row t of the panel, L x = p, by columns (x_j = p_j / L_jj, then
-// p_i -= L_ij x_j for i > j), with L's columns broadcast from LDS in groups
-// of 16 reads.  The row is read before it is written.
+constexpr int TRSM_ROWS = 128;   // panel rows per workgroup (more workgroups: the panel is latency-bound)
+
 template <typename T>
-__global__ __launch_bounds__(TRSM_ROWS) void trsm_sub_kernel(T* P, int64_t ldp, int64_t m, int kb, const T* L,
-                                                             int64_t ldl, const T* rdiag, const int* info)
+__global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int64_t m, int kb,
+                                                         const T* linv, const int* info)
 {
-    __shared__ T lt[NB][NB + 1];   // lt[j][i] = L[i][j]
-    __shared__ T rd[NB];
+    typedef typename PanelMf<T>::v4 v4;
+    __shared__ T sl[NB][NB + 1];
     if (*info != 0) return;
     const int tid = threadIdx.x;
-    {
-        constexpr int PER = NB * NB / TRSM_ROWS;
-        T v[PER];
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int idx = tid + TRSM_ROWS * e, i = idx & (NB - 1), j = idx >> 6;
-            v[e] = L[min(i, kb - 1) + (int64_t)min(j, kb - 1) * ldl];
-        }
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int idx = tid + TRSM_ROWS * e, i = idx & (NB - 1), j = idx >> 6;
-            lt[j][i] = (i > j && i < kb) ? v[e] : T(0);
-        }
-        if (tid < NB) rd[tid] = tid < kb ? rdiag[tid] : T(0);
-    }
+    block64_to_lds<256>(sl, linv, tid);   // sl[i][j] = Linv[i][j]
     __syncthreads();
-    const int64_t row = (int64_t)blockIdx.x * TRSM_ROWS + tid;
-    if (row >= m) return;
-    T x[NB];
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    constexpr int KS = NB / 4, NT = NB / 16;
+    for (int half = 0; half < TRSM_ROWS / 128; ++half) {
+        const int64_t i0 = (int64_t)blockIdx.x * TRSM_ROWS + 128 * half + 32 * wave;
+        if (i0 >= m) break;
+        T av[KS][2];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) x[j] = P[row + (int64_t)min(j, kb - 1) * ldp];
+        for (int s = 0; s < KS; ++s) {
+            const int kk = 4 * s + lk;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) asm volatile("" ::"v"(x[j]));
-#pragma clang loop unroll(full)
-    for (int j = 0; j < NB; ++j) {
-        x[j] *= rd[j];                       // zero past kb
-#pragma clang loop unroll(full)
-        for (int g = 0; g < NB / 16; ++g) {
-            if (16 * g + 15 <= j) continue;
-            T v[16];
+            for (int h = 0; h < 2; ++h) {
+                const int64_t ri = i0 + 16 * h + lm;
+                av[s][h] = load_or_zero((const T*)P, ri + (int64_t)kk * ldp, kk < kb && ri < m);
+            }
+        }
 #pragma unroll
-            for (int u = 0; u < 16; ++u) v[u] = lt[j][16 * g + u];
+        for (int b = 0; b < NT; ++b) {
+            v4 acc[2] = {v4{0, 0, 0, 0}, v4{0, 0, 0, 0}};
+            const int cc = 16 * b + lm;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) asm volatile("" ::"v"(v[u]));
+            for (int s = 0; s < 4 * (b + 1); ++s) {
+                const int kk = 4 * s + lk;
+                const T bv = (kk < kb && cc < kb) ? sl[cc][kk] : T(0);   // Linv^T[kk][cc]
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (16 * g + u > j) x[16 * g + u] -= v[u] * x[j];
-            // the updated values consumed here too: otherwise the compiler
-            // defers the updates to the steps that use them and holds every
-            // step's reads in registers (which then spill)
+                for (int h = 0; h < 2; ++h) acc[h] = PanelMf<T>::run(av[s][h], bv, acc[h]);
+            }
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (16 * g + u > j) asm volatile("" ::"v"(x[16 * g + u]));
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t row = i0 + 16 * h + PanelMf<T>::row(lk, r);
+                    if (row < m && cc < kb) P[row + (int64_t)cc * ldp] = acc[h][r];
+                }
         }
     }
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-        if (j < kb) P[row + (int64_t)j * ldp] = x[j];
 }
 
-// U's 64 x 64 diagonal blocks = the inverses of L's (one wave per block, all
-// blocks in one launch, after the factorisation): block b at rows and
-// columns [64 b, 64 b + kb).
-template <typename T>
-__global__ __launch_bounds__(64) void inv_blocks_kernel(int64_t n, const T* A, int64_t lda, T* U, int64_t ldu,
-                                                        const int* info)
-{
-    __shared__ T lt[NB][NB + 1];   // lt[k][i] = L[i][k]
-    __shared__ T rdiag[NB];
-    if (*info != 0) return;
-    const int64_t k0 = (int64_t)blockIdx.x * NB;
-    const int kb = (int)min((int64_t)NB, n - k0);
-    const T* Lb = A + k0 + k0 * lda;
-    const int r = threadIdx.x;
-    T v[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) v[c] = Lb[min(r, kb - 1) + (int64_t)min(c, kb - 1) * lda];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) lt[c][r] = (r < kb && c < kb && r >= c) ? v[c] : T(0);
-    __syncthreads();
-    rdiag[r] = r < kb ? T(1) / lt[r][r] : T(0);
-    wave_inv64_to_lds<T>(lt, rdiag);
-    // lane i stores row i of U: U[i][c] = lt[c][i], consecutive lanes at consecutive addresses
-    T* Ub = U + k0 + k0 * ldu;
-#pragma unroll
-    for (int cc = 0; cc < NB; ++cc)
-        if (r < kb && cc < kb) Ub[r + cc * ldu] = lt[cc][r];
-}
 
 template <typename T>
 __global__ void fill_kernel(int64_t m, int64_t n, T* A, int64_t lda, T v, int mode)
@@ -881,25 +875,26 @@ int fill_t(cfmi_ctx* c, int64_t m, int64_t n, void* A, int64_t lda, double v, in
 // outputs are only as wide as K: far less traffic than a rank-64 update of
 // all the panel's remaining columns after every 64-column step.
 template <typename T>
-int potrf_panel(cfmi_ctx* c, int64_t n, T* A, int64_t lda, T* rdiag, int64_t k0, int64_t w, int slices)
+int potrf_panel(cfmi_ctx* c, int64_t n, T* A, int64_t lda, T* U, int64_t ldu, T* linv, int64_t k0, int64_t w,
+                int slices)
 {
     if (w <= NB) {
         const int kb = (int)w;
         hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(64), 0, c->stream, A + k0 + k0 * lda, lda, kb,
-                           c->d_info, k0, rdiag);
+                           c->d_info, k0, linv, U ? U + k0 + k0 * ldu : (T*)nullptr, ldu);
         CFMI_LAUNCH_CHECK(c);
         const int64_t m = n - k0 - kb;
         if (m > 0) {
-            hipLaunchKernelGGL((trsm_sub_kernel<T>), dim3((unsigned)((m + TRSM_ROWS - 1) / TRSM_ROWS)),
-                               dim3(TRSM_ROWS), 0, c->stream, A + (k0 + kb) + k0 * lda, lda, m, kb,
-                               (const T*)(A + k0 + k0 * lda), lda, (const T*)rdiag, (const int*)c->d_info);
+            hipLaunchKernelGGL((trsm_panel_kernel<T>), dim3((unsigned)((m + TRSM_ROWS - 1) / TRSM_ROWS)), dim3(256),
+                               0, c->stream, A + (k0 + kb) + k0 * lda, lda, m, kb, (const T*)linv,
+                               (const int*)c->d_info);
             CFMI_LAUNCH_CHECK(c);
         }
         return CFMI_OK;
     }
     const int64_t blocks = (w + NB - 1) / NB;
     const int64_t h = (blocks + 1) / 2 * NB;
-    int rc = potrf_panel<T>(c, n, A, lda, rdiag, k0, h, slices);
+    int rc = potrf_panel<T>(c, n, A, lda, U, ldu, linv, k0, h, slices);
     if (rc) return rc;
     const int64_t m1 = n - (k0 + h);
     T* P = A + (k0 + h) + k0 * lda;
@@ -913,7 +908,7 @@ int potrf_panel(cfmi_ctx* c, int64_t n, T* A, int64_t lda, T* rdiag, int64_t k0,
         rc = gemm_t<T>(c, 0, 1, m1, w - h, h, -1.0, P, lda, P, lda, 1.0, C, lda, 16);
     }
     if (rc) return rc;
-    return potrf_panel<T>(c, n, A, lda, rdiag, k0 + h, w - h, slices);
+    return potrf_panel<T>(c, n, A, lda, U, ldu, linv, k0 + h, w - h, slices);
 }
 
 template <typename T>
@@ -921,7 +916,7 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
 {
     T* A = (T*)Av;
     T* U = (T*)Uv;
-    T* rdiag = (T*)c->d_red;  // 1 / L_jj of the current diagonal block (reduction staging)
+    T* linv = (T*)c->d_red;  // 64 x 64 staging inside the reduction buffer
     CFMI_HIP(c, hipMemsetAsync(c->d_info, 0, sizeof(int), c->stream));
     if (U) {
         int rc = fill_t<T>(c, n, n, U, ldu, 0.0, 0);
@@ -933,7 +928,7 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
     // narrower update of the whole trailing matrix is bound by re-reading C).
     for (int64_t K0 = 0; K0 < n; K0 += OB2) {
         const int64_t ob2 = std::min<int64_t>(OB2, n - K0);
-        int rc = potrf_panel<T>(c, n, A, lda, rdiag, K0, ob2, slices);
+        int rc = potrf_panel<T>(c, n, A, lda, U, ldu, linv, K0, ob2, slices);
         if (rc) return rc;
         const int64_t m2 = n - K0 - ob2;
         if (m2 > 0) {
@@ -948,11 +943,6 @@ int potrf_t(cfmi_ctx* c, int64_t n, void* Av, int64_t lda, void* Uv, int64_t ldu
             }
             if (rc) return rc;
         }
-    }
-    if (U) {
-        hipLaunchKernelGGL((inv_blocks_kernel<T>), dim3((unsigned)((n + NB - 1) / NB)), dim3(64), 0, c->stream, n,
-                           (const T*)A, lda, U, ldu, (const int*)c->d_info);
-        CFMI_LAUNCH_CHECK(c);
     }
     int rc = fill_t<T>(c, n, n, A, lda, 0.0, 1);
     if (rc) return rc;
