@@ -815,6 +815,101 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(T* P, int64_t ldp, int6
 }
 
 
+// The Cholesky's narrow in-panel updates, C (m x w, lower trapezoid: only
+// entries with column <= row) -= P P[0:w]^T with P m x K, K <= 128 and
+// w <= min(m, 128): one workgroup per 64 x 64 tile of C, its stored values,
+// and both operand blocks (one 64-deep K chunk at a time, staged in LDS,
+// k-major), fetched with every load of a round in flight; wave w takes the
+// 32 x 32 quadrant (w / 2, w % 2) on MFMA.  (A general GEMM tile for these
+// shapes spends its time waiting on one K-slab after another.)
+constexpr int CU_LDT = 72;   // LDS row pitch of the staged operands (64 rows + pad)
+
+template <typename T>
+__global__ __launch_bounds__(256) void chol_update_kernel(int64_t m, int w, int K, const T* __restrict__ P,
+                                                          int64_t ldp, T* __restrict__ C, int64_t ldc)
+{
+    typedef typename PanelMf<T>::v4 v4;
+    __shared__ T sa[NB][CU_LDT];
+    __shared__ T sb[NB][CU_LDT];
+    // tile (ti, tj >= 0) of the trapezoid, column-major over the tiles
+    const int64_t tm = (m + NB - 1) / NB;
+    int64_t b = blockIdx.x, tj = 0;
+    while (b >= tm - tj) {
+        b -= tm - tj;
+        ++tj;
+    }
+    const int64_t ti = tj + b;
+    const int64_t r0 = ti * NB, c0 = tj * NB;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wa = wave >> 1, wb = wave & 1, lm = lane & 15, lk = lane >> 4;
+    const bool diag = ti == tj;
+    const bool active = !(diag && wb > wa);
+    // the stored quadrant (MFMA output layout)
+    T cv[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * wa + 16 * x + PanelMf<T>::row(lk, r);
+                const int64_t col = c0 + 32 * wb + 16 * y + lm;
+                cv[x][y][r] = load_or_zero((const T*)C, row + col * ldc, active && row < m && col < w && col <= row);
+            }
+    v4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
+    for (int kc = 0; kc < K; kc += NB) {
+        const int kw = min(NB, K - kc);
+        T va[NB / 4], vb[NB / 4];
+#pragma unroll
+        for (int p = 0; p < NB / 4; ++p) {
+            const int k = wave + 4 * p;
+            const T* col = P + (int64_t)(kc + (k < kw ? k : 0)) * ldp;
+            const int64_t ra = r0 + lane, rb = c0 + lane;
+            va[p] = load_or_zero(col, ra, k < kw && ra < m);
+            vb[p] = load_or_zero(col, rb, k < kw && rb < w);
+        }
+        if (kc) __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NB / 4; ++p) {
+            const int k = wave + 4 * p;
+            sa[k][lane] = va[p];
+            sb[k][lane] = vb[p];
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int s4 = 0; s4 < NB / 4; ++s4) {
+                const int k = 4 * s4 + lk;
+                T av[2], bv[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    av[h] = sa[k][32 * wa + 16 * h + lm];
+                    bv[h] = sb[k][32 * wb + 16 * h + lm];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) acc[x][y] = PanelMf<T>::run(av[x], bv[y], acc[x][y]);
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * wa + 16 * x + PanelMf<T>::row(lk, r);
+                const int64_t col = c0 + 32 * wb + 16 * y + lm;
+                if (row < m && col < w && col <= row) C[row + col * ldc] = cv[x][y][r] - acc[x][y][r];
+            }
+}
+
 template <typename T>
 __global__ void fill_kernel(int64_t m, int64_t n, T* A, int64_t lda, T v, int mode)
 {
@@ -899,7 +994,14 @@ int potrf_panel(cfmi_ctx* c, int64_t n, T* A, int64_t lda, T* U, int64_t ldu, T*
     const int64_t m1 = n - (k0 + h);
     T* P = A + (k0 + h) + k0 * lda;
     T* C = A + (k0 + h) + (k0 + h) * lda;
-    if constexpr (std::is_same<T, double>::value) {
+    if (h <= 2 * NB) {
+        // the narrow updates (K = 64, 128): 64 x 64 tiles with all loads of a round in flight
+        const int64_t tm = (m1 + NB - 1) / NB, tw = (w - h + NB - 1) / NB;
+        const int64_t tiles = tw * tm - tw * (tw - 1) / 2;
+        hipLaunchKernelGGL((chol_update_kernel<T>), dim3((unsigned)tiles), dim3(256), 0, c->stream, m1, (int)(w - h),
+                           (int)h, (const T*)P, lda, C, lda);
+        CFMI_LAUNCH_CHECK(c);
+    } else if constexpr (std::is_same<T, double>::value) {
         // sliced (FP64-accurate on the int8 matrix cores) once K is deep
         // enough for the products to outweigh the slicing
         if (slices && h >= S8_PANEL_MIN_K) rc = cfmi_s8_update_lower(c, slices, m1, w - h, h, -1.0, P, lda, C, lda);
