@@ -1105,25 +1105,83 @@ __global__ void diag_scale_kernel(int64_t n, const double* A, int64_t lda, doubl
     }
 }
 
+// Column-blocked grid for the elementwise m x n passes: blockIdx.y walks the
+// columns, the x-blocks cover EW_ROWS rows of a column each (no 64-bit
+// divisions per element, and each thread's loads of a column in flight).
+constexpr int EW_PER = 8;                    // elements per thread and column
+constexpr int64_t EW_ROWS = 256 * EW_PER;
+
+inline dim3 ew_grid(int64_t m, int64_t n)
+{
+    const int64_t bx = (m + EW_ROWS - 1) / EW_ROWS;
+    return dim3((unsigned)(bx > 0 ? bx : 1), (unsigned)(n < 65535 ? (n > 0 ? n : 1) : 65535));
+}
+
+__global__ void scale_flat_kernel(int mode, int64_t m, int64_t n, double* A, int64_t lda, const double* d)
+{
+    // the flat form for short columns; mode as scale_kernel
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % m, j = idx / m;
+        A[i + j * lda] *= mode == 0 ? d[j] : mode == 1 ? d[i] : d[i] * d[j];
+    }
+}
+
 __global__ void scale_kernel(int mode, int64_t m, int64_t n, double* A, int64_t lda, const double* d)
 {
     // mode 0: columns, 1: rows, 2: both (D A D)
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+        const double dj = mode == 1 ? 1.0 : d[j];
+        double* col = A + j * lda;
+        for (int64_t i0 = (int64_t)blockIdx.x * EW_ROWS + threadIdx.x; i0 < m; i0 += (int64_t)gridDim.x * EW_ROWS) {
+            double v[EW_PER], di[EW_PER];
+#pragma unroll
+            for (int e = 0; e < EW_PER; ++e) {
+                const int64_t i = i0 + 256 * e;
+                v[e] = load_or_zero((const double*)col, i, i < m);
+                di[e] = load_or_zero(d, mode == 0 ? 0 : i, mode != 0 && i < m);   // d has n entries in mode 0
+            }
+#pragma unroll
+            for (int e = 0; e < EW_PER; ++e) {
+                const int64_t i = i0 + 256 * e;
+                if (i < m) col[i] = v[e] * (mode == 0 ? dj : (mode == 1 ? di[e] : di[e] * dj));
+            }
+        }
+    }
+}
+
+// The flat form for short columns (one element per column, a strided diagonal):
+__global__ void axpby_flat_kernel(int64_t m, int64_t n, double alpha, const double* A, int64_t lda, double beta,
+                                  double* B, int64_t ldb)
+{
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = idx % m, j = idx / m;
-        double s = mode == 0 ? d[j] : mode == 1 ? d[i] : d[i] * d[j];
-        A[i + j * lda] *= s;
+        const int64_t i = idx % m, j = idx / m;
+        double* p = B + i + j * ldb;
+        *p = alpha * A[i + j * lda] + (beta == 0.0 ? 0.0 : beta * *p);
     }
 }
 
 __global__ void axpby_kernel(int64_t m, int64_t n, double alpha, const double* A, int64_t lda, double beta,
                              double* B, int64_t ldb)
 {
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m * n;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = idx % m, j = idx / m;
-        double* p = B + i + j * ldb;
-        *p = alpha * A[i + j * lda] + (beta == 0.0 ? 0.0 : beta * *p);
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+        const double* a = A + j * lda;
+        double* b = B + j * ldb;
+        for (int64_t i0 = (int64_t)blockIdx.x * EW_ROWS + threadIdx.x; i0 < m; i0 += (int64_t)gridDim.x * EW_ROWS) {
+            double va[EW_PER], vb[EW_PER];
+#pragma unroll
+            for (int e = 0; e < EW_PER; ++e) {
+                const int64_t i = i0 + 256 * e;
+                va[e] = load_or_zero(a, i, i < m);
+                vb[e] = load_or_zero((const double*)b, i, i < m);   // not used when beta = 0
+            }
+#pragma unroll
+            for (int e = 0; e < EW_PER; ++e) {
+                const int64_t i = i0 + 256 * e;
+                if (i < m) b[i] = alpha * va[e] + (beta == 0.0 ? 0.0 : beta * vb[e]);
+            }
+        }
     }
 }
 
@@ -1309,9 +1367,7 @@ int cfmi_jacobi_scale(cfmi_ctx* c, int64_t n, double* A, int64_t lda, double* d)
     hipLaunchKernelGGL(diag_scale_kernel, elem_grid(n), dim3(256), 0, c->stream, n,
                        (const double*)A, lda, d);
     CFMI_LAUNCH_CHECK(c);
-    int64_t tot = n * n;
-    hipLaunchKernelGGL(scale_kernel, elem_grid(tot), dim3(256), 0, c->stream, 2, n, n,
-                       A, lda, (const double*)d);
+    hipLaunchKernelGGL(scale_kernel, ew_grid(n, n), dim3(256), 0, c->stream, 2, n, n, A, lda, (const double*)d);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
@@ -1320,9 +1376,11 @@ int cfmi_scale(cfmi_ctx* c, int side, int64_t m, int64_t n, double* A, int64_t l
 {
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
-    int64_t tot = m * n;
-    hipLaunchKernelGGL(scale_kernel, elem_grid(tot), dim3(256), 0, c->stream,
-                       side ? 1 : 0, m, n, A, lda, d);
+    if (m >= 256)
+        hipLaunchKernelGGL(scale_kernel, ew_grid(m, n), dim3(256), 0, c->stream, side ? 1 : 0, m, n, A, lda, d);
+    else
+        hipLaunchKernelGGL(scale_flat_kernel, elem_grid(m * n), dim3(256), 0, c->stream, side ? 1 : 0, m, n, A, lda,
+                           d);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
@@ -1332,9 +1390,11 @@ int cfmi_axpby(cfmi_ctx* c, int64_t m, int64_t n, double alpha, const double* A,
 {
     if (!c) return CFMI_EINVAL;
     if (m <= 0 || n <= 0) return CFMI_OK;
-    int64_t tot = m * n;
-    hipLaunchKernelGGL(axpby_kernel, elem_grid(tot), dim3(256), 0, c->stream, m, n,
-                       alpha, A, lda, beta, B, ldb);
+    if (m >= 256)
+        hipLaunchKernelGGL(axpby_kernel, ew_grid(m, n), dim3(256), 0, c->stream, m, n, alpha, A, lda, beta, B, ldb);
+    else
+        hipLaunchKernelGGL(axpby_flat_kernel, elem_grid(m * n), dim3(256), 0, c->stream, m, n, alpha, A, lda, beta,
+                           B, ldb);
     CFMI_LAUNCH_CHECK(c);
     return CFMI_OK;
 }
