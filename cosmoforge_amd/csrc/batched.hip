@@ -782,6 +782,10 @@ __device__ __forceinline__ bool step_item(const StepArgs& a, const int64_t* off,
 // MFMA; each wave takes 32-row tiles and reads a tile completely before
 // overwriting it.  One load of L_kk^-1 into LDS serves SPR rows.
 constexpr int SPR = 256;    // panel rows per workgroup
+// Panels per trailing-matrix update (sw_update_kernel).  3 measured 4% faster
+// in mixed mode (164 -> 158 ms per disc batch) but the deeper FP32 sums moved
+// the mixed -2 ln L from 4.6e-4 to 6.7e-4 of the FP64 batch (target 1e-3).
+constexpr int SLA = 2;
 
 template <typename TS>
 __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
@@ -838,13 +842,14 @@ __global__ __launch_bounds__(256) void sw_panel_kernel(StepArgs a)
     }
 }
 
-// Trailing updates with one step of look-ahead, so that the trailing matrix
-// is read and written once per two panels.  Step k (k0 = 64 k, r0 = k0 +
-// kb the first row below the diagonal block):
-//   k even: the next block column only, A[r0:, r0:r0+64] -= P_k P_k[r0:r0+64]^T
-//           (tiles (ti, 0));
-//   k odd:  the whole trailing matrix from both panels, A[r0:, r0:] -=
-//           [P_{k-1} P_k] [P_{k-1} P_k]^T (K = 64 + kb, columns k0 - 64 ..).
+// Trailing updates with look-ahead, so that the trailing matrix is read and
+// written once per SLA panels.  Step k (k0 = 64 k, r0 = k0 + kb the first
+// row below the diagonal block), phase ph = k mod SLA:
+//   ph < SLA - 1: the next block column only, from the panels since the last
+//           trailing update, A[r0:, r0:r0+64] -= Q Q[r0:r0+64]^T with
+//           Q = [P_{k-ph} .. P_k] (K = 64 ph + kb; tiles (ti, 0));
+//   ph = SLA - 1: the whole trailing matrix from the last SLA panels,
+//           A[r0:, r0:] -= Q Q^T, Q = [P_{k-SLA+1} .. P_k].
 // The workgroup stages the two 64-row operand blocks in LDS one 64-deep K
 // chunk at a time (k-major, rows contiguous; a diagonal tile stages one),
 // with coalesced column loads, and fetches its stored tile at the start;
@@ -864,7 +869,8 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
     int job;
     int64_t tile;
     if (!step_item(a, a.seg_uoff, a.seg_upj, job, tile)) return;
-    const bool wide = (a.k0 / SNB) & 1;
+    const int ph = (a.k0 / SNB) % SLA;
+    const bool wide = ph == SLA - 1;
     int ti, tj;
     if (wide) tri_tile((int)tile, ti, tj);
     else ti = (int)tile, tj = 0;
@@ -872,7 +878,7 @@ __global__ __launch_bounds__(BT) void sw_update_kernel(StepArgs a)
     const int k0 = a.k0;
     const int kb = min(SNB, n - k0);
     const int r0 = k0 + kb;
-    const int kbeg = wide ? k0 - SNB : k0, kend = k0 + kb;
+    const int kbeg = k0 - ph * SNB, kend = k0 + kb;
     const int iA = r0 + ti * SNB, iB = r0 + tj * SNB;
     const bool diag = ti == tj;
     TS* A = (TS*)a.work + a.job_work[job];
@@ -1357,8 +1363,8 @@ void plan_steps(cfmi_batch* b, int npatch, const int64_t* n, const std::vector<i
             const int64_t kb = std::min<int64_t>(SNB, g.n - k0);
             const int64_t t = (g.n - k0 - kb + SNB - 1) / SNB;      // trailing 64-tiles per side
             const int64_t tp = (g.n - k0 - kb + SPR - 1) / SPR;     // panel workgroups
-            // look-ahead: even steps update the next block column, odd steps the trailing matrix
-            const int64_t tu = (k0 / SNB) & 1 ? t * (t + 1) / 2 : t;
+            // look-ahead: the next block column, or every SLA-th step the trailing matrix
+            const int64_t tu = (k0 / SNB) % SLA == SLA - 1 ? t * (t + 1) / 2 : t;
             q0.push_back(g.q0);
             ppj.push_back(tp);
             upj.push_back(tu);

@@ -207,6 +207,41 @@ def test_factor_and_inverse(ctx, n):
     assert f.logdet == pytest.approx(LA.matrix_slogdet_symm(C)[1], rel=1e-12, abs=1e-10)
 
 
+@pytest.mark.parametrize("n", [700, 2500])
+def test_potrf_fp32(ctx, n):
+    """cfmi_potrf in FP32 (one-wave FP32 diagonal factor and inverse, FP32
+    panel solves and updates): the factor equals the FP64 Cholesky factor to
+    FP32 rounding times the growth of the factorisation, the diagonal-block
+    inverses are the inverses of its 64 x 64 blocks, the strict upper triangle
+    is zero."""
+    import ctypes
+
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(n + 3)
+    X = rng.standard_normal((n, n // 2))
+    A = X @ X.T / n + np.eye(n)
+    d = 1.0 / np.sqrt(np.diag(A))
+    A = d[:, None] * A * d[None, :]
+    C = ctx.upload_cm(A, dtype=torch.float32)
+    U = torch.zeros((n, n), dtype=torch.float32, device=ctx.device)
+    info = ctypes.c_int(-1)
+    rc = ctx.lib.cfmi_potrf(ctx.h, engine.F32, n, engine.P(C), n, engine.P(U), n, ctypes.byref(info))
+    assert rc == 0 and info.value == 0
+    L = C.cpu().numpy().T.astype(np.float64)
+    Lref = np.linalg.cholesky(A)
+    assert not np.triu(L, 1).any()
+    assert np.max(np.abs(L - Lref)) < 2e-5
+    Uh = U.cpu().numpy().T.astype(np.float64)
+    for k0 in (0, 64 * ((n // 64) - 1), 64 * (n // 64)):
+        kb = min(64, n - k0)
+        if kb <= 0:
+            continue
+        blk = L[k0:k0 + kb, k0:k0 + kb]
+        np.testing.assert_allclose(Uh[k0:k0 + kb, k0:k0 + kb] @ blk, np.eye(kb), atol=1e-5)
+
+
 def test_factor_not_pd_raises(ctx):
     from cosmoforge_amd import engine
 
