@@ -44,11 +44,11 @@ sys.path.insert(0, ROOT)
 METRIC = "Fisher+QML time-to-solution & picslike evals/s, mixed vs FP64 at stated error"
 INT8_MFMA_PEAK_TOPS = 5000.0       # MI355X dense int8 MFMA
 FP64_MFMA_PEAK_TFLOPS = 78.6
-# HBM bytes per gemm_s8 launch of the config-2 mixed solve (mean over its 140
+# HBM bytes per gemm_s8 launch of the config-2 mixed solve (mean over its 150
 # launches), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one
 # solve, each scaled by what the counter reads for a plain 1 GiB device copy
 # (FETCH x 2.0, WRITE x 1.0); not measured live.  Keyed by config.
-S8_TRAFFIC_BYTES_PER_LAUNCH = {2: 4.62e9}
+S8_TRAFFIC_BYTES_PER_LAUNCH = {2: 4.31e9}
 
 
 def log(msg):

@@ -24,6 +24,16 @@ struct cfmi_ctx {
     // Growable workspace for split-K partials and trtri temporaries.
     void* d_work = nullptr;
     size_t work_bytes = 0;
+    // The blocked Cholesky's look-ahead: a second stream (default priority,
+    // created on first use) with a workspace of its own, the two events that
+    // order it against the context's stream, and the int8 slices of the
+    // current panel, which both streams read.
+    hipStream_t aux_stream = nullptr;
+    void* aux_work = nullptr;
+    size_t aux_work_bytes = 0;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void* d_panel = nullptr;
+    size_t panel_bytes = 0;
     // Sliced-GEMM launch record (cfmi_s8_stats_*): HIP events around every
     // gemm_s8 kernel launch on the stream while enabled, with each launch's
     // algorithmic flops and slice count.

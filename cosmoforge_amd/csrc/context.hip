@@ -78,6 +78,12 @@ int cfmi_destroy(cfmi_ctx* c)
         hipEventDestroy(l.stop);
     }
     for (auto e : c->s8_event_pool) hipEventDestroy(e);
+    if (c->aux_stream) hipStreamSynchronize(c->aux_stream);
+    if (c->aux_work) hipFree(c->aux_work);
+    if (c->d_panel) hipFree(c->d_panel);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->aux_stream) hipStreamDestroy(c->aux_stream);
     if (c->d_work) hipFree(c->d_work);
     if (c->d_info) hipFree(c->d_info);
     if (c->d_red) hipFree(c->d_red);

@@ -242,6 +242,47 @@ def test_potrf_fp32(ctx, n):
         np.testing.assert_allclose(Uh[k0:k0 + kb, k0:k0 + kb] @ blk, np.eye(kb), atol=1e-5)
 
 
+@pytest.mark.parametrize("n", [6500, 8200])
+def test_potrf_lookahead(ctx, n):
+    """Factorisations past two 2048-column panels update the next panel's
+    columns on the context's stream and the rest of the trailing matrix on a
+    second stream while the next panel is factored.  The FP64 factor equals
+    the host Cholesky factor, the FP64 and sliced factors are the same bits
+    on a repeat (the ordering between the streams leaves no race), and the
+    sliced factor equals the FP64 one to the slicing level.  n = 8200 ends
+    with an 8-column panel."""
+    import ctypes
+
+    import torch
+    from cosmoforge_amd import engine
+
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n // 4))
+    A = X @ X.T / n + np.eye(n)
+    d = 1.0 / np.sqrt(np.diag(A))
+    A = d[:, None] * A * d[None, :]
+    dA = ctx.upload_cm(A)
+    runs = {}
+    for mode in ("fp64", "fp64", "s8", "s8"):
+        C = dA.clone()
+        info = ctypes.c_int(-1)
+        if mode == "fp64":
+            rc = ctx.lib.cfmi_potrf(ctx.h, engine.F64, n, engine.P(C), n, None, n, ctypes.byref(info))
+        else:
+            rc = ctx.lib.cfmi_potrf_s8(ctx.h, 5, n, engine.P(C), n, None, n, ctypes.byref(info))
+        assert rc == 0 and info.value == 0
+        runs.setdefault(mode, []).append(C.cpu().numpy().T.copy())
+        del C
+        torch.cuda.empty_cache()
+    for mode, (a, b) in runs.items():
+        np.testing.assert_array_equal(a, b, err_msg=mode)
+    L = runs["fp64"][0]
+    assert not np.triu(L, 1).any()
+    Lref = np.linalg.cholesky(A)
+    assert np.max(np.abs(L - Lref)) < 1e-12
+    assert np.max(np.abs(runs["s8"][0] - L)) < 1e-8
+
+
 def test_factor_not_pd_raises(ctx):
     from cosmoforge_amd import engine
 
