@@ -268,14 +268,17 @@ int cfmi_jacobi_scale(cfmi_ctx *ctx, int64_t n, double *A, int64_t lda, double *
 
 /* Lower Cholesky A = L L^T in place (upper triangle zeroed).  If U != NULL
  * the diagonal 64 x 64 blocks of L^-1 are written into U (rest zeroed), to
- * be completed by cfmi_trtri.  *info (host) receives the LAPACK info.      */
+ * be completed by cfmi_trtri.  *info (host) receives the LAPACK info.
+ * For n > 4096 the trailing updates past the next 2048-column panel run on
+ * a second stream that the context owns (look-ahead), joined to the
+ * context's stream before the call synchronizes that stream and returns. */
 int cfmi_potrf(cfmi_ctx *ctx, int dtype, int64_t n, void *A, int64_t lda, void *U, int64_t ldu,
                int *info);
 
 /* Launch record of the sliced GEMM on this context (no reference
- * counterpart; measurement): while enabled, HIP events on the context's
+ * counterpart; measurement): while enabled, HIP events on the launch's
  * stream bracket every gemm_s8 kernel launch, including those the library
- * issues itself (cfmi_potrf_s8).  enable(on) synchronizes and clears the
+ * issues itself (cfmi_potrf_s8, some of them on its look-ahead stream).  enable(on) synchronizes and clears the
  * record; read synchronizes and returns the launch count, the summed
  * launch time (ms), the algorithmic flops (2 k per entry computed: 2 m n k,
  * k (m n - n (n - 1) / 2) for a lower or symmetric result, i.e. about m n k
